@@ -1,0 +1,38 @@
+"""Copies the reference's own acceptance fixtures (spec/files/*.png, data
+produced by the reference's `zig build spec -Dupdate=true`) that cover the
+fill/composite hot path into tests/golden/spec/, with a SHA-256 manifest.
+Reads /root/reference; run once where the reference is mounted."""
+import glob
+import hashlib
+import os
+import shutil
+import sys
+
+REF = os.environ.get("Z2D_REFERENCE", "/root/reference")
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "spec")
+PREFIXES = ["001", "002", "003", "004", "005", "006", "007", "013", "016", "023", "024",
+            "025", "026", "028", "031", "032", "033", "046", "047", "048", "049", "050",
+            "051", "052", "053", "059", "061", "062", "063", "064", "065", "066", "067",
+            "068", "069", "070", "071", "072", "075", "076", "077", "078", "079", "080"]
+
+
+def main():
+    src_dir = os.path.join(REF, "spec", "files")
+    if not os.path.isdir(src_dir):
+        sys.exit(f"reference fixtures not found at {src_dir}")
+    os.makedirs(OUT, exist_ok=True)
+    lines = []
+    for pre in PREFIXES:
+        for f in sorted(glob.glob(os.path.join(src_dir, pre + "_*.png"))):
+            dst = os.path.join(OUT, os.path.basename(f))
+            shutil.copyfile(f, dst)
+            h = hashlib.sha256(open(dst, "rb").read()).hexdigest()
+            lines.append(f"{h}  {os.path.basename(f)}")
+    with open(os.path.join(OUT, "SHA256SUMS"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(f"copied {len(lines)} fixtures")
+
+
+if __name__ == "__main__":
+    main()

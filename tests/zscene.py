@@ -1,0 +1,370 @@
+"""Test-scene builder: a small Python restatement of the reference's managed
+Context/Path front end (src/Context.zig, src/Path.zig,
+src/Transformation.zig) that produces the same PathNode arrays, patterns and
+options the painter receives.  Scenes recorded with it replay on any backend
+(the oracle or the HIP library) through `Scene.run(backend)`."""
+import ctypes as C
+import math
+from fractions import Fraction
+
+import numpy as np
+
+from z2d_amd import abi
+
+I24_MIN, I24_MAX = -(1 << 23), (1 << 23) - 1
+
+
+def cf(x):
+    """Comptime-float arithmetic helper: exact rational -> nearest f64 (Zig
+    evaluates comptime float expressions at f128 and coerces once)."""
+    return float(Fraction(x))
+
+
+class Transformation:
+    """src/Transformation.zig (identity, mul, translate, scale, rotate,
+    inverse, userToDevice)."""
+
+    def __init__(self, ax=1.0, by=0.0, cx=0.0, dy=1.0, tx=0.0, ty=0.0):
+        self.ax, self.by, self.cx, self.dy, self.tx, self.ty = ax, by, cx, dy, tx, ty
+
+    def tuple(self):
+        return (self.ax, self.by, self.cx, self.dy, self.tx, self.ty)
+
+    def is_identity(self):
+        return self.tuple() == abi.IDENTITY
+
+    def mul(self, b):
+        a = self
+        return Transformation(
+            ax=a.ax * b.ax + a.by * b.cx,
+            by=a.ax * b.by + a.by * b.dy,
+            cx=a.cx * b.ax + a.dy * b.cx,
+            dy=a.cx * b.by + a.dy * b.dy,
+            tx=a.ax * b.tx + a.by * b.ty + a.tx,
+            ty=a.cx * b.tx + a.dy * b.ty + a.ty,
+        )
+
+    def translate(self, tx, ty):
+        return self.mul(Transformation(tx=float(tx), ty=float(ty)))
+
+    def scale(self, sx, sy):
+        return self.mul(Transformation(ax=float(sx), dy=float(sy)))
+
+    def rotate(self, angle):
+        s, c = math.sin(angle), math.cos(angle)
+        return self.mul(Transformation(ax=c, by=-s, cx=s, dy=c))
+
+    def inverse(self):
+        a = self
+        if a.by == 0 and a.cx == 0:
+            if a.ax == 0 or a.dy == 0:
+                raise ValueError("InvalidMatrix")
+            if a.ax != 1 or a.dy != 1:
+                return Transformation(1 / a.ax, 0.0, 0.0, 1 / a.dy, -a.tx / a.ax, -a.ty / a.dy)
+            return Transformation(1.0, 0.0, 0.0, 1.0, -a.tx, -a.ty)
+        det = a.ax * a.dy - a.by * a.cx
+        if det == 0:
+            raise ValueError("InvalidMatrix")
+        adj = (a.dy, -a.by, -a.cx, a.ax, a.by * a.ty - a.dy * a.tx, a.cx * a.tx - a.ax * a.ty)
+        inv = 1 / det
+        ax, by, cx, dy, tx, ty = adj
+        return Transformation(ax * inv, by * inv, cx * inv, dy * inv, tx * inv, ty * inv)
+
+    def user_to_device(self, x, y):
+        ox = self.ax * x + self.by * y
+        oy = self.cx * x + self.dy * y
+        return ox + self.tx, oy + self.ty
+
+
+def clamp_i24(v):
+    return min(max(float(v), float(I24_MIN)), float(I24_MAX))
+
+
+class Path:
+    """src/Path.zig:116-262, :447-477, :570-572."""
+
+    def __init__(self):
+        self.nodes = []
+        self.initial = None
+        self.current = None
+        self.transformation = Transformation()
+
+    def reset(self):
+        self.nodes = []
+        self.initial = None
+        self.current = None
+
+    def _pt(self, x, y):
+        return self.transformation.user_to_device(clamp_i24(x), clamp_i24(y))
+
+    def move_to(self, x, y):
+        p = self._pt(x, y)
+        if self.nodes and self.nodes[-1][0] == abi.MOVE_TO and self.nodes[-1][1:3] == p:
+            return
+        self.nodes.append((abi.MOVE_TO, p[0], p[1]))
+        self.initial = p
+        self.current = p
+
+    def line_to(self, x, y):
+        if self.current is None:
+            return self.move_to(x, y)
+        p = self._pt(x, y)
+        self.nodes.append((abi.LINE_TO, p[0], p[1]))
+        self.current = p
+
+    def curve_to(self, x1, y1, x2, y2, x3, y3):
+        if self.current is None:
+            raise ValueError("NoCurrentPoint")
+        p1, p2, p3 = self._pt(x1, y1), self._pt(x2, y2), self._pt(x3, y3)
+        self.nodes.append((abi.CURVE_TO, p1[0], p1[1], p2[0], p2[1], p3[0], p3[1]))
+        self.current = p3
+
+    def close(self):
+        if self.current is None:
+            return
+        self.nodes.append((abi.CLOSE_PATH,))
+        self.nodes.append((abi.MOVE_TO, self.initial[0], self.initial[1]))
+
+    def node_array(self):
+        return abi.nodes_array(self.nodes)
+
+
+class Surface:
+    """A host surface buffer (src/surface.zig:97-184 init/initPixel)."""
+
+    def __init__(self, fmt, width, height, init_px=None):
+        self.format, self.width, self.height = fmt, width, height
+        self.buf = np.zeros(abi.surface_nbytes(fmt, width, height) + 8, dtype=np.uint8)
+        self.nbytes = abi.surface_nbytes(fmt, width, height)
+        if init_px is not None:
+            self.paint(init_px)
+
+    def desc(self):
+        d = abi.SurfaceDesc()
+        d.format = self.format
+        d.width = self.width
+        d.height = self.height
+        d.data = self.buf.ctypes.data
+        d.len = self.nbytes
+        return d
+
+    def paint(self, px):
+        """ImageSurface.initBuffer/paintPixel: every pixel = T.fromPixel(px)."""
+        raw = pixel_bytes(self.format, px)
+        bits = abi.FORMAT_BITS[self.format]
+        if bits == 32:
+            self.buf[: self.nbytes] = np.tile(np.frombuffer(bytes(raw), np.uint8),
+                                              self.width * self.height)
+        elif bits == 8:
+            self.buf[: self.nbytes] = raw[0]
+        else:
+            v = raw[0]
+            packed = 0
+            sh = 0
+            while sh <= 8 - bits:
+                packed |= v << sh
+                sh += bits
+            self.buf[: self.nbytes] = packed if v != 0 else 0
+
+    def copy(self):
+        s = Surface.__new__(Surface)
+        s.format, s.width, s.height, s.nbytes = self.format, self.width, self.height, self.nbytes
+        s.buf = self.buf.copy()
+        return s
+
+    def data(self):
+        return self.buf[: self.nbytes]
+
+
+def _scale_alpha(v, from_bits, to_bits):
+    """Alpha.scaleIntoIntAlpha (src/pixel.zig:582-630)."""
+    if from_bits == to_bits or v == 0:
+        return v
+    if from_bits == 1:
+        return v * ((1 << to_bits) - 1)
+    if from_bits > to_bits:
+        return v >> (from_bits - to_bits)
+    d = to_bits - from_bits
+    if d in (2, 4):
+        return (v << d) + v
+    return (v << 6) | (v << 4) | (v << 2) | v
+
+
+def pixel_rgba(px):
+    """pixel.RGBA.fromPixel (src/pixel.zig:399-432)."""
+    f = px.format
+    if f in (abi.RGBA, abi.ARGB):
+        return px.r, px.g, px.b, px.a
+    if f in (abi.RGB, abi.XRGB):
+        return px.r, px.g, px.b, 255
+    return 0, 0, 0, _scale_alpha(px.a, abi.FORMAT_BITS[f], 8)
+
+
+def pixel_bytes(fmt, px):
+    """Bytes of T.fromPixel(px) for a surface of format `fmt`."""
+    r, g, b, a = pixel_rgba(px)
+    if fmt == abi.RGBA:
+        return [r, g, b, a]
+    if fmt == abi.ARGB:
+        return [b, g, r, a]
+    if fmt == abi.RGB:
+        return [r, g, b, 0]
+    if fmt == abi.XRGB:
+        return [b, g, r, 0]
+    if px.format in (abi.RGB, abi.XRGB):
+        return [(1 << abi.FORMAT_BITS[fmt]) - 1 if abi.FORMAT_BITS[fmt] < 8 else 255]
+    bits = abi.FORMAT_BITS[fmt]
+    src_bits = abi.FORMAT_BITS[px.format] if px.format >= abi.ALPHA8 else 8
+    return [_scale_alpha(px.a if px.format >= abi.ALPHA8 else a, src_bits, bits)]
+
+
+def color_rgba_encode(r, g, b, a=1.0):
+    """Pixel.fromColor for a linear rgb(a) color (src/pixel.zig:118-121):
+    round(255*c) per channel (f32) then RGBA.multiply."""
+    f32 = np.float32
+    vals = [int(math.floor(float(f32(255.0) * f32(v)) + 0.5)) for v in (r, g, b, a)]
+    aa = vals[3]
+    return abi.rgba(vals[0] * aa // 255, vals[1] * aa // 255, vals[2] * aa // 255, aa)
+
+
+class Context:
+    """Recorder for the managed Context (src/Context.zig) subset the hot-path
+    scenes use.  Each fill/stroke/composite is recorded with the options in
+    effect at the call."""
+
+    def __init__(self, surface):
+        self.surface = surface
+        self.path = Path()
+        self.pattern = abi.solid_pattern(abi.rgba(0, 0, 0, 255))
+        self.keep = []
+        self.aa = abi.AA_DEFAULT
+        self.fill_rule = abi.NON_ZERO
+        self.op = abi.OP["src_over"]
+        self.precision = abi.PRECISION_INTEGER
+        self.tolerance = 0.1
+        self.dither = abi.DITHER_NONE
+        self.line_width = 2.0
+        self.cap = abi.CAP_BUTT
+        self.join = abi.JOIN_MITER
+        self.miter_limit = 10.0
+        self.dashes = []
+        self.dash_offset = 0.0
+        self.transformation = Transformation()
+        self.ops = []
+
+    # --- setters --------------------------------------------------------
+    def set_source_pixel(self, px):
+        self.pattern = abi.solid_pattern(px)
+
+    def set_source_gradient(self, grad):
+        self.keep.append(grad)
+        self.pattern = grad.pattern()
+
+    def set_transformation(self, t):
+        self.transformation = t
+        self.path.transformation = t
+
+    def translate(self, tx, ty):
+        self.set_transformation(self.transformation.translate(tx, ty))
+
+    def scale(self, sx, sy):
+        self.set_transformation(self.transformation.scale(sx, sy))
+
+    def rotate(self, a):
+        self.set_transformation(self.transformation.rotate(a))
+
+    def set_identity(self):
+        self.set_transformation(Transformation())
+
+    # --- path -----------------------------------------------------------
+    def move_to(self, x, y):
+        self.path.move_to(x, y)
+
+    def line_to(self, x, y):
+        self.path.line_to(x, y)
+
+    def curve_to(self, *a):
+        self.path.curve_to(*a)
+
+    def close_path(self):
+        self.path.close()
+
+    def reset_path(self):
+        self.path.reset()
+
+    # --- drawing ----------------------------------------------------------
+    def _wrapped_pattern(self):
+        """Context.wrapDither (src/Context.zig:679-699)."""
+        if self.dither == abi.DITHER_NONE:
+            return self.pattern
+        p = self.pattern
+        scale = {abi.ALPHA1: 1, abi.ALPHA2: 2, abi.ALPHA4: 4}.get(self.surface.format, 8)
+        d = abi.PatternDesc()
+        d.kind = abi.PATTERN_DITHER
+        d.dither.type = self.dither
+        d.dither.scale = scale
+        if p.kind == abi.PATTERN_OPAQUE:
+            d.dither.source = abi.DITHER_SRC_PIXEL
+            d.dither.pixel = p.pixel
+        elif p.kind == abi.PATTERN_GRADIENT:
+            d.dither.source = abi.DITHER_SRC_GRADIENT
+            d.dither.gradient = p.gradient
+        else:
+            return p
+        return d
+
+    def fill(self):
+        opts = abi.fill_opts(self.aa, self.fill_rule, self.op, self.precision, self.tolerance)
+        self.ops.append(("fill", self._wrapped_pattern(), self.path.node_array(), opts))
+
+    def stroke(self):
+        o = abi.StrokeOpts()
+        o.anti_aliasing_mode = self.aa
+        o.line_cap_mode = self.cap
+        o.line_join_mode = self.join
+        o.op = self.op
+        o.precision = self.precision
+        o.hairline = 0
+        dashes = (C.c_double * max(1, len(self.dashes)))(*self.dashes)
+        self.keep.append(dashes)
+        o.dashes = C.cast(dashes, C.POINTER(C.c_double))
+        o.n_dashes = len(self.dashes)
+        o.dash_offset = self.dash_offset
+        o.line_width = self.line_width
+        o.miter_limit = self.miter_limit
+        o.tolerance = self.tolerance
+        o.transformation[:] = self.transformation.tuple()
+        self.ops.append(("stroke", self._wrapped_pattern(), self.path.node_array(), o))
+
+
+class Scene:
+    """A recorded scene: a target surface plus the painter calls made on it."""
+
+    def __init__(self, name, surface, ops, keep=(), extra_surfaces=()):
+        self.name = name
+        self.surface = surface
+        self.ops = ops
+        self.keep = list(keep)
+        self.extra = list(extra_surfaces)
+
+    def run(self, backend):
+        """Replays the calls on a fresh copy of the target; returns it."""
+        sfc = self.surface.copy()
+        for op in self.ops:
+            kind = op[0]
+            if kind == "fill":
+                _, pat, nodes, opts = op
+                rc = backend.fill(sfc, pat, nodes, opts)
+            elif kind == "stroke":
+                _, pat, nodes, opts = op
+                rc = backend.stroke(sfc, pat, nodes, opts)
+            elif kind == "composite":
+                _, src_fn, dst_x, dst_y, ops, precision = op
+                rc = backend.composite(sfc, dst_x, dst_y, ops, precision)
+            elif kind == "call":
+                rc = op[1](sfc, backend)
+            else:
+                raise ValueError(kind)
+            if rc != 0:
+                raise RuntimeError(f"{self.name}: {kind} returned {rc}")
+        return sfc
