@@ -1,0 +1,127 @@
+/*
+ * z2d_gpu.h - C ABI of the MI355X backend for z2d's painter hot path.
+ *
+ * The reference's boundary is painter.fill / painter.stroke
+ * (src/painter.zig:66-143, :214-344) plus the compositor entry points
+ * SurfaceCompositor.run (src/compositor.zig:302-440) and Surface.composite
+ * (src/surface.zig:225-241).  A Zig shim keeping those signatures marshals
+ * its tagged unions into the POD descriptors of z2d_types.h and calls the
+ * functions below; INTEGRATION.md shows that binding.
+ *
+ * All functions return a z2d_status code (0 = ok).  Argument checks run on
+ * the host before any device work, in the reference's order.
+ */
+#ifndef Z2D_GPU_H
+#define Z2D_GPU_H
+
+#include "z2d_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct z2d_gpu_ctx z2d_gpu_ctx;
+typedef struct z2d_gpu_surface z2d_gpu_surface;
+typedef struct z2d_gpu_batch z2d_gpu_batch;
+
+/* A z2d_surface_desc whose `reserved` field holds this value refers to
+ * device memory (from z2d_gpu_surface_desc) instead of a host buffer. */
+#define Z2D_SURFACE_DEVICE 1u
+
+/* ---- context -------------------------------------------------------------
+ * One context per host thread; it owns a HIP stream and a device arena.
+ * No reference counterpart (the reference allocates per call through its
+ * `alloc` parameter, src/painter.zig:67). */
+int32_t z2d_gpu_create(int32_t device, z2d_gpu_ctx **out);
+void z2d_gpu_destroy(z2d_gpu_ctx *ctx);
+const char *z2d_gpu_last_error(z2d_gpu_ctx *ctx);
+
+/* ---- painter ---------------------------------------------------------------
+ * painter.fill (src/painter.zig:66-143).  Synchronous: on return the caller's
+ * surface buffer holds the result (host buffers are uploaded and read back;
+ * device descriptors are rendered in place).  Errors:
+ * PIXEL_SOURCE_NOT_PREMULTIPLIED, PATH_NOT_CLOSED, INVALID_STATE; zero nodes
+ * is a no-op, as in the reference. */
+int32_t z2d_gpu_fill(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                     const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                     uint64_t n_nodes, const z2d_fill_opts *opts);
+
+/* painter.stroke (src/painter.zig:214-344): stroke tessellation runs on the
+ * host and feeds the same rasterizer (non-zero rule). */
+int32_t z2d_gpu_stroke(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                       const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                       uint64_t n_nodes, const z2d_stroke_opts *opts);
+
+/* The rasterizer step of painter.fill/stroke on an existing edge list
+ * (direct.run / supersample.run / multisample.run,
+ * src/internal/raster/{direct,supersample,multisample}.zig), with extents as
+ * Polygon.extent_{top,bottom,left,right}; edges are in the scaled space of
+ * the AA mode (x1 for none, x4 otherwise). */
+int32_t z2d_gpu_fill_polygon(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                             const z2d_pattern_desc *pattern, const z2d_edge *edges,
+                             uint64_t n_edges, const double extents[4],
+                             uint32_t anti_aliasing_mode, uint32_t fill_rule,
+                             uint32_t op, uint32_t precision);
+
+/* ---- compositor ----------------------------------------------------------
+ * SurfaceCompositor.run (src/compositor.zig:302-440): `n_ops` operations
+ * applied at (dst_x, dst_y).  Surface parameters may be host or device
+ * descriptors.  If `out` is non-NULL the result goes to `out` (same format
+ * and size as dst) and dst is left untouched; NULL = in place, as the
+ * reference. */
+int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x,
+                          int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
+                          uint32_t precision, z2d_surface_desc *out);
+
+/* ---- device-resident surfaces ----------------------------------------------
+ * Surface.init / initBuffer (src/surface.zig:97-184) for surfaces that stay
+ * in HBM across calls. */
+int32_t z2d_gpu_surface_create(z2d_gpu_ctx *ctx, uint32_t format, int32_t width,
+                               int32_t height, z2d_gpu_surface **out);
+int32_t z2d_gpu_surface_upload(z2d_gpu_surface *s, const void *host, uint64_t len);
+int32_t z2d_gpu_surface_download(z2d_gpu_surface *s, void *host, uint64_t len);
+/* Fills a descriptor (reserved = Z2D_SURFACE_DEVICE) for use with the
+ * painter and compositor calls above. */
+int32_t z2d_gpu_surface_desc(z2d_gpu_surface *s, z2d_surface_desc *out);
+void z2d_gpu_surface_destroy(z2d_gpu_surface *s);
+
+/* ---- batches ----------------------------------------------------------------
+ * Many fills on many resident surfaces rendered by one pipeline launch
+ * (edge build, binning, one persistent tile-render launch).  Draws on one
+ * surface are applied in the order they were added, exactly like
+ * successive painter.fill calls.  No reference counterpart: it batches the
+ * reference's per-call painter.fill. */
+int32_t z2d_gpu_batch_create(z2d_gpu_ctx *ctx, z2d_gpu_batch **out);
+int32_t z2d_gpu_batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                           const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                           uint64_t n_nodes, const z2d_fill_opts *opts);
+int32_t z2d_gpu_batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                             const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                             uint64_t n_nodes, const z2d_stroke_opts *opts);
+int32_t z2d_gpu_batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                                   const z2d_pattern_desc *pattern, const z2d_edge *edges,
+                                   uint64_t n_edges, const double extents[4],
+                                   uint32_t anti_aliasing_mode, uint32_t fill_rule,
+                                   uint32_t op, uint32_t precision);
+/* Uploads the batch's inputs to HBM (no rendering). */
+int32_t z2d_gpu_batch_prepare(z2d_gpu_batch *b);
+/* Renders the prepared batch.  mode 0: one persistent launch for all
+ * surfaces; mode 1: one launch per surface.  Asynchronous. */
+int32_t z2d_gpu_batch_render(z2d_gpu_batch *b, uint32_t mode);
+int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b);
+/* Device time of the last render: whole pipeline and the tile-render
+ * kernel(s) alone, in milliseconds (HIP events on the context stream). */
+int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms);
+/* Number of draws / surfaces / flattened nodes in the batch. */
+int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]);
+int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b);
+void z2d_gpu_batch_destroy(z2d_gpu_batch *b);
+
+/* Synchronizes the context stream. */
+int32_t z2d_gpu_sync(z2d_gpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* Z2D_GPU_H */

@@ -1,0 +1,1450 @@
+// host.cpp - host runtime behind include/z2d_gpu.h.
+//
+// Mirrors the reference's painter entry points (src/painter.zig:66-344) and
+// SurfaceCompositor.run (src/compositor.zig:302-440): argument checks on the
+// host in the reference's order, curve flattening with the reference's
+// Spline algorithm, then one device pipeline per batch (render.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/z2d_gpu.h"
+#include "render.h"
+
+using namespace z2d;
+
+// launchers (render.hip)
+hipError_t z2d_launch_edge_build(const JobView &J, hipStream_t st);
+hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n,
+                              z2d_edge *sedges, int32_t *sx, const uint32_t *soff,
+                              hipStream_t st);
+hipError_t z2d_launch_binning(const JobView &J, hipStream_t st);
+hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
+                             uint32_t *cursor, uint32_t n_blocks, hipStream_t st);
+hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
+                                bool stream, hipStream_t st);
+hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+
+static const uint16_t kBlueNoise[4096] = {
+#include "blue_noise_64x64.inc"
+};
+
+// ---------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
+
+struct DevBuf {
+  uint8_t *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(n, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return e;
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct HostBuf {
+  uint8_t *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(n, (size_t)1 << 20);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), want, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Sub-allocation plan inside one arena.
+struct Layout {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = off;
+    off += align_up(bytes ? bytes : 1);
+    return o;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context / surfaces
+// ---------------------------------------------------------------------------
+struct z2d_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint16_t *blue_noise = nullptr;
+  std::string err;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf comp_arena;
+  HostBuf comp_stage;
+  int n_cus = 256;
+};
+
+struct z2d_gpu_surface {
+  z2d_gpu_ctx *ctx = nullptr;
+  uint32_t format = 0;
+  int32_t w = 0, h = 0;
+  uint8_t *data = nullptr;
+  uint64_t nbytes = 0;
+  bool owned = true;
+};
+
+static int32_t fail(z2d_gpu_ctx *ctx, hipError_t e, const char *what) {
+  if (ctx) {
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+  }
+  return Z2D_ERR_DEVICE;
+}
+#define HIP_OK(ctx, expr)                                 \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return fail((ctx), _e, #expr);  \
+  } while (0)
+
+static inline int fbits(uint32_t f) { return fmt_bits(f); }
+static uint64_t surface_bytes(uint32_t fmt, int32_t w, int32_t h) {
+  uint64_t n = (uint64_t)w * (uint64_t)h;
+  int b = fbits(fmt);
+  return b == 32 ? n * 4 : (n * (uint64_t)b + 7) / 8;
+}
+// Device allocations are padded to whole 32-bit words (+4) so packed-alpha
+// atomics on the last word stay inside the allocation.
+static uint64_t padded_bytes(uint64_t n) { return ((n + 3) & ~3ull) + 4; }
+
+// ---------------------------------------------------------------------------
+// patterns and gradients (host side of DevPaint / DevGradient)
+// ---------------------------------------------------------------------------
+namespace {
+
+inline float clampf_(float v, float lo, float hi) { return fmaxf(lo, fminf(v, hi)); }
+inline float zmodf_h(float x, float y) {
+  float a = fmodf(x, y);
+  return x < 0.0f ? fmodf(a + y, y) : a;
+}
+
+// Color.init clamping (src/color.zig:81-92, :216-223, :416-423).
+DevStop stop_from(const z2d_stop &s) {
+  DevStop o;
+  o.offset = clampf_(s.offset, 0.0f, 1.0f);
+  o.space = s.color.space;
+  if (s.color.space == Z2D_COLOR_HSL) {
+    float h = s.color.v[0];
+    o.v[0] = (h < 0.0f || h > 360.0f) ? zmodf_h(h, 360.0f) : h;
+    for (int i = 1; i < 4; i++) o.v[i] = clampf_(s.color.v[i], 0.0f, 1.0f);
+  } else {
+    for (int i = 0; i < 4; i++) o.v[i] = clampf_(s.color.v[i], 0.0f, 1.0f);
+  }
+  return o;
+}
+
+// Host copy of zpowf (same algorithm as the device's).
+float zpowf_h(float x, float y) {
+  if (y == 0.0f || x == 1.0f) return 1.0f;
+  if (std::isnan(x) || std::isnan(y)) return NAN;
+  if (y == 1.0f) return x;
+  if (x == 0.0f) {
+    float ip;
+    bool odd = std::modf(y, &ip) == 0.0f && std::fmod(std::fabs(ip), 2.0f) == 1.0f;
+    if (y < 0.0f) return odd ? std::copysign(INFINITY, x) : INFINITY;
+    return odd ? x : 0.0f;
+  }
+  if (std::isinf(y) || std::isinf(x)) return std::pow(x, y);
+  if (y == 0.5f) return std::sqrt(x);
+  if (y == -0.5f) return 1.0f / std::sqrt(x);
+  float yi;
+  float yf = std::modf(std::fabs(y), &yi);
+  if (yf != 0.0f && x < 0.0f) return NAN;
+  float a1 = 1.0f;
+  int ae = 0;
+  if (yf != 0.0f) {
+    if (yf > 0.5f) {
+      yf -= 1.0f;
+      yi += 1.0f;
+    }
+    a1 = expf(yf * logf(x));
+  }
+  int xe;
+  float x1 = frexpf(x, &xe);
+  int i = (int)yi;
+  while (i != 0) {
+    if (xe < -(1 << 9) || (1 << 9) < xe) {
+      ae += xe;
+      break;
+    }
+    if (i & 1) {
+      a1 *= x1;
+      ae += xe;
+    }
+    x1 *= x1;
+    xe <<= 1;
+    if (x1 < 0.5f) {
+      x1 += x1;
+      xe -= 1;
+    }
+    i >>= 1;
+  }
+  if (y < 0.0f) {
+    a1 = 1.0f / a1;
+    ae = -ae;
+  }
+  return scalbnf(a1, ae);
+}
+
+void hsl_to_rgb_h(const float *h, float *o) {
+  float hue = fmodf(h[0], 360.0f);
+  if (hue < 0.0f) hue += 360.0f;
+  const float ns[3] = {0.0f, 8.0f, 4.0f};
+  for (int i = 0; i < 3; i++) {
+    float k = fmodf(ns[i] + hue / 30.0f, 12.0f);
+    float a = h[1] * fminf(h[2], 1.0f - h[2]);
+    o[i] = h[2] - a * fmaxf(-1.0f, fminf(fminf(k - 3.0f, 9.0f - k), 1.0f));
+  }
+  o[3] = h[3];
+}
+
+// LinearRGB.fromColor (src/color.zig:226-248) of an initialized color.
+void color_linear_h(const DevStop &s, float *o) {
+  if (s.space == Z2D_COLOR_HSL) {
+    hsl_to_rgb_h(s.v, o);
+    return;
+  }
+  if (s.space == Z2D_COLOR_LINEAR_RGB) {
+    for (int i = 0; i < 4; i++) o[i] = s.v[i];
+    return;
+  }
+  for (int i = 0; i < 3; i++) o[i] = zpowf_h(zpowf_h(s.v[i], 2.2f), 1.0f);
+  o[3] = s.v[3];
+}
+
+I4 pixel_rgba(const z2d_pixel &p) {
+  I4 o{0, 0, 0, 0};
+  switch (p.format) {
+    case Z2D_FORMAT_RGBA:
+    case Z2D_FORMAT_ARGB: o = I4{p.r, p.g, p.b, p.a}; break;
+    case Z2D_FORMAT_RGB:
+    case Z2D_FORMAT_XRGB: o = I4{p.r, p.g, p.b, 255}; break;
+    case Z2D_FORMAT_ALPHA8: o.a = p.a; break;
+    case Z2D_FORMAT_ALPHA4: o.a = (uint32_t)p.a * 17u; break;
+    case Z2D_FORMAT_ALPHA2: o.a = (uint32_t)p.a * 85u; break;
+    default: o.a = (uint32_t)p.a * 255u; break;
+  }
+  return o;
+}
+bool pixel_opaque(const z2d_pixel &p) {
+  switch (p.format) {
+    case Z2D_FORMAT_XRGB:
+    case Z2D_FORMAT_RGB: return true;
+    case Z2D_FORMAT_ARGB:
+    case Z2D_FORMAT_RGBA:
+    case Z2D_FORMAT_ALPHA8: return p.a == 255;
+    case Z2D_FORMAT_ALPHA4: return p.a == 15;
+    case Z2D_FORMAT_ALPHA2: return p.a == 3;
+    default: return p.a == 1;
+  }
+}
+// RGBA.canDemultiply (src/pixel.zig:504-517).
+bool can_demultiply(const z2d_pixel &p) {
+  if (p.a == 0) return true;
+  return (uint32_t)p.r * 255u / p.a <= 255u && (uint32_t)p.g * 255u / p.a <= 255u &&
+         (uint32_t)p.b * 255u / p.a <= 255u;
+}
+
+// Accumulates paints, gradients and stops for one upload.
+struct PaintTable {
+  std::vector<DevPaint> paints;
+  std::vector<DevGradient> grads;
+  std::vector<DevStop> stops;
+  std::vector<uint32_t> grad_stop_off;  // per gradient
+  std::vector<int32_t> paint_grad;      // per paint: gradient index or -1
+
+  uint32_t add_gradient(const z2d_gradient_desc *g) {
+    DevGradient d;
+    std::memset(&d, 0, sizeof(d));
+    d.type = g->type;
+    d.method = g->method;
+    d.polar = g->polar;
+    d.n_stops = g->n_stops;
+    for (int i = 0; i < 6; i++) {
+      d.geom[i] = g->geom[i];
+      d.m[i] = g->matrix[i];
+    }
+    d.identity = (g->matrix[0] == 1.0 && g->matrix[1] == 0.0 && g->matrix[2] == 0.0 &&
+                  g->matrix[3] == 1.0 && g->matrix[4] == 0.0 && g->matrix[5] == 0.0);
+    if (g->type == Z2D_GRADIENT_RADIAL) {
+      // Radial.initBuffer (src/gradient.zig:440-470)
+      d.r1 = std::max(0.0, g->geom[2]);
+      d.r2 = std::max(0.0, g->geom[5]);
+      d.cdx = g->geom[3] - g->geom[0];
+      d.cdy = g->geom[4] - g->geom[1];
+      d.dr = d.r2 - d.r1;
+      d.min_dr = -d.r1;
+      double a = 0.0;
+      a += d.cdx * d.cdx;
+      a += d.cdy * d.cdy;
+      a += d.dr * (-d.dr);
+      d.a = a;
+      d.inv_a = a != 0.0 ? 1.0 / a : 0.0;
+    } else if (g->type == Z2D_GRADIENT_CONIC) {
+      // Conic.initBuffer: @mod(angle, pi*2)
+      const double two_pi = 3.14159265358979323846 * 2.0;
+      double a = std::fmod(g->geom[2], two_pi);
+      d.conic_angle = g->geom[2] < 0.0 ? std::fmod(a + two_pi, two_pi) : a;
+    }
+    // Stop.List.addAssumeCapacity: clamp, Color.init, stable sort by offset.
+    uint32_t base = (uint32_t)stops.size();
+    for (uint32_t i = 0; i < g->n_stops; i++) {
+      DevStop s = stop_from(g->stops[i]);
+      size_t j = stops.size();
+      stops.push_back(s);
+      while (j > base && stops[j - 1].offset > s.offset) {
+        stops[j] = stops[j - 1];
+        j--;
+      }
+      stops[j] = s;
+    }
+    grads.push_back(d);
+    grad_stop_off.push_back(base);
+    return (uint32_t)grads.size() - 1;
+  }
+
+  uint32_t add(const z2d_pattern_desc *p) {
+    DevPaint d;
+    std::memset(&d, 0, sizeof(d));
+    d.kind = p->kind;
+    int32_t gi = -1;
+    if (p->kind == Z2D_PATTERN_OPAQUE) {
+      d.px_format = p->pixel.format;
+      d.px = pixel_rgba(p->pixel);
+      d.px_opaque = pixel_opaque(p->pixel);
+    } else if (p->kind == Z2D_PATTERN_GRADIENT) {
+      gi = (int32_t)add_gradient(p->gradient);
+    } else {
+      const z2d_dither_desc &dd = p->dither;
+      d.dither_type = dd.type;
+      d.dither_source = dd.source;
+      d.dither_scale = dd.scale;
+      if (dd.source == Z2D_DITHER_SRC_GRADIENT) {
+        gi = (int32_t)add_gradient(dd.gradient);
+      } else if (dd.source == Z2D_DITHER_SRC_PIXEL) {
+        // LinearRGB.decodeRGBA: u16 demultiply, then /255
+        I4 v = pixel_rgba(dd.pixel);
+        uint32_t r = v.r, g = v.g, b = v.b;
+        if (v.a == 0) {
+          r = g = b = 0;
+        } else {
+          r = (r * 255u / v.a) & 0xFFFF;
+          g = (g * 255u / v.a) & 0xFFFF;
+          b = (b * 255u / v.a) & 0xFFFF;
+        }
+        d.dither_color = F4{(float)r / 255.0f, (float)g / 255.0f, (float)b / 255.0f,
+                            (float)v.a / 255.0f};
+      } else {
+        z2d_stop tmp;
+        tmp.offset = 0.0f;
+        tmp.color = dd.color;
+        DevStop s = stop_from(tmp);
+        float c[4];
+        color_linear_h(s, c);
+        d.dither_color = F4{c[0], c[1], c[2], c[3]};
+      }
+    }
+    paints.push_back(d);
+    paint_grad.push_back(gi);
+    return (uint32_t)paints.size() - 1;
+  }
+
+  void clear() {
+    paints.clear();
+    grads.clear();
+    stops.clear();
+    grad_stop_off.clear();
+    paint_grad.clear();
+  }
+
+  // Writes the three tables at their device addresses.
+  void write(uint8_t *host_base, uint8_t *dev_base, size_t paint_off, size_t grad_off,
+             size_t stop_off) {
+    DevPaint *hp = reinterpret_cast<DevPaint *>(host_base + paint_off);
+    DevGradient *hg = reinterpret_cast<DevGradient *>(host_base + grad_off);
+    DevStop *hs = reinterpret_cast<DevStop *>(host_base + stop_off);
+    DevGradient *dg = reinterpret_cast<DevGradient *>(dev_base + grad_off);
+    DevStop *ds = reinterpret_cast<DevStop *>(dev_base + stop_off);
+    for (size_t i = 0; i < grads.size(); i++) {
+      hg[i] = grads[i];
+      hg[i].stops = ds + grad_stop_off[i];
+    }
+    if (!stops.empty()) std::memcpy(hs, stops.data(), stops.size() * sizeof(DevStop));
+    for (size_t i = 0; i < paints.size(); i++) {
+      hp[i] = paints[i];
+      hp[i].gradient = paint_grad[i] >= 0 ? dg + paint_grad[i] : nullptr;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Flattening: fill_plotter's node state machine with the reference's Spline
+// subdivision (src/internal/tess/fill_plotter.zig:21-118,
+// src/internal/tess/Spline.zig:37-165).  Curves become the exact line_to
+// sequence the spline would plot; the device repeats the state machine to
+// build edges.  Also reports whether every contour returns to its start.
+// ---------------------------------------------------------------------------
+struct Pt {
+  double x, y;
+};
+inline bool peq(Pt a, Pt b) { return a.x == b.x && a.y == b.y; }
+
+struct Flattener {
+  std::vector<FNode> *out;
+  Pt buf[3];
+  int len = 0;
+  double tol_sq = 0.01;
+  double ymin = INFINITY, ymax = -INFINITY;
+
+  void track(Pt p) {
+    ymin = std::min(ymin, p.y);
+    ymax = std::max(ymax, p.y);
+  }
+  void emit(uint32_t tag, Pt p) {
+    FNode n;
+    n.x = p.x;
+    n.y = p.y;
+    n.tag = tag;
+    n.pad = 0;
+    out->push_back(n);
+  }
+  void add(Pt p) {
+    if (len < 3) {
+      buf[len++] = p;
+    } else {
+      buf[1] = buf[2];
+      buf[2] = p;
+    }
+  }
+  // line_to through the point buffer (dedupe against the last point).
+  void line_to(Pt p) {
+    emit(Z2D_NODE_LINE_TO, p);
+    track(p);
+    if (!peq(buf[len - 1], p)) add(p);
+  }
+  struct Knots {
+    Pt a, b, c, d;
+  };
+  static Pt half(Pt a, Pt b) { return Pt{a.x + ((b.x - a.x) / 2.0), a.y + ((b.y - a.y) / 2.0)}; }
+  static double err_sq(const Knots &k) {
+    double bx = k.b.x - k.a.x, by = k.b.y - k.a.y, cx = k.c.x - k.a.x, cy = k.c.y - k.a.y;
+    if (k.a.x != k.d.x || k.a.y != k.d.y) {
+      double dx = k.d.x - k.a.x, dy = k.d.y - k.a.y;
+      double dd = dx * dx + dy * dy;
+      double bd = bx * dx + by * dy;
+      if (bd >= dd) {
+        bx -= dx;
+        by -= dy;
+      } else {
+        bx -= bd / dd * dx;
+        by -= bd / dd * dy;
+      }
+      double cd = cx * dx + cy * dy;
+      if (cd >= dd) {
+        cx -= dx;
+        cy -= dy;
+      } else {
+        cx -= cd / dd * dx;
+        cy -= cd / dd * dy;
+      }
+    }
+    double be = bx * bx + by * by, ce = cx * cx + cy * cy;
+    return be > ce ? be : ce;
+  }
+  static Knots split(Knots &k) {
+    Pt ab = half(k.a, k.b), bc = half(k.b, k.c), cd = half(k.c, k.d);
+    Pt abbc = half(ab, bc), bccd = half(bc, cd), fin = half(abbc, bccd);
+    Knots r{fin, bccd, cd, k.d};
+    k.b = ab;
+    k.c = abbc;
+    k.d = fin;
+    return r;
+  }
+  void decompose_into(Knots &s1, Pt start, int depth) {
+    if (err_sq(s1) < tol_sq || depth > 64) {
+      if (!peq(s1.a, start)) line_to(s1.a);
+      return;
+    }
+    Knots s2 = split(s1);
+    decompose_into(s1, start, depth + 1);
+    decompose_into(s2, start, depth + 1);
+  }
+  void curve(Pt b, Pt c, Pt d) {
+    Pt a = buf[len - 1];
+    if (peq(a, b) && peq(c, d)) {
+      line_to(d);
+      return;
+    }
+    Knots s1{a, b, c, d};
+    decompose_into(s1, a, 0);
+    line_to(d);
+  }
+};
+
+// Flattens one node set.  Returns a z2d_status; balanced_out reports whether
+// every contour's last point equals its first point.
+int32_t flatten_nodes(const z2d_path_node *nodes, uint64_t n, double tolerance,
+                      std::vector<FNode> &out, bool &balanced_out, double &ymin,
+                      double &ymax) {
+  Flattener f;
+  f.out = &out;
+  f.tol_sq = tolerance * tolerance;
+  bool balanced = true;
+  for (uint64_t i = 0; i < n; i++) {
+    const z2d_path_node &nd = nodes[i];
+    Pt p{nd.v[0], nd.v[1]};
+    switch (nd.tag) {
+      case Z2D_NODE_MOVE_TO:
+        if (i == n - 1) {
+          f.emit(Z2D_NODE_MOVE_TO, p);
+          goto done;
+        }
+        if (f.len > 0 && !peq(f.buf[f.len - 1], f.buf[0])) balanced = false;
+        f.len = 0;
+        f.add(p);
+        f.emit(Z2D_NODE_MOVE_TO, p);
+        f.track(p);
+        break;
+      case Z2D_NODE_LINE_TO:
+        if (f.len == 0) return Z2D_ERR_INVALID_STATE;
+        f.line_to(p);
+        break;
+      case Z2D_NODE_CURVE_TO:
+        if (f.len == 0) return Z2D_ERR_INVALID_STATE;
+        f.track(Pt{nd.v[0], nd.v[1]});
+        f.track(Pt{nd.v[2], nd.v[3]});
+        f.curve(Pt{nd.v[0], nd.v[1]}, Pt{nd.v[2], nd.v[3]}, Pt{nd.v[4], nd.v[5]});
+        break;
+      case Z2D_NODE_CLOSE_PATH:
+        f.emit(Z2D_NODE_CLOSE_PATH, Pt{0, 0});
+        if (f.len >= 3 && !peq(f.buf[f.len - 1], f.buf[0])) f.add(f.buf[0]);
+        break;
+      default: return Z2D_ERR_INVALID_ARGUMENT;
+    }
+  }
+done:
+  if (f.len > 0 && !peq(f.buf[f.len - 1], f.buf[0])) balanced = false;
+  balanced_out = balanced;
+  ymin = f.ymin;
+  ymax = f.ymax;
+  return Z2D_OK;
+}
+
+// PathNode.isClosedNodeSet (src/internal/path_nodes.zig:23-36).
+bool is_closed_node_set(const z2d_path_node *nodes, uint64_t n) {
+  if (n == 0) return false;
+  bool closed = false;
+  for (uint64_t i = 0; i < n; i++) {
+    switch (nodes[i].tag) {
+      case Z2D_NODE_MOVE_TO:
+        if (!closed && i != 0) return closed;
+        break;
+      case Z2D_NODE_CLOSE_PATH: closed = true; break;
+      default: closed = false; break;
+    }
+  }
+  return closed;
+}
+
+int32_t check_pattern(const z2d_pattern_desc *p) {
+  if (!p) return Z2D_ERR_INVALID_ARGUMENT;
+  if (p->kind == Z2D_PATTERN_OPAQUE &&
+      (p->pixel.format == Z2D_FORMAT_ARGB || p->pixel.format == Z2D_FORMAT_RGBA) &&
+      !can_demultiply(p->pixel))
+    return Z2D_ERR_PIXEL_SOURCE_NOT_PREMULTIPLIED;
+  if (p->kind == Z2D_PATTERN_GRADIENT && !p->gradient) return Z2D_ERR_INVALID_ARGUMENT;
+  if (p->kind == Z2D_PATTERN_DITHER && p->dither.source == Z2D_DITHER_SRC_GRADIENT &&
+      !p->dither.gradient)
+    return Z2D_ERR_INVALID_ARGUMENT;
+  return Z2D_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+struct HostDraw {
+  DevDraw d;
+  uint32_t surface_slot;  // index into batch surfaces
+  uint32_t seq;           // insertion order
+  uint32_t band_lo, band_hi;  // estimated band range (inclusive)
+};
+
+struct z2d_gpu_batch {
+  z2d_gpu_ctx *ctx = nullptr;
+  std::vector<z2d_gpu_surface *> surfaces;
+  std::vector<FNode> nodes;
+  std::vector<z2d_edge> in_edges;
+  std::vector<HostDraw> draws;
+  PaintTable paints;
+  bool prepared = false;
+  // device state of the prepared batch
+  DevBuf arena;
+  HostBuf stage;
+  JobView view{};
+  uint32_t n_rowprog = 0;
+  uint32_t *d_rowprog_list = nullptr;
+  uint32_t *d_rowprog_soff = nullptr;
+  z2d_edge *d_scratch_edges = nullptr;
+  int32_t *d_scratch_x = nullptr;
+  std::vector<uint32_t> surf_tile_off;  // per surface slot, + total
+  float last_pipeline_ms = 0.0f, last_render_ms = 0.0f;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+  uint32_t slot_of(z2d_gpu_surface *s) {
+    for (uint32_t i = 0; i < surfaces.size(); i++)
+      if (surfaces[i] == s) return i;
+    surfaces.push_back(s);
+    return (uint32_t)surfaces.size() - 1;
+  }
+};
+
+static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
+                                const z2d_pattern_desc *pattern, uint32_t aa, uint32_t rule,
+                                uint32_t op, uint32_t precision, HostDraw &hd, double ymin,
+                                double ymax) {
+  (void)pattern;
+  hd.surface_slot = b->slot_of(s);
+  hd.seq = (uint32_t)b->draws.size();
+  DevDraw &d = hd.d;
+  // Alpha1 targets force AA off (src/painter.zig:88-91).
+  uint32_t eff_aa = (s->format == Z2D_FORMAT_ALPHA1) ? Z2D_AA_NONE : aa;
+  d.mode = eff_aa == Z2D_AA_NONE ? MODE_DIRECT : eff_aa == Z2D_AA_SUPERSAMPLE_4X ? MODE_SSAA
+                                                                             : MODE_MSAA;
+  d.rule = rule;
+  d.op = op;
+  d.precision = op_requires_float(op) ? Z2D_PRECISION_FLOAT : precision;
+  d.paint = b->paints.add(pattern);
+  const bool bounded = op_is_bounded(op);
+  if (d.mode == MODE_DIRECT && !bounded) d.flags |= DRAW_ROWPROG;
+  if (d.flags & DRAW_UNBALANCED) d.flags |= DRAW_ROWPROG;
+  // Band estimate from the input points (pixel rows).
+  const int32_t nb = (s->h + TILE_H - 1) / TILE_H;
+  if (!bounded || !std::isfinite(ymin) || !std::isfinite(ymax)) {
+    hd.band_lo = 0;
+    hd.band_hi = (uint32_t)(nb - 1);
+  } else {
+    double lo = std::floor(ymin) - 2.0, hi = std::ceil(ymax) + 2.0;
+    lo = std::max(0.0, std::min(lo, (double)(s->h - 1)));
+    hi = std::max(0.0, std::min(hi, (double)(s->h - 1)));
+    hd.band_lo = (uint32_t)lo / TILE_H;
+    hd.band_hi = (uint32_t)hi / TILE_H;
+  }
+  b->draws.push_back(hd);
+  b->prepared = false;
+  return Z2D_OK;
+}
+
+static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
+                          const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o) {
+  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
+  // painter.fill checks, in order (src/painter.zig:73-82)
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  if (n == 0) return Z2D_OK;
+  if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!is_closed_node_set(nodes, n)) return Z2D_ERR_PATH_NOT_CLOSED;
+  if (o->op >= Z2D_OP_COUNT || o->fill_rule > 1 || o->anti_aliasing_mode > 3)
+    return Z2D_ERR_INVALID_ARGUMENT;
+  HostDraw hd;
+  std::memset(&hd, 0, sizeof(hd));
+  const size_t before = b->nodes.size();
+  bool balanced = true;
+  double ymin, ymax;
+  double tol = std::max(o->tolerance, 0.001);
+  rc = flatten_nodes(nodes, n, tol, b->nodes, balanced, ymin, ymax);
+  if (rc) {
+    b->nodes.resize(before);
+    return rc;
+  }
+  hd.d.node_off = (uint32_t)before;
+  hd.d.node_cnt = (uint32_t)(b->nodes.size() - before);
+  if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
+  return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
+                          o->precision, hd, ymin, ymax);
+}
+
+static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
+                                  const z2d_pattern_desc *pattern, const z2d_edge *edges,
+                                  uint64_t n_edges, const double ext[4], uint32_t aa,
+                                  uint32_t rule, uint32_t op, uint32_t precision,
+                                  bool balanced) {
+  if (!b || !s || !ext) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  if (op >= Z2D_OP_COUNT || rule > 1 || aa > 3) return Z2D_ERR_INVALID_ARGUMENT;
+  HostDraw hd;
+  std::memset(&hd, 0, sizeof(hd));
+  // Edge slots are reserved in the node range (one slot per edge).
+  const size_t before = b->nodes.size();
+  b->nodes.resize(before + n_edges);
+  hd.d.node_off = (uint32_t)before;
+  hd.d.node_cnt = (uint32_t)n_edges;
+  hd.d.input_edges = 1;
+  hd.d.in_edge_off = (uint32_t)b->in_edges.size();
+  b->in_edges.insert(b->in_edges.end(), edges, edges + n_edges);
+  for (int i = 0; i < 4; i++) hd.d.in_ext[i] = ext[i];
+  if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
+  // extents are in scaled space
+  const double sc = (aa == Z2D_AA_NONE || s->format == Z2D_FORMAT_ALPHA1) ? 1.0 : 4.0;
+  double ymin = n_edges ? ext[0] / sc : NAN, ymax = n_edges ? ext[1] / sc : NAN;
+  return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, ymin, ymax);
+}
+
+static int32_t batch_prepare(z2d_gpu_batch *b) {
+  z2d_gpu_ctx *ctx = b->ctx;
+  if (b->prepared) return Z2D_OK;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  const uint32_t ns = (uint32_t)b->surfaces.size();
+  // Draws grouped by surface, insertion order kept within a surface.
+  std::stable_sort(b->draws.begin(), b->draws.end(), [](const HostDraw &a, const HostDraw &c) {
+    return a.surface_slot < c.surface_slot;
+  });
+  std::vector<uint32_t> surf_draw_off(ns + 1, 0), surf_band_off(ns + 1, 0);
+  b->surf_tile_off.assign(ns + 1, 0);
+  for (const HostDraw &h : b->draws) surf_draw_off[h.surface_slot + 1]++;
+  for (uint32_t i = 0; i < ns; i++) {
+    surf_draw_off[i + 1] += surf_draw_off[i];
+    const z2d_gpu_surface *s = b->surfaces[i];
+    uint32_t bands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
+    uint32_t tcols = (uint32_t)((s->w + TILE_W - 1) / TILE_W);
+    surf_band_off[i + 1] = surf_band_off[i] + bands;
+    b->surf_tile_off[i + 1] = b->surf_tile_off[i] + bands * tcols;
+  }
+  const uint32_t n_bands = surf_band_off[ns];
+  const uint32_t n_tiles = b->surf_tile_off[ns];
+  const uint32_t nd = (uint32_t)b->draws.size();
+  uint64_t max_entries = 0, max_rows = 0, scratch = 0;
+  std::vector<uint32_t> rowprog_list, rowprog_soff;
+  for (uint32_t i = 0; i < nd; i++) {
+    HostDraw &h = b->draws[i];
+    h.d.surface = h.surface_slot;
+    max_entries += h.band_hi - h.band_lo + 1;
+    if (h.d.flags & DRAW_ROWPROG) {
+      const z2d_gpu_surface *s = b->surfaces[h.surface_slot];
+      h.d.row_off = (uint32_t)max_rows;
+      max_rows += (uint64_t)(h.d.mode == MODE_DIRECT ? 1 : 4) * (uint64_t)s->h + 4;
+      rowprog_list.push_back(i);
+      rowprog_soff.push_back((uint32_t)scratch);
+      scratch += h.d.node_cnt + 1;
+    }
+  }
+  if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull)
+    return Z2D_ERR_OUT_OF_MEMORY;
+  b->n_rowprog = (uint32_t)rowprog_list.size();
+
+  // ---- arena layout
+  Layout L;
+  const size_t o_nodes = L.take(b->nodes.size() * sizeof(FNode));
+  const size_t o_inedges = L.take(b->in_edges.size() * sizeof(z2d_edge));
+  const size_t o_draws = L.take(nd * sizeof(DevDraw));
+  const size_t o_surfs = L.take(ns * sizeof(DevSurface));
+  const size_t o_paints = L.take(b->paints.paints.size() * sizeof(DevPaint));
+  const size_t o_grads = L.take(b->paints.grads.size() * sizeof(DevGradient));
+  const size_t o_stops = L.take(b->paints.stops.size() * sizeof(DevStop));
+  const size_t o_sdo = L.take((ns + 1) * sizeof(uint32_t));
+  const size_t o_sbo = L.take((ns + 1) * sizeof(uint32_t));
+  const size_t o_tiles = L.take((size_t)n_tiles * sizeof(TileRef));
+  const size_t o_rpl = L.take(rowprog_list.size() * sizeof(uint32_t));
+  const size_t o_rps = L.take(rowprog_soff.size() * sizeof(uint32_t));
+  const size_t o_bn = L.take(sizeof(kBlueNoise));
+  const size_t input_bytes = L.off;
+  const size_t o_edges = L.take(b->nodes.size() * sizeof(z2d_edge));
+  const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
+  const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
+  const size_t o_counters = L.take(16 * sizeof(uint32_t));
+  const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
+  const size_t o_rows = L.take(max_rows * sizeof(RowRec));
+  const size_t o_sedges = L.take(scratch * sizeof(z2d_edge));
+  const size_t o_sx = L.take(scratch * sizeof(int32_t));
+  const size_t total = L.off;
+  hipError_t e = b->arena.ensure(total);
+  if (e != hipSuccess) return fail(ctx, e, "batch arena");
+  e = b->stage.ensure(input_bytes);
+  if (e != hipSuccess) return fail(ctx, e, "batch staging");
+  uint8_t *H = b->stage.p;
+  uint8_t *D = b->arena.p;
+
+  if (!b->nodes.empty()) std::memcpy(H + o_nodes, b->nodes.data(), b->nodes.size() * sizeof(FNode));
+  if (!b->in_edges.empty())
+    std::memcpy(H + o_inedges, b->in_edges.data(), b->in_edges.size() * sizeof(z2d_edge));
+  DevDraw *hdraws = reinterpret_cast<DevDraw *>(H + o_draws);
+  for (uint32_t i = 0; i < nd; i++) hdraws[i] = b->draws[i].d;
+  DevSurface *hs = reinterpret_cast<DevSurface *>(H + o_surfs);
+  for (uint32_t i = 0; i < ns; i++) {
+    const z2d_gpu_surface *s = b->surfaces[i];
+    hs[i].data = s->data;
+    hs[i].w = s->w;
+    hs[i].h = s->h;
+    hs[i].fmt = s->format;
+    hs[i].pad = 0;
+    hs[i].nbytes = s->nbytes;
+  }
+  b->paints.write(H, D, o_paints, o_grads, o_stops);
+  std::memcpy(H + o_sdo, surf_draw_off.data(), (ns + 1) * sizeof(uint32_t));
+  std::memcpy(H + o_sbo, surf_band_off.data(), (ns + 1) * sizeof(uint32_t));
+  TileRef *ht = reinterpret_cast<TileRef *>(H + o_tiles);
+  for (uint32_t i = 0; i < ns; i++) {
+    const z2d_gpu_surface *s = b->surfaces[i];
+    const uint32_t bands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
+    const uint32_t tcols = (uint32_t)((s->w + TILE_W - 1) / TILE_W);
+    uint32_t k = b->surf_tile_off[i];
+    for (uint32_t ty = 0; ty < bands; ty++)
+      for (uint32_t tx = 0; tx < tcols; tx++) ht[k++] = TileRef{i, (uint16_t)tx, (uint16_t)ty};
+  }
+  if (!rowprog_list.empty()) {
+    std::memcpy(H + o_rpl, rowprog_list.data(), rowprog_list.size() * sizeof(uint32_t));
+    std::memcpy(H + o_rps, rowprog_soff.data(), rowprog_soff.size() * sizeof(uint32_t));
+  }
+  std::memcpy(H + o_bn, kBlueNoise, sizeof(kBlueNoise));
+  HIP_OK(ctx, hipMemcpyAsync(D, H, input_bytes, hipMemcpyHostToDevice, ctx->stream));
+
+  JobView &J = b->view;
+  J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
+  J.in_edges = reinterpret_cast<const z2d_edge *>(D + o_inedges);
+  J.edges = reinterpret_cast<z2d_edge *>(D + o_edges);
+  J.draws = reinterpret_cast<DevDraw *>(D + o_draws);
+  J.n_draws = nd;
+  J.surfaces = reinterpret_cast<DevSurface *>(D + o_surfs);
+  J.n_surfaces = ns;
+  J.paints = reinterpret_cast<const DevPaint *>(D + o_paints);
+  J.blue_noise = reinterpret_cast<const uint16_t *>(D + o_bn);
+  J.surf_draw_off = reinterpret_cast<const uint32_t *>(D + o_sdo);
+  J.surf_band_off = reinterpret_cast<const uint32_t *>(D + o_sbo);
+  J.n_bands = n_bands;
+  J.band_cnt = reinterpret_cast<uint32_t *>(D + o_bandcnt);
+  J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
+  J.max_entries = (uint32_t)max_entries;
+  J.entry_edge_idx = nullptr;
+  J.max_entry_edges = 0;
+  J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
+  J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);
+  J.rows = reinterpret_cast<RowRec *>(D + o_rows);
+  J.max_rows = (uint32_t)max_rows;
+  J.tiles = reinterpret_cast<const TileRef *>(D + o_tiles);
+  J.n_tiles = n_tiles;
+  b->d_rowprog_list = reinterpret_cast<uint32_t *>(D + o_rpl);
+  b->d_rowprog_soff = reinterpret_cast<uint32_t *>(D + o_rps);
+  b->d_scratch_edges = reinterpret_cast<z2d_edge *>(D + o_sedges);
+  b->d_scratch_x = reinterpret_cast<int32_t *>(D + o_sx);
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  b->prepared = true;
+  return Z2D_OK;
+}
+
+static int32_t batch_render(z2d_gpu_batch *b, uint32_t mode) {
+  z2d_gpu_ctx *ctx = b->ctx;
+  if (!b->prepared) {
+    int32_t rc = batch_prepare(b);
+    if (rc) return rc;
+  }
+  if (b->draws.empty()) return Z2D_OK;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  JobView &J = b->view;
+  const uint32_t ns = J.n_surfaces;
+  for (int i = 0; i < 3; i++)
+    if (!b->ev[i]) HIP_OK(ctx, hipEventCreate(&b->ev[i]));
+  HIP_OK(ctx, hipEventRecord(b->ev[0], ctx->stream));
+  HIP_OK(ctx, z2d_launch_fill_u32(J.counters, 16, 0u, ctx->stream));
+  HIP_OK(ctx, z2d_launch_fill_u32(J.cursors, ns + 1, 0u, ctx->stream));
+  HIP_OK(ctx, z2d_launch_edge_build(J, ctx->stream));
+  HIP_OK(ctx, z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,
+                                 b->d_scratch_x, b->d_rowprog_soff, ctx->stream));
+  HIP_OK(ctx, z2d_launch_binning(J, ctx->stream));
+  HIP_OK(ctx, hipEventRecord(b->ev[1], ctx->stream));
+  const uint32_t max_blocks = (uint32_t)ctx->n_cus * 8;
+  if (mode == 0) {
+    uint32_t nb = std::min(J.n_tiles, max_blocks);
+    HIP_OK(ctx, z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, ctx->stream));
+  } else {
+    for (uint32_t s = 0; s < ns; s++) {
+      uint32_t t0 = b->surf_tile_off[s], tn = b->surf_tile_off[s + 1] - t0;
+      uint32_t nb = std::min(tn, max_blocks);
+      HIP_OK(ctx, z2d_launch_render(J, t0, tn, J.cursors + s, nb, ctx->stream));
+    }
+  }
+  HIP_OK(ctx, hipEventRecord(b->ev[2], ctx->stream));
+  return Z2D_OK;
+}
+
+static int32_t batch_wait(z2d_gpu_batch *b) {
+  z2d_gpu_ctx *ctx = b->ctx;
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  if (b->ev[0] && b->ev[2]) {
+    float a = 0.0f, r = 0.0f;
+    if (hipEventElapsedTime(&a, b->ev[0], b->ev[2]) == hipSuccess) b->last_pipeline_ms = a;
+    if (hipEventElapsedTime(&r, b->ev[1], b->ev[2]) == hipSuccess) b->last_render_ms = r;
+  }
+  return Z2D_OK;
+}
+
+static void batch_clear(z2d_gpu_batch *b) {
+  b->surfaces.clear();
+  b->nodes.clear();
+  b->in_edges.clear();
+  b->draws.clear();
+  b->paints.clear();
+  b->prepared = false;
+}
+
+// ---------------------------------------------------------------------------
+// surfaces from descriptors
+// ---------------------------------------------------------------------------
+static int32_t surface_alloc(z2d_gpu_ctx *ctx, uint32_t fmt, int32_t w, int32_t h,
+                             z2d_gpu_surface **out) {
+  if (w < 1) return Z2D_ERR_INVALID_WIDTH;
+  if (h < 1) return Z2D_ERR_INVALID_HEIGHT;
+  if (fmt > Z2D_FORMAT_ALPHA1) return Z2D_ERR_INVALID_ARGUMENT;
+  auto *s = new (std::nothrow) z2d_gpu_surface();
+  if (!s) return Z2D_ERR_OUT_OF_MEMORY;
+  s->ctx = ctx;
+  s->format = fmt;
+  s->w = w;
+  s->h = h;
+  s->nbytes = surface_bytes(fmt, w, h);
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(&s->data, padded_bytes(s->nbytes));
+  if (e != hipSuccess) {
+    delete s;
+    return fail(ctx, e, "surface alloc");
+  }
+  e = hipMemsetAsync(s->data, 0, padded_bytes(s->nbytes), ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(s->data);
+    delete s;
+    return fail(ctx, e, "surface clear");
+  }
+  *out = s;
+  return Z2D_OK;
+}
+
+// A descriptor as a surface: device descriptors are wrapped, host ones are
+// uploaded into a temporary device surface (copied back by the caller).
+struct SurfRef {
+  z2d_gpu_surface *s = nullptr;
+  bool temp = false;
+  z2d_surface_desc *desc = nullptr;
+};
+
+static int32_t surface_from_desc(z2d_gpu_ctx *ctx, z2d_surface_desc *d, SurfRef &r) {
+  if (!d || !d->data) return Z2D_ERR_INVALID_ARGUMENT;
+  if (d->width < 1) return Z2D_ERR_INVALID_WIDTH;
+  if (d->height < 1) return Z2D_ERR_INVALID_HEIGHT;
+  if (d->format > Z2D_FORMAT_ALPHA1) return Z2D_ERR_INVALID_ARGUMENT;
+  uint64_t need = surface_bytes(d->format, d->width, d->height);
+  if (d->len < need) return Z2D_ERR_INVALID_ARGUMENT;
+  r.desc = d;
+  if (d->reserved == Z2D_SURFACE_DEVICE) {
+    auto *s = new (std::nothrow) z2d_gpu_surface();
+    if (!s) return Z2D_ERR_OUT_OF_MEMORY;
+    s->ctx = ctx;
+    s->format = d->format;
+    s->w = d->width;
+    s->h = d->height;
+    s->nbytes = need;
+    s->data = d->data;
+    s->owned = false;
+    r.s = s;
+    r.temp = false;
+    return Z2D_OK;
+  }
+  int32_t rc = surface_alloc(ctx, d->format, d->width, d->height, &r.s);
+  if (rc) return rc;
+  r.temp = true;
+  HIP_OK(ctx, hipMemcpyAsync(r.s->data, d->data, need, hipMemcpyHostToDevice, ctx->stream));
+  return Z2D_OK;
+}
+
+static void surface_free(z2d_gpu_surface *s) {
+  if (!s) return;
+  if (s->owned && s->data) (void)hipFree(s->data);
+  delete s;
+}
+
+static int32_t surface_release(z2d_gpu_ctx *ctx, SurfRef &r, bool copy_back) {
+  int32_t rc = Z2D_OK;
+  if (r.s && r.temp && copy_back) {
+    hipError_t e =
+        hipMemcpyAsync(r.desc->data, r.s->data, r.s->nbytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = fail(ctx, e, "surface download");
+  }
+  if (r.s) {
+    if (r.temp) (void)hipStreamSynchronize(ctx->stream);
+    surface_free(r.s);
+  }
+  r.s = nullptr;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// compositor
+// ---------------------------------------------------------------------------
+static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t dst_x,
+                              int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
+                              uint32_t precision, z2d_surface_desc *out_d) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (!dst_d || (n_ops && !ops) || n_ops > 16) return Z2D_ERR_INVALID_ARGUMENT;
+  if (n_ops == 0) return Z2D_OK;
+  if (dst_x >= dst_d->width || dst_y >= dst_d->height) return Z2D_OK;
+  for (uint64_t i = 0; i < n_ops; i++)
+    if (ops[i].op >= Z2D_OP_COUNT) return Z2D_ERR_INVALID_ARGUMENT;
+  // precision: float if any operator requires it
+  for (uint64_t i = 0; i < n_ops; i++)
+    if (op_requires_float(ops[i].op)) precision = Z2D_PRECISION_FLOAT;
+  int32_t sw, sh;
+  switch (ops[0].src.kind) {
+    case Z2D_PARAM_NONE: return Z2D_OK;
+    case Z2D_PARAM_SURFACE:
+      if (!ops[0].src.surface) return Z2D_ERR_INVALID_ARGUMENT;
+      sw = ops[0].src.surface->width;
+      sh = ops[0].src.surface->height;
+      break;
+    default:
+      if (dst_x != 0 || dst_y != 0) return Z2D_OK;
+      sw = dst_d->width;
+      sh = dst_d->height;
+      break;
+  }
+  const int32_t ssx = dst_x < 0 ? -dst_x : 0, ssy = dst_y < 0 ? -dst_y : 0;
+  const int32_t width = (sw + dst_x > dst_d->width) ? dst_d->width - dst_x : sw;
+  const int32_t height = (sh + dst_y > dst_d->height) ? dst_d->height - dst_y : sh;
+  if (ssx >= width || ssy >= height) return Z2D_OK;
+
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  SurfRef dst, out;
+  int32_t rc = surface_from_desc(ctx, dst_d, dst);
+  if (rc) return rc;
+  if (out_d) {
+    if (out_d->format != dst_d->format || out_d->width != dst_d->width ||
+        out_d->height != dst_d->height) {
+      surface_release(ctx, dst, false);
+      return Z2D_ERR_INVALID_ARGUMENT;
+    }
+    rc = surface_from_desc(ctx, out_d, out);
+    if (rc) {
+      surface_release(ctx, dst, false);
+      return rc;
+    }
+  }
+  std::vector<SurfRef> srcs;
+  PaintTable pt;
+  CompJob job;
+  std::memset(&job, 0, sizeof(job));
+  auto devsurf = [](const z2d_gpu_surface *s) {
+    DevSurface d;
+    d.data = s->data;
+    d.w = s->w;
+    d.h = s->h;
+    d.fmt = s->format;
+    d.pad = 0;
+    d.nbytes = s->nbytes;
+    return d;
+  };
+  job.dst = devsurf(dst.s);
+  job.out = out_d ? devsurf(out.s) : job.dst;
+  job.x0 = ssx + dst_x;
+  job.y0 = ssy + dst_y;
+  job.width = width - ssx;
+  job.height = (height > ssy ? height : ssy) - ssy;
+  job.src_row0 = ssy;
+  job.n_ops = (uint32_t)n_ops;
+  job.precision = precision;
+  std::vector<int32_t> paint_idx(2 * n_ops, -1);
+  for (uint64_t k = 0; k < n_ops && rc == Z2D_OK; k++) {
+    job.ops[k].op = ops[k].op;
+    for (int side = 0; side < 2; side++) {
+      const z2d_comp_param &p = side == 0 ? ops[k].dst : ops[k].src;
+      DevCompParam &q = side == 0 ? job.ops[k].dst : job.ops[k].src;
+      q.kind = p.kind;
+      switch (p.kind) {
+        case Z2D_PARAM_NONE: break;
+        case Z2D_PARAM_PIXEL: q.px = pixel_rgba(p.pixel); break;
+        case Z2D_PARAM_SURFACE: {
+          SurfRef r;
+          rc = surface_from_desc(ctx, const_cast<z2d_surface_desc *>(p.surface), r);
+          if (rc) break;
+          srcs.push_back(r);
+          q.surf = devsurf(r.s);
+          // getStride(x, y): dst-side params at the destination start,
+          // src-side params at (src_start_x, src_y).
+          int32_t bx = side == 0 ? job.x0 : ssx;
+          int32_t by = side == 0 ? job.y0 : ssy;
+          if (bx < 0 || by < 0 || bx >= r.s->w || by >= r.s->h) {
+            rc = Z2D_ERR_INVALID_ARGUMENT;  // the reference reads an empty stride
+            break;
+          }
+          q.base = (int64_t)r.s->w * by + bx;
+          // rows of the source stride advance by the source's own width
+        } break;
+        case Z2D_PARAM_GRADIENT: {
+          z2d_pattern_desc tmp;
+          std::memset(&tmp, 0, sizeof(tmp));
+          tmp.kind = Z2D_PATTERN_GRADIENT;
+          tmp.gradient = p.gradient;
+          if (!p.gradient) {
+            rc = Z2D_ERR_INVALID_ARGUMENT;
+            break;
+          }
+          paint_idx[2 * k + side] = (int32_t)pt.add(&tmp);
+        } break;
+        case Z2D_PARAM_DITHER: {
+          z2d_pattern_desc tmp;
+          std::memset(&tmp, 0, sizeof(tmp));
+          tmp.kind = Z2D_PATTERN_DITHER;
+          tmp.dither = p.dither;
+          paint_idx[2 * k + side] = (int32_t)pt.add(&tmp);
+        } break;
+        default: rc = Z2D_ERR_INVALID_ARGUMENT; break;
+      }
+      if (rc) break;
+    }
+  }
+  if (rc == Z2D_OK) {
+    Layout L;
+    const size_t o_job = L.take(sizeof(CompJob));
+    const size_t o_paints = L.take(pt.paints.size() * sizeof(DevPaint));
+    const size_t o_grads = L.take(pt.grads.size() * sizeof(DevGradient));
+    const size_t o_stops = L.take(pt.stops.size() * sizeof(DevStop));
+    const size_t o_bn = L.take(sizeof(kBlueNoise));
+    hipError_t e = ctx->comp_arena.ensure(L.off);
+    if (e == hipSuccess) e = ctx->comp_stage.ensure(L.off);
+    if (e != hipSuccess) {
+      rc = fail(ctx, e, "composite arena");
+    } else {
+      uint8_t *H = ctx->comp_stage.p, *D = ctx->comp_arena.p;
+      pt.write(H, D, o_paints, o_grads, o_stops);
+      const DevPaint *dp = reinterpret_cast<const DevPaint *>(D + o_paints);
+      for (uint64_t k = 0; k < n_ops; k++) {
+        if (paint_idx[2 * k] >= 0) job.ops[k].dst.paint = dp + paint_idx[2 * k];
+        if (paint_idx[2 * k + 1] >= 0) job.ops[k].src.paint = dp + paint_idx[2 * k + 1];
+      }
+      job.blue_noise = reinterpret_cast<const uint16_t *>(D + o_bn);
+      std::memcpy(H + o_job, &job, sizeof(job));
+      std::memcpy(H + o_bn, kBlueNoise, sizeof(kBlueNoise));
+      e = hipMemcpyAsync(D, H, L.off, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) {
+        // Streaming path: whole 4-byte-format surfaces, aligned rows, every
+        // surface source covering the destination at the origin.
+        bool stream = fbits(job.dst.fmt) == 32 && fbits(job.out.fmt) == 32 && job.x0 == 0 &&
+                      job.y0 == 0 && job.width == job.dst.w && job.height == job.dst.h &&
+                      (job.dst.w % 4) == 0;
+        for (uint64_t k = 0; k < n_ops && stream; k++)
+          for (int side = 0; side < 2; side++) {
+            const DevCompParam &q = side == 0 ? job.ops[k].dst : job.ops[k].src;
+            if (q.kind == Z2D_PARAM_SURFACE && (q.base != 0 || q.surf.w != job.dst.w))
+              stream = false;
+          }
+        e = z2d_launch_composite(reinterpret_cast<const CompJob *>(D + o_job), job, stream,
+                                 ctx->stream);
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) rc = fail(ctx, e, "composite launch");
+    }
+  }
+  for (SurfRef &r : srcs) surface_release(ctx, r, false);
+  int32_t rc2 = Z2D_OK;
+  if (out_d) {
+    rc2 = surface_release(ctx, out, rc == Z2D_OK);
+    surface_release(ctx, dst, false);
+  } else {
+    rc2 = surface_release(ctx, dst, rc == Z2D_OK);
+  }
+  return rc ? rc : rc2;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int32_t z2d_gpu_create(int32_t device, z2d_gpu_ctx **out) {
+  if (!out) return Z2D_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return Z2D_ERR_DEVICE;
+  if (device < 0 || device >= count) return Z2D_ERR_INVALID_ARGUMENT;
+  auto *ctx = new (std::nothrow) z2d_gpu_ctx();
+  if (!ctx) return Z2D_ERR_OUT_OF_MEMORY;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return Z2D_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->n_cus = prop.multiProcessorCount;
+  *out = ctx;
+  return Z2D_OK;
+}
+
+void z2d_gpu_destroy(z2d_gpu_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->comp_arena.release();
+  ctx->comp_stage.release();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *z2d_gpu_last_error(z2d_gpu_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int32_t z2d_gpu_sync(z2d_gpu_ctx *ctx) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_surface_create(z2d_gpu_ctx *ctx, uint32_t format, int32_t width, int32_t height,
+                               z2d_gpu_surface **out) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (!out) return Z2D_ERR_INVALID_ARGUMENT;
+  return surface_alloc(ctx, format, width, height, out);
+}
+
+int32_t z2d_gpu_surface_upload(z2d_gpu_surface *s, const void *host, uint64_t len) {
+  if (!s || !host || len < s->nbytes) return Z2D_ERR_INVALID_ARGUMENT;
+  z2d_gpu_ctx *ctx = s->ctx;
+  HIP_OK(ctx, hipMemcpyAsync(s->data, host, s->nbytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_surface_download(z2d_gpu_surface *s, void *host, uint64_t len) {
+  if (!s || !host || len < s->nbytes) return Z2D_ERR_INVALID_ARGUMENT;
+  z2d_gpu_ctx *ctx = s->ctx;
+  HIP_OK(ctx, hipMemcpyAsync(host, s->data, s->nbytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_surface_desc(z2d_gpu_surface *s, z2d_surface_desc *out) {
+  if (!s || !out) return Z2D_ERR_INVALID_ARGUMENT;
+  out->format = s->format;
+  out->width = s->w;
+  out->height = s->h;
+  out->reserved = Z2D_SURFACE_DEVICE;
+  out->data = s->data;
+  out->len = s->nbytes;
+  return Z2D_OK;
+}
+
+void z2d_gpu_surface_destroy(z2d_gpu_surface *s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  surface_free(s);
+}
+
+int32_t z2d_gpu_batch_create(z2d_gpu_ctx *ctx, z2d_gpu_batch **out) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (!out) return Z2D_ERR_INVALID_ARGUMENT;
+  auto *b = new (std::nothrow) z2d_gpu_batch();
+  if (!b) return Z2D_ERR_OUT_OF_MEMORY;
+  b->ctx = ctx;
+  *out = b;
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                           const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                           uint64_t n_nodes, const z2d_fill_opts *opts) {
+  return batch_fill(b, surface, pattern, nodes, n_nodes, opts);
+}
+
+int32_t z2d_gpu_batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                                   const z2d_pattern_desc *pattern, const z2d_edge *edges,
+                                   uint64_t n_edges, const double extents[4],
+                                   uint32_t aa, uint32_t rule, uint32_t op,
+                                   uint32_t precision) {
+  if (n_edges && !edges) return Z2D_ERR_INVALID_ARGUMENT;
+  return batch_fill_polygon(b, surface, pattern, edges, n_edges, extents, aa, rule, op,
+                            precision, false);
+}
+
+int32_t z2d_gpu_batch_prepare(z2d_gpu_batch *b) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  return batch_prepare(b);
+}
+
+int32_t z2d_gpu_batch_render(z2d_gpu_batch *b, uint32_t mode) {
+  if (!b || mode > 1) return Z2D_ERR_INVALID_ARGUMENT;
+  return batch_render(b, mode);
+}
+
+int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  return batch_wait(b);
+}
+
+int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  if (pipeline_ms) *pipeline_ms = b->last_pipeline_ms;
+  if (render_ms) *render_ms = b->last_render_ms;
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]) {
+  if (!b || !out) return Z2D_ERR_INVALID_ARGUMENT;
+  out[0] = b->draws.size();
+  out[1] = b->surfaces.size();
+  out[2] = b->nodes.size();
+  out[3] = b->surf_tile_off.empty() ? 0 : b->surf_tile_off.back();
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_clear(b);
+  return Z2D_OK;
+}
+
+void z2d_gpu_batch_destroy(z2d_gpu_batch *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  b->arena.release();
+  b->stage.release();
+  for (int i = 0; i < 3; i++)
+    if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+  delete b;
+}
+
+// One painter call on a descriptor through a single-draw batch.
+static int32_t single_draw(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                           int32_t (*add)(z2d_gpu_batch *, z2d_gpu_surface *, const void *),
+                           const void *arg) {
+  SurfRef ref;
+  int32_t rc = surface_from_desc(ctx, surface, ref);
+  if (rc) return rc;
+  z2d_gpu_batch b;
+  b.ctx = ctx;
+  rc = add(&b, ref.s, arg);
+  if (rc == Z2D_OK && !b.draws.empty()) {
+    rc = batch_prepare(&b);
+    if (rc == Z2D_OK) rc = batch_render(&b, 0);
+    if (rc == Z2D_OK) rc = batch_wait(&b);
+  }
+  b.arena.release();
+  b.stage.release();
+  for (int i = 0; i < 3; i++)
+    if (b.ev[i]) (void)hipEventDestroy(b.ev[i]);
+  int32_t rc2 = surface_release(ctx, ref, rc == Z2D_OK);
+  return rc ? rc : rc2;
+}
+
+struct FillArgs {
+  const z2d_pattern_desc *pattern;
+  const z2d_path_node *nodes;
+  uint64_t n;
+  const z2d_fill_opts *opts;
+};
+struct PolyArgs {
+  const z2d_pattern_desc *pattern;
+  const z2d_edge *edges;
+  uint64_t n;
+  const double *ext;
+  uint32_t aa, rule, op, precision;
+};
+
+int32_t z2d_gpu_fill(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, const z2d_pattern_desc *pattern,
+                     const z2d_path_node *nodes, uint64_t n_nodes, const z2d_fill_opts *opts) {
+  if (!opts || !surface) return Z2D_ERR_INVALID_ARGUMENT;
+  // Host-side checks first, in the reference's order, so they also work
+  // without a device.
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  if (n_nodes == 0) return Z2D_OK;
+  if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!is_closed_node_set(nodes, n_nodes)) return Z2D_ERR_PATH_NOT_CLOSED;
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  FillArgs a{pattern, nodes, n_nodes, opts};
+  return single_draw(
+      ctx, surface,
+      [](z2d_gpu_batch *b, z2d_gpu_surface *s, const void *p) -> int32_t {
+        const FillArgs &f = *static_cast<const FillArgs *>(p);
+        return batch_fill(b, s, f.pattern, f.nodes, f.n, f.opts);
+      },
+      &a);
+}
+
+int32_t z2d_gpu_fill_polygon(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                             const z2d_pattern_desc *pattern, const z2d_edge *edges,
+                             uint64_t n_edges, const double extents[4], uint32_t aa,
+                             uint32_t rule, uint32_t op, uint32_t precision) {
+  if (!surface || !extents || (n_edges && !edges)) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  PolyArgs a{pattern, edges, n_edges, extents, aa, rule, op, precision};
+  return single_draw(
+      ctx, surface,
+      [](z2d_gpu_batch *b, z2d_gpu_surface *s, const void *p) -> int32_t {
+        const PolyArgs &f = *static_cast<const PolyArgs *>(p);
+        return batch_fill_polygon(b, s, f.pattern, f.edges, f.n, f.ext, f.aa, f.rule, f.op,
+                                  f.precision, false);
+      },
+      &a);
+}
+
+int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x, int32_t dst_y,
+                          const z2d_comp_op *ops, uint64_t n_ops, uint32_t precision,
+                          z2d_surface_desc *out) {
+  return composite_impl(ctx, dst, dst_x, dst_y, ops, n_ops, precision, out);
+}
+
+int32_t z2d_gpu_stroke(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                       const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                       uint64_t n_nodes, const z2d_stroke_opts *opts) {
+  (void)ctx; (void)surface; (void)nodes; (void)n_nodes;
+  if (!opts) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  return Z2D_ERR_UNSUPPORTED;
+}
+
+int32_t z2d_gpu_batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                             const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                             uint64_t n_nodes, const z2d_stroke_opts *opts) {
+  (void)b; (void)surface; (void)pattern; (void)nodes; (void)n_nodes; (void)opts;
+  return Z2D_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

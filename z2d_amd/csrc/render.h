@@ -1,0 +1,144 @@
+// render.h - device job layout shared by the host runtime (host.cpp) and the
+// kernels (render.hip).  A job is a set of surfaces and an ordered list of
+// draws (fills / stroke polygons) on them; one pipeline launch renders all of
+// them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_pixel.h"
+
+namespace z2d {
+
+// Tile shape of the renderer: TILE_W pixels x TILE_H rows per workgroup of
+// 256 lanes (4 wave64s; lane = column, each wave owns TILE_H/4 rows).
+constexpr int TILE_W = 64;
+constexpr int TILE_H = 16;
+constexpr int RENDER_THREADS = 256;
+constexpr int ROWS_PER_THREAD = TILE_H / 4;
+constexpr int MAX_SUBROWS = TILE_H * 4;
+constexpr int LIST_CAP = 24;     // crossings per sub-row held in LDS
+constexpr int EDGE_CACHE = 256;  // band edges staged in LDS per pass
+
+// Flattened path node (move_to / line_to / close_path only; curves are
+// flattened on the host by the reference's Spline algorithm first).
+struct FNode {
+  double x, y;
+  uint32_t tag, pad;
+};
+
+struct DevSurface {
+  uint8_t *data;
+  int32_t w, h;
+  uint32_t fmt, pad;
+  uint64_t nbytes;
+};
+
+enum DrawFlags : uint32_t {
+  DRAW_VISIBLE = 1u << 0,     // passed Polygon.inBox and has a non-empty region
+  DRAW_UNBOUNDED = 1u << 1,   // operator is unbounded (src_in/dst_in/src_out/dst_atop)
+  DRAW_FAST_SRC = 1u << 2,    // solid pattern and op reduces to source
+  DRAW_UNBALANCED = 1u << 3,  // some contour does not return to its start
+  DRAW_ROWPROG = 1u << 4,     // needs the serial row emulation
+  DRAW_HAS_EDGES = 1u << 5,
+};
+
+enum DrawMode : uint32_t { MODE_DIRECT = 0, MODE_MSAA = 1, MODE_SSAA = 2 };
+
+struct DevDraw {
+  // inputs
+  uint32_t node_off, node_cnt;  // flattened nodes; edge slots use the same range
+  uint32_t surface;
+  uint32_t mode;       // DrawMode (alpha1 targets already forced to direct)
+  uint32_t rule, op;
+  uint32_t precision;  // already upgraded to float if op requires it
+  uint32_t paint;
+  uint32_t input_edges;  // nonzero: edges given directly (stroke polygons)
+  uint32_t in_edge_off;  // their offset in JobView::in_edges
+  double in_ext[4];      // extents when edges are given directly
+  // written by k_edge_build
+  uint32_t edge_cnt;
+  uint32_t flags;
+  double ext[4];  // top, bottom, left, right (scaled space)
+  int32_t rx0, ry0, rx1, ry1;  // composite region [rx0,rx1) x [ry0,ry1)
+  int32_t sx, ex, sy, ey;      // MSAA window (rows sy..ey inclusive)
+  int32_t mask_bits;           // SSAA mask depth
+  int32_t row_y0, row_n;       // sub-rows covered by the row records
+  uint32_t row_off;            // offset into the row-record array
+};
+
+// Per sub-row record of the serial emulation (rowprog): unbalanced cutoff
+// and, for direct-mode unbounded draws, the surviving span.
+struct RowRec {
+  int32_t cutoff;  // crossings >= cutoff are in a dropped trailing span
+  int32_t kind;    // direct unbounded: 0 clear row, 1 untouched, 2 span on dst, 3 span on zero
+  int32_t s, e;    // surviving span [s, e)
+};
+
+struct BandEntry {
+  uint32_t draw;
+  uint32_t edge_off;  // into the entry-edge index array
+  uint32_t edge_cnt;
+  uint32_t pad;
+};
+
+struct TileRef {
+  uint32_t surface;
+  uint16_t tx, ty;
+};
+
+struct JobView {
+  const FNode *nodes;
+  const z2d_edge *in_edges;  // edges of polygon draws
+  z2d_edge *edges;  // edge slots, indexed like nodes
+  DevDraw *draws;
+  uint32_t n_draws;
+  DevSurface *surfaces;
+  uint32_t n_surfaces;
+  const DevPaint *paints;
+  const uint16_t *blue_noise;
+  // surface -> draw range, band range
+  const uint32_t *surf_draw_off;  // n_surfaces + 1
+  const uint32_t *surf_band_off;  // n_surfaces + 1
+  uint32_t n_bands;
+  // binning outputs
+  uint32_t *band_cnt;    // n_bands + 1 (scanned in place)
+  BandEntry *entries;    // capacity max_entries
+  uint32_t max_entries;
+  uint32_t *entry_edge_idx;
+  uint32_t max_entry_edges;
+  uint32_t *counters;    // [2] entry overflow flag
+  uint32_t *cursors;     // tile work cursors, one per render launch
+  RowRec *rows;
+  uint32_t max_rows;
+  const TileRef *tiles;
+  uint32_t n_tiles;
+};
+
+// SurfaceCompositor.run batch on device (src/compositor.zig:302-440).
+struct DevCompParam {
+  uint32_t kind;  // Z2D_PARAM_*
+  uint32_t pad;
+  I4 px;
+  const DevPaint *paint;  // gradient / dither sources
+  DevSurface surf;        // surface sources
+  int64_t base;           // pixel index of the stride start in row 0
+};
+struct DevCompOp {
+  uint32_t op, pad;
+  DevCompParam dst, src;
+};
+struct CompJob {
+  DevSurface dst;
+  DevSurface out;  // equal to dst for in-place runs
+  int32_t x0, y0;  // first destination pixel of the region
+  int32_t width, height;
+  int32_t src_row0;  // source row of region row 0 (src_start_y)
+  uint32_t n_ops;
+  uint32_t precision;
+  DevCompOp ops[16];
+  const uint16_t *blue_noise;
+};
+
+}  // namespace z2d

@@ -1,0 +1,220 @@
+"""ctypes binding of libz2d_gpu.so (include/z2d_gpu.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback
+on the product path."""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libz2d_gpu.so")
+
+_lib = None
+
+P = C.POINTER
+_SIGS = {
+    "z2d_gpu_create": ([C.c_int32, P(C.c_void_p)], C.c_int32),
+    "z2d_gpu_destroy": ([C.c_void_p], None),
+    "z2d_gpu_last_error": ([C.c_void_p], C.c_char_p),
+    "z2d_gpu_sync": ([C.c_void_p], C.c_int32),
+    "z2d_gpu_fill": ([C.c_void_p, P(abi.SurfaceDesc), P(abi.PatternDesc), C.c_void_p,
+                      C.c_uint64, P(abi.FillOpts)], C.c_int32),
+    "z2d_gpu_stroke": ([C.c_void_p, P(abi.SurfaceDesc), P(abi.PatternDesc), C.c_void_p,
+                        C.c_uint64, P(abi.StrokeOpts)], C.c_int32),
+    "z2d_gpu_fill_polygon": ([C.c_void_p, P(abi.SurfaceDesc), P(abi.PatternDesc), C.c_void_p,
+                              C.c_uint64, P(C.c_double), C.c_uint32, C.c_uint32, C.c_uint32,
+                              C.c_uint32], C.c_int32),
+    "z2d_gpu_composite": ([C.c_void_p, P(abi.SurfaceDesc), C.c_int32, C.c_int32, P(abi.CompOp),
+                           C.c_uint64, C.c_uint32, P(abi.SurfaceDesc)], C.c_int32),
+    "z2d_gpu_surface_create": ([C.c_void_p, C.c_uint32, C.c_int32, C.c_int32, P(C.c_void_p)],
+                               C.c_int32),
+    "z2d_gpu_surface_upload": ([C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
+    "z2d_gpu_surface_download": ([C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
+    "z2d_gpu_surface_desc": ([C.c_void_p, P(abi.SurfaceDesc)], C.c_int32),
+    "z2d_gpu_surface_destroy": ([C.c_void_p], None),
+    "z2d_gpu_batch_create": ([C.c_void_p, P(C.c_void_p)], C.c_int32),
+    "z2d_gpu_batch_fill": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p, C.c_uint64,
+                            P(abi.FillOpts)], C.c_int32),
+    "z2d_gpu_batch_stroke": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p,
+                              C.c_uint64, P(abi.StrokeOpts)], C.c_int32),
+    "z2d_gpu_batch_fill_polygon": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p,
+                                    C.c_uint64, P(C.c_double), C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_uint32], C.c_int32),
+    "z2d_gpu_batch_prepare": ([C.c_void_p], C.c_int32),
+    "z2d_gpu_batch_render": ([C.c_void_p, C.c_uint32], C.c_int32),
+    "z2d_gpu_batch_wait": ([C.c_void_p], C.c_int32),
+    "z2d_gpu_batch_timing": ([C.c_void_p, P(C.c_double), P(C.c_double)], C.c_int32),
+    "z2d_gpu_batch_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
+    "z2d_gpu_batch_destroy": ([C.c_void_p], None),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().z2d_gpu_create(device, C.byref(h))
+        if rc != 0:
+            raise GpuError(f"z2d_gpu_create failed: {rc}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().z2d_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self):
+        return lib().z2d_gpu_last_error(self.h).decode()
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise GpuError(f"{what} returned {rc}: {self.last_error()}")
+
+    def sync(self):
+        self.check(lib().z2d_gpu_sync(self.h), "sync")
+
+
+class DeviceSurface:
+    """A surface resident in HBM (z2d_gpu_surface)."""
+
+    def __init__(self, ctx, fmt, width, height):
+        self.ctx = ctx
+        self.format, self.width, self.height = fmt, width, height
+        self.nbytes = abi.surface_nbytes(fmt, width, height)
+        h = C.c_void_p()
+        ctx.check(lib().z2d_gpu_surface_create(ctx.h, fmt, width, height, C.byref(h)),
+                  "surface_create")
+        self.h = h
+
+    def upload(self, arr):
+        self.ctx.check(lib().z2d_gpu_surface_upload(self.h, arr.ctypes.data, arr.nbytes),
+                       "surface_upload")
+
+    def download(self, arr):
+        self.ctx.check(lib().z2d_gpu_surface_download(self.h, arr.ctypes.data, arr.nbytes),
+                       "surface_download")
+
+    def desc(self):
+        d = abi.SurfaceDesc()
+        self.ctx.check(lib().z2d_gpu_surface_desc(self.h, C.byref(d)), "surface_desc")
+        return d
+
+    def close(self):
+        if self.h:
+            lib().z2d_gpu_surface_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many fills on resident surfaces rendered by one pipeline launch."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().z2d_gpu_batch_create(ctx.h, C.byref(h)), "batch_create")
+        self.h = h
+
+    def fill(self, surface, pattern, nodes, opts):
+        return lib().z2d_gpu_batch_fill(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
+                                        len(nodes), C.byref(opts))
+
+    def fill_polygon(self, surface, pattern, edges, extents, aa, rule, op, precision):
+        ext = (C.c_double * 4)(*extents)
+        return lib().z2d_gpu_batch_fill_polygon(self.h, surface.h, C.byref(pattern),
+                                                edges.ctypes.data, len(edges), ext, aa, rule, op,
+                                                precision)
+
+    def prepare(self):
+        self.ctx.check(lib().z2d_gpu_batch_prepare(self.h), "batch_prepare")
+
+    def render(self, mode=0):
+        self.ctx.check(lib().z2d_gpu_batch_render(self.h, mode), "batch_render")
+
+    def wait(self):
+        self.ctx.check(lib().z2d_gpu_batch_wait(self.h), "batch_wait")
+
+    def timing(self):
+        a, r = C.c_double(), C.c_double()
+        lib().z2d_gpu_batch_timing(self.h, C.byref(a), C.byref(r))
+        return a.value, r.value
+
+    def stats(self):
+        out = (C.c_uint64 * 4)()
+        lib().z2d_gpu_batch_stats(self.h, out)
+        return tuple(out)
+
+    def reset(self):
+        lib().z2d_gpu_batch_reset(self.h)
+
+    def close(self):
+        if self.h:
+            lib().z2d_gpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuBackend:
+    """Replays scene calls through the C ABI (host-buffer painter calls)."""
+
+    name = "gpu"
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or Context()
+
+    def fill(self, sfc, pattern, nodes, opts):
+        d = sfc.desc()
+        rc = lib().z2d_gpu_fill(self.ctx.h, C.byref(d), C.byref(pattern), nodes.ctypes.data,
+                                len(nodes), C.byref(opts))
+        return rc
+
+    def stroke(self, sfc, pattern, nodes, opts):
+        d = sfc.desc()
+        return lib().z2d_gpu_stroke(self.ctx.h, C.byref(d), C.byref(pattern), nodes.ctypes.data,
+                                    len(nodes), C.byref(opts))
+
+    def composite(self, sfc, dst_x, dst_y, ops, precision, out=None):
+        d = sfc.desc()
+        arr = (abi.CompOp * len(ops))(*ops)
+        od = out.desc() if out is not None else None
+        return lib().z2d_gpu_composite(self.ctx.h, C.byref(d), dst_x, dst_y, arr, len(ops),
+                                       precision, C.byref(od) if od is not None else None)
