@@ -1,0 +1,275 @@
+"""Benchmark of the fill + composite hot path on one MI355X.
+
+Headline workload (the configuration the metric is quoted on): one
+3840x2160 RGBA8 scene of 10,000 random closed paths (70% lines / 30% cubic
+Beziers, 3-64 segments, log-uniform 4-512 px boxes, 60% solid / 40%
+gradient paints, 50/50 non-zero / even-odd, src_over, 4x multisample AA),
+inputs resident in HBM.  One step = one pass of the hot path over the whole
+scene: edge build from the flattened nodes, binning, and the tile
+rasterize + composite of all 10,000 paths in order.
+
+  value          pixel-draws per second (Mpix/s): pixels with nonzero
+                 coverage summed over all paths, counted on the device
+  roofline       algorithmic bytes of the render kernel (2*bpp per pixel-draw
+                 + 32 B per edge + paint parameters) / its mean duration,
+                 against MI355X HBM3E peak (8 TB/s)
+  cpu_baseline   the C oracle (a scalar restatement of the reference
+                 painter) rendering the first 4,000 paths of the same scene
+                 on one host core
+  extra          512x512 star latency (p50/p99), 4096x4096 compositor
+                 passes (GB/s), batched uneven surfaces vs one launch per
+                 surface
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from z2d_amd import abi, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+METRIC = "fill+composite Mpix/s (% HBM peak); p50/p99 latency of batched render reqs"
+
+
+def paint_bytes(pattern):
+    if pattern.kind == abi.PATTERN_GRADIENT:
+        return 96 + 20 * pattern.gradient.contents.n_stops
+    return 8
+
+
+def build_batch(gpu, ctx, sc, n=None):
+    surf = gpu.DeviceSurface(ctx, sc["format"], sc["width"], sc["height"])
+    surf.upload(sc["buf"])
+    b = gpu.Batch(ctx)
+    draws = sc["draws"] if n is None else sc["draws"][:n]
+    for pat, nodes, opts in draws:
+        rc = b.fill(surf, pat, nodes, opts)
+        if rc != 0:
+            raise RuntimeError(f"batch_fill rc={rc}")
+    b.prepare()
+    return surf, b
+
+
+def count_work(b):
+    b.counters(True)
+    b.render(0)
+    b.wait()
+    pix, edges, _, overflow = b.counters(False)
+    if overflow:
+        raise RuntimeError("bin overflow while counting")
+    return pix, edges
+
+
+def bench_scene(gpu, ctx, sc, steps, warmup):
+    surf, b = build_batch(gpu, ctx, sc)
+    for _ in range(warmup):
+        b.render(0)
+        b.wait()
+    pix, edges = count_work(b)
+    ctx.sync()
+    render_ms, pipe_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.render(0)
+        b.wait()
+        p, r = b.timing()
+        pipe_ms.append(p)
+        render_ms.append(r)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    pbytes = sum(paint_bytes(d[0]) for d in sc["draws"])
+    alg_bytes = 2 * 4 * pix + 32 * edges + pbytes
+    res = dict(pix=pix, edges=edges, wall=wall, render_ms=float(np.mean(render_ms)),
+               pipe_ms=float(np.mean(pipe_ms)), alg_bytes=alg_bytes)
+    b.close()
+    surf.close()
+    return res
+
+
+def bench_star_latency(gpu, ctx, iters=300):
+    sc = scenes.star_512()
+    surf, b = build_batch(gpu, ctx, sc)
+    for _ in range(10):
+        b.render(0)
+        b.wait()
+    lat = []
+    for _ in range(iters):
+        t = time.perf_counter()
+        b.render(0)
+        b.wait()
+        lat.append((time.perf_counter() - t) * 1e3)
+    b.close()
+    surf.close()
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def bench_compositor(gpu, ctx, ops=("src_over", "multiply", "soft_light"), reps=5):
+    w = h = 4096
+    src, dst, _mask = scenes.compositor_inputs(w, h)
+    s_src = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+    s_dst = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+    s_out = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+    s_src.upload(src)
+    s_dst.upload(dst)
+    d_src, d_dst, d_out = s_src.desc(), s_dst.desc(), s_out.desc()
+    out = {}
+    for name in ops:
+        o = abi.CompOp()
+        o.op = abi.OP[name]
+        o.src.kind = abi.PARAM_SURFACE
+        o.src.surface = C.pointer(d_src)
+        arr = (abi.CompOp * 1)(o)
+        f = gpu.lib().z2d_gpu_composite
+        f(ctx.h, C.byref(d_dst), 0, 0, arr, 1, 0, C.byref(d_out))
+        t = time.perf_counter()
+        for _ in range(reps):
+            rc = f(ctx.h, C.byref(d_dst), 0, 0, arr, 1, 0, C.byref(d_out))
+            if rc:
+                raise RuntimeError(f"composite rc={rc}")
+        dt = (time.perf_counter() - t) / reps
+        out[name] = round(w * h * 12 / dt / 1e9, 1)
+    for s in (s_src, s_dst, s_out):
+        s.close()
+    return out
+
+
+def bench_uneven_batch(gpu, ctx, n_surfaces=512, seed=0xC4):
+    """Uneven fill-only surface mix (sides log-uniform 16-1024 px, 80% RGBA8
+    / 20% alpha8, 1-50 paths each): one persistent launch vs one launch per
+    surface over the same prepared batch."""
+    rng = scenes.SplitMix64(seed)
+    b = gpu.Batch(ctx)
+    surfs = []
+    for _ in range(n_surfaces):
+        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
+        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
+        s = gpu.DeviceSurface(ctx, fmt, side, side)
+        surfs.append(s)
+        draws, keep = scenes.random_paths(rng, rng.randint(1, 50), side, side,
+                                          max_side=min(512.0, side))
+        surfs.append(keep)
+        for pat, nodes, opts in draws:
+            b.fill(s, pat, nodes, opts)
+    b.prepare()
+    pix, _ = count_work(b)
+    res = {}
+    for mode, name in ((0, "batched"), (1, "per_surface")):
+        for _ in range(2):
+            b.render(mode)
+            b.wait()
+        t = time.perf_counter()
+        for _ in range(3):
+            b.render(mode)
+            b.wait()
+        dt = (time.perf_counter() - t) / 3
+        res[name] = round(pix / dt / 1e6, 1)
+    res["speedup"] = round(res["batched"] / res["per_surface"], 2)
+    res["surfaces"] = n_surfaces
+    b.close()
+    return res
+
+
+def cpu_baseline(gpu, ctx, sc, n_paths):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_backend import OracleBackend  # the checker, timed as the baseline
+    from zscene import Surface
+    _, b = build_batch(gpu, ctx, sc, n_paths)
+    pix, _ = count_work(b)
+    b.close()
+    s = Surface(sc["format"], sc["width"], sc["height"])
+    s.buf[: s.nbytes] = sc["buf"]
+    ob = OracleBackend()
+    t = time.perf_counter()
+    for pat, nodes, opts in sc["draws"][:n_paths]:
+        ob.fill(s, pat, nodes, opts)
+    dt = time.perf_counter() - t
+    return dict(value=round(pix / dt / 1e6, 2), unit="Mpix/s", cores=1, kind="port",
+                sample=f"first {n_paths} of the 10,000 paths of the same 3840x2160 scene, "
+                       f"C oracle (-O2), one core, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--paths", type=int, default=10000)
+    ap.add_argument("--cpu-paths", type=int, default=4000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the stream-copy kernel on this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py renders on one GPU only (got --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md and not run", file=sys.stderr)
+        sys.exit(2)
+    from z2d_amd import gpu
+    ctx = gpu.Context()
+    if args.copy_probe:
+        g = C.c_double()
+        ctx.check(gpu.lib().z2d_gpu_stream_copy(ctx.h, args.copy_probe << 20, 1, C.byref(g)),
+                  "stream_copy")
+        print(json.dumps({"stream_copy_MiB": args.copy_probe, "GBps": round(g.value, 1)}))
+        return
+    log("generating scene")
+    sc = scenes.scene_4k(n_paths=args.paths)
+    log("rendering scene")
+    r = bench_scene(gpu, ctx, sc, args.steps, args.warmup)
+    log(f"scene: {r['render_ms']:.3f} ms render, {r['pipe_ms']:.3f} ms pipeline")
+    ms = r["wall"] * 1e3 / args.steps
+    value = r["pix"] * args.steps / r["wall"] / 1e6
+    achieved = r["alg_bytes"] / (r["render_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "scene_3840x2160_10k_paths", "width": 3840, "height": 2160,
+                   "paths": args.paths, "seed": "0xC3", "aa": "multisample_4x",
+                   "op": "src_over", "pixel_draws_per_step": int(r["pix"]),
+                   "edges_per_step": int(r["edges"])},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_render",
+                     "kernel_ms": round(r["render_ms"], 4),
+                     "pipeline_ms": round(r["pipe_ms"], 4)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(gpu, ctx, sc, min(args.cpu_paths, args.paths))
+    if not args.no_extra:
+        log("extras")
+        p50, p99 = bench_star_latency(gpu, ctx)
+        g = C.c_double()
+        ctx.check(gpu.lib().z2d_gpu_stream_copy(ctx.h, 1 << 30, 20, C.byref(g)), "stream_copy")
+        line["extra"] = {
+            "stream_copy_GBps": round(g.value, 1),
+            "star_512_latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
+            "compositor_4096_GBps": bench_compositor(gpu, ctx),
+            "uneven_fill_batch_Mpix_s": bench_uneven_batch(gpu, ctx),
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

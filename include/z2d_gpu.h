@@ -114,8 +114,18 @@ int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b);
 int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms);
 /* Number of draws / surfaces / flattened nodes in the batch. */
 int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]);
+/* Work counters of the last render (enable before rendering): out[0] =
+ * pixel-draws (pixels with nonzero coverage summed over draws), out[1] =
+ * edges built, out[3] = nonzero if a bin overflowed.  No reference
+ * counterpart: the benchmark's unit of work. */
+int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4]);
 int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b);
 void z2d_gpu_batch_destroy(z2d_gpu_batch *b);
+
+/* Measurement utility with no reference counterpart: copies `bytes` between
+ * two device buffers `iters` times with a plain stream-copy kernel and
+ * reports read+write GB/s (the achievable-bandwidth reference). */
+int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
 /* Synchronizes the context stream. */
 int32_t z2d_gpu_sync(z2d_gpu_ctx *ctx);

@@ -31,6 +31,7 @@ hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 
 static const uint16_t kBlueNoise[4096] = {
 #include "blue_noise_64x64.inc"
@@ -626,6 +627,7 @@ struct z2d_gpu_batch {
   int32_t *d_scratch_x = nullptr;
   std::vector<uint32_t> surf_tile_off;  // per surface slot, + total
   float last_pipeline_ms = 0.0f, last_render_ms = 0.0f;
+  uint32_t count_stats = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
   uint32_t slot_of(z2d_gpu_surface *s) {
@@ -855,6 +857,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.entry_edge_idx = nullptr;
   J.max_entry_edges = 0;
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
+  J.count_stats = b->count_stats;
   J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);
   J.rows = reinterpret_cast<RowRec *>(D + o_rows);
   J.max_rows = (uint32_t)max_rows;
@@ -1330,6 +1333,26 @@ int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]) {
   return Z2D_OK;
 }
 
+int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4]) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  b->count_stats = enable ? 1u : 0u;
+  b->view.count_stats = b->count_stats;
+  if (out) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (b->prepared && b->view.counters) {
+      uint32_t c[16];
+      HIP_OK(b->ctx, hipMemcpyAsync(c, b->view.counters, sizeof(c), hipMemcpyDeviceToHost,
+                                    b->ctx->stream));
+      HIP_OK(b->ctx, hipStreamSynchronize(b->ctx->stream));
+      out[0] = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
+      out[1] = c[6];
+      out[2] = b->view.n_bands ? 0 : 0;
+      out[3] = c[2];
+    }
+  }
+  return Z2D_OK;
+}
+
 int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b) {
   if (!b) return Z2D_ERR_INVALID_ARGUMENT;
   (void)hipStreamSynchronize(b->ctx->stream);
@@ -1428,6 +1451,39 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           const z2d_comp_op *ops, uint64_t n_ops, uint32_t precision,
                           z2d_surface_desc *out) {
   return composite_impl(ctx, dst, dst_x, dst_y, ops, n_ops, precision, out);
+}
+
+int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (bytes < 16 || iters == 0) return Z2D_ERR_INVALID_ARGUMENT;
+  bytes &= ~15ull;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  void *a = nullptr, *b = nullptr;
+  HIP_OK(ctx, hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, e, "stream copy alloc");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  e = hipMemsetAsync(a, 1, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = z2d_launch_stream_copy(a, b, bytes, ctx->stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+  for (uint32_t i = 0; i < iters && e == hipSuccess; i++)
+    e = z2d_launch_stream_copy(a, b, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess && gbps) *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(ctx, e, "stream copy");
+  return Z2D_OK;
 }
 
 int32_t z2d_gpu_stroke(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,

@@ -108,7 +108,8 @@ struct JobView {
   uint32_t max_entries;
   uint32_t *entry_edge_idx;
   uint32_t max_entry_edges;
-  uint32_t *counters;    // [2] entry overflow flag
+  uint32_t *counters;    // [2] entry overflow flag; [4..5] pixel-draws (u64), [6] edges
+  uint32_t count_stats;  // nonzero: accumulate pixel-draw / edge counts
   uint32_t *cursors;     // tile work cursors, one per render launch
   RowRec *rows;
   uint32_t max_rows;

@@ -164,6 +164,7 @@ __global__ void k_edge_build(JobView J) {
     }
   }
   D.edge_cnt = acc.n;
+  if (J.count_stats && acc.n) atomicAdd(&J.counters[6], acc.n);
   D.ext[0] = acc.top; D.ext[1] = acc.bottom; D.ext[2] = acc.left; D.ext[3] = acc.right;
 
   // Raster entry logic: visibility, draw window and composite region.
@@ -611,6 +612,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
       // ---- phase B: per-pixel coverage and composite
       const bool overflow = sh.overflow != 0;
       const RowRec *R = (D.flags & DRAW_ROWPROG) ? J.rows + D.row_off : nullptr;
+      uint32_t n_cov = 0;
 #pragma unroll
       for (int i = 0; i < ROWS_PER_THREAD; i++) {
         const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
@@ -668,7 +670,15 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
             if (in && xs_base + c_lo + j < cutoff) k++;
           }
         }
+        n_cov += k > 0;
         pix[i] = composite_cov(pix[i], S.fmt, k, SC * SC, D, P, J.blue_noise, px, py);
+      }
+      if (J.count_stats) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) n_cov += __shfl_xor(n_cov, off, 64);
+        if (lane == 0 && n_cov)
+          atomicAdd(reinterpret_cast<unsigned long long *>(&J.counters[4]),
+                    (unsigned long long)n_cov);
       }
       __syncthreads();
     }
@@ -790,6 +800,24 @@ __global__ void __launch_bounds__(256) k_composite_stream(const CompJob *__restr
 }  // namespace z2d
 
 using namespace z2d;
+
+// Plain stream copy (16 B per lane per iteration): the bandwidth reference
+// and the known-size kernel that calibrates HBM byte counters.
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
+  uint64_t n = bytes / 16;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_stream_copy, dim3((uint32_t)blocks), dim3(256), 0, st,
+                     reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n);
+  return hipGetLastError();
+}
 
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

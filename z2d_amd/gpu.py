@@ -46,8 +46,10 @@ _SIGS = {
     "z2d_gpu_batch_wait": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_timing": ([C.c_void_p, P(C.c_double), P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_destroy": ([C.c_void_p], None),
+    "z2d_gpu_stream_copy": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
 }
 
 
@@ -176,6 +178,11 @@ class Batch:
     def stats(self):
         out = (C.c_uint64 * 4)()
         lib().z2d_gpu_batch_stats(self.h, out)
+        return tuple(out)
+
+    def counters(self, enable=True):
+        out = (C.c_uint64 * 4)()
+        self.ctx.check(lib().z2d_gpu_batch_counters(self.h, 1 if enable else 0, out), "counters")
         return tuple(out)
 
     def reset(self):
