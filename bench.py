@@ -18,7 +18,8 @@ rasterize + composite of all 10,000 paths in order.
                  on one host core
   extra          512x512 star latency (p50/p99), 4096x4096 compositor
                  passes (GB/s), batched uneven surfaces vs one launch per
-                 surface
+                 surface, and the request server under open-loop Poisson
+                 arrivals (Mpix/s, p50/p99 latency per QPS level)
 """
 import argparse
 import ctypes as C
@@ -180,6 +181,84 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=512, seed=0xC4):
     return res
 
 
+def serving_requests(n, seed=0x5EF):
+    """Request mix of the serving form: one surface job each, side
+    log-uniform 16-1024 px, 80% RGBA8 / 20% alpha8, 1-50 paths of the 4K
+    scene's distribution, 30% with an alpha8 clip mask."""
+    rng = scenes.SplitMix64(seed)
+    reqs = []
+    for _ in range(n):
+        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
+        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
+        draws, keep = scenes.random_paths(rng, rng.randint(1, 50), side, side,
+                                          max_side=min(512.0, side))
+        clip = None
+        if rng.uniform() < 0.3:
+            clip = scenes.splitmix_stream(rng.next(), (side * side + 7) // 8).view(np.uint8)
+            clip = clip[: side * side].copy()
+        reqs.append(dict(fmt=fmt, w=side, h=side, draws=draws, keep=keep, clip=clip,
+                         out=np.empty(abi.surface_nbytes(fmt, side, side), np.uint8)))
+    return reqs
+
+
+def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000), seconds=2.0, seed=0x9015):
+    """Open-loop Poisson arrivals at each QPS level, cycling through the
+    request pool (and one saturation run with twice the pool queued at
+    t=0).  Latency = completion - scheduled
+    arrival; throughput = pixel-draws counted on the device / wall time."""
+    import threading
+    srv = gpu.Server(ctx, 2 << 30, 64)
+    white = abi.pixel(abi.RGBA, 255, 255, 255, 255)
+    a_clear = abi.pixel(abi.ALPHA8, a=0)
+    rng = np.random.default_rng(seed)
+    out = {}
+    levels = list(qps_levels) + ["saturation"]
+    for lvl in levels:
+        n = 2 * len(reqs) if lvl == "saturation" else int(lvl * seconds)
+        gaps = (np.zeros(n) if lvl == "saturation" else rng.exponential(1.0 / lvl, n))
+        sched = np.cumsum(gaps)
+        st0 = srv.stats()
+        done_t = {}
+
+        def collect():
+            while len(done_t) < n:
+                got = srv.wait_any(30000.0)
+                if got is None:
+                    return
+                done_t[got[0]] = (time.perf_counter(), got[1])
+
+        th = threading.Thread(target=collect)
+        th.start()
+        ids = []
+        t0 = time.perf_counter()
+        for i in range(n):
+            rq = reqs[i % len(reqs)]
+            while time.perf_counter() - t0 < sched[i]:
+                pass
+            rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], rq["draws"],
+                                 clear=white if rq["fmt"] == abi.RGBA else a_clear,
+                                 clip=rq["clip"], out=rq["out"])
+            if rc:
+                raise RuntimeError(f"server submit rc={rc}")
+            ids.append(rid)
+        th.join()
+        t_end = time.perf_counter()
+        if len(done_t) < n or any(v[1] for v in done_t.values()):
+            raise RuntimeError(f"serving level {lvl}: {len(done_t)}/{n} completed")
+        lat = np.array([(done_t[rid][0] - t0 - sched[i]) * 1e3 for i, rid in enumerate(ids)])
+        st1 = srv.stats()
+        pix = st1["pixel_draws"] - st0["pixel_draws"]
+        out[str(lvl)] = dict(requests=n, Mpix_s=round(pix / (t_end - t0) / 1e6, 1),
+                             req_s=round(n / (t_end - t0), 1),
+                             p50_ms=round(float(np.percentile(lat, 50)), 3),
+                             p99_ms=round(float(np.percentile(lat, 99)), 3),
+                             batches=int(st1["batches"] - st0["batches"]))
+        log(f"serving {lvl}: {out[str(lvl)]}")
+    out["peak_hbm_MiB"] = round(srv.stats()["peak_bytes"] / 2**20, 1)
+    srv.close()
+    return out
+
+
 def cpu_baseline(gpu, ctx, sc, n_paths):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_backend import OracleBackend  # the checker, timed as the baseline
@@ -208,6 +287,8 @@ def main():
     ap.add_argument("--cpu-paths", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="distinct requests in the serving pool (cycled at each QPS level)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the stream-copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -267,6 +348,7 @@ def main():
             "star_512_latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
             "compositor_4096_GBps": bench_compositor(gpu, ctx),
             "uneven_fill_batch_Mpix_s": bench_uneven_batch(gpu, ctx),
+            "serving": bench_serving(gpu, ctx, serving_requests(args.serve_requests)),
         }
     print(json.dumps(line), flush=True)
 

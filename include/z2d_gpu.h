@@ -122,6 +122,51 @@ int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4
 int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b);
 void z2d_gpu_batch_destroy(z2d_gpu_batch *b);
 
+/* ---- serving ------------------------------------------------------------------
+ * A request loop with continuous batching: each request is one surface job
+ * (a cleared surface, fills applied in order, then an optional alpha8 clip
+ * mask applied as Surface.composite(mask, .dst_in)).  Requests are copied at
+ * submit, admitted while their device memory fits `hbm_budget`, and rendered
+ * in batches on two streams so requests arriving while others run are
+ * batched and overlapped on the device.  No reference counterpart (the
+ * reference renders one call at a time). */
+typedef struct z2d_gpu_server z2d_gpu_server;
+
+typedef struct z2d_fill_item {
+  const z2d_pattern_desc *pattern;
+  const z2d_path_node *nodes;
+  uint64_t n_nodes;
+  z2d_fill_opts opts;
+} z2d_fill_item;
+
+typedef struct z2d_request_desc {
+  uint32_t format;
+  int32_t width;
+  int32_t height;
+  z2d_pixel clear;            /* initial contents (Surface.initPixel) */
+  const uint8_t *clip_mask;   /* optional alpha8 width*height mask */
+  const z2d_fill_item *fills;
+  uint64_t n_fills;
+} z2d_request_desc;
+
+int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch,
+                              z2d_gpu_server **out);
+/* `out` (optional, >= the surface's byte size) receives the result and must
+ * stay valid until the request completes.  BUDGET_EXCEEDED if the request
+ * can never fit. */
+int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *req, void *out,
+                              uint64_t out_len, uint64_t *id);
+/* 1 = done (status filled, request released), 0 = pending. */
+int32_t z2d_gpu_server_poll(z2d_gpu_server *s, uint64_t id, int32_t *status);
+int32_t z2d_gpu_server_wait(z2d_gpu_server *s, uint64_t id, int32_t *status, double timeout_ms);
+/* Next completed request in completion order (1), or 0 on timeout. */
+int32_t z2d_gpu_server_wait_any(z2d_gpu_server *s, uint64_t *id, int32_t *status,
+                                double timeout_ms);
+/* submitted, completed, rejected, batches, pixel-draws, bytes in use, peak
+ * bytes, queued. */
+int32_t z2d_gpu_server_stats(z2d_gpu_server *s, uint64_t out[8]);
+void z2d_gpu_server_destroy(z2d_gpu_server *s);
+
 /* Measurement utility with no reference counterpart: copies `bytes` between
  * two device buffers `iters` times with a plain stream-copy kernel and
  * reports read+write GB/s (the achievable-bandwidth reference). */

@@ -1,0 +1,160 @@
+"""The request server (continuous batching under an HBM budget), checked on
+the GPU through the C ABI.  Each request is one surface: initialised from a
+pixel (ImageSurface.initPixel, src/surface.zig), the fills applied in order
+(painter.fill, src/painter.zig:66-143), then an optional alpha8 clip mask
+applied as Surface.composite(mask, .dst_in) (src/surface.zig:225-241).  The
+expected surface is the oracle's rendering of the same calls; a request
+renders byte-identically alone and inside a mix of other requests; a budget
+smaller than the whole load still completes every request."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from z2d_amd import abi, scenes
+from gpu_util import diff_report, gpu_backend
+from oracle_backend import OracleBackend
+from zscene import Surface
+
+pytestmark = pytest.mark.gpu
+
+
+def _clear_pixel(rng, fmt):
+    if fmt in (abi.RGBA, abi.ARGB):
+        a = rng.randint(0, 255)
+        return abi.pixel(fmt, rng.randint(0, a), rng.randint(0, a), rng.randint(0, a), a)
+    if fmt in (abi.RGB, abi.XRGB):
+        return abi.pixel(fmt, rng.randint(0, 255), rng.randint(0, 255), rng.randint(0, 255))
+    return abi.pixel(fmt, a=rng.randint(0, (1 << abi.FORMAT_BITS[fmt]) - 1))
+
+
+def make_request(rng, fmt=None, side_max=300, with_clip=None):
+    fmt = fmt if fmt is not None else rng.randint(0, 7)
+    w = rng.randint(8, side_max)
+    h = rng.randint(8, side_max)
+    aa = [abi.AA_NONE, abi.AA_DEFAULT, abi.AA_MULTISAMPLE_4X][rng.randint(0, 2)]
+    draws, keep = scenes.random_paths(rng, rng.randint(0, 12), w, h,
+                                      max_side=min(200.0, max(w, h)), aa=aa)
+    clip = None
+    if with_clip if with_clip is not None else rng.uniform() < 0.4:
+        clip = rng.u8_array(w * h)
+    return dict(fmt=fmt, w=w, h=h, clear=_clear_pixel(rng, fmt), draws=draws, keep=keep,
+                clip=clip)
+
+
+def oracle_render(rq):
+    s = Surface(rq["fmt"], rq["w"], rq["h"], init_px=rq["clear"])
+    ob = OracleBackend()
+    for pat, nodes, opts in rq["draws"]:
+        assert ob.fill(s, pat, nodes, opts) == 0
+    if rq["clip"] is not None:
+        m = Surface(abi.ALPHA8, rq["w"], rq["h"])
+        m.buf[: m.nbytes] = rq["clip"]
+        md = m.desc()
+        op = abi.CompOp()
+        op.op = abi.OP["dst_in"]
+        op.src.kind = abi.PARAM_SURFACE
+        op.src.surface = C.pointer(md)
+        ob.composite(s, 0, 0, [op], abi.PRECISION_INTEGER)
+    return np.asarray(s.data(), np.uint8)[: s.nbytes].copy()
+
+
+def submit(srv, rq):
+    out = np.zeros(abi.surface_nbytes(rq["fmt"], rq["w"], rq["h"]), np.uint8)
+    rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], rq["draws"], clear=rq["clear"],
+                         clip=rq["clip"], out=out)
+    assert rc == 0, f"submit rc={rc}"
+    return rid, out
+
+
+def _server(budget, max_batch=64):
+    from z2d_amd.gpu import Server
+    return Server(gpu_backend().ctx, budget, max_batch)
+
+
+def test_server_requests_match_oracle():
+    rng = scenes.SplitMix64(0x5E5)
+    reqs = [make_request(rng) for _ in range(48)]
+    srv = _server(256 << 20)
+    try:
+        subs = [submit(srv, rq) for rq in reqs]
+        for i, ((rid, out), rq) in enumerate(zip(subs, reqs)):
+            assert srv.wait(rid) == 0, f"request {i}"
+            msg = diff_report(out, oracle_render(rq))
+            assert not msg, f"request {i} fmt={rq['fmt']} {rq['w']}x{rq['h']}: {msg}"
+        st = srv.stats()
+        assert st["completed"] == st["submitted"] == len(reqs)
+        assert st["bytes_in_use"] == 0 and st["queued"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_same_alone_and_in_mix():
+    rng = scenes.SplitMix64(0xA11)
+    target = make_request(rng, fmt=abi.RGBA, side_max=400, with_clip=True)
+    others = [make_request(rng) for _ in range(40)]
+    srv = _server(512 << 20, max_batch=16)
+    try:
+        rid, alone = submit(srv, target)
+        assert srv.wait(rid) == 0
+        subs = []
+        for i, rq in enumerate(others):
+            subs.append(submit(srv, rq))
+            if i == 17:
+                mixed_id, mixed = submit(srv, target)
+        assert srv.wait(mixed_id) == 0
+        for rid2, _ in subs:
+            assert srv.wait(rid2) == 0
+        assert not diff_report(mixed, alone)
+        assert not diff_report(alone, oracle_render(target))
+        assert srv.stats()["batches"] >= 2
+    finally:
+        srv.close()
+
+
+def test_server_budget_admission():
+    """A 1 MiB budget admits a few requests at a time; the rest wait in the
+    queue and still complete.  A request that can never fit is refused."""
+    rng = scenes.SplitMix64(0xB0D)
+    reqs = [make_request(rng, fmt=abi.RGBA, side_max=256, with_clip=False) for _ in range(24)]
+    budget = 1 << 20
+    srv = _server(budget)
+    try:
+        big = np.zeros(600 * 600 * 4, np.uint8)
+        rc, _ = srv.submit(abi.RGBA, 600, 600, [], out=big)
+        assert rc == abi.ERR_BUDGET_EXCEEDED
+        subs = [submit(srv, rq) for rq in reqs]
+        done = {}
+        while len(done) < len(subs):
+            got = srv.wait_any(60000.0)
+            assert got is not None, "server stalled"
+            done[got[0]] = got[1]
+        assert all(v == 0 for v in done.values())
+        for (rid, out), rq in zip(subs, reqs):
+            assert not diff_report(out, oracle_render(rq))
+        st = srv.stats()
+        assert st["rejected"] == 1
+        assert 0 < st["peak_bytes"] <= budget
+        assert st["batches"] > 1
+    finally:
+        srv.close()
+
+
+def test_server_error_status():
+    """An open path fails its request with PathNotClosed (src/painter.zig:82);
+    the other requests in the batch are unaffected."""
+    rng = scenes.SplitMix64(0xE77)
+    good = make_request(rng, fmt=abi.RGBA, with_clip=False)
+    bad = make_request(rng, fmt=abi.RGBA, with_clip=False)
+    open_nodes = abi.nodes_array([(abi.MOVE_TO, 1.0, 1.0), (abi.LINE_TO, 20.0, 3.0),
+                                  (abi.LINE_TO, 9.0, 15.0)])
+    bad["draws"] = [(abi.solid_pattern(abi.rgba(255, 0, 0)), open_nodes, abi.fill_opts())]
+    srv = _server(64 << 20)
+    try:
+        rid_b, _ = submit(srv, bad)
+        rid_g, out = submit(srv, good)
+        assert srv.wait(rid_b) == abi.ERR_PATH_NOT_CLOSED
+        assert srv.wait(rid_g) == 0
+        assert not diff_report(out, oracle_render(good))
+    finally:
+        srv.close()

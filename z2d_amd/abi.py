@@ -123,6 +123,17 @@ class CompOp(C.Structure):
     _fields_ = [("op", C.c_uint32), ("dst", CompParam), ("src", CompParam)]
 
 
+class FillItem(C.Structure):
+    _fields_ = [("pattern", C.POINTER(PatternDesc)), ("nodes", C.c_void_p),
+                ("n_nodes", C.c_uint64), ("opts", FillOpts)]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [("format", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32),
+                ("clear", Pixel), ("clip_mask", C.c_void_p), ("fills", C.POINTER(FillItem)),
+                ("n_fills", C.c_uint64)]
+
+
 IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
 
 NODE_DTYPE = np.dtype([("tag", "<u4"), ("reserved", "<u4"), ("v", "<f8", (6,))])

@@ -7,7 +7,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -120,6 +128,7 @@ struct z2d_gpu_surface {
   uint8_t *data = nullptr;
   uint64_t nbytes = 0;
   bool owned = true;
+  const uint8_t *clip = nullptr;  // alpha8 clip applied after a batch's draws
 };
 
 static int32_t fail(z2d_gpu_ctx *ctx, hipError_t e, const char *what) {
@@ -610,6 +619,7 @@ struct HostDraw {
 
 struct z2d_gpu_batch {
   z2d_gpu_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;  // null: the context stream
   std::vector<z2d_gpu_surface *> surfaces;
   std::vector<FNode> nodes;
   std::vector<z2d_edge> in_edges;
@@ -637,6 +647,10 @@ struct z2d_gpu_batch {
     return (uint32_t)surfaces.size() - 1;
   }
 };
+
+static inline hipStream_t bstream(const z2d_gpu_batch *b) {
+  return b->stream ? b->stream : b->ctx->stream;
+}
 
 static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
                                 const z2d_pattern_desc *pattern, uint32_t aa, uint32_t rule,
@@ -818,6 +832,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     hs[i].fmt = s->format;
     hs[i].pad = 0;
     hs[i].nbytes = s->nbytes;
+    hs[i].clip = s->clip;
   }
   b->paints.write(H, D, o_paints, o_grads, o_stops);
   std::memcpy(H + o_sdo, surf_draw_off.data(), (ns + 1) * sizeof(uint32_t));
@@ -836,7 +851,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     std::memcpy(H + o_rps, rowprog_soff.data(), rowprog_soff.size() * sizeof(uint32_t));
   }
   std::memcpy(H + o_bn, kBlueNoise, sizeof(kBlueNoise));
-  HIP_OK(ctx, hipMemcpyAsync(D, H, input_bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_OK(ctx, hipMemcpyAsync(D, H, input_bytes, hipMemcpyHostToDevice, bstream(b)));
 
   JobView &J = b->view;
   J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
@@ -867,7 +882,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   b->d_rowprog_soff = reinterpret_cast<uint32_t *>(D + o_rps);
   b->d_scratch_edges = reinterpret_cast<z2d_edge *>(D + o_sedges);
   b->d_scratch_x = reinterpret_cast<int32_t *>(D + o_sx);
-  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_OK(ctx, hipStreamSynchronize(bstream(b)));
   b->prepared = true;
   return Z2D_OK;
 }
@@ -878,38 +893,40 @@ static int32_t batch_render(z2d_gpu_batch *b, uint32_t mode) {
     int32_t rc = batch_prepare(b);
     if (rc) return rc;
   }
-  if (b->draws.empty()) return Z2D_OK;
+  bool any_clip = false;
+  for (const z2d_gpu_surface *s : b->surfaces) any_clip |= s->clip != nullptr;
+  if (b->draws.empty() && !any_clip) return Z2D_OK;
   HIP_OK(ctx, hipSetDevice(ctx->device));
   JobView &J = b->view;
   const uint32_t ns = J.n_surfaces;
   for (int i = 0; i < 3; i++)
     if (!b->ev[i]) HIP_OK(ctx, hipEventCreate(&b->ev[i]));
-  HIP_OK(ctx, hipEventRecord(b->ev[0], ctx->stream));
-  HIP_OK(ctx, z2d_launch_fill_u32(J.counters, 16, 0u, ctx->stream));
-  HIP_OK(ctx, z2d_launch_fill_u32(J.cursors, ns + 1, 0u, ctx->stream));
-  HIP_OK(ctx, z2d_launch_edge_build(J, ctx->stream));
+  HIP_OK(ctx, hipEventRecord(b->ev[0], bstream(b)));
+  HIP_OK(ctx, z2d_launch_fill_u32(J.counters, 16, 0u, bstream(b)));
+  HIP_OK(ctx, z2d_launch_fill_u32(J.cursors, ns + 1, 0u, bstream(b)));
+  HIP_OK(ctx, z2d_launch_edge_build(J, bstream(b)));
   HIP_OK(ctx, z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,
-                                 b->d_scratch_x, b->d_rowprog_soff, ctx->stream));
-  HIP_OK(ctx, z2d_launch_binning(J, ctx->stream));
-  HIP_OK(ctx, hipEventRecord(b->ev[1], ctx->stream));
+                                 b->d_scratch_x, b->d_rowprog_soff, bstream(b)));
+  HIP_OK(ctx, z2d_launch_binning(J, bstream(b)));
+  HIP_OK(ctx, hipEventRecord(b->ev[1], bstream(b)));
   const uint32_t max_blocks = (uint32_t)ctx->n_cus * 8;
   if (mode == 0) {
     uint32_t nb = std::min(J.n_tiles, max_blocks);
-    HIP_OK(ctx, z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, ctx->stream));
+    HIP_OK(ctx, z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, bstream(b)));
   } else {
     for (uint32_t s = 0; s < ns; s++) {
       uint32_t t0 = b->surf_tile_off[s], tn = b->surf_tile_off[s + 1] - t0;
       uint32_t nb = std::min(tn, max_blocks);
-      HIP_OK(ctx, z2d_launch_render(J, t0, tn, J.cursors + s, nb, ctx->stream));
+      HIP_OK(ctx, z2d_launch_render(J, t0, tn, J.cursors + s, nb, bstream(b)));
     }
   }
-  HIP_OK(ctx, hipEventRecord(b->ev[2], ctx->stream));
+  HIP_OK(ctx, hipEventRecord(b->ev[2], bstream(b)));
   return Z2D_OK;
 }
 
 static int32_t batch_wait(z2d_gpu_batch *b) {
   z2d_gpu_ctx *ctx = b->ctx;
-  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_OK(ctx, hipStreamSynchronize(bstream(b)));
   if (b->ev[0] && b->ev[2]) {
     float a = 0.0f, r = 0.0f;
     if (hipEventElapsedTime(&a, b->ev[0], b->ev[2]) == hipSuccess) b->last_pipeline_ms = a;
@@ -1079,6 +1096,7 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
     d.fmt = s->format;
     d.pad = 0;
     d.nbytes = s->nbytes;
+    d.clip = nullptr;
     return d;
   };
   job.dst = devsurf(dst.s);
@@ -1342,8 +1360,8 @@ int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4
     if (b->prepared && b->view.counters) {
       uint32_t c[16];
       HIP_OK(b->ctx, hipMemcpyAsync(c, b->view.counters, sizeof(c), hipMemcpyDeviceToHost,
-                                    b->ctx->stream));
-      HIP_OK(b->ctx, hipStreamSynchronize(b->ctx->stream));
+                                    bstream(b)));
+      HIP_OK(b->ctx, hipStreamSynchronize(bstream(b)));
       out[0] = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
       out[1] = c[6];
       out[2] = b->view.n_bands ? 0 : 0;
@@ -1355,7 +1373,7 @@ int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4
 
 int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b) {
   if (!b) return Z2D_ERR_INVALID_ARGUMENT;
-  (void)hipStreamSynchronize(b->ctx->stream);
+  (void)hipStreamSynchronize(bstream(b));
   batch_clear(b);
   return Z2D_OK;
 }
@@ -1501,6 +1519,446 @@ int32_t z2d_gpu_batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *surface,
                              uint64_t n_nodes, const z2d_stroke_opts *opts) {
   (void)b; (void)surface; (void)pattern; (void)nodes; (void)n_nodes; (void)opts;
   return Z2D_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: a request loop with continuous batching under an HBM budget.
+// Requests (one surface job each) are copied at submit, admitted FIFO while
+// their surface + clip mask fit the budget, and rendered in batches on two
+// streams: while one batch runs on the device the next is admitted, built
+// and launched on the other, so work from requests that arrive while others
+// are in flight overlaps on the device.  Surfaces live in one pre-allocated
+// HBM region managed by a first-fit allocator.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HbmPool {
+  uint8_t *base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_blocks;  // offset -> size
+  hipError_t init(size_t n) {
+    size = (n + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&base, size);
+    if (e == hipSuccess) free_blocks[0] = size;
+    return e;
+  }
+  uint8_t *alloc(size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+      if (it->second >= n) {
+        size_t off = it->first, rem = it->second - n;
+        free_blocks.erase(it);
+        if (rem) free_blocks[off + n] = rem;
+        return base + off;
+      }
+    }
+    return nullptr;
+  }
+  void release(uint8_t *p, size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    size_t off = (size_t)(p - base);
+    auto it = free_blocks.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_blocks.end() && off + it->second == nx->first) {
+      it->second += nx->second;
+      free_blocks.erase(nx);
+    }
+    if (it != free_blocks.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_blocks.erase(it);
+      }
+    }
+  }
+  void destroy() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+  }
+};
+
+struct SrvFill {
+  z2d_pattern_desc pat;
+  z2d_gradient_desc grad;
+  std::vector<z2d_stop> stops;
+  std::vector<z2d_path_node> nodes;
+  z2d_fill_opts opts;
+};
+
+struct SrvRequest {
+  uint64_t id = 0;
+  uint32_t format = 0;
+  int32_t w = 0, h = 0;
+  z2d_pixel clear{};
+  std::vector<uint8_t> mask;
+  std::vector<std::unique_ptr<SrvFill>> fills;
+  void *out = nullptr;
+  uint64_t out_len = 0;
+  size_t surf_bytes = 0, mask_bytes = 0;
+  z2d_gpu_surface surf;
+  uint8_t *mask_dev = nullptr;
+  int32_t status = Z2D_OK;
+  bool done = false;
+};
+
+struct SrvSlot {
+  z2d_gpu_batch batch;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<SrvRequest *> reqs;
+  HostBuf result_stage;
+  std::vector<size_t> stage_off;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct z2d_gpu_server {
+  z2d_gpu_ctx *ctx = nullptr;
+  HbmPool pool;
+  uint64_t budget = 0;
+  uint32_t max_batch = 64;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<SrvRequest *> queue;
+  std::unordered_map<uint64_t, SrvRequest *> reqs;
+  std::deque<uint64_t> completed;  // completion order, for wait_any
+  uint64_t next_id = 1;
+  uint64_t used = 0, peak = 0;
+  uint64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_batches = 0, pixel_draws = 0;
+  bool stop = false;
+  SrvSlot slots[2];
+  std::thread worker;
+  std::string err;
+
+  void finalize(SrvSlot &sl) {
+    uint64_t cnt[4] = {0, 0, 0, 0};
+    (void)z2d_gpu_batch_counters(&sl.batch, 1, cnt);
+    for (size_t i = 0; i < sl.reqs.size(); i++) {
+      SrvRequest *r = sl.reqs[i];
+      if (cnt[3] && r->status == Z2D_OK) r->status = Z2D_ERR_DEVICE;  // bin overflow
+      if (r->out && r->status == Z2D_OK)
+        std::memcpy(r->out, sl.result_stage.p + sl.stage_off[i],
+                    std::min<uint64_t>(r->out_len, r->surf.nbytes));
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    for (SrvRequest *r : sl.reqs) {
+      pool.release(r->surf.data, r->surf_bytes);
+      if (r->mask_dev) pool.release(r->mask_dev, r->mask_bytes);
+      used -= r->surf_bytes + r->mask_bytes;
+      r->done = true;
+      completed.push_back(r->id);
+      n_completed++;
+    }
+    pixel_draws += cnt[0];
+    sl.reqs.clear();
+    sl.busy = false;
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+
+  // Builds and launches one batch on a free slot (requests already admitted
+  // and their device memory reserved).
+  void launch(SrvSlot &sl, std::vector<SrvRequest *> &take) {
+    hipStream_t st = sl.stream;
+    batch_clear(&sl.batch);
+    sl.batch.ctx = ctx;
+    sl.batch.stream = st;
+    sl.batch.count_stats = 1;
+    size_t stage = 0;
+    sl.stage_off.clear();
+    for (SrvRequest *r : take) {
+      sl.stage_off.push_back(stage);
+      stage += (r->surf.nbytes + 255) & ~(size_t)255;
+    }
+    hipError_t e = sl.result_stage.ensure(stage);
+    for (SrvRequest *r : take) {
+      if (e != hipSuccess) break;
+      // Surface.initPixel: every pixel = T.fromPixel(clear)
+      I4 c = pixel_rgba(r->clear);
+      uint32_t raw = 0;
+      switch (r->format) {
+        case Z2D_FORMAT_RGBA: raw = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24); break;
+        case Z2D_FORMAT_ARGB: raw = c.b | (c.g << 8) | (c.r << 16) | (c.a << 24); break;
+        case Z2D_FORMAT_RGB: raw = c.r | (c.g << 8) | (c.b << 16); break;
+        case Z2D_FORMAT_XRGB: raw = c.b | (c.g << 8) | (c.r << 16); break;
+        default: {
+          int bits = fbits(r->format);
+          uint32_t v = (bits == 8) ? c.a : (c.a & 0xFF) >> (8 - bits);
+          uint32_t byte = 0;
+          if (v)
+            for (int sh = 0; sh <= 8 - bits; sh += bits) byte |= v << sh;
+          raw = byte | (byte << 8) | (byte << 16) | (byte << 24);
+        } break;
+      }
+      e = z2d_launch_fill_u32(reinterpret_cast<uint32_t *>(r->surf.data),
+                              padded_bytes(r->surf.nbytes) / 4, raw, st);
+      if (e == hipSuccess && r->mask_dev)
+        e = hipMemcpyAsync(r->mask_dev, r->mask.data(), r->mask.size(), hipMemcpyHostToDevice, st);
+      r->surf.clip = r->mask_dev;
+      // a clipped surface takes part in the render even with no fills
+      if (r->mask_dev) sl.batch.slot_of(&r->surf);
+      for (auto &f : r->fills) {
+        int32_t rc = batch_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(), f->nodes.size(),
+                                &f->opts);
+        if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
+      }
+    }
+    int32_t rc = e == hipSuccess ? batch_prepare(&sl.batch) : Z2D_ERR_DEVICE;
+    if (rc == Z2D_OK) rc = batch_render(&sl.batch, 0);
+    for (size_t i = 0; i < take.size() && rc == Z2D_OK; i++) {
+      hipError_t ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], take[i]->surf.data,
+                                     take[i]->surf.nbytes, hipMemcpyDeviceToHost, st);
+      if (ce != hipSuccess) rc = Z2D_ERR_DEVICE;
+    }
+    if (rc != Z2D_OK)
+      for (SrvRequest *r : take)
+        if (r->status == Z2D_OK) r->status = rc;
+    (void)hipEventRecord(sl.done, st);
+    sl.reqs = take;
+    sl.busy = true;
+    std::lock_guard<std::mutex> lk(mu);
+    n_batches++;
+  }
+
+  void run() {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+      bool progressed = false;
+      for (SrvSlot &sl : slots) {
+        if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+          finalize(sl);
+          progressed = true;
+        }
+      }
+      SrvSlot *free_slot = nullptr;
+      for (SrvSlot &sl : slots)
+        if (!sl.busy) {
+          free_slot = &sl;
+          break;
+        }
+      std::vector<SrvRequest *> take;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        if (stop && queue.empty() && !slots[0].busy && !slots[1].busy) return;
+        if (free_slot) {
+          while (!queue.empty() && take.size() < max_batch) {
+            SrvRequest *r = queue.front();
+            size_t need = r->surf_bytes + r->mask_bytes;
+            if (used + need > budget) break;
+            uint8_t *sp = pool.alloc(r->surf_bytes);
+            if (!sp) break;
+            uint8_t *mp = nullptr;
+            if (r->mask_bytes) {
+              mp = pool.alloc(r->mask_bytes);
+              if (!mp) {
+                pool.release(sp, r->surf_bytes);
+                break;
+              }
+            }
+            r->surf.data = sp;
+            r->mask_dev = mp;
+            used += need;
+            peak = std::max(peak, used);
+            queue.pop_front();
+            take.push_back(r);
+          }
+        }
+        if (take.empty() && !progressed) {
+          bool any_busy = slots[0].busy || slots[1].busy;
+          if (any_busy)
+            cv_work.wait_for(lk, std::chrono::microseconds(20));
+          else
+            cv_work.wait_for(lk, std::chrono::milliseconds(5),
+                             [&] { return stop || !queue.empty(); });
+        }
+      }
+      if (!take.empty()) launch(*free_slot, take);
+    }
+  }
+};
+
+static void copy_pattern(SrvFill &f, const z2d_pattern_desc *p) {
+  f.pat = *p;
+  const z2d_gradient_desc *g = nullptr;
+  if (p->kind == Z2D_PATTERN_GRADIENT) g = p->gradient;
+  if (p->kind == Z2D_PATTERN_DITHER && p->dither.source == Z2D_DITHER_SRC_GRADIENT)
+    g = p->dither.gradient;
+  if (g) {
+    f.grad = *g;
+    f.stops.assign(g->stops, g->stops + g->n_stops);
+    f.grad.stops = f.stops.data();
+    if (p->kind == Z2D_PATTERN_GRADIENT)
+      f.pat.gradient = &f.grad;
+    else
+      f.pat.dither.gradient = &f.grad;
+  }
+}
+
+extern "C" {
+
+int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch,
+                              z2d_gpu_server **out) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (!out || hbm_budget < 4096) return Z2D_ERR_INVALID_ARGUMENT;
+  auto *s = new (std::nothrow) z2d_gpu_server();
+  if (!s) return Z2D_ERR_OUT_OF_MEMORY;
+  s->ctx = ctx;
+  s->budget = hbm_budget;
+  s->max_batch = max_batch ? max_batch : 64;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  hipError_t e = s->pool.init(hbm_budget + 256 * 1024);
+  for (int i = 0; i < 2 && e == hipSuccess; i++) {
+    e = hipStreamCreate(&s->slots[i].stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->slots[i].done, hipEventDisableTiming);
+    s->slots[i].batch.ctx = ctx;
+  }
+  if (e != hipSuccess) {
+    s->pool.destroy();
+    delete s;
+    return fail(ctx, e, "server create");
+  }
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, void *out,
+                              uint64_t out_len, uint64_t *id) {
+  if (!s || !rq || !id) return Z2D_ERR_INVALID_ARGUMENT;
+  if (rq->width < 1) return Z2D_ERR_INVALID_WIDTH;
+  if (rq->height < 1) return Z2D_ERR_INVALID_HEIGHT;
+  if (rq->format > Z2D_FORMAT_ALPHA1 || (rq->n_fills && !rq->fills))
+    return Z2D_ERR_INVALID_ARGUMENT;
+  auto r = std::make_unique<SrvRequest>();
+  r->format = rq->format;
+  r->w = rq->width;
+  r->h = rq->height;
+  r->clear = rq->clear;
+  r->out = out;
+  r->out_len = out_len;
+  r->surf.ctx = s->ctx;
+  r->surf.format = rq->format;
+  r->surf.w = rq->width;
+  r->surf.h = rq->height;
+  r->surf.nbytes = surface_bytes(rq->format, rq->width, rq->height);
+  r->surf.owned = false;
+  r->surf_bytes = padded_bytes(r->surf.nbytes);
+  if (out && out_len < r->surf.nbytes) return Z2D_ERR_INVALID_ARGUMENT;
+  if (rq->clip_mask) {
+    size_t n = (size_t)rq->width * (size_t)rq->height;
+    r->mask.assign(rq->clip_mask, rq->clip_mask + n);
+    r->mask_bytes = n;
+  }
+  for (uint64_t i = 0; i < rq->n_fills; i++) {
+    const z2d_fill_item &it = rq->fills[i];
+    int32_t rc = check_pattern(it.pattern);
+    if (rc) return rc;
+    auto f = std::make_unique<SrvFill>();
+    copy_pattern(*f, it.pattern);
+    f->nodes.assign(it.nodes, it.nodes + it.n_nodes);
+    f->opts = it.opts;
+    r->fills.push_back(std::move(f));
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (r->surf_bytes + r->mask_bytes > s->budget) {
+    s->n_rejected++;
+    return Z2D_ERR_BUDGET_EXCEEDED;
+  }
+  r->id = s->next_id++;
+  *id = r->id;
+  SrvRequest *raw = r.release();
+  s->reqs[raw->id] = raw;
+  s->queue.push_back(raw);
+  s->n_submitted++;
+  s->cv_work.notify_all();
+  return Z2D_OK;
+}
+
+static int32_t take_done(z2d_gpu_server *s, uint64_t id, int32_t *status) {
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!it->second->done) return 0;
+  if (status) *status = it->second->status;
+  delete it->second;
+  s->reqs.erase(it);
+  auto c = std::find(s->completed.begin(), s->completed.end(), id);
+  if (c != s->completed.end()) s->completed.erase(c);
+  return 1;
+}
+
+int32_t z2d_gpu_server_poll(z2d_gpu_server *s, uint64_t id, int32_t *status) {
+  if (!s) return Z2D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  return take_done(s, id, status);
+}
+
+int32_t z2d_gpu_server_wait(z2d_gpu_server *s, uint64_t id, int32_t *status, double timeout_ms) {
+  if (!s) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto deadline = std::chrono::steady_clock::now() +
+                  std::chrono::microseconds((int64_t)(timeout_ms * 1000.0));
+  for (;;) {
+    int32_t rc = take_done(s, id, status);
+    if (rc != 0) return rc;
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) return take_done(s, id, status);
+  }
+}
+
+int32_t z2d_gpu_server_wait_any(z2d_gpu_server *s, uint64_t *id, int32_t *status,
+                                double timeout_ms) {
+  if (!s || !id) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto deadline = std::chrono::steady_clock::now() +
+                  std::chrono::microseconds((int64_t)(timeout_ms * 1000.0));
+  for (;;) {
+    if (!s->completed.empty()) {
+      uint64_t got = s->completed.front();
+      *id = got;
+      return take_done(s, got, status);
+    }
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && s->completed.empty())
+      return 0;
+  }
+}
+
+int32_t z2d_gpu_server_stats(z2d_gpu_server *s, uint64_t out[8]) {
+  if (!s || !out) return Z2D_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  out[0] = s->n_submitted;
+  out[1] = s->n_completed;
+  out[2] = s->n_rejected;
+  out[3] = s->n_batches;
+  out[4] = s->pixel_draws;
+  out[5] = s->used;
+  out[6] = s->peak;
+  out[7] = s->queue.size();
+  return Z2D_OK;
+}
+
+void z2d_gpu_server_destroy(z2d_gpu_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  for (SrvSlot &sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    sl.batch.arena.release();
+    sl.batch.stage.release();
+    for (int i = 0; i < 3; i++)
+      if (sl.batch.ev[i]) (void)hipEventDestroy(sl.batch.ev[i]);
+    sl.result_stage.release();
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto &kv : s->reqs) delete kv.second;
+  s->pool.destroy();
+  delete s;
 }
 
 }  // extern "C"

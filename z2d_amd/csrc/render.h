@@ -33,6 +33,7 @@ struct DevSurface {
   int32_t w, h;
   uint32_t fmt, pad;
   uint64_t nbytes;
+  const uint8_t *clip;  // optional alpha8 clip mask (w*h): dst_in after all draws
 };
 
 enum DrawFlags : uint32_t {

@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     const uint32_t band = J.surf_band_off[T.surface] + T.ty;
     const uint32_t e0 = min(J.band_cnt[band], J.max_entries);
     const uint32_t e1 = min(J.band_cnt[band + 1], J.max_entries);
-    if (e0 == e1) continue;
+    if (e0 == e1 && !S.clip) continue;
     const int32_t px = tx0 + (int32_t)lane;
     const bool col_ok = px < W;
 
@@ -683,6 +683,16 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
       __syncthreads();
     }
 
+    if (S.clip) {
+      // Clip stage: Surface.composite(dst, mask, .dst_in) after the draws.
+#pragma unroll
+      for (int i = 0; i < ROWS_PER_THREAD; i++) {
+        const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+        if (!col_ok || py >= H) continue;
+        const I4 m{0, 0, 0, S.clip[(int64_t)py * W + px]};
+        pix[i] = pack(op_int(Z2D_OP_DST_IN, unpack(pix[i], S.fmt), m), S.fmt);
+      }
+    }
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
       const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;

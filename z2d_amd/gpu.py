@@ -50,6 +50,15 @@ _SIGS = {
     "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_destroy": ([C.c_void_p], None),
     "z2d_gpu_stream_copy": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
+    "z2d_gpu_server_create": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)], C.c_int32),
+    "z2d_gpu_server_submit": ([C.c_void_p, P(abi.RequestDesc), C.c_void_p, C.c_uint64,
+                               P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_server_poll": ([C.c_void_p, C.c_uint64, P(C.c_int32)], C.c_int32),
+    "z2d_gpu_server_wait": ([C.c_void_p, C.c_uint64, P(C.c_int32), C.c_double], C.c_int32),
+    "z2d_gpu_server_wait_any": ([C.c_void_p, P(C.c_uint64), P(C.c_int32), C.c_double],
+                                C.c_int32),
+    "z2d_gpu_server_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_server_destroy": ([C.c_void_p], None),
 }
 
 
@@ -225,3 +234,81 @@ class GpuBackend:
         od = out.desc() if out is not None else None
         return lib().z2d_gpu_composite(self.ctx.h, C.byref(d), dst_x, dst_y, arr, len(ops),
                                        precision, C.byref(od) if od is not None else None)
+
+
+class Server:
+    """Request loop with continuous batching under an HBM budget
+    (z2d_gpu_server_*).  Each request is one surface: cleared to `clear`,
+    the fills applied in order, then the optional alpha8 clip mask."""
+
+    STATS = ("submitted", "completed", "rejected", "batches", "pixel_draws", "bytes_in_use",
+             "peak_bytes", "queued")
+
+    def __init__(self, ctx, hbm_budget, max_batch=64):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().z2d_gpu_server_create(ctx.h, hbm_budget, max_batch, C.byref(h)),
+                  "server_create")
+        self.h = h
+        self._pending = {}
+
+    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None):
+        """fills: list of (pattern, nodes, opts).  Returns (rc, id); `out`
+        (numpy uint8, >= surface bytes) receives the result."""
+        items = (abi.FillItem * max(1, len(fills)))()
+        for i, (pat, nodes, opts) in enumerate(fills):
+            items[i].pattern = C.pointer(pat)
+            items[i].nodes = nodes.ctypes.data
+            items[i].n_nodes = len(nodes)
+            items[i].opts = opts
+        rq = abi.RequestDesc()
+        rq.format, rq.width, rq.height = fmt, width, height
+        rq.clear = clear if clear is not None else abi.Pixel()
+        rq.clip_mask = clip.ctypes.data if clip is not None else None
+        rq.fills = items
+        rq.n_fills = len(fills)
+        rid = C.c_uint64()
+        rc = lib().z2d_gpu_server_submit(self.h, C.byref(rq),
+                                         out.ctypes.data if out is not None else None,
+                                         out.nbytes if out is not None else 0, C.byref(rid))
+        if rc == 0:
+            self._pending[rid.value] = out  # keeps the result buffer alive
+        return rc, rid.value
+
+    def wait(self, rid, timeout_ms=60000.0):
+        st = C.c_int32()
+        rc = lib().z2d_gpu_server_wait(self.h, rid, C.byref(st), timeout_ms)
+        if rc == 1:
+            self._pending.pop(rid, None)
+            return st.value
+        if rc == 0:
+            raise TimeoutError(f"request {rid} not done after {timeout_ms} ms")
+        raise GpuError(f"server_wait returned {rc}")
+
+    def wait_any(self, timeout_ms=60000.0):
+        """(id, status) of the next completed request, or None on timeout."""
+        rid, st = C.c_uint64(), C.c_int32()
+        rc = lib().z2d_gpu_server_wait_any(self.h, C.byref(rid), C.byref(st), timeout_ms)
+        if rc == 1:
+            self._pending.pop(rid.value, None)
+            return rid.value, st.value
+        if rc == 0:
+            return None
+        raise GpuError(f"server_wait_any returned {rc}")
+
+    def stats(self):
+        out = (C.c_uint64 * 8)()
+        lib().z2d_gpu_server_stats(self.h, out)
+        return dict(zip(self.STATS, out))
+
+    def close(self):
+        if self.h:
+            lib().z2d_gpu_server_destroy(self.h)
+            self.h = None
+            self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
