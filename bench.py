@@ -66,7 +66,7 @@ def count_work(b):
     b.counters(True)
     b.render(0)
     b.wait()
-    pix, edges, _, overflow = b.counters(False)
+    pix, edges, _, overflow = b.counters(False)[:4]
     if overflow:
         raise RuntimeError("bin overflow while counting")
     return pix, edges

@@ -116,9 +116,11 @@ int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *rend
 int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]);
 /* Work counters of the last render (enable before rendering): out[0] =
  * pixel-draws (pixels with nonzero coverage summed over draws), out[1] =
- * edges built, out[3] = nonzero if a bin overflowed.  No reference
+ * edges built, out[2] = tile-draw pairs rasterized, out[3] = nonzero if a
+ * bin overflowed, out[4] = tile-draw pairs on the unpacked (very many edges)
+ * path, out[5] = capacity of the per-band edge lists.  No reference
  * counterpart: the benchmark's unit of work. */
-int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4]);
+int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[8]);
 int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b);
 void z2d_gpu_batch_destroy(z2d_gpu_batch *b);
 

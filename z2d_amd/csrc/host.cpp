@@ -615,7 +615,37 @@ struct HostDraw {
   uint32_t surface_slot;  // index into batch surfaces
   uint32_t seq;           // insertion order
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
+  uint64_t edge_pairs;        // upper bound of (edge, band) pairs
 };
+
+// Bands [floor(lo / TILE_H), floor(hi / TILE_H)] that an edge between pixel
+// rows a and b can meet, capped at the surface's band count.
+static uint64_t band_span(double a, double b, uint32_t nbands) {
+  const double lo = std::min(a, b), hi = std::max(a, b);
+  if (!std::isfinite(lo) || !std::isfinite(hi)) return nbands;
+  const double s = std::floor(hi / TILE_H) - std::floor(lo / TILE_H) + 1.0;
+  return (uint64_t)std::min(s, (double)nbands);
+}
+
+// Upper bound of the (edge, band) pairs of a flattened draw, following the
+// fill plotter's edge sources: a line joins the previous point to its own,
+// a close joins the last point to the subpath's first.
+static uint64_t node_edge_pairs(const FNode *n, size_t cnt, uint32_t nbands) {
+  uint64_t total = 0;
+  double cur = 0.0, init = 0.0;
+  for (size_t i = 0; i < cnt; i++) {
+    if (n[i].tag == Z2D_NODE_MOVE_TO) {
+      cur = init = n[i].y;
+    } else if (n[i].tag == Z2D_NODE_LINE_TO) {
+      total += band_span(cur, n[i].y, nbands);
+      cur = n[i].y;
+    } else if (n[i].tag == Z2D_NODE_CLOSE_PATH) {
+      total += band_span(cur, init, nbands);
+      cur = init;
+    }
+  }
+  return total;
+}
 
 struct z2d_gpu_batch {
   z2d_gpu_ctx *ctx = nullptr;
@@ -712,6 +742,8 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
   }
   hd.d.node_off = (uint32_t)before;
   hd.d.node_cnt = (uint32_t)(b->nodes.size() - before);
+  hd.edge_pairs = node_edge_pairs(b->nodes.data() + before, hd.d.node_cnt,
+                                  (uint32_t)((s->h + TILE_H - 1) / TILE_H));
   if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
   return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
                           o->precision, hd, ymin, ymax);
@@ -741,6 +773,9 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   // extents are in scaled space
   const double sc = (aa == Z2D_AA_NONE || s->format == Z2D_FORMAT_ALPHA1) ? 1.0 : 4.0;
   double ymin = n_edges ? ext[0] / sc : NAN, ymax = n_edges ? ext[1] / sc : NAN;
+  const uint32_t nbands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
+  for (uint64_t i = 0; i < n_edges; i++)
+    hd.edge_pairs += band_span(edges[i].y0 / sc, edges[i].y1 / sc, nbands);
   return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, ymin, ymax);
 }
 
@@ -767,12 +802,13 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const uint32_t n_bands = surf_band_off[ns];
   const uint32_t n_tiles = b->surf_tile_off[ns];
   const uint32_t nd = (uint32_t)b->draws.size();
-  uint64_t max_entries = 0, max_rows = 0, scratch = 0;
+  uint64_t max_entries = 0, max_rows = 0, scratch = 0, max_entry_edges = 0;
   std::vector<uint32_t> rowprog_list, rowprog_soff;
   for (uint32_t i = 0; i < nd; i++) {
     HostDraw &h = b->draws[i];
     h.d.surface = h.surface_slot;
     max_entries += h.band_hi - h.band_lo + 1;
+    max_entry_edges += h.edge_pairs;
     if (h.d.flags & DRAW_ROWPROG) {
       const z2d_gpu_surface *s = b->surfaces[h.surface_slot];
       h.d.row_off = (uint32_t)max_rows;
@@ -782,7 +818,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
       scratch += h.d.node_cnt + 1;
     }
   }
-  if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull)
+  if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull ||
+      max_entry_edges > 0xFFFFFFF0ull)
     return Z2D_ERR_OUT_OF_MEMORY;
   b->n_rowprog = (uint32_t)rowprog_list.size();
 
@@ -805,6 +842,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_edges = L.take(b->nodes.size() * sizeof(z2d_edge));
   const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
+  const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
+  const size_t o_eidx = L.take(max_entry_edges * sizeof(uint32_t));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
   const size_t o_rows = L.take(max_rows * sizeof(RowRec));
@@ -869,8 +908,9 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.band_cnt = reinterpret_cast<uint32_t *>(D + o_bandcnt);
   J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
   J.max_entries = (uint32_t)max_entries;
-  J.entry_edge_idx = nullptr;
-  J.max_entry_edges = 0;
+  J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);
+  J.entry_edge_idx = reinterpret_cast<uint32_t *>(D + o_eidx);
+  J.max_entry_edges = (uint32_t)max_entry_edges;
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
   J.count_stats = b->count_stats;
   J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);
@@ -1351,12 +1391,12 @@ int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]) {
   return Z2D_OK;
 }
 
-int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4]) {
+int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[8]) {
   if (!b) return Z2D_ERR_INVALID_ARGUMENT;
   b->count_stats = enable ? 1u : 0u;
   b->view.count_stats = b->count_stats;
   if (out) {
-    out[0] = out[1] = out[2] = out[3] = 0;
+    for (int i = 0; i < 8; i++) out[i] = 0;
     if (b->prepared && b->view.counters) {
       uint32_t c[16];
       HIP_OK(b->ctx, hipMemcpyAsync(c, b->view.counters, sizeof(c), hipMemcpyDeviceToHost,
@@ -1364,8 +1404,10 @@ int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[4
       HIP_OK(b->ctx, hipStreamSynchronize(bstream(b)));
       out[0] = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
       out[1] = c[6];
-      out[2] = b->view.n_bands ? 0 : 0;
+      out[2] = c[7];
       out[3] = c[2];
+      out[4] = c[8];
+      out[5] = b->view.max_entry_edges;
     }
   }
   return Z2D_OK;
@@ -1634,7 +1676,7 @@ struct z2d_gpu_server {
   std::string err;
 
   void finalize(SrvSlot &sl) {
-    uint64_t cnt[4] = {0, 0, 0, 0};
+    uint64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     (void)z2d_gpu_batch_counters(&sl.batch, 1, cnt);
     for (size_t i = 0; i < sl.reqs.size(); i++) {
       SrvRequest *r = sl.reqs[i];

@@ -18,8 +18,11 @@ constexpr int TILE_H = 16;
 constexpr int RENDER_THREADS = 256;
 constexpr int ROWS_PER_THREAD = TILE_H / 4;
 constexpr int MAX_SUBROWS = TILE_H * 4;
-constexpr int LIST_CAP = 24;     // crossings per sub-row held in LDS
-constexpr int EDGE_CACHE = 256;  // band edges staged in LDS per pass
+constexpr int ACC_WORDS = TILE_W * 2;  // 4 sample columns per pixel, two int16 per word
+// Winding deltas are accumulated as int16 pairs in LDS; an entry whose band
+// holds more edges than this cannot overflow them and takes the exact
+// per-thread path instead.
+constexpr uint32_t PACKED_EDGE_LIMIT = 32767;
 
 // Flattened path node (move_to / line_to / close_path only; curves are
 // flattened on the host by the reference's Spline algorithm first).
@@ -80,8 +83,8 @@ struct RowRec {
 struct BandEntry {
   uint32_t draw;
   uint32_t edge_off;  // into the entry-edge index array
-  uint32_t edge_cnt;
-  uint32_t pad;
+  uint32_t edge_cnt;  // edges of the draw whose y-range meets the band
+  uint32_t band;
 };
 
 struct TileRef {
@@ -107,9 +110,12 @@ struct JobView {
   uint32_t *band_cnt;    // n_bands + 1 (scanned in place)
   BandEntry *entries;    // capacity max_entries
   uint32_t max_entries;
-  uint32_t *entry_edge_idx;
+  uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place
+  uint32_t *entry_edge_idx;  // capacity max_entry_edges
   uint32_t max_entry_edges;
-  uint32_t *counters;    // [2] entry overflow flag; [4..5] pixel-draws (u64), [6] edges
+  // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;
+  // [7] tile-draw entries rasterized; [8] entries on the unpacked path
+  uint32_t *counters;
   uint32_t count_stats;  // nonzero: accumulate pixel-draw / edge counts
   uint32_t *cursors;     // tile work cursors, one per render launch
   RowRec *rows;

@@ -8,13 +8,15 @@
 //   k_rowprog     serial row emulation, only for draws whose coverage depends
 //                 on the reference's sort order (unbalanced contours) or whose
 //                 direct-mode unbounded clears depend on whole rows
-//   k_band_count / k_scan / k_band_fill
+//   k_band_pass / k_scan
 //                 ordered per-band draw lists (TILE_H-row bins, draw order kept)
+//   k_entry_edges / k_scan
+//                 per (draw, band) entry, the draw's edges that meet the band
 //   k_render      one workgroup per TILE_W x TILE_H tile: the tile stays in
 //                 registers while every draw touching its band is rasterized
-//                 (exact f64 intercepts, LDS crossing lists, per-pixel winding)
-//                 and composited in draw order; HBM sees one read and one
-//                 write per tile pixel
+//                 (exact f64 intercepts accumulated as winding deltas in LDS,
+//                 wave prefix scans for per-sample winding) and composited in
+//                 draw order; HBM sees one read and one write per tile pixel
 //   k_composite   SurfaceCompositor.run batches (src/compositor.zig:302-440)
 #include <hip/hip_runtime.h>
 
@@ -376,7 +378,7 @@ __global__ void k_band_pass(JobView J) {
         e.draw = d + lane;
         e.edge_off = 0;
         e.edge_cnt = 0;
-        e.pad = 0;
+        e.band = b;
         J.entries[base + pos] = e;
       } else {
         atomicOr(&J.counters[2], 1u);
@@ -427,16 +429,95 @@ __global__ void k_scan(uint32_t *a, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// k_entry_edges: per (draw, band) entry, the indices of the draw's edges whose
+// y-range meets the band's sample rows (top < last mid && bottom >= first mid,
+// the active-edge test of src/internal/raster/multisample.zig and direct.zig
+// taken over the band).  Tiles then walk only those edges.  One wave per
+// entry; FILL=false counts, FILL=true writes after the scan.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void k_entry_edges(JobView J) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (e >= J.max_entries) return;
+  const uint32_t total = min(J.band_cnt[J.n_bands], J.max_entries);
+  if (e >= total) {
+    if (!FILL && lane == 0) J.entry_ecnt[e] = 0;
+    return;
+  }
+  BandEntry &E = J.entries[e];
+  const DevDraw &D = J.draws[E.draw];
+  const bool direct_unb = D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED);
+  const int SC = D.mode == MODE_DIRECT ? 1 : 4;
+  const int32_t y0 = (int32_t)(E.band - J.surf_band_off[D.surface]) * TILE_H;
+  const double ys_lo = (double)(y0 * SC) + 0.5;
+  const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+  const z2d_edge *edges = J.edges + D.node_off;
+  const uint32_t ne = direct_unb ? 0u : D.edge_cnt;
+  const uint32_t base = FILL ? J.entry_ecnt[e] : 0u;
+  uint32_t count = 0;
+  for (uint32_t k0 = 0; k0 < ne; k0 += 64) {
+    bool hit = false;
+    if (k0 + lane < ne) {
+      const z2d_edge ed = edges[k0 + lane];
+      const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
+      const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+      hit = top < ys_hi && bot >= ys_lo;
+    }
+    const uint64_t m = __ballot(hit);
+    if (FILL && hit) {
+      const uint32_t pos = base + count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (pos < J.max_entry_edges)
+        J.entry_edge_idx[pos] = k0 + lane;
+      else
+        atomicOr(&J.counters[2], 1u);
+    }
+    count += (uint32_t)__popcll(m);
+  }
+  if (lane == 0) {
+    if (!FILL) {
+      J.entry_ecnt[e] = count;
+    } else {
+      // only indices below the capacity were stored
+      E.edge_off = base;
+      E.edge_cnt = base >= J.max_entry_edges ? 0u : min(count, J.max_entry_edges - base);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_render: tile renderer.
 // ---------------------------------------------------------------------------
 struct TileShared {
-  z2d_edge edges[EDGE_CACHE];
-  int32_t list[MAX_SUBROWS][LIST_CAP];  // (xr << 8) | (d + 128)
-  int32_t left[MAX_SUBROWS];
-  int32_t cnt[MAX_SUBROWS];
-  uint32_t n_cached;
-  uint32_t overflow;
+  // Winding deltas per sample row and sample column of the tile, two int16
+  // columns per word (column 2i in the low half).  Zero between entries:
+  // the coverage phase clears what it reads.
+  uint32_t acc[MAX_SUBROWS][ACC_WORDS];
+  z2d_edge cedge[RENDER_THREADS];    // staged band edges
+  int32_t clo[RENDER_THREADS];       // first active sample row in the tile
+  uint32_t coff[RENDER_THREADS];     // exclusive offsets of active-row counts
+  uint32_t wsum[RENDER_THREADS / 64];
+  uint32_t s_tile;
 };
+
+// Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
+// 16-lane rows, then row broadcasts across rows).
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return x;
+}
+
+// Splits a word of two int16 deltas accumulated with 32-bit adds (the low
+// half's borrows land in the high half and are undone here).
+__device__ __forceinline__ void unpack_pair(uint32_t w, int32_t &lo, int32_t &hi) {
+  lo = (int32_t)(int16_t)(w & 0xFFFFu);
+  hi = (int32_t)(w - (uint32_t)lo) >> 16;
+}
 
 __device__ __forceinline__ uint32_t ssaa_mask(int k, int bits) {
   switch (bits) {
@@ -491,18 +572,45 @@ __device__ __forceinline__ bool msaa_preclear(const DevDraw &D, int W, int x, in
   return false;
 }
 
+// Active sample rows of an edge inside the tile: the rows sr in [0, SR) with
+// top < ys_base + sr + 0.5 <= bottom (the reference's active-edge test).  The
+// floor estimate is corrected with that exact comparison.
+__device__ __forceinline__ int32_t active_rows(double top, double bot, int32_t ys_base, int SR,
+                                               int32_t &lo_out) {
+  const double flo = floor(top - 0.5 - (double)ys_base) + 1.0;
+  const double fhi = floor(bot - 0.5 - (double)ys_base);
+  if (!(flo <= fhi + 1.0 && flo <= (double)SR && fhi >= -1.0)) return 0;  // NaN too
+  int32_t lo = (int32_t)fmin(fmax(flo, 0.0), (double)SR);
+  int32_t hi = (int32_t)fmin(fmax(fhi, -1.0), (double)(SR - 1));
+  if (lo > 0 && top < (double)(ys_base + lo - 1) + 0.5)
+    lo--;
+  else if (lo < SR && !(top < (double)(ys_base + lo) + 0.5))
+    lo++;
+  if (hi < SR - 1 && (double)(ys_base + hi + 1) + 0.5 <= bot)
+    hi++;
+  else if (hi >= 0 && !((double)(ys_base + hi) + 0.5 <= bot))
+    hi--;
+  lo_out = lo;
+  return hi >= lo ? hi - lo + 1 : 0;
+}
+
+__device__ __forceinline__ bool wind_in(int32_t w, bool eo) { return eo ? (w & 1) != 0 : w != 0; }
+
 __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t tile_base,
                                                            uint32_t tile_count,
                                                            uint32_t *cursor) {
   __shared__ TileShared sh;
-  __shared__ uint32_t s_tile;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // The accumulator starts at zero and every entry leaves it at zero.
+  for (uint32_t i = tid; i < (uint32_t)(MAX_SUBROWS * ACC_WORDS); i += RENDER_THREADS)
+    (&sh.acc[0][0])[i] = 0u;
 
   for (;;) {
     // persistent work loop over the tile queue
-    if (tid == 0) s_tile = atomicAdd(cursor, 1u);
+    if (tid == 0) sh.s_tile = atomicAdd(cursor, 1u);
     __syncthreads();
-    const uint32_t ti = s_tile;
+    const uint32_t ti = sh.s_tile;
     __syncthreads();
     if (ti >= tile_count) break;
     const TileRef T = J.tiles[tile_base + ti];
@@ -527,14 +635,15 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     for (int i = 0; i < ROWS_PER_THREAD; i++) orig[i] = pix[i];
 
     for (uint32_t e = e0; e < e1; e++) {
-      const uint32_t di = J.entries[e].draw;
-      const DevDraw &D = J.draws[di];
+      const BandEntry E = J.entries[e];
+      const DevDraw &D = J.draws[E.draw];
       if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;  // uniform
       const DevPaint &P = J.paints[D.paint];
       const int SC = D.mode == MODE_DIRECT ? 1 : 4;
-      const int SR = TILE_H * SC;  // sub-rows in this tile
+      const int SR = TILE_H * SC;  // sample rows in this tile
       const int32_t ys_base = ty0 * SC;
       const int32_t xs_base = tx0 * SC;
+      const int32_t cols = TILE_W * SC;
       const bool direct_unb = D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED);
 
       if (direct_unb) {
@@ -556,97 +665,160 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
         }
         continue;
       }
+      if (J.count_stats && tid == 0) atomicAdd(&J.counters[7], 1u);
 
-      // ---- phase A: crossings of this draw's edges in the tile's sub-rows
-      if (tid < MAX_SUBROWS) {
-        sh.left[tid] = 0;
-        sh.cnt[tid] = 0;
-      }
-      if (tid == 0) sh.overflow = 0;
-      __syncthreads();
-      const double ys_lo = (double)ys_base + 0.5;
-      const double ys_hi = (double)(ys_base + SR - 1) + 0.5;
       const z2d_edge *edges = J.edges + D.node_off;
-      const uint32_t ne = D.edge_cnt;
+      const uint32_t *eidx = J.entry_edge_idx + E.edge_off;
+      const uint32_t ne = E.edge_off >= J.max_entry_edges
+                              ? 0u
+                              : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
       const bool eo = D.rule == Z2D_FILL_EVEN_ODD;
-      for (uint32_t c0 = 0; c0 < ne; c0 += RENDER_THREADS) {
-        if (tid == 0) sh.n_cached = 0;
-        __syncthreads();
-        const uint32_t k = c0 + tid;
-        if (k < ne) {
-          const z2d_edge ed = edges[k];
-          const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-          const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-          if (top < ys_hi && bot >= ys_lo) {
-            uint32_t slot = atomicAdd(&sh.n_cached, 1u);
-            sh.edges[slot] = ed;
+      const bool packed = ne <= PACKED_EDGE_LIMIT;  // uniform
+
+      // ---- phase A: winding deltas of this draw's band edges.  Each
+      // crossing adds its direction at its sample column (crossings left of
+      // the tile at column 0, right of it nowhere); work items are the
+      // (edge, active sample row) pairs, compacted by a block scan.
+      if (packed) {
+        for (uint32_t c0 = 0; c0 < ne; c0 += RENDER_THREADS) {
+          const uint32_t k = c0 + tid;
+          int32_t n_sr = 0, lo = 0;
+          z2d_edge ed{0.0, 0.0, 0.0, 0.0};
+          if (k < ne) {
+            ed = edges[eidx[k]];
+            const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
+            const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+            n_sr = active_rows(top, bot, ys_base, SR, lo);
+            if (n_sr) {
+              // every crossing right of the tile: no effect here
+              const double xb = ed.x_start + ed.x_inc * (bot - top);
+              if (fmin(ed.x_start, xb) > (double)(xs_base + cols) + 2.0) n_sr = 0;
+            }
           }
-        }
-        __syncthreads();
-        const uint32_t nc = sh.n_cached;
-        const uint32_t items = nc * (uint32_t)SR;
-        for (uint32_t it = tid; it < items; it += RENDER_THREADS) {
-          const uint32_t ei = it / (uint32_t)SR;
-          const int sr = (int)(it - ei * (uint32_t)SR);
-          const z2d_edge ed = sh.edges[ei];
-          const double mid = (double)(ys_base + sr) + 0.5;
-          const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-          const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-          if (!(top < mid && bot >= mid)) continue;
-          const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - top)));
-          const int32_t dv = eo ? 1 : (ed.y0 < ed.y1 ? -1 : 1);
-          const int32_t xr = x - xs_base;
-          if (xr < 0) {
-            atomicAdd(&sh.left[sr], dv);
-          } else if (xr < TILE_W * SC) {
-            int32_t slot = atomicAdd(&sh.cnt[sr], 1);
-            if (slot < LIST_CAP)
-              sh.list[sr][slot] = (xr << 8) | (dv + 128);
-            else
-              sh.overflow = 1;
+          sh.cedge[tid] = ed;
+          sh.clo[tid] = lo;
+          const int32_t incl = wave_incl_scan(n_sr);
+          if (lane == 63) sh.wsum[wave] = (uint32_t)incl;
+          __syncthreads();
+          uint32_t woff = 0, total = 0;
+#pragma unroll
+          for (uint32_t w = 0; w < RENDER_THREADS / 64; w++) {
+            const uint32_t v = sh.wsum[w];
+            woff += w < wave ? v : 0u;
+            total += v;
           }
+          sh.coff[tid] = woff + (uint32_t)(incl - n_sr);
+          __syncthreads();
+          for (uint32_t it = tid; it < total; it += RENDER_THREADS) {
+            // owning edge: the last j with coff[j] <= it
+            uint32_t lj = 0, hj = RENDER_THREADS;
+            while (hj - lj > 1) {
+              const uint32_t mj = (lj + hj) >> 1;
+              if (sh.coff[mj] <= it) lj = mj; else hj = mj;
+            }
+            const z2d_edge ce = sh.cedge[lj];
+            const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
+            const double top = ce.y0 < ce.y1 ? ce.y0 : ce.y1;
+            const double mid = (double)(ys_base + sr) + 0.5;
+            const int32_t x = (int32_t)round(ce.x_start + (ce.x_inc * (mid - top)));
+            const int32_t xr = x - xs_base;
+            if (xr >= cols) continue;
+            const int32_t col = xr < 0 ? 0 : xr;
+            const int32_t dv = eo ? 1 : (ce.y0 < ce.y1 ? -1 : 1);
+            atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? ((uint32_t)dv << 16) : (uint32_t)dv);
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
 
-      // ---- phase B: per-pixel coverage and composite
-      const bool overflow = sh.overflow != 0;
+      // ---- phase B: coverage per pixel (samples covered of SC x SC), then
+      // the composite in draw order.
       const RowRec *R = (D.flags & DRAW_ROWPROG) ? J.rows + D.row_off : nullptr;
-      uint32_t n_cov = 0;
+      const int c_lo = (int)lane * SC;
+      int kk[ROWS_PER_THREAD];
+      if (packed && SC == 4) {
 #pragma unroll
-      for (int i = 0; i < ROWS_PER_THREAD; i++) {
-        const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
-        const int32_t py = ty0 + ly;
-        if (!col_ok || py >= H) continue;
-        const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
-        if (!in_region) continue;
-        if (D.mode == MODE_MSAA && (D.flags & DRAW_UNBOUNDED) && msaa_preclear(D, W, px, py))
-          pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
-        int k = 0;
-        for (int o = 0; o < SC; o++) {
-          const int sr = ly * SC + o;
-          int32_t w[4];
-          const int32_t lw = sh.left[sr];
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
+          int k = 0;
 #pragma unroll
-          for (int j = 0; j < 4; j++) w[j] = lw;
-          const int c_lo = (int)lane * SC;
-          if (!overflow) {
-            const int n = min(sh.cnt[sr], LIST_CAP);
-            for (int q = 0; q < n; q++) {
-              const int32_t v = sh.list[sr][q];
-              const int32_t xr = v >> 8, dv = (v & 0xFF) - 128;
+          for (int pr = 0; pr < 2; pr++) {
+            int32_t d[2][4], tot[2];
 #pragma unroll
-              for (int j = 0; j < 4; j++)
-                if (j < SC && xr <= c_lo + j) w[j] += dv;
+            for (int q = 0; q < 2; q++) {
+              uint2 *slot = reinterpret_cast<uint2 *>(&sh.acc[ly * 4 + pr * 2 + q][lane * 2]);
+              const uint2 v = *slot;
+              *slot = make_uint2(0u, 0u);
+              unpack_pair(v.x, d[q][0], d[q][1]);
+              unpack_pair(v.y, d[q][2], d[q][3]);
+              tot[q] = d[q][0] + d[q][1] + d[q][2] + d[q][3];
             }
-          } else {
-            // slow path: this thread recomputes the sub-row's crossings
-            const int32_t ys = ys_base + sr;
-            const double mid = (double)ys + 0.5;
+            // both sample rows in one scan: int16 halves of one word
+            const uint32_t pk = (uint32_t)tot[0] + ((uint32_t)tot[1] << 16);
+            const uint32_t exc = (uint32_t)wave_incl_scan((int32_t)pk) - pk;
+            int32_t base[2];
+            unpack_pair(exc, base[0], base[1]);
 #pragma unroll
-            for (int j = 0; j < 4; j++) w[j] = 0;
+            for (int q = 0; q < 2; q++) {
+              const int32_t ys = ys_base + ly * 4 + pr * 2 + q;
+              int32_t cutoff = INT32_MAX;
+              if (R) {
+                const int32_t rr = ys - D.row_y0;
+                if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+              }
+              int32_t w = base[q];
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                w += d[q][j];
+                if (wind_in(w, eo) && xs_base + c_lo + j < cutoff) k++;
+              }
+            }
+          }
+          kk[i] = k;
+        }
+      } else if (packed) {
+        // one sample per pixel: rows paired into one scan
+#pragma unroll
+        for (int pr = 0; pr < ROWS_PER_THREAD / 2; pr++) {
+          int32_t d[2];
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            uint32_t *slot = &sh.acc[wave * ROWS_PER_THREAD + pr * 2 + q][lane >> 1];
+            const uint32_t v = *slot;
+            if ((lane & 1) == 0) *slot = 0u;
+            int32_t lo, hi;
+            unpack_pair(v, lo, hi);
+            d[q] = (lane & 1) ? hi : lo;
+          }
+          const uint32_t pk = (uint32_t)d[0] + ((uint32_t)d[1] << 16);
+          const uint32_t inc = (uint32_t)wave_incl_scan((int32_t)pk);
+          int32_t w[2];
+          unpack_pair(inc, w[0], w[1]);
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            const int32_t ys = ys_base + wave * ROWS_PER_THREAD + pr * 2 + q;
+            int32_t cutoff = INT32_MAX;
+            if (R) {
+              const int32_t rr = ys - D.row_y0;
+              if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+            }
+            kk[pr * 2 + q] = (wind_in(w[q], eo) && xs_base + c_lo < cutoff) ? 1 : 0;
+          }
+        }
+      } else {
+        // more band edges than the packed deltas can hold: each thread
+        // evaluates the crossings of its own sample rows
+        if (J.count_stats && tid == 0) atomicAdd(&J.counters[8], 1u);
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
+          int k = 0;
+          for (int o = 0; o < SC; o++) {
+            const int32_t ys = ys_base + ly * SC + o;
+            const double mid = (double)ys + 0.5;
+            int32_t w[4] = {0, 0, 0, 0};
             for (uint32_t q = 0; q < ne; q++) {
-              const z2d_edge ed = edges[q];
+              const z2d_edge ed = edges[eidx[q]];
               const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
               const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
               if (!(top < mid && bot >= mid)) continue;
@@ -657,21 +829,30 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
               for (int j = 0; j < 4; j++)
                 if (j < SC && xr <= c_lo + j) w[j] += dv;
             }
-          }
-          int32_t cutoff = INT32_MAX;
-          if (R) {
-            const int32_t rr = ys_base + sr - D.row_y0;
-            if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
-          }
+            int32_t cutoff = INT32_MAX;
+            if (R) {
+              const int32_t rr = ys - D.row_y0;
+              if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+            }
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            if (j >= SC) break;
-            const bool in = eo ? (w[j] & 1) != 0 : w[j] != 0;
-            if (in && xs_base + c_lo + j < cutoff) k++;
+            for (int j = 0; j < 4; j++)
+              if (j < SC && wind_in(w[j], eo) && xs_base + c_lo + j < cutoff) k++;
           }
+          kk[i] = k;
         }
-        n_cov += k > 0;
-        pix[i] = composite_cov(pix[i], S.fmt, k, SC * SC, D, P, J.blue_noise, px, py);
+      }
+
+      uint32_t n_cov = 0;
+#pragma unroll
+      for (int i = 0; i < ROWS_PER_THREAD; i++) {
+        const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+        if (!col_ok || py >= H) continue;
+        const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
+        if (!in_region) continue;
+        if (D.mode == MODE_MSAA && (D.flags & DRAW_UNBOUNDED) && msaa_preclear(D, W, px, py))
+          pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
+        n_cov += kk[i] > 0;
+        pix[i] = composite_cov(pix[i], S.fmt, kk[i], SC * SC, D, P, J.blue_noise, px, py);
       }
       if (J.count_stats) {
 #pragma unroll
@@ -862,6 +1043,12 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   hipLaunchKernelGGL(k_band_pass<false>, grid, dim3(256), 0, st, J);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.band_cnt, J.n_bands);
   hipLaunchKernelGGL(k_band_pass<true>, grid, dim3(256), 0, st, J);
+  if (J.max_entries) {
+    dim3 egrid((J.max_entries + 3) / 4);
+    hipLaunchKernelGGL(k_entry_edges<false>, egrid, dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.entry_ecnt, J.max_entries);
+    hipLaunchKernelGGL(k_entry_edges<true>, egrid, dim3(256), 0, st, J);
+  }
   return hipGetLastError();
 }
 

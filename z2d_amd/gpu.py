@@ -190,7 +190,9 @@ class Batch:
         return tuple(out)
 
     def counters(self, enable=True):
-        out = (C.c_uint64 * 4)()
+        """(pixel_draws, edges, tile_entries, bin_overflow, unpacked_entries,
+        edge_list_capacity, 0, 0) of the last render."""
+        out = (C.c_uint64 * 8)()
         self.ctx.check(lib().z2d_gpu_batch_counters(self.h, 1 if enable else 0, out), "counters")
         return tuple(out)
 
