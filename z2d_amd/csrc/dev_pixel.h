@@ -486,6 +486,17 @@ struct DevGradient {
   double r1, r2, cdx, cdy, dr, min_dr, a, inv_a;
   double conic_angle;  // @mod(angle, 2*pi)
   const DevStop *stops;
+  const float *offs;   // stop offsets, dense
+  const float4 *pre;   // per stop: its color in the interpolation space
+  // linear: the gradient vector and 1/|v|^2 (Linear.getOffset's constants)
+  double lin_dx, lin_dy, lin_inv;
+  uint32_t lin_degenerate, pad1;
+};
+
+// Where a draw's stops are read from (LDS staging or the gradient itself).
+struct StopView {
+  const float *off;
+  const float4 *pre;
 };
 
 // A resolved paint (Pattern, src/pattern.zig:32-50).
@@ -513,17 +524,12 @@ __device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
   double px = (double)x + 0.5, py = (double)y + 0.5;
   if (g.type == Z2D_GRADIENT_LINEAR) {
     if (!g.identity) user_to_device(g.m, px, py);
-    double dx = g.geom[2] - g.geom[0], dy = g.geom[3] - g.geom[1];
-    double dist = 0.0;
-    dist += dx * dx;
-    dist += dy * dy;
-    if (dist == 0.0) return -1.0f;
-    double inv = 1.0 / dist;
+    if (g.lin_degenerate) return -1.0f;
     double spx = px - g.geom[0], spy = py - g.geom[1];
     double dt = 0.0;
-    dt += dx * spx;
-    dt += dy * spy;
-    return (float)dclamp01(dt * inv);
+    dt += g.lin_dx * spx;
+    dt += g.lin_dy * spy;
+    return (float)dclamp01(dt * g.lin_inv);
   }
   if (g.type == Z2D_GRADIENT_RADIAL) {
     if (g.r1 == 0.0 && g.r2 == 0.0) return -1.0f;
@@ -562,9 +568,9 @@ __device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
 }
 
 // Stop.List.search.
-__device__ inline void stops_search(const DevGradient &g, float offset, int &i0, int &i1,
-                                    float &t, bool &transparent) {
-  const int n = (int)g.n_stops;
+__device__ inline void stops_search(const float *st, uint32_t n_stops, float offset, int &i0,
+                                    int &i1, float &t, bool &transparent) {
+  const int n = (int)n_stops;
   transparent = false;
   if (offset < 0.0f || n == 0) {
     transparent = true;
@@ -572,35 +578,34 @@ __device__ inline void stops_search(const DevGradient &g, float offset, int &i0,
     t = 0.0f;
     return;
   }
-  const DevStop *st = g.stops;
   float o = fminz(offset, 1.0f);
   int left = 0, right = n, mid = 0;
   while (left < right) {
     mid = left + (right - left) / 2;
-    if (o >= st[mid].offset && (mid == n - 1 || o <= st[mid + 1].offset)) break;
-    if (o < st[mid].offset) {
+    if (o >= st[mid] && (mid == n - 1 || o <= st[mid + 1])) break;
+    if (o < st[mid]) {
       right = mid;
       continue;
     }
-    if (o > st[mid].offset) {
+    if (o > st[mid]) {
       left = mid + 1;
       continue;
     }
   }
   if (mid == n - 1) {
     i0 = i1 = mid;
-    t = o - st[mid].offset;
+    t = o - st[mid];
     return;
   }
-  if (mid == 0 && o < st[mid].offset) {
+  if (mid == 0 && o < st[mid]) {
     i0 = i1 = mid;
-    t = o / st[mid].offset;
+    t = o / st[mid];
     return;
   }
   i0 = mid;
   i1 = mid + 1;
-  float rl = st[mid + 1].offset - st[mid].offset;
-  t = rl != 0.0f ? (o - st[mid].offset) / rl : 0.0f;
+  float rl = st[mid + 1] - st[mid];
+  t = rl != 0.0f ? (o - st[mid]) / rl : 0.0f;
 }
 
 // HSL.toRGB (src/color.zig:470-488).
@@ -679,27 +684,34 @@ __device__ __forceinline__ I4 encode_premul(const float *v) {
 
 // Gradient color at (x, y): encode=true -> interpolateEncodeVec (premultiplied
 // RGBA16); encode=false -> interpolateVec (linear, straight alpha f32).
-__device__ inline void gradient_sample(const DevGradient &g, int x, int y, bool encode, I4 &oi,
-                                       F4 &of) {
+// A stop's color in the gradient's interpolation space, the per-stop part of
+// interpolateVec / interpolateEncodeVec: premultiplied linear or sRGB RGB, or
+// HSL with premultiplied saturation and lightness.  Evaluated once per stop
+// (k_grad_setup); a transparent (offset < 0) sample uses all zeros, which is
+// what this gives for the transparent linear stop in every space.
+__device__ inline float4 stop_interp_color(const DevGradient &g, const DevStop &s) {
+  float a[4];
+  if (g.method == Z2D_INTERP_LINEAR_RGB || g.method == Z2D_INTERP_SRGB) {
+    color_to_rgb(s, g.method == Z2D_INTERP_SRGB, a);
+    return make_float4(a[0] * a[3], a[1] * a[3], a[2] * a[3], a[3]);
+  }
+  color_to_hsl(s, a);
+  return make_float4(a[0], a[1] * a[3], a[2] * a[3], a[3]);
+}
+
+__device__ inline void gradient_sample(const DevGradient &g, const StopView &sv, int x, int y,
+                                       bool encode, I4 &oi, F4 &of) {
   int i0, i1;
   float t;
   bool transparent;
-  stops_search(g, gradient_offset(g, x, y), i0, i1, t, transparent);
-  DevStop c0, c1;
-  if (transparent) {
-    c0 = DevStop{0.0f, Z2D_COLOR_LINEAR_RGB, {0, 0, 0, 0}};
-    c1 = c0;
-  } else {
-    c0 = g.stops[i0];
-    c1 = g.stops[i1];
-  }
-  float a[4], b[4], v[4];
+  stops_search(sv.off, g.n_stops, gradient_offset(g, x, y), i0, i1, t, transparent);
+  const float4 A = transparent ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : sv.pre[i0];
+  const float4 B = transparent ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : sv.pre[i1];
+  const float am[4] = {A.x, A.y, A.z, A.w};
+  const float bm[4] = {B.x, B.y, B.z, B.w};
+  float v[4];
   if (g.method == Z2D_INTERP_LINEAR_RGB || g.method == Z2D_INTERP_SRGB) {
     bool srgb = g.method == Z2D_INTERP_SRGB;
-    color_to_rgb(c0, srgb, a);
-    color_to_rgb(c1, srgb, b);
-    float am[4] = {a[0] * a[3], a[1] * a[3], a[2] * a[3], a[3]};
-    float bm[4] = {b[0] * b[3], b[1] * b[3], b[2] * b[3], b[3]};
     for (int i = 0; i < 4; i++) v[i] = lerpf_(am[i], bm[i], t);
     if (!srgb && encode) {
       oi = I4{f2u8(255.0f * v[0]), f2u8(255.0f * v[1]), f2u8(255.0f * v[2]), f2u8(255.0f * v[3])};
@@ -713,10 +725,6 @@ __device__ inline void gradient_sample(const DevGradient &g, int x, int y, bool 
     if (srgb)
       for (int i = 0; i < 3; i++) v[i] = zpowf(v[i], 2.2f);
   } else {
-    color_to_hsl(c0, a);
-    color_to_hsl(c1, b);
-    float am[4] = {a[0], a[1] * a[3], a[2] * a[3], a[3]};
-    float bm[4] = {b[0], b[1] * b[3], b[2] * b[3], b[3]};
     float ah = am[0], bh = bm[0];
     switch (g.polar) {
       case Z2D_POLAR_SHORTER: {
@@ -764,11 +772,12 @@ __device__ __forceinline__ float bayer8(int x, int y) {
 }
 
 // fromDitherVec for one lane: linear straight-alpha f32.
-__device__ inline F4 dither_sample(const DevPaint &p, const uint16_t *blue_noise, int x, int y) {
+__device__ inline F4 dither_sample(const DevPaint &p, const StopView &sv,
+                                   const uint16_t *blue_noise, int x, int y) {
   F4 c;
   if (p.dither_source == Z2D_DITHER_SRC_GRADIENT) {
     I4 unused;
-    gradient_sample(*p.gradient, x, y, false, unused, c);
+    gradient_sample(*p.gradient, sv, x, y, false, unused, c);
   } else {
     c = p.dither_color;
   }
@@ -786,27 +795,35 @@ __device__ inline F4 dither_sample(const DevPaint &p, const uint16_t *blue_noise
             clampz(c.b + m * sc, 0.0f, 1.0f), clampz(c.a + m * sc, 0.0f, 1.0f)};
 }
 
-__device__ inline I4 paint_int(const DevPaint &p, const uint16_t *bn, int x, int y) {
+__device__ inline I4 paint_int(const DevPaint &p, const StopView &sv, const uint16_t *bn, int x,
+                               int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return p.px;
   if (p.kind == Z2D_PATTERN_GRADIENT) {
     I4 o;
     F4 unused;
-    gradient_sample(*p.gradient, x, y, true, o, unused);
+    gradient_sample(*p.gradient, sv, x, y, true, o, unused);
     return o;
   }
-  F4 f = dither_sample(p, bn, x, y);
+  F4 f = dither_sample(p, sv, bn, x, y);
   float v[4] = {f.r, f.g, f.b, f.a};
   return encode_premul(v);
 }
-__device__ inline F4 paint_float(const DevPaint &p, const uint16_t *bn, int x, int y) {
+__device__ inline F4 paint_float(const DevPaint &p, const StopView &sv, const uint16_t *bn,
+                                 int x, int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return decode_raw(p.px);
   if (p.kind == Z2D_PATTERN_GRADIENT) {
     I4 unused;
     F4 o;
-    gradient_sample(*p.gradient, x, y, false, unused, o);
+    gradient_sample(*p.gradient, sv, x, y, false, unused, o);
     return o;
   }
-  return dither_sample(p, bn, x, y);
+  return dither_sample(p, sv, bn, x, y);
+}
+
+// The stops of a paint as stored with its gradient.
+__device__ __forceinline__ StopView paint_stops(const DevPaint &p) {
+  if (!p.gradient) return StopView{nullptr, nullptr};
+  return StopView{p.gradient->offs, p.gradient->pre};
 }
 
 }  // namespace z2d

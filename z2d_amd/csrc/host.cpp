@@ -40,6 +40,7 @@ hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob 
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st);
 
 static const uint16_t kBlueNoise[4096] = {
 #include "blue_noise_64x64.inc"
@@ -326,6 +327,15 @@ struct PaintTable {
       a += d.dr * (-d.dr);
       d.a = a;
       d.inv_a = a != 0.0 ? 1.0 / a : 0.0;
+    } else if (g->type == Z2D_GRADIENT_LINEAR) {
+      // Linear.getOffset's per-gradient terms (src/gradient.zig)
+      d.lin_dx = g->geom[2] - g->geom[0];
+      d.lin_dy = g->geom[3] - g->geom[1];
+      double dist = 0.0;
+      dist += d.lin_dx * d.lin_dx;
+      dist += d.lin_dy * d.lin_dy;
+      d.lin_degenerate = dist == 0.0;
+      d.lin_inv = dist != 0.0 ? 1.0 / dist : 0.0;
     } else if (g->type == Z2D_GRADIENT_CONIC) {
       // Conic.initBuffer: @mod(angle, pi*2)
       const double two_pi = 3.14159265358979323846 * 2.0;
@@ -404,18 +414,38 @@ struct PaintTable {
   }
 
   // Writes the three tables at their device addresses.
-  void write(uint8_t *host_base, uint8_t *dev_base, size_t paint_off, size_t grad_off,
-             size_t stop_off) {
-    DevPaint *hp = reinterpret_cast<DevPaint *>(host_base + paint_off);
-    DevGradient *hg = reinterpret_cast<DevGradient *>(host_base + grad_off);
-    DevStop *hs = reinterpret_cast<DevStop *>(host_base + stop_off);
-    DevGradient *dg = reinterpret_cast<DevGradient *>(dev_base + grad_off);
-    DevStop *ds = reinterpret_cast<DevStop *>(dev_base + stop_off);
+  // Layout: paints, gradients, stops, dense stop offsets, per-stop colors
+  // (the last filled on the device by k_grad_setup).
+  struct Offsets {
+    size_t paints, grads, stops, offs, pre;
+  };
+  Offsets take(Layout &L) const {
+    Offsets o;
+    o.paints = L.take(paints.size() * sizeof(DevPaint));
+    o.grads = L.take(grads.size() * sizeof(DevGradient));
+    o.stops = L.take(stops.size() * sizeof(DevStop));
+    o.offs = L.take(stops.size() * sizeof(float));
+    o.pre = L.take(stops.size() * sizeof(float4));
+    return o;
+  }
+
+  void write(uint8_t *host_base, uint8_t *dev_base, const Offsets &o) {
+    DevPaint *hp = reinterpret_cast<DevPaint *>(host_base + o.paints);
+    DevGradient *hg = reinterpret_cast<DevGradient *>(host_base + o.grads);
+    DevStop *hs = reinterpret_cast<DevStop *>(host_base + o.stops);
+    float *hoff = reinterpret_cast<float *>(host_base + o.offs);
+    DevGradient *dg = reinterpret_cast<DevGradient *>(dev_base + o.grads);
+    DevStop *ds = reinterpret_cast<DevStop *>(dev_base + o.stops);
+    float *doff = reinterpret_cast<float *>(dev_base + o.offs);
+    float4 *dpre = reinterpret_cast<float4 *>(dev_base + o.pre);
     for (size_t i = 0; i < grads.size(); i++) {
       hg[i] = grads[i];
       hg[i].stops = ds + grad_stop_off[i];
+      hg[i].offs = doff + grad_stop_off[i];
+      hg[i].pre = dpre + grad_stop_off[i];
     }
     if (!stops.empty()) std::memcpy(hs, stops.data(), stops.size() * sizeof(DevStop));
+    for (size_t i = 0; i < stops.size(); i++) hoff[i] = stops[i].offset;
     for (size_t i = 0; i < paints.size(); i++) {
       hp[i] = paints[i];
       hp[i].gradient = paint_grad[i] >= 0 ? dg + paint_grad[i] : nullptr;
@@ -829,9 +859,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_inedges = L.take(b->in_edges.size() * sizeof(z2d_edge));
   const size_t o_draws = L.take(nd * sizeof(DevDraw));
   const size_t o_surfs = L.take(ns * sizeof(DevSurface));
-  const size_t o_paints = L.take(b->paints.paints.size() * sizeof(DevPaint));
-  const size_t o_grads = L.take(b->paints.grads.size() * sizeof(DevGradient));
-  const size_t o_stops = L.take(b->paints.stops.size() * sizeof(DevStop));
+  const PaintTable::Offsets o_pt = b->paints.take(L);
   const size_t o_sdo = L.take((ns + 1) * sizeof(uint32_t));
   const size_t o_sbo = L.take((ns + 1) * sizeof(uint32_t));
   const size_t o_tiles = L.take((size_t)n_tiles * sizeof(TileRef));
@@ -873,7 +901,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     hs[i].nbytes = s->nbytes;
     hs[i].clip = s->clip;
   }
-  b->paints.write(H, D, o_paints, o_grads, o_stops);
+  b->paints.write(H, D, o_pt);
   std::memcpy(H + o_sdo, surf_draw_off.data(), (ns + 1) * sizeof(uint32_t));
   std::memcpy(H + o_sbo, surf_band_off.data(), (ns + 1) * sizeof(uint32_t));
   TileRef *ht = reinterpret_cast<TileRef *>(H + o_tiles);
@@ -891,6 +919,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   }
   std::memcpy(H + o_bn, kBlueNoise, sizeof(kBlueNoise));
   HIP_OK(ctx, hipMemcpyAsync(D, H, input_bytes, hipMemcpyHostToDevice, bstream(b)));
+  HIP_OK(ctx, z2d_launch_grad_setup(reinterpret_cast<DevGradient *>(D + o_pt.grads),
+                                    (uint32_t)b->paints.grads.size(), bstream(b)));
 
   JobView &J = b->view;
   J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
@@ -900,7 +930,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.n_draws = nd;
   J.surfaces = reinterpret_cast<DevSurface *>(D + o_surfs);
   J.n_surfaces = ns;
-  J.paints = reinterpret_cast<const DevPaint *>(D + o_paints);
+  J.paints = reinterpret_cast<const DevPaint *>(D + o_pt.paints);
   J.blue_noise = reinterpret_cast<const uint16_t *>(D + o_bn);
   J.surf_draw_off = reinterpret_cast<const uint32_t *>(D + o_sdo);
   J.surf_band_off = reinterpret_cast<const uint32_t *>(D + o_sbo);
@@ -1201,9 +1231,7 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
   if (rc == Z2D_OK) {
     Layout L;
     const size_t o_job = L.take(sizeof(CompJob));
-    const size_t o_paints = L.take(pt.paints.size() * sizeof(DevPaint));
-    const size_t o_grads = L.take(pt.grads.size() * sizeof(DevGradient));
-    const size_t o_stops = L.take(pt.stops.size() * sizeof(DevStop));
+    const PaintTable::Offsets o_pt = pt.take(L);
     const size_t o_bn = L.take(sizeof(kBlueNoise));
     hipError_t e = ctx->comp_arena.ensure(L.off);
     if (e == hipSuccess) e = ctx->comp_stage.ensure(L.off);
@@ -1211,8 +1239,8 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
       rc = fail(ctx, e, "composite arena");
     } else {
       uint8_t *H = ctx->comp_stage.p, *D = ctx->comp_arena.p;
-      pt.write(H, D, o_paints, o_grads, o_stops);
-      const DevPaint *dp = reinterpret_cast<const DevPaint *>(D + o_paints);
+      pt.write(H, D, o_pt);
+      const DevPaint *dp = reinterpret_cast<const DevPaint *>(D + o_pt.paints);
       for (uint64_t k = 0; k < n_ops; k++) {
         if (paint_idx[2 * k] >= 0) job.ops[k].dst.paint = dp + paint_idx[2 * k];
         if (paint_idx[2 * k + 1] >= 0) job.ops[k].src.paint = dp + paint_idx[2 * k + 1];
@@ -1221,6 +1249,9 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
       std::memcpy(H + o_job, &job, sizeof(job));
       std::memcpy(H + o_bn, kBlueNoise, sizeof(kBlueNoise));
       e = hipMemcpyAsync(D, H, L.off, hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess)
+        e = z2d_launch_grad_setup(reinterpret_cast<DevGradient *>(D + o_pt.grads),
+                                  (uint32_t)pt.grads.size(), ctx->stream);
       if (e == hipSuccess) {
         // Streaming path: whole 4-byte-format surfaces, aligned rows, every
         // surface source covering the destination at the origin.

@@ -11,13 +11,17 @@
 
 namespace z2d {
 
-// Tile shape of the renderer: TILE_W pixels x TILE_H rows per workgroup of
-// 256 lanes (4 wave64s; lane = column, each wave owns TILE_H/4 rows).
+// Tile shape of the renderer: TILE_W pixels x TILE_H rows per band entry;
+// workgroups of 256 lanes (4 wave64s) hold four independent strip workers.
 constexpr int TILE_W = 64;
 constexpr int TILE_H = 16;
 constexpr int RENDER_THREADS = 256;
 constexpr int ROWS_PER_THREAD = TILE_H / 4;
 constexpr int MAX_SUBROWS = TILE_H * 4;
+// Each wave renders one strip of ROWS_PER_THREAD rows of a tile on its own
+// (no workgroup barriers): lane = column, one register per strip row.
+constexpr int STRIPS_PER_TILE = TILE_H / ROWS_PER_THREAD;
+constexpr int STRIP_SUBROWS = ROWS_PER_THREAD * 4;
 constexpr int ACC_WORDS = TILE_W * 2;  // 4 sample columns per pixel, two int16 per word
 // Winding deltas are accumulated as int16 pairs in LDS; an entry whose band
 // holds more edges than this cannot overflow them and takes the exact

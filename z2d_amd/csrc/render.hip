@@ -488,17 +488,30 @@ __global__ void k_entry_edges(JobView J) {
 // ---------------------------------------------------------------------------
 // k_render: tile renderer.
 // ---------------------------------------------------------------------------
-struct TileShared {
-  // Winding deltas per sample row and sample column of the tile, two int16
+// Gradients with up to this many stops are read from LDS while a draw is
+// composited.
+constexpr uint32_t STAGED_STOPS = 64;
+
+// LDS of one strip worker (one wave).
+struct StripShared {
+  // Winding deltas per sample row and sample column of the strip, two int16
   // columns per word (column 2i in the low half).  Zero between entries:
   // the coverage phase clears what it reads.
-  uint32_t acc[MAX_SUBROWS][ACC_WORDS];
-  z2d_edge cedge[RENDER_THREADS];    // staged band edges
-  int32_t clo[RENDER_THREADS];       // first active sample row in the tile
-  uint32_t coff[RENDER_THREADS];     // exclusive offsets of active-row counts
-  uint32_t wsum[RENDER_THREADS / 64];
-  uint32_t s_tile;
+  uint32_t acc[STRIP_SUBROWS][ACC_WORDS];
+  z2d_edge cedge[64];  // staged band edges
+  int32_t clo[64];     // first active sample row in the strip
+  uint32_t coff[64];   // exclusive offsets of active-row counts
+  float soff[STAGED_STOPS];   // the draw's gradient stops, staged
+  float4 spre[STAGED_STOPS];
 };
+
+// Orders this wave's LDS accesses before and after the call (lanes of one
+// wave exchanging data through LDS).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
 // 16-lane rows, then row broadcasts across rows).
@@ -528,40 +541,59 @@ __device__ __forceinline__ uint32_t ssaa_mask(int k, int bits) {
   }
 }
 
-// One pixel through one draw, given its sample coverage k (of kfull).
+// One pixel through one draw, given its sample coverage k (of kfull): the
+// paint masked by the coverage (DST_IN), then the draw's operator.  The
+// supersample mode masks every pixel of its box, even at zero coverage; the
+// solid fast path of painter.fill stays in integer precision.
 __device__ __forceinline__ uint32_t composite_cov(uint32_t raw, uint32_t fmt, int k, int kfull,
                                                   const DevDraw &D, const DevPaint &P,
-                                                  const uint16_t *bn, int x, int y) {
+                                                  const StopView &sv, const uint16_t *bn, int x,
+                                                  int y) {
+  uint32_t mask_a = 255;
+  bool masked = true;
   if (D.mode == MODE_SSAA) {
-    I4 mk{0, 0, 0, ssaa_mask(k, D.mask_bits)};
-    if (D.precision == Z2D_PRECISION_INTEGER) {
-      I4 src = op_int(Z2D_OP_DST_IN, paint_int(P, bn, x, y), mk);
-      return pack(op_int(D.op, unpack(raw, fmt), src), fmt);
+    mask_a = ssaa_mask(k, D.mask_bits);
+  } else {
+    if (k == 0) return raw;
+    if (D.op == Z2D_OP_CLEAR) return pack(I4{0, 0, 0, 0}, fmt);
+    if (k >= kfull) {
+      if (D.flags & DRAW_FAST_SRC) return pack(P.px, fmt);
+      masked = false;
+    } else {
+      mask_a = (uint32_t)clampi(k * 16 - 1, 0, 255);
     }
-    F4 src = op_float(Z2D_OP_DST_IN, paint_float(P, bn, x, y), decode_raw(mk));
-    return pack(encode_raw(op_float(D.op, decode_raw(unpack(raw, fmt)), src)), fmt);
   }
-  if (k == 0) return raw;
-  if (D.op == Z2D_OP_CLEAR) return pack(I4{0, 0, 0, 0}, fmt);
-  if (k >= kfull) {
-    if (D.flags & DRAW_FAST_SRC) return pack(P.px, fmt);
-    if (D.precision == Z2D_PRECISION_INTEGER)
-      return pack(op_int(D.op, unpack(raw, fmt), paint_int(P, bn, x, y)), fmt);
-    return pack(encode_raw(op_float(D.op, decode_raw(unpack(raw, fmt)), paint_float(P, bn, x, y))),
-                fmt);
-  }
-  const uint32_t alpha = (uint32_t)clampi(k * 16 - 1, 0, 255);
-  const I4 mk{0, 0, 0, alpha};
-  if (D.flags & DRAW_FAST_SRC) {
-    I4 src = op_int(Z2D_OP_DST_IN, P.px, mk);
+  const bool int_path = D.precision == Z2D_PRECISION_INTEGER ||
+                        (D.mode != MODE_SSAA && (D.flags & DRAW_FAST_SRC));
+  if (int_path) {
+    I4 src = P.kind == Z2D_PATTERN_OPAQUE ? P.px : paint_int(P, sv, bn, x, y);
+    if (masked) src = op_int(Z2D_OP_DST_IN, src, I4{0, 0, 0, mask_a});
     return pack(op_int(D.op, unpack(raw, fmt), src), fmt);
   }
-  if (D.precision == Z2D_PRECISION_INTEGER) {
-    I4 src = op_int(Z2D_OP_DST_IN, paint_int(P, bn, x, y), mk);
-    return pack(op_int(D.op, unpack(raw, fmt), src), fmt);
-  }
-  F4 src = op_float(Z2D_OP_DST_IN, paint_float(P, bn, x, y), decode_raw(mk));
+  F4 src = P.kind == Z2D_PATTERN_OPAQUE ? decode_raw(P.px) : paint_float(P, sv, bn, x, y);
+  if (masked) src = op_float(Z2D_OP_DST_IN, src, decode_raw(I4{0, 0, 0, mask_a}));
   return pack(encode_raw(op_float(D.op, decode_raw(unpack(raw, fmt)), src)), fmt);
+}
+
+// composite_cov for the common case, kept inline: src_over in integer
+// precision without a supersample mask (the same steps: DST_IN of the paint
+// by the coverage alpha, then SRC_OVER onto the pixel).
+__device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, int k,
+                                                       int kfull, const DevDraw &D,
+                                                       const DevPaint &P, const StopView &sv,
+                                                       const uint16_t *bn, int x, int y) {
+  if (k == 0) return raw;
+  if (k >= kfull && (D.flags & DRAW_FAST_SRC)) return pack(P.px, fmt);
+  I4 s = P.kind == Z2D_PATTERN_OPAQUE ? P.px : paint_int(P, sv, bn, x, y);
+  if (k < kfull) {
+    const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
+    s = I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
+  }
+  const I4 d = unpack(raw, fmt);
+  const uint32_t sa = s.a;
+  const I4 o{u16(s.r + iinvmul(d.r, sa)), u16(s.g + iinvmul(d.g, sa)),
+             u16(s.b + iinvmul(d.b, sa)), u16(sa + d.a - imul(sa, d.a))};
+  return pack(o, fmt);
 }
 
 __device__ __forceinline__ bool msaa_preclear(const DevDraw &D, int W, int x, int y) {
@@ -596,27 +628,86 @@ __device__ __forceinline__ int32_t active_rows(double top, double bot, int32_t y
 
 __device__ __forceinline__ bool wind_in(int32_t w, bool eo) { return eo ? (w & 1) != 0 : w != 0; }
 
+// Covered samples (of 4 x 4) per pixel of the lane's column for the strip's
+// rows: each sample row's winding is the prefix sum of the deltas up to the
+// sample column (lane-local over the pixel's 4 columns, a wave scan across
+// lanes, two sample rows packed into one scan).  Reads and clears the deltas.
+// CUT: the row records' cutoff columns apply (unbalanced contours).
+template <bool EO, bool CUT>
+__device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, int32_t xs_base,
+                                              int32_t ys_base, const RowRec *R, const DevDraw &D,
+                                              int *kk) {
+  const int c_lo = (int)lane * 4;
+#pragma unroll
+  for (int i = 0; i < ROWS_PER_THREAD; i++) {
+    int k = 0;
+#pragma unroll
+    for (int pr = 0; pr < 2; pr++) {
+      int32_t d[2][4], tot[2];
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        uint2 *slot = reinterpret_cast<uint2 *>(&sh.acc[i * 4 + pr * 2 + q][lane * 2]);
+        const uint2 v = *slot;
+        *slot = make_uint2(0u, 0u);
+        unpack_pair(v.x, d[q][0], d[q][1]);
+        unpack_pair(v.y, d[q][2], d[q][3]);
+        tot[q] = d[q][0] + d[q][1] + d[q][2] + d[q][3];
+      }
+      const uint32_t pk = (uint32_t)tot[0] + ((uint32_t)tot[1] << 16);
+      const uint32_t exc = (uint32_t)wave_incl_scan((int32_t)pk) - pk;
+      int32_t base[2];
+      unpack_pair(exc, base[0], base[1]);
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        int32_t cutoff = INT32_MAX;
+        if (CUT) {
+          const int32_t rr = ys_base + i * 4 + pr * 2 + q - D.row_y0;
+          if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+        }
+        int32_t w = base[q];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          w += d[q][j];
+          const bool in = EO ? (w & 1) != 0 : w != 0;
+          if (CUT)
+            k += (in && xs_base + c_lo + j < cutoff) ? 1 : 0;
+          else
+            k += in ? 1 : 0;
+        }
+      }
+    }
+    kk[i] = k;
+  }
+}
+
+// Persistent strip workers: each wave takes (tile, strip) items from the
+// queue and renders every draw of the tile's band over its ROWS_PER_THREAD x
+// TILE_W pixels, which stay in registers from the first draw to the last.
 __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t tile_base,
                                                            uint32_t tile_count,
                                                            uint32_t *cursor) {
-  __shared__ TileShared sh;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ StripShared shw[RENDER_THREADS / 64];
+  const uint32_t lane = threadIdx.x & 63;
+  StripShared &sh = shw[threadIdx.x >> 6];
+  const uint32_t n_items = tile_count * STRIPS_PER_TILE;
 
   // The accumulator starts at zero and every entry leaves it at zero.
-  for (uint32_t i = tid; i < (uint32_t)(MAX_SUBROWS * ACC_WORDS); i += RENDER_THREADS)
+  for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
     (&sh.acc[0][0])[i] = 0u;
+  wave_lds_sync();
 
   for (;;) {
-    // persistent work loop over the tile queue
-    if (tid == 0) sh.s_tile = atomicAdd(cursor, 1u);
-    __syncthreads();
-    const uint32_t ti = sh.s_tile;
-    __syncthreads();
-    if (ti >= tile_count) break;
-    const TileRef T = J.tiles[tile_base + ti];
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(cursor, 1u);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= n_items) break;
+    const TileRef T = J.tiles[tile_base + item / STRIPS_PER_TILE];
+    const uint32_t strip = item % STRIPS_PER_TILE;
     const DevSurface S = J.surfaces[T.surface];
     const int32_t W = S.w, H = S.h;
-    const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * TILE_H;
+    const int32_t tx0 = (int32_t)T.tx * TILE_W;
+    const int32_t ty0 = (int32_t)T.ty * TILE_H + (int32_t)strip * ROWS_PER_THREAD;
+    if (ty0 >= H) continue;
     const uint32_t band = J.surf_band_off[T.surface] + T.ty;
     const uint32_t e0 = min(J.band_cnt[band], J.max_entries);
     const uint32_t e1 = min(J.band_cnt[band + 1], J.max_entries);
@@ -627,7 +718,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     uint32_t pix[ROWS_PER_THREAD];
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
-      const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+      const int32_t py = ty0 + i;
       pix[i] = (col_ok && py < H) ? load_raw(S, (int64_t)py * W + px) : 0u;
     }
     uint32_t orig[ROWS_PER_THREAD];
@@ -637,22 +728,44 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     for (uint32_t e = e0; e < e1; e++) {
       const BandEntry E = J.entries[e];
       const DevDraw &D = J.draws[E.draw];
-      if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;  // uniform
+      // uniform: the draw's composite region misses this strip
+      if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;
+      if (D.ry1 <= ty0 || D.ry0 >= ty0 + ROWS_PER_THREAD) continue;
       const DevPaint &P = J.paints[D.paint];
+      StopView sv{nullptr, nullptr};
+      if (P.gradient) {
+        const DevGradient &G = *P.gradient;
+        if (G.n_stops <= STAGED_STOPS) {
+          wave_lds_sync();
+          if (lane < G.n_stops) {
+            sh.soff[lane] = G.offs[lane];
+            sh.spre[lane] = G.pre[lane];
+          }
+          wave_lds_sync();
+          sv = StopView{sh.soff, sh.spre};
+        } else {
+          sv = StopView{G.offs, G.pre};
+        }
+      }
       const int SC = D.mode == MODE_DIRECT ? 1 : 4;
-      const int SR = TILE_H * SC;  // sample rows in this tile
+      const int SR = ROWS_PER_THREAD * SC;  // sample rows in this strip
       const int32_t ys_base = ty0 * SC;
       const int32_t xs_base = tx0 * SC;
       const int32_t cols = TILE_W * SC;
       const bool direct_unb = D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED);
 
+      if (J.count_stats && lane == 0) atomicAdd(&J.counters[7], 1u);
+      int kk[ROWS_PER_THREAD];
       if (direct_unb) {
         // Row records give the whole result of the reference's quirky
-        // per-pair clears (src/internal/raster/direct.zig:88-124).
+        // per-pair clears (src/internal/raster/direct.zig:88-124): cleared
+        // pixels become transparent, the rest composite at full coverage
+        // onto themselves or onto a cleared pixel.
         const RowRec *R = J.rows + D.row_off;
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
-          const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+          const int32_t py = ty0 + i;
+          kk[i] = 0;
           if (!col_ok || py >= H) continue;
           const RowRec rr = R[py - D.row_y0];
           if (rr.kind == 1) continue;
@@ -660,199 +773,179 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
             pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
             continue;
           }
-          const uint32_t base = rr.kind == 2 ? pix[i] : pack(I4{0, 0, 0, 0}, S.fmt);
-          pix[i] = composite_cov(base, S.fmt, 1, 1, D, P, J.blue_noise, px, py);
+          if (rr.kind != 2) pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
+          kk[i] = 1;
         }
-        continue;
-      }
-      if (J.count_stats && tid == 0) atomicAdd(&J.counters[7], 1u);
+      } else {
+        const z2d_edge *edges = J.edges + D.node_off;
+        const uint32_t *eidx = J.entry_edge_idx + E.edge_off;
+        const uint32_t ne = E.edge_off >= J.max_entry_edges
+                                ? 0u
+                                : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
+        const bool eo = D.rule == Z2D_FILL_EVEN_ODD;
+        const bool packed = ne <= PACKED_EDGE_LIMIT;  // uniform
 
-      const z2d_edge *edges = J.edges + D.node_off;
-      const uint32_t *eidx = J.entry_edge_idx + E.edge_off;
-      const uint32_t ne = E.edge_off >= J.max_entry_edges
-                              ? 0u
-                              : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
-      const bool eo = D.rule == Z2D_FILL_EVEN_ODD;
-      const bool packed = ne <= PACKED_EDGE_LIMIT;  // uniform
-
-      // ---- phase A: winding deltas of this draw's band edges.  Each
-      // crossing adds its direction at its sample column (crossings left of
-      // the tile at column 0, right of it nowhere); work items are the
-      // (edge, active sample row) pairs, compacted by a block scan.
-      if (packed) {
-        for (uint32_t c0 = 0; c0 < ne; c0 += RENDER_THREADS) {
-          const uint32_t k = c0 + tid;
-          int32_t n_sr = 0, lo = 0;
-          z2d_edge ed{0.0, 0.0, 0.0, 0.0};
-          if (k < ne) {
-            ed = edges[eidx[k]];
-            const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-            const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-            n_sr = active_rows(top, bot, ys_base, SR, lo);
-            if (n_sr) {
-              // every crossing right of the tile: no effect here
-              const double xb = ed.x_start + ed.x_inc * (bot - top);
-              if (fmin(ed.x_start, xb) > (double)(xs_base + cols) + 2.0) n_sr = 0;
+        // ---- phase A: winding deltas of this draw's band edges.  Each
+        // crossing adds its direction at its sample column (crossings left of
+        // the strip at column 0, right of it nowhere); work items are the
+        // (edge, active sample row) pairs, compacted by a wave scan.
+        if (packed) {
+          for (uint32_t c0 = 0; c0 < ne; c0 += 64) {
+            const uint32_t k = c0 + lane;
+            int32_t n_sr = 0, lo = 0;
+            z2d_edge ed{0.0, 0.0, 0.0, 0.0};
+            if (k < ne) {
+              ed = edges[eidx[k]];
+              const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
+              const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+              n_sr = active_rows(top, bot, ys_base, SR, lo);
+              if (n_sr) {
+                // every crossing right of the strip: no effect here
+                const double xb = ed.x_start + ed.x_inc * (bot - top);
+                if (fmin(ed.x_start, xb) > (double)(xs_base + cols) + 2.0) n_sr = 0;
+              }
             }
-          }
-          sh.cedge[tid] = ed;
-          sh.clo[tid] = lo;
-          const int32_t incl = wave_incl_scan(n_sr);
-          if (lane == 63) sh.wsum[wave] = (uint32_t)incl;
-          __syncthreads();
-          uint32_t woff = 0, total = 0;
-#pragma unroll
-          for (uint32_t w = 0; w < RENDER_THREADS / 64; w++) {
-            const uint32_t v = sh.wsum[w];
-            woff += w < wave ? v : 0u;
-            total += v;
-          }
-          sh.coff[tid] = woff + (uint32_t)(incl - n_sr);
-          __syncthreads();
-          for (uint32_t it = tid; it < total; it += RENDER_THREADS) {
-            // owning edge: the last j with coff[j] <= it
-            uint32_t lj = 0, hj = RENDER_THREADS;
-            while (hj - lj > 1) {
-              const uint32_t mj = (lj + hj) >> 1;
-              if (sh.coff[mj] <= it) lj = mj; else hj = mj;
+            const int32_t incl = wave_incl_scan(n_sr);
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+            if (total == 0) continue;  // uniform
+            sh.cedge[lane] = ed;
+            sh.clo[lane] = lo;
+            sh.coff[lane] = (uint32_t)(incl - n_sr);
+            wave_lds_sync();
+            for (uint32_t it = lane; it < total; it += 64) {
+              // owning edge: the last j with coff[j] <= it
+              uint32_t lj = 0, hj = 64;
+              while (hj - lj > 1) {
+                const uint32_t mj = (lj + hj) >> 1;
+                if (sh.coff[mj] <= it) lj = mj; else hj = mj;
+              }
+              const z2d_edge ce = sh.cedge[lj];
+              const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
+              const double top = ce.y0 < ce.y1 ? ce.y0 : ce.y1;
+              const double mid = (double)(ys_base + sr) + 0.5;
+              const int32_t x = (int32_t)round(ce.x_start + (ce.x_inc * (mid - top)));
+              const int32_t xr = x - xs_base;
+              if (xr >= cols) continue;
+              const int32_t col = xr < 0 ? 0 : xr;
+              const int32_t dv = eo ? 1 : (ce.y0 < ce.y1 ? -1 : 1);
+              atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? ((uint32_t)dv << 16) : (uint32_t)dv);
             }
-            const z2d_edge ce = sh.cedge[lj];
-            const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
-            const double top = ce.y0 < ce.y1 ? ce.y0 : ce.y1;
-            const double mid = (double)(ys_base + sr) + 0.5;
-            const int32_t x = (int32_t)round(ce.x_start + (ce.x_inc * (mid - top)));
-            const int32_t xr = x - xs_base;
-            if (xr >= cols) continue;
-            const int32_t col = xr < 0 ? 0 : xr;
-            const int32_t dv = eo ? 1 : (ce.y0 < ce.y1 ? -1 : 1);
-            atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? ((uint32_t)dv << 16) : (uint32_t)dv);
+            wave_lds_sync();
           }
-          __syncthreads();
         }
-      }
 
-      // ---- phase B: coverage per pixel (samples covered of SC x SC), then
-      // the composite in draw order.
-      const RowRec *R = (D.flags & DRAW_ROWPROG) ? J.rows + D.row_off : nullptr;
-      const int c_lo = (int)lane * SC;
-      int kk[ROWS_PER_THREAD];
-      if (packed && SC == 4) {
-#pragma unroll
-        for (int i = 0; i < ROWS_PER_THREAD; i++) {
-          const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
-          int k = 0;
-#pragma unroll
-          for (int pr = 0; pr < 2; pr++) {
-            int32_t d[2][4], tot[2];
-#pragma unroll
+        // ---- phase B: coverage per pixel (samples covered of SC x SC), then
+        // the composite in draw order.
+        const RowRec *R = (D.flags & DRAW_ROWPROG) ? J.rows + D.row_off : nullptr;
+        const int c_lo = (int)lane * SC;
+        if (packed && SC == 4) {
+          if (eo) {
+            if (R) coverage_msaa<true, true>(sh, lane, xs_base, ys_base, R, D, kk);
+            else coverage_msaa<true, false>(sh, lane, xs_base, ys_base, R, D, kk);
+          } else {
+            if (R) coverage_msaa<false, true>(sh, lane, xs_base, ys_base, R, D, kk);
+            else coverage_msaa<false, false>(sh, lane, xs_base, ys_base, R, D, kk);
+          }
+        } else if (packed) {
+          // one sample per pixel: rows paired into one scan
+  #pragma unroll
+          for (int pr = 0; pr < ROWS_PER_THREAD / 2; pr++) {
+            int32_t d[2];
+  #pragma unroll
             for (int q = 0; q < 2; q++) {
-              uint2 *slot = reinterpret_cast<uint2 *>(&sh.acc[ly * 4 + pr * 2 + q][lane * 2]);
-              const uint2 v = *slot;
-              *slot = make_uint2(0u, 0u);
-              unpack_pair(v.x, d[q][0], d[q][1]);
-              unpack_pair(v.y, d[q][2], d[q][3]);
-              tot[q] = d[q][0] + d[q][1] + d[q][2] + d[q][3];
+              uint32_t *slot = &sh.acc[pr * 2 + q][lane >> 1];
+              const uint32_t v = *slot;
+              if ((lane & 1) == 0) *slot = 0u;
+              int32_t lo, hi;
+              unpack_pair(v, lo, hi);
+              d[q] = (lane & 1) ? hi : lo;
             }
-            // both sample rows in one scan: int16 halves of one word
-            const uint32_t pk = (uint32_t)tot[0] + ((uint32_t)tot[1] << 16);
-            const uint32_t exc = (uint32_t)wave_incl_scan((int32_t)pk) - pk;
-            int32_t base[2];
-            unpack_pair(exc, base[0], base[1]);
-#pragma unroll
+            const uint32_t pk = (uint32_t)d[0] + ((uint32_t)d[1] << 16);
+            const uint32_t inc = (uint32_t)wave_incl_scan((int32_t)pk);
+            int32_t w[2];
+            unpack_pair(inc, w[0], w[1]);
+  #pragma unroll
             for (int q = 0; q < 2; q++) {
-              const int32_t ys = ys_base + ly * 4 + pr * 2 + q;
+              int32_t cutoff = INT32_MAX;
+              if (R) {
+                const int32_t rr = ys_base + pr * 2 + q - D.row_y0;
+                if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+              }
+              kk[pr * 2 + q] = (wind_in(w[q], eo) && xs_base + c_lo < cutoff) ? 1 : 0;
+            }
+          }
+        } else {
+          // more band edges than the packed deltas can hold: each lane
+          // evaluates the crossings of its own sample rows
+          if (J.count_stats && lane == 0) atomicAdd(&J.counters[8], 1u);
+  #pragma unroll
+          for (int i = 0; i < ROWS_PER_THREAD; i++) {
+            int k = 0;
+            for (int o = 0; o < SC; o++) {
+              const int32_t ys = ys_base + i * SC + o;
+              const double mid = (double)ys + 0.5;
+              int32_t w[4] = {0, 0, 0, 0};
+              for (uint32_t q = 0; q < ne; q++) {
+                const z2d_edge ed = edges[eidx[q]];
+                const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
+                const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+                if (!(top < mid && bot >= mid)) continue;
+                const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - top)));
+                const int32_t dv = eo ? 1 : (ed.y0 < ed.y1 ? -1 : 1);
+                const int32_t xr = x - xs_base;
+  #pragma unroll
+                for (int j = 0; j < 4; j++)
+                  if (j < SC && xr <= c_lo + j) w[j] += dv;
+              }
               int32_t cutoff = INT32_MAX;
               if (R) {
                 const int32_t rr = ys - D.row_y0;
                 if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
               }
-              int32_t w = base[q];
-#pragma unroll
-              for (int j = 0; j < 4; j++) {
-                w += d[q][j];
-                if (wind_in(w, eo) && xs_base + c_lo + j < cutoff) k++;
-              }
-            }
-          }
-          kk[i] = k;
-        }
-      } else if (packed) {
-        // one sample per pixel: rows paired into one scan
-#pragma unroll
-        for (int pr = 0; pr < ROWS_PER_THREAD / 2; pr++) {
-          int32_t d[2];
-#pragma unroll
-          for (int q = 0; q < 2; q++) {
-            uint32_t *slot = &sh.acc[wave * ROWS_PER_THREAD + pr * 2 + q][lane >> 1];
-            const uint32_t v = *slot;
-            if ((lane & 1) == 0) *slot = 0u;
-            int32_t lo, hi;
-            unpack_pair(v, lo, hi);
-            d[q] = (lane & 1) ? hi : lo;
-          }
-          const uint32_t pk = (uint32_t)d[0] + ((uint32_t)d[1] << 16);
-          const uint32_t inc = (uint32_t)wave_incl_scan((int32_t)pk);
-          int32_t w[2];
-          unpack_pair(inc, w[0], w[1]);
-#pragma unroll
-          for (int q = 0; q < 2; q++) {
-            const int32_t ys = ys_base + wave * ROWS_PER_THREAD + pr * 2 + q;
-            int32_t cutoff = INT32_MAX;
-            if (R) {
-              const int32_t rr = ys - D.row_y0;
-              if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
-            }
-            kk[pr * 2 + q] = (wind_in(w[q], eo) && xs_base + c_lo < cutoff) ? 1 : 0;
-          }
-        }
-      } else {
-        // more band edges than the packed deltas can hold: each thread
-        // evaluates the crossings of its own sample rows
-        if (J.count_stats && tid == 0) atomicAdd(&J.counters[8], 1u);
-#pragma unroll
-        for (int i = 0; i < ROWS_PER_THREAD; i++) {
-          const int32_t ly = (int32_t)wave * ROWS_PER_THREAD + i;
-          int k = 0;
-          for (int o = 0; o < SC; o++) {
-            const int32_t ys = ys_base + ly * SC + o;
-            const double mid = (double)ys + 0.5;
-            int32_t w[4] = {0, 0, 0, 0};
-            for (uint32_t q = 0; q < ne; q++) {
-              const z2d_edge ed = edges[eidx[q]];
-              const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-              const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-              if (!(top < mid && bot >= mid)) continue;
-              const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - top)));
-              const int32_t dv = eo ? 1 : (ed.y0 < ed.y1 ? -1 : 1);
-              const int32_t xr = x - xs_base;
-#pragma unroll
+  #pragma unroll
               for (int j = 0; j < 4; j++)
-                if (j < SC && xr <= c_lo + j) w[j] += dv;
+                if (j < SC && wind_in(w[j], eo) && xs_base + c_lo + j < cutoff) k++;
             }
-            int32_t cutoff = INT32_MAX;
-            if (R) {
-              const int32_t rr = ys - D.row_y0;
-              if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-              if (j < SC && wind_in(w[j], eo) && xs_base + c_lo + j < cutoff) k++;
+            kk[i] = k;
           }
-          kk[i] = k;
         }
+        wave_lds_sync();
       }
 
       uint32_t n_cov = 0;
+      if (D.op == Z2D_OP_SRC_OVER && D.precision == Z2D_PRECISION_INTEGER &&
+          D.mode != MODE_SSAA) {
 #pragma unroll
-      for (int i = 0; i < ROWS_PER_THREAD; i++) {
-        const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
-        if (!col_ok || py >= H) continue;
-        const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
-        if (!in_region) continue;
-        if (D.mode == MODE_MSAA && (D.flags & DRAW_UNBOUNDED) && msaa_preclear(D, W, px, py))
-          pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
-        n_cov += kk[i] > 0;
-        pix[i] = composite_cov(pix[i], S.fmt, kk[i], SC * SC, D, P, J.blue_noise, px, py);
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          const int32_t py = ty0 + i;
+          if (!col_ok || py >= H) continue;
+          const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
+          if (!in_region) continue;
+          n_cov += kk[i] > 0;
+          pix[i] = composite_src_over(pix[i], S.fmt, kk[i], SC * SC, D, P, sv, J.blue_noise,
+                                      px, py);
+        }
+      } else {
+        // every other operator / precision / mode: one composite instance,
+        // rows in a rolled loop over a copy of the strip's pixels
+        uint32_t tp[ROWS_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) tp[i] = pix[i];
+#pragma unroll 1
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          const int32_t py = ty0 + i;
+          if (!col_ok || py >= H) continue;
+          const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
+          if (!in_region) continue;
+          uint32_t v = tp[i];
+          if (D.mode == MODE_MSAA && (D.flags & DRAW_UNBOUNDED) && msaa_preclear(D, W, px, py))
+            v = pack(I4{0, 0, 0, 0}, S.fmt);
+          const int k = kk[i];
+          n_cov += k > 0;
+          tp[i] = composite_cov(v, S.fmt, k, SC * SC, D, P, sv, J.blue_noise, px, py);
+        }
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) pix[i] = tp[i];
       }
       if (J.count_stats) {
 #pragma unroll
@@ -861,14 +954,13 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
           atomicAdd(reinterpret_cast<unsigned long long *>(&J.counters[4]),
                     (unsigned long long)n_cov);
       }
-      __syncthreads();
     }
 
     if (S.clip) {
       // Clip stage: Surface.composite(dst, mask, .dst_in) after the draws.
 #pragma unroll
       for (int i = 0; i < ROWS_PER_THREAD; i++) {
-        const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+        const int32_t py = ty0 + i;
         if (!col_ok || py >= H) continue;
         const I4 m{0, 0, 0, S.clip[(int64_t)py * W + px]};
         pix[i] = pack(op_int(Z2D_OP_DST_IN, unpack(pix[i], S.fmt), m), S.fmt);
@@ -876,10 +968,21 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     }
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
-      const int32_t py = ty0 + (int32_t)wave * ROWS_PER_THREAD + i;
+      const int32_t py = ty0 + i;
       if (col_ok && py < H && pix[i] != orig[i]) store_raw(S, (int64_t)py * W + px, pix[i]);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// k_grad_setup: per-stop interpolation colors of every gradient of a job.
+// ---------------------------------------------------------------------------
+__global__ void k_grad_setup(DevGradient *grads, uint32_t n_grads) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n_grads) return;
+  const DevGradient &g = grads[gi];
+  float4 *pre = const_cast<float4 *>(g.pre);
+  for (uint32_t i = 0; i < g.n_stops; i++) pre[i] = stop_interp_color(g, g.stops[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -894,7 +997,7 @@ __device__ __forceinline__ I4 comp_param_int(const DevCompParam &p, const CompJo
       if (idx < 0 || idx >= (int64_t)p.surf.w * p.surf.h) return I4{0, 0, 0, 0};
       return unpack(load_raw(p.surf, idx), p.surf.fmt);
     }
-    default: return paint_int(*p.paint, J.blue_noise, x, y);
+    default: return paint_int(*p.paint, paint_stops(*p.paint), J.blue_noise, x, y);
   }
 }
 __device__ __forceinline__ F4 comp_param_float(const DevCompParam &p, const CompJob &J,
@@ -906,7 +1009,7 @@ __device__ __forceinline__ F4 comp_param_float(const DevCompParam &p, const Comp
       if (idx < 0 || idx >= (int64_t)p.surf.w * p.surf.h) return F4{0, 0, 0, 0};
       return decode_raw(unpack(load_raw(p.surf, idx), p.surf.fmt));
     }
-    default: return paint_float(*p.paint, J.blue_noise, x, y);
+    default: return paint_float(*p.paint, paint_stops(*p.paint), J.blue_noise, x, y);
   }
 }
 
@@ -1013,6 +1116,12 @@ hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hi
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st) {
+  if (n_grads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grad_setup, dim3((n_grads + 63) / 64), dim3(64), 0, st, grads, n_grads);
+  return hipGetLastError();
 }
 
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
