@@ -772,12 +772,14 @@ __device__ __forceinline__ float bayer8(int x, int y) {
 }
 
 // fromDitherVec for one lane: linear straight-alpha f32.
-__device__ inline F4 dither_sample(const DevPaint &p, const StopView &sv,
+// g: the paint's gradient (p.gradient or a copy of it); read only for a
+// gradient source.
+__device__ inline F4 dither_sample(const DevPaint &p, const DevGradient *g, const StopView &sv,
                                    const uint16_t *blue_noise, int x, int y) {
   F4 c;
   if (p.dither_source == Z2D_DITHER_SRC_GRADIENT) {
     I4 unused;
-    gradient_sample(*p.gradient, sv, x, y, false, unused, c);
+    gradient_sample(*g, sv, x, y, false, unused, c);
   } else {
     c = p.dither_color;
   }
@@ -795,7 +797,23 @@ __device__ inline F4 dither_sample(const DevPaint &p, const StopView &sv,
             clampz(c.b + m * sc, 0.0f, 1.0f), clampz(c.a + m * sc, 0.0f, 1.0f)};
 }
 
-__device__ inline I4 paint_int(const DevPaint &p, const StopView &sv, const uint16_t *bn, int x,
+// Integer paint sample (premultiplied RGBA, 0-255 per channel); g is the
+// paint's gradient (p.gradient or a copy of it) when it has one.
+__device__ __forceinline__ I4 paint_int_g(const DevPaint &p, const DevGradient *g,
+                                          const StopView &sv, const uint16_t *bn, int x, int y) {
+  if (p.kind == Z2D_PATTERN_OPAQUE) return p.px;
+  if (p.kind == Z2D_PATTERN_GRADIENT) {
+    I4 o;
+    F4 unused;
+    gradient_sample(*g, sv, x, y, true, o, unused);
+    return o;
+  }
+  F4 f = dither_sample(p, g, sv, bn, x, y);
+  float v[4] = {f.r, f.g, f.b, f.a};
+  return encode_premul(v);
+}
+
+__device__ __noinline__ I4 paint_int(const DevPaint &p, const StopView &sv, const uint16_t *bn, int x,
                                int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return p.px;
   if (p.kind == Z2D_PATTERN_GRADIENT) {
@@ -804,7 +822,7 @@ __device__ inline I4 paint_int(const DevPaint &p, const StopView &sv, const uint
     gradient_sample(*p.gradient, sv, x, y, true, o, unused);
     return o;
   }
-  F4 f = dither_sample(p, sv, bn, x, y);
+  F4 f = dither_sample(p, p.gradient, sv, bn, x, y);
   float v[4] = {f.r, f.g, f.b, f.a};
   return encode_premul(v);
 }
@@ -817,7 +835,7 @@ __device__ inline F4 paint_float(const DevPaint &p, const StopView &sv, const ui
     gradient_sample(*p.gradient, sv, x, y, false, unused, o);
     return o;
   }
-  return dither_sample(p, sv, bn, x, y);
+  return dither_sample(p, p.gradient, sv, bn, x, y);
 }
 
 // The stops of a paint as stored with its gradient.

@@ -503,7 +503,21 @@ struct StripShared {
   uint32_t coff[64];   // exclusive offsets of active-row counts
   float soff[STAGED_STOPS];   // the draw's gradient stops, staged
   float4 spre[STAGED_STOPS];
+  uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples of the strip (RGBA8)
+  DevPaint paint;      // the draw's paint and gradient, staged
+  DevGradient grad;
 };
+
+// Loads a value every lane of the wave reads at the same address through the
+// scalar data cache (constant address space: read-only for the kernel).
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *p) {
+#if __HIP_DEVICE_COMPILE__
+  return *((const __attribute__((address_space(4))) T *)p);
+#else
+  return *p;  // host pass only parses device code
+#endif
+}
 
 // Orders this wave's LDS accesses before and after the call (lanes of one
 // wave exchanging data through LDS).
@@ -576,15 +590,11 @@ __device__ __forceinline__ uint32_t composite_cov(uint32_t raw, uint32_t fmt, in
 }
 
 // composite_cov for the common case, kept inline: src_over in integer
-// precision without a supersample mask (the same steps: DST_IN of the paint
-// by the coverage alpha, then SRC_OVER onto the pixel).
+// precision without a supersample mask, given the paint sample s (the same
+// steps: DST_IN of the paint by the coverage alpha, then SRC_OVER).
 __device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, int k,
-                                                       int kfull, const DevDraw &D,
-                                                       const DevPaint &P, const StopView &sv,
-                                                       const uint16_t *bn, int x, int y) {
-  if (k == 0) return raw;
-  if (k >= kfull && (D.flags & DRAW_FAST_SRC)) return pack(P.px, fmt);
-  I4 s = P.kind == Z2D_PATTERN_OPAQUE ? P.px : paint_int(P, sv, bn, x, y);
+                                                       int kfull, const DevDraw &D, I4 s) {
+  if (k >= kfull && (D.flags & DRAW_FAST_SRC)) return pack(s, fmt);
   if (k < kfull) {
     const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
     s = I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
@@ -662,7 +672,7 @@ __device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, in
         int32_t cutoff = INT32_MAX;
         if (CUT) {
           const int32_t rr = ys_base + i * 4 + pr * 2 + q - D.row_y0;
-          if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+          if (rr >= 0 && rr < D.row_n) cutoff = load_uniform(&R[rr]).cutoff;
         }
         int32_t w = base[q];
 #pragma unroll
@@ -701,16 +711,16 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     if (lane == 0) item = atomicAdd(cursor, 1u);
     item = __builtin_amdgcn_readfirstlane(item);
     if (item >= n_items) break;
-    const TileRef T = J.tiles[tile_base + item / STRIPS_PER_TILE];
+    const TileRef T = load_uniform(&J.tiles[tile_base + item / STRIPS_PER_TILE]);
     const uint32_t strip = item % STRIPS_PER_TILE;
-    const DevSurface S = J.surfaces[T.surface];
+    const DevSurface S = load_uniform(&J.surfaces[T.surface]);
     const int32_t W = S.w, H = S.h;
     const int32_t tx0 = (int32_t)T.tx * TILE_W;
     const int32_t ty0 = (int32_t)T.ty * TILE_H + (int32_t)strip * ROWS_PER_THREAD;
     if (ty0 >= H) continue;
-    const uint32_t band = J.surf_band_off[T.surface] + T.ty;
-    const uint32_t e0 = min(J.band_cnt[band], J.max_entries);
-    const uint32_t e1 = min(J.band_cnt[band + 1], J.max_entries);
+    const uint32_t band = load_uniform(&J.surf_band_off[T.surface]) + T.ty;
+    const uint32_t e0 = min(load_uniform(&J.band_cnt[band]), J.max_entries);
+    const uint32_t e1 = min(load_uniform(&J.band_cnt[band + 1]), J.max_entries);
     if (e0 == e1 && !S.clip) continue;
     const int32_t px = tx0 + (int32_t)lane;
     const bool col_ok = px < W;
@@ -726,15 +736,31 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     for (int i = 0; i < ROWS_PER_THREAD; i++) orig[i] = pix[i];
 
     for (uint32_t e = e0; e < e1; e++) {
-      const BandEntry E = J.entries[e];
-      const DevDraw &D = J.draws[E.draw];
+      const BandEntry E = load_uniform(&J.entries[e]);
+      const DevDraw D = load_uniform(&J.draws[E.draw]);
       // uniform: the draw's composite region misses this strip
       if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;
       if (D.ry1 <= ty0 || D.ry0 >= ty0 + ROWS_PER_THREAD) continue;
       const DevPaint &P = J.paints[D.paint];
       StopView sv{nullptr, nullptr};
+      if (P.kind != Z2D_PATTERN_OPAQUE) {
+        // the paint and its gradient in LDS for the samplers
+        wave_lds_sync();
+        constexpr uint32_t PW = sizeof(DevPaint) / 4, GW = sizeof(DevGradient) / 4;
+        static_assert(PW <= 64 && GW <= 128, "staging sizes");
+        if (lane < PW) reinterpret_cast<uint32_t *>(&sh.paint)[lane] =
+            reinterpret_cast<const uint32_t *>(&J.paints[D.paint])[lane];
+        if (P.gradient) {
+          const uint32_t *gs = reinterpret_cast<const uint32_t *>(P.gradient);
+          uint32_t *gd = reinterpret_cast<uint32_t *>(&sh.grad);
+          if (lane < GW) gd[lane] = gs[lane];
+          if (lane + 64 < GW) gd[lane + 64] = gs[lane + 64];
+          if (lane == 0) sh.paint.gradient = &sh.grad;
+        }
+        wave_lds_sync();
+      }
       if (P.gradient) {
-        const DevGradient &G = *P.gradient;
+        const DevGradient G = load_uniform(P.gradient);
         if (G.n_stops <= STAGED_STOPS) {
           wave_lds_sync();
           if (lane < G.n_stops) {
@@ -767,7 +793,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
           const int32_t py = ty0 + i;
           kk[i] = 0;
           if (!col_ok || py >= H) continue;
-          const RowRec rr = R[py - D.row_y0];
+          const RowRec rr = load_uniform(&R[py - D.row_y0]);
           if (rr.kind == 1) continue;
           if (rr.kind == 0 || px < rr.s || px >= rr.e) {
             pix[i] = pack(I4{0, 0, 0, 0}, S.fmt);
@@ -869,7 +895,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
               int32_t cutoff = INT32_MAX;
               if (R) {
                 const int32_t rr = ys_base + pr * 2 + q - D.row_y0;
-                if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+                if (rr >= 0 && rr < D.row_n) cutoff = load_uniform(&R[rr]).cutoff;
               }
               kk[pr * 2 + q] = (wind_in(w[q], eo) && xs_base + c_lo < cutoff) ? 1 : 0;
             }
@@ -900,7 +926,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
               int32_t cutoff = INT32_MAX;
               if (R) {
                 const int32_t rr = ys - D.row_y0;
-                if (rr >= 0 && rr < D.row_n) cutoff = R[rr].cutoff;
+                if (rr >= 0 && rr < D.row_n) cutoff = load_uniform(&R[rr]).cutoff;
               }
   #pragma unroll
               for (int j = 0; j < 4; j++)
@@ -915,15 +941,35 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
       uint32_t n_cov = 0;
       if (D.op == Z2D_OP_SRC_OVER && D.precision == Z2D_PRECISION_INTEGER &&
           D.mode != MODE_SSAA) {
+        uint32_t need = 0;  // covered pixels inside the composite region
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
           const int32_t py = ty0 + i;
-          if (!col_ok || py >= H) continue;
-          const bool in_region = px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1;
-          if (!in_region) continue;
-          n_cov += kk[i] > 0;
-          pix[i] = composite_src_over(pix[i], S.fmt, kk[i], SC * SC, D, P, sv, J.blue_noise,
-                                      px, py);
+          if (col_ok && py < H && px >= D.rx0 && px < D.rx1 && py >= D.ry0 && py < D.ry1 &&
+              kk[i] > 0)
+            need |= 1u << i;
+        }
+        const bool solid = P.kind == Z2D_PATTERN_OPAQUE;
+        if (!solid) {
+          // paint samples of the covered pixels: one evaluation site for all
+          // rows, kept in LDS (each lane reads back only its own)
+#pragma unroll 1
+          for (int i = 0; i < ROWS_PER_THREAD; i++) {
+            if (!((need >> i) & 1u)) continue;
+            const I4 c = paint_int(sh.paint, sv, J.blue_noise, px, ty0 + i);
+            sh.ppaint[i][lane] = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          if (!((need >> i) & 1u)) continue;
+          n_cov++;
+          I4 src = P.px;
+          if (!solid) {
+            const uint32_t pv = sh.ppaint[i][lane];
+            src = I4{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+          }
+          pix[i] = composite_src_over(pix[i], S.fmt, kk[i], SC * SC, D, src);
         }
       } else {
         // every other operator / precision / mode: one composite instance,
