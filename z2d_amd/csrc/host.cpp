@@ -869,6 +869,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t input_bytes = L.off;
   const size_t o_edges = L.take(b->nodes.size() * sizeof(z2d_edge));
   const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
+  const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
   const size_t o_eidx = L.take(max_entry_edges * sizeof(uint32_t));
@@ -927,6 +928,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.in_edges = reinterpret_cast<const z2d_edge *>(D + o_inedges);
   J.edges = reinterpret_cast<z2d_edge *>(D + o_edges);
   J.draws = reinterpret_cast<DevDraw *>(D + o_draws);
+  J.draw_rows = reinterpret_cast<int2 *>(D + o_drows);
   J.n_draws = nd;
   J.surfaces = reinterpret_cast<DevSurface *>(D + o_surfs);
   J.n_surfaces = ns;

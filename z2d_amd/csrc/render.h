@@ -101,6 +101,7 @@ struct JobView {
   const z2d_edge *in_edges;  // edges of polygon draws
   z2d_edge *edges;  // edge slots, indexed like nodes
   DevDraw *draws;
+  int2 *draw_rows;  // per draw: composite rows [y0, y1), empty when not visible
   uint32_t n_draws;
   DevSurface *surfaces;
   uint32_t n_surfaces;

@@ -124,6 +124,7 @@ __global__ void k_edge_build(JobView J) {
   const DevSurface S = J.surfaces[D.surface];
   const double scale = D.mode == MODE_DIRECT ? 1.0 : 4.0;
   z2d_edge *out = J.edges + D.node_off;
+  J.draw_rows[d] = make_int2(0, 0);  // not visible unless set below
   ExtAcc acc{0, 0.0, 0.0, 0.0, 0.0};
   uint32_t flags = D.flags & (DRAW_UNBALANCED | DRAW_ROWPROG);
   if (D.input_edges) {
@@ -133,34 +134,48 @@ __global__ void k_edge_build(JobView J) {
     acc.left = D.in_ext[2]; acc.right = D.in_ext[3];
   } else {
     // fill_plotter.plot (src/internal/tess/fill_plotter.zig:21-97) with the
-    // PointBuffer(1, 3) of src/internal/tess/point_buffer.zig.
-    double bx[3], by[3];
+    // PointBuffer(1, 3) of src/internal/tess/point_buffer.zig.  Only the
+    // buffer's first point, its last point and its fill count (capped at 3)
+    // are ever read, so those are the whole state.  Nodes are fetched in
+    // groups so their loads overlap.
+    double fx = 0.0, fy = 0.0, lx = 0.0, ly = 0.0;
     int len = 0;
     const FNode *nodes = J.nodes + D.node_off;
     const uint32_t n = D.node_cnt;
-    for (uint32_t i = 0; i < n; i++) {
-      FNode nd = nodes[i];
-      if (nd.tag == Z2D_NODE_MOVE_TO) {
-        if (i == n - 1) break;
-        len = 0;
-        bx[0] = nd.x; by[0] = nd.y; len = 1;
-      } else if (nd.tag == Z2D_NODE_LINE_TO) {
-        if (len == 0) continue;  // InvalidState is rejected on the host
-        double lx = bx[len - 1], ly = by[len - 1];
-        if (!pt_eq(lx, ly, nd.x, nd.y)) {
-          add_edge(out, acc, scale, lx, ly, nd.x, nd.y);
-          if (len < 3) {
-            bx[len] = nd.x; by[len] = nd.y; len++;
-          } else {
-            bx[1] = bx[2]; by[1] = by[2]; bx[2] = nd.x; by[2] = nd.y;
+    constexpr uint32_t GROUP = 8;
+    bool done = false;
+    for (uint32_t base = 0; base < n && !done; base += GROUP) {
+      FNode buf[GROUP];
+#pragma unroll
+      for (uint32_t j = 0; j < GROUP; j++)
+        if (base + j < n) buf[j] = nodes[base + j];
+#pragma unroll
+      for (uint32_t j = 0; j < GROUP; j++) {
+        const uint32_t i = base + j;
+        if (i >= n || done) break;
+        const FNode nd = buf[j];
+        if (nd.tag == Z2D_NODE_MOVE_TO) {
+          if (i == n - 1) {
+            done = true;
+            break;
           }
-        }
-      } else if (nd.tag == Z2D_NODE_CLOSE_PATH) {
-        if (len >= 3) {
-          double lx = bx[len - 1], ly = by[len - 1];
-          if (pt_eq(lx, ly, bx[0], by[0])) continue;
-          add_edge(out, acc, scale, lx, ly, bx[0], by[0]);
-          bx[1] = bx[2]; by[1] = by[2]; bx[2] = bx[0]; by[2] = by[0];
+          fx = lx = nd.x;
+          fy = ly = nd.y;
+          len = 1;
+        } else if (nd.tag == Z2D_NODE_LINE_TO) {
+          if (len == 0) continue;  // InvalidState is rejected on the host
+          if (!pt_eq(lx, ly, nd.x, nd.y)) {
+            add_edge(out, acc, scale, lx, ly, nd.x, nd.y);
+            lx = nd.x;
+            ly = nd.y;
+            if (len < 3) len++;
+          }
+        } else if (nd.tag == Z2D_NODE_CLOSE_PATH) {
+          if (len >= 3 && !pt_eq(lx, ly, fx, fy)) {
+            add_edge(out, acc, scale, lx, ly, fx, fy);
+            lx = fx;
+            ly = fy;
+          }
         }
       }
     }
@@ -241,6 +256,7 @@ __global__ void k_edge_build(JobView J) {
     D.row_n = ey - sy + 1;
   }
   D.flags = flags | DRAW_VISIBLE;
+  J.draw_rows[d] = make_int2(D.ry0, D.ry1);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,8 +383,8 @@ __global__ void k_band_pass(JobView J) {
   for (uint32_t d = d0; d < d1; d += 64) {
     bool hit = false;
     if (d + lane < d1) {
-      const DevDraw &D = J.draws[d + lane];
-      hit = (D.flags & DRAW_VISIBLE) && D.ry0 < y1 && D.ry1 > y0;
+      const int2 r = J.draw_rows[d + lane];
+      hit = r.x < y1 && r.y > y0;
     }
     uint64_t m = __ballot(hit);
     if (FILL && hit) {
