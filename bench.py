@@ -115,40 +115,64 @@ def bench_star_latency(gpu, ctx, iters=300):
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
-def bench_compositor(gpu, ctx, ops=("src_over", "multiply", "soft_light"), reps=5):
-    w = h = 4096
-    src, dst, _mask = scenes.compositor_inputs(w, h)
+def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3):
+    """Compositor sweep at 4096x4096 RGBA8 (seed 0xC2): every operator in
+    enum order out-of-place, out = op(src, dst), then the alpha8-mask variant
+    {dst_in dst=src src=mask}, {op}.  GB/s counts 12 B/px (13 with the mask):
+    src and dst read once, out written once."""
+    src, dst, mask = scenes.compositor_inputs(w, h)
     s_src = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
     s_dst = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
     s_out = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+    s_mask = gpu.DeviceSurface(ctx, abi.ALPHA8, w, h)
     s_src.upload(src)
     s_dst.upload(dst)
-    d_src, d_dst, d_out = s_src.desc(), s_dst.desc(), s_out.desc()
-    out = {}
-    for name in ops:
-        o = abi.CompOp()
-        o.op = abi.OP[name]
-        o.src.kind = abi.PARAM_SURFACE
-        o.src.surface = C.pointer(d_src)
-        arr = (abi.CompOp * 1)(o)
-        f = gpu.lib().z2d_gpu_composite
-        f(ctx.h, C.byref(d_dst), 0, 0, arr, 1, 0, C.byref(d_out))
-        t = time.perf_counter()
-        for _ in range(reps):
-            rc = f(ctx.h, C.byref(d_dst), 0, 0, arr, 1, 0, C.byref(d_out))
-            if rc:
-                raise RuntimeError(f"composite rc={rc}")
-        dt = (time.perf_counter() - t) / reps
-        out[name] = round(w * h * 12 / dt / 1e9, 1)
-    for s in (s_src, s_dst, s_out):
+    s_mask.upload(mask)
+    d_src, d_dst, d_out, d_mask = s_src.desc(), s_dst.desc(), s_out.desc(), s_mask.desc()
+    f = gpu.lib().z2d_gpu_composite
+    per_op, per_op_mask = {}, {}
+    tot_bytes = tot_time = 0.0
+    for masked in (False, True):
+        for name in abi.OPERATORS:
+            if masked:
+                m = abi.CompOp()
+                m.op = abi.OP["dst_in"]
+                m.dst.kind = abi.PARAM_SURFACE
+                m.dst.surface = C.pointer(d_src)
+                m.src.kind = abi.PARAM_SURFACE
+                m.src.surface = C.pointer(d_mask)
+                o = abi.CompOp()
+                o.op = abi.OP[name]
+                arr = (abi.CompOp * 2)(m, o)
+            else:
+                o = abi.CompOp()
+                o.op = abi.OP[name]
+                o.src.kind = abi.PARAM_SURFACE
+                o.src.surface = C.pointer(d_src)
+                arr = (abi.CompOp * 1)(o)
+            n = len(arr)
+            ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
+            t = time.perf_counter()
+            for _ in range(reps):
+                ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
+            dt = (time.perf_counter() - t) / reps
+            nbytes = w * h * (13 if masked else 12)
+            (per_op_mask if masked else per_op)[name] = round(nbytes / dt / 1e9, 1)
+            tot_bytes += nbytes
+            tot_time += dt
+    for s in (s_src, s_dst, s_out, s_mask):
         s.close()
-    return out
+    return {"GBps_all_passes": round(tot_bytes / tot_time / 1e9, 1), "per_op_GBps": per_op,
+            "mask_per_op_GBps": per_op_mask, "passes": 2 * len(abi.OPERATORS),
+            "note": "host-timed per call (upload of the op list, launch, sync)"}
 
 
-def bench_uneven_batch(gpu, ctx, n_surfaces=512, seed=0xC4):
-    """Uneven fill-only surface mix (sides log-uniform 16-1024 px, 80% RGBA8
-    / 20% alpha8, 1-50 paths each): one persistent launch vs one launch per
-    surface over the same prepared batch."""
+def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
+    """Batched form: 4096 surfaces (sides log-uniform 16-1024 px, 80% RGBA8 /
+    20% alpha8, cleared to 0, 1-50 paths each of the 4K scene's
+    distribution), filled only: stroke tessellation is not built yet, so the
+    30% stroke share is drawn as fills.  One persistent launch vs one launch
+    per surface over the same prepared batch."""
     rng = scenes.SplitMix64(seed)
     b = gpu.Batch(ctx)
     surfs = []
@@ -177,14 +201,16 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=512, seed=0xC4):
         res[name] = round(pix / dt / 1e6, 1)
     res["speedup"] = round(res["batched"] / res["per_surface"], 2)
     res["surfaces"] = n_surfaces
+    res["pixel_draws"] = int(pix)
+    res["strokes"] = "drawn as fills (stroke tessellation not built)"
     b.close()
     return res
 
 
-def serving_requests(n, seed=0x5EF):
+def serving_requests(gpu, n, seed=0xC5):
     """Request mix of the serving form: one surface job each, side
-    log-uniform 16-1024 px, 80% RGBA8 / 20% alpha8, 1-50 paths of the 4K
-    scene's distribution, 30% with an alpha8 clip mask."""
+    log-uniform 16-1024 px, 80% RGBA8 / 20% alpha8 cleared to 0, 1-50 paths
+    of the 4K scene's distribution, half with an alpha8 clip mask."""
     rng = scenes.SplitMix64(seed)
     reqs = []
     for _ in range(n):
@@ -193,23 +219,23 @@ def serving_requests(n, seed=0x5EF):
         draws, keep = scenes.random_paths(rng, rng.randint(1, 50), side, side,
                                           max_side=min(512.0, side))
         clip = None
-        if rng.uniform() < 0.3:
+        if rng.uniform() < 0.5:
             clip = scenes.splitmix_stream(rng.next(), (side * side + 7) // 8).view(np.uint8)
             clip = clip[: side * side].copy()
-        reqs.append(dict(fmt=fmt, w=side, h=side, draws=draws, keep=keep, clip=clip,
+        prep = gpu.Server.prepare(fmt, side, side, draws, clear=abi.pixel(fmt), clip=clip)
+        reqs.append(dict(prep=prep, keep=keep,
                          out=np.empty(abi.surface_nbytes(fmt, side, side), np.uint8)))
     return reqs
 
 
-def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000), seconds=2.0, seed=0x9015):
+def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2.0,
+                  seed=0x9015, budget=16 << 30):
     """Open-loop Poisson arrivals at each QPS level, cycling through the
     request pool (and one saturation run with twice the pool queued at
     t=0).  Latency = completion - scheduled
     arrival; throughput = pixel-draws counted on the device / wall time."""
     import threading
-    srv = gpu.Server(ctx, 2 << 30, 64)
-    white = abi.pixel(abi.RGBA, 255, 255, 255, 255)
-    a_clear = abi.pixel(abi.ALPHA8, a=0)
+    srv = gpu.Server(ctx, budget, 64)
     rng = np.random.default_rng(seed)
     out = {}
     levels = list(qps_levels) + ["saturation"]
@@ -235,9 +261,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000), seconds=2.0, see
             rq = reqs[i % len(reqs)]
             while time.perf_counter() - t0 < sched[i]:
                 pass
-            rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], rq["draws"],
-                                 clear=white if rq["fmt"] == abi.RGBA else a_clear,
-                                 clip=rq["clip"], out=rq["out"])
+            rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
             if rc:
                 raise RuntimeError(f"server submit rc={rc}")
             ids.append(rid)
@@ -254,7 +278,10 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000), seconds=2.0, see
                              p99_ms=round(float(np.percentile(lat, 99)), 3),
                              batches=int(st1["batches"] - st0["batches"]))
         log(f"serving {lvl}: {out[str(lvl)]}")
-    out["peak_hbm_MiB"] = round(srv.stats()["peak_bytes"] / 2**20, 1)
+    st = srv.stats()
+    out["peak_hbm_MiB"] = round(st["peak_bytes"] / 2**20, 1)
+    out["hbm_budget_GiB"] = budget / 2**30
+    out["rejected"] = int(st["rejected"])
     srv.close()
     return out
 
@@ -335,6 +362,13 @@ def main():
                      "kernel_ms": round(r["render_ms"], 4),
                      "pipeline_ms": round(r["pipe_ms"], 4)},
     }
+    tj = os.path.join(ROOT, "profiles", "k_render_scene_3840x2160_10k.traffic.json")
+    if os.path.exists(tj) and args.paths == 10000:
+        # HBM bytes per k_render launch from rocprofv3 FETCH_SIZE/WRITE_SIZE
+        # runs of this workload, scaled by the stream-copy calibration
+        with open(tj) as fh:
+            line["roofline"]["traffic"] = json.load(fh)["traffic_bytes_per_launch"]
+        line["roofline"]["traffic_source"] = "profiles/k_render_scene_3840x2160_10k.md"
     if not args.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(gpu, ctx, sc, min(args.cpu_paths, args.paths))
@@ -348,7 +382,7 @@ def main():
             "star_512_latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
             "compositor_4096_GBps": bench_compositor(gpu, ctx),
             "uneven_fill_batch_Mpix_s": bench_uneven_batch(gpu, ctx),
-            "serving": bench_serving(gpu, ctx, serving_requests(args.serve_requests)),
+            "serving": bench_serving(gpu, ctx, serving_requests(gpu, args.serve_requests)),
         }
     print(json.dumps(line), flush=True)
 

@@ -217,10 +217,8 @@ def dither_pattern(dtype, scale, px=None, col=None, gradient=None):
 def nodes_array(nodes):
     """nodes: list of (tag, v...) tuples -> numpy structured array."""
     arr = np.zeros(len(nodes), dtype=NODE_DTYPE)
-    for i, nd in enumerate(nodes):
-        arr[i]["tag"] = nd[0]
-        vals = list(nd[1:]) + [0.0] * (6 - len(nd[1:]))
-        arr[i]["v"] = vals
+    arr["tag"] = [nd[0] for nd in nodes]
+    arr["v"] = [tuple(nd[1:]) + (0.0,) * (7 - len(nd)) for nd in nodes]
     return arr
 
 

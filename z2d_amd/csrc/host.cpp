@@ -774,6 +774,11 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
   hd.d.node_cnt = (uint32_t)(b->nodes.size() - before);
   hd.edge_pairs = node_edge_pairs(b->nodes.data() + before, hd.d.node_cnt,
                                   (uint32_t)((s->h + TILE_H - 1) / TILE_H));
+  for (size_t i = before + 1; i < b->nodes.size(); i++)
+    if (b->nodes[i].tag == Z2D_NODE_LINE_TO && b->nodes[i - 1].tag == Z2D_NODE_CLOSE_PATH) {
+      hd.d.flags |= DRAW_SERIAL_PLOT;
+      break;
+    }
   if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
   return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
                           o->precision, hd, ymin, ymax);

@@ -50,6 +50,7 @@ enum DrawFlags : uint32_t {
   DRAW_UNBALANCED = 1u << 3,  // some contour does not return to its start
   DRAW_ROWPROG = 1u << 4,     // needs the serial row emulation
   DRAW_HAS_EDGES = 1u << 5,
+  DRAW_SERIAL_PLOT = 1u << 6,  // a line follows a close: plot node by node
 };
 
 enum DrawMode : uint32_t { MODE_DIRECT = 0, MODE_MSAA = 1, MODE_SSAA = 2 };

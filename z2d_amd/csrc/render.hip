@@ -65,8 +65,20 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
+// 16-lane rows, then row broadcasts across rows).
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return x;
+}
+
 // ---------------------------------------------------------------------------
-// k_edge_build: one thread per draw.
+// k_edge_build: one wave per draw.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool pt_eq(double ax, double ay, double bx, double by) {
   return ax == bx && ay == by;
@@ -117,65 +129,172 @@ __device__ bool in_box(const double *ext, double scale, int32_t w, int32_t h) {
   return true;
 }
 
-__global__ void k_edge_build(JobView J) {
-  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+// Inclusive max over the lanes of a wave up to this one.
+__device__ __forceinline__ int32_t wave_max_scan(int32_t v) {
+  const int32_t lane = (int32_t)(threadIdx.x & 63);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v = max(v, t);
+  }
+  return v;
+}
+
+// fill_plotter.plot (src/internal/tess/fill_plotter.zig:21-97) with the
+// PointBuffer(1, 3) of src/internal/tess/point_buffer.zig, one lane per
+// node.  Only the buffer's first point, last point and fill count (capped
+// at 3) are ever read.  The last point before a node is the previous node's
+// point when that node is a move or a line (a line to an equal point leaves
+// it equal); the host sends draws with a line right after a close to the
+// serial form below.  A line emits when it moves; a close emits when it
+// starts a run of closes, at least two lines moved since the subpath's
+// move, and the last point is not the first.  Edge slots are a wave scan of
+// the emit flags; the extents are the running min/max, with the first edge
+// seeding them as in Polygon.addEdge (a NaN there sticks).
+__device__ void plot_parallel(const FNode *nodes, uint32_t n, double scale, z2d_edge *out,
+                              ExtAcc &acc) {
+  const uint32_t lane = threadIdx.x & 63;
+  const double inf = __builtin_huge_val();
+  double c_fx = 0.0, c_fy = 0.0, c_px = 0.0, c_py = 0.0;
+  uint32_t c_ptag = 0xFFFFFFFFu;
+  bool c_have = false;
+  int32_t c_cnt = 0;
+  uint32_t ebase = 0;
+  double m_top = inf, m_bot = -inf, m_left = inf, m_right = -inf;
+  double f_top = 0.0, f_bot = 0.0, f_left = 0.0, f_right = 0.0;
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t k = base + lane;
+    FNode nd{0.0, 0.0, 0xFFFFFFFFu, 0u};
+    if (k < n) nd = nodes[k];
+    uint32_t tag = nd.tag;
+    if (tag == Z2D_NODE_MOVE_TO && k == n - 1) tag = 0xFFFFFFFFu;  // a final move is ignored
+    double px = __shfl_up(nd.x, 1, 64), py = __shfl_up(nd.y, 1, 64);
+    uint32_t ptag = __shfl_up(tag, 1, 64);
+    if (lane == 0) {
+      px = c_px;
+      py = c_py;
+      ptag = c_ptag;
+    }
+    const int32_t head = wave_max_scan(tag == Z2D_NODE_MOVE_TO ? (int32_t)lane : -1);
+    const int32_t hsrc = head >= 0 ? head : 0;
+    const double hx = __shfl(nd.x, hsrc, 64), hy = __shfl(nd.y, hsrc, 64);
+    const double fx = head >= 0 ? hx : c_fx, fy = head >= 0 ? hy : c_fy;
+    const bool have = head >= 0 || c_have;
+    const bool line_moves = tag == Z2D_NODE_LINE_TO && have && !pt_eq(px, py, nd.x, nd.y);
+    const int32_t a = line_moves ? 1 : 0;
+    const int32_t ps = wave_incl_scan(a);
+    const int32_t ps_head = __shfl(ps, hsrc, 64);
+    const int32_t cnt_before = head >= 0 ? ps - a - ps_head : c_cnt + ps - a;
+    const bool close_emits = tag == Z2D_NODE_CLOSE_PATH && have &&
+                             ptag != Z2D_NODE_CLOSE_PATH && cnt_before >= 2 &&
+                             !pt_eq(px, py, fx, fy);
+    const double tx = line_moves ? nd.x : fx, ty = line_moves ? nd.y : fy;
+    // Polygon.addEdge (src/internal/tess/Polygon.zig:61-109)
+    const double ax = px * scale, ay = py * scale, bx = tx * scale, by = ty * scale;
+    const bool emit = (line_moves || close_emits) && (ay < by || ay > by);
+    const uint64_t m = __ballot(emit);
+    double top = 0.0, bot = 0.0, left = 0.0, right = 0.0;
+    if (emit) {
+      z2d_edge e;
+      e.y0 = ay;
+      e.y1 = by;
+      if (ay < by) {
+        e.x_start = ax;
+        e.x_inc = (bx - ax) / (by - ay);
+      } else {
+        e.x_start = bx;
+        e.x_inc = (ax - bx) / (ay - by);
+      }
+      top = e.y0 < e.y1 ? e.y0 : e.y1;
+      bot = e.y0 < e.y1 ? e.y1 : e.y0;
+      left = ax < bx ? ax : bx;
+      right = ax < bx ? bx : ax;
+      out[ebase + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = e;
+      m_top = fmin(m_top, top);
+      m_bot = fmax(m_bot, bot);
+      m_left = fmin(m_left, left);
+      m_right = fmax(m_right, right);
+    }
+    if (ebase == 0 && m != 0) {
+      const int fl = __ffsll((unsigned long long)m) - 1;
+      f_top = __shfl(top, fl, 64);
+      f_bot = __shfl(bot, fl, 64);
+      f_left = __shfl(left, fl, 64);
+      f_right = __shfl(right, fl, 64);
+    }
+    ebase += (uint32_t)__popcll(m);
+    c_px = __shfl(nd.x, 63, 64);
+    c_py = __shfl(nd.y, 63, 64);
+    c_ptag = __shfl(tag, 63, 64);
+    c_fx = __shfl(fx, 63, 64);
+    c_fy = __shfl(fy, 63, 64);
+    c_have = __shfl(have ? 1 : 0, 63, 64) != 0;
+    c_cnt = __shfl(cnt_before + a, 63, 64);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    m_top = fmin(m_top, __shfl_xor(m_top, off, 64));
+    m_bot = fmax(m_bot, __shfl_xor(m_bot, off, 64));
+    m_left = fmin(m_left, __shfl_xor(m_left, off, 64));
+    m_right = fmax(m_right, __shfl_xor(m_right, off, 64));
+  }
+  acc.n = ebase;
+  if (ebase) {
+    acc.top = f_top != f_top ? f_top : m_top;
+    acc.bottom = f_bot != f_bot ? f_bot : m_bot;
+    acc.left = f_left != f_left ? f_left : m_left;
+    acc.right = f_right != f_right ? f_right : m_right;
+  } else {
+    acc.top = acc.bottom = acc.left = acc.right = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_edge_build(JobView J) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t d = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (d >= J.n_draws) return;
   DevDraw &D = J.draws[d];
   const DevSurface S = J.surfaces[D.surface];
   const double scale = D.mode == MODE_DIRECT ? 1.0 : 4.0;
   z2d_edge *out = J.edges + D.node_off;
-  J.draw_rows[d] = make_int2(0, 0);  // not visible unless set below
+  uint32_t flags = D.flags & (DRAW_UNBALANCED | DRAW_ROWPROG | DRAW_SERIAL_PLOT);
   ExtAcc acc{0, 0.0, 0.0, 0.0, 0.0};
-  uint32_t flags = D.flags & (DRAW_UNBALANCED | DRAW_ROWPROG);
   if (D.input_edges) {
-    for (uint32_t k = 0; k < D.node_cnt; k++) out[k] = J.in_edges[D.in_edge_off + k];
+    for (uint32_t k = lane; k < D.node_cnt; k += 64) out[k] = J.in_edges[D.in_edge_off + k];
     acc.n = D.node_cnt;
     acc.top = D.in_ext[0]; acc.bottom = D.in_ext[1];
     acc.left = D.in_ext[2]; acc.right = D.in_ext[3];
-  } else {
-    // fill_plotter.plot (src/internal/tess/fill_plotter.zig:21-97) with the
-    // PointBuffer(1, 3) of src/internal/tess/point_buffer.zig.  Only the
-    // buffer's first point, its last point and its fill count (capped at 3)
-    // are ever read, so those are the whole state.  Nodes are fetched in
-    // groups so their loads overlap.
+  } else if (!(flags & DRAW_SERIAL_PLOT)) {
+    plot_parallel(J.nodes + D.node_off, D.node_cnt, scale, out, acc);
+  }
+  if (lane != 0) return;
+  J.draw_rows[d] = make_int2(0, 0);  // not visible unless set below
+  if (!D.input_edges && (flags & DRAW_SERIAL_PLOT)) {
+    // The same state machine node by node (a line may follow a close).
     double fx = 0.0, fy = 0.0, lx = 0.0, ly = 0.0;
     int len = 0;
     const FNode *nodes = J.nodes + D.node_off;
     const uint32_t n = D.node_cnt;
-    constexpr uint32_t GROUP = 8;
-    bool done = false;
-    for (uint32_t base = 0; base < n && !done; base += GROUP) {
-      FNode buf[GROUP];
-#pragma unroll
-      for (uint32_t j = 0; j < GROUP; j++)
-        if (base + j < n) buf[j] = nodes[base + j];
-#pragma unroll
-      for (uint32_t j = 0; j < GROUP; j++) {
-        const uint32_t i = base + j;
-        if (i >= n || done) break;
-        const FNode nd = buf[j];
-        if (nd.tag == Z2D_NODE_MOVE_TO) {
-          if (i == n - 1) {
-            done = true;
-            break;
-          }
-          fx = lx = nd.x;
-          fy = ly = nd.y;
-          len = 1;
-        } else if (nd.tag == Z2D_NODE_LINE_TO) {
-          if (len == 0) continue;  // InvalidState is rejected on the host
-          if (!pt_eq(lx, ly, nd.x, nd.y)) {
-            add_edge(out, acc, scale, lx, ly, nd.x, nd.y);
-            lx = nd.x;
-            ly = nd.y;
-            if (len < 3) len++;
-          }
-        } else if (nd.tag == Z2D_NODE_CLOSE_PATH) {
-          if (len >= 3 && !pt_eq(lx, ly, fx, fy)) {
-            add_edge(out, acc, scale, lx, ly, fx, fy);
-            lx = fx;
-            ly = fy;
-          }
+    for (uint32_t i = 0; i < n; i++) {
+      const FNode nd = nodes[i];
+      if (nd.tag == Z2D_NODE_MOVE_TO) {
+        if (i == n - 1) break;
+        fx = lx = nd.x;
+        fy = ly = nd.y;
+        len = 1;
+      } else if (nd.tag == Z2D_NODE_LINE_TO) {
+        if (len == 0) continue;  // InvalidState is rejected on the host
+        if (!pt_eq(lx, ly, nd.x, nd.y)) {
+          add_edge(out, acc, scale, lx, ly, nd.x, nd.y);
+          lx = nd.x;
+          ly = nd.y;
+          if (len < 3) len++;
+        }
+      } else if (nd.tag == Z2D_NODE_CLOSE_PATH) {
+        if (len >= 3 && !pt_eq(lx, ly, fx, fy)) {
+          add_edge(out, acc, scale, lx, ly, fx, fy);
+          lx = fx;
+          ly = fy;
         }
       }
     }
@@ -543,18 +662,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
-// 16-lane rows, then row broadcasts across rows).
-__device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
-  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
-  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
-  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
-  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
-  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
-  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
-  return x;
-}
-
 // Splits a word of two int16 deltas accumulated with 32-bit adds (the low
 // half's borrows land in the high half and are undone here).
 __device__ __forceinline__ void unpack_pair(uint32_t w, int32_t &lo, int32_t &hi) {
@@ -709,7 +816,7 @@ __device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, in
 // Persistent strip workers: each wave takes (tile, strip) items from the
 // queue and renders every draw of the tile's band over its ROWS_PER_THREAD x
 // TILE_W pixels, which stay in registers from the first draw to the last.
-__global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t tile_base,
+__global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_t tile_base,
                                                            uint32_t tile_count,
                                                            uint32_t *cursor) {
   __shared__ StripShared shw[RENDER_THREADS / 64];
@@ -722,11 +829,26 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
     (&sh.acc[0][0])[i] = 0u;
   wave_lds_sync();
 
+  uint32_t stat_pix = 0;  // work counters (count_stats), per lane
+  uint32_t stat_entries = 0, stat_unpacked = 0;
   for (;;) {
     uint32_t item = 0;
     if (lane == 0) item = atomicAdd(cursor, 1u);
     item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= n_items) break;
+    if (item >= n_items) {
+      if (J.count_stats) {
+        // one set of atomics per wave (per-entry atomics on one address
+        // serialize in L2)
+        unsigned long long wpix = stat_pix;
+        for (int off = 32; off > 0; off >>= 1) wpix += __shfl_xor(wpix, off, 64);
+        if (lane == 0) {
+          if (wpix) atomicAdd(reinterpret_cast<unsigned long long *>(&J.counters[4]), wpix);
+          if (stat_entries) atomicAdd(&J.counters[7], stat_entries);
+          if (stat_unpacked) atomicAdd(&J.counters[8], stat_unpacked);
+        }
+      }
+      break;
+    }
     const TileRef T = load_uniform(&J.tiles[tile_base + item / STRIPS_PER_TILE]);
     const uint32_t strip = item % STRIPS_PER_TILE;
     const DevSurface S = load_uniform(&J.surfaces[T.surface]);
@@ -796,7 +918,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
       const int32_t cols = TILE_W * SC;
       const bool direct_unb = D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED);
 
-      if (J.count_stats && lane == 0) atomicAdd(&J.counters[7], 1u);
+      if (J.count_stats) stat_entries++;
       int kk[ROWS_PER_THREAD];
       if (direct_unb) {
         // Row records give the whole result of the reference's quirky
@@ -919,7 +1041,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
         } else {
           // more band edges than the packed deltas can hold: each lane
           // evaluates the crossings of its own sample rows
-          if (J.count_stats && lane == 0) atomicAdd(&J.counters[8], 1u);
+          if (J.count_stats) stat_unpacked++;
   #pragma unroll
           for (int i = 0; i < ROWS_PER_THREAD; i++) {
             int k = 0;
@@ -1009,13 +1131,7 @@ __global__ void __launch_bounds__(RENDER_THREADS) k_render(JobView J, uint32_t t
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) pix[i] = tp[i];
       }
-      if (J.count_stats) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) n_cov += __shfl_xor(n_cov, off, 64);
-        if (lane == 0 && n_cov)
-          atomicAdd(reinterpret_cast<unsigned long long *>(&J.counters[4]),
-                    (unsigned long long)n_cov);
-      }
+      if (J.count_stats) stat_pix += n_cov;  // per lane
     }
 
     if (S.clip) {
@@ -1194,8 +1310,8 @@ hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t 
 
 hipError_t z2d_launch_edge_build(const JobView &J, hipStream_t st) {
   if (J.n_draws == 0) return hipSuccess;
-  dim3 grid((J.n_draws + 63) / 64);
-  hipLaunchKernelGGL(k_edge_build, grid, dim3(64), 0, st, J);
+  dim3 grid((J.n_draws + 3) / 4);  // one wave per draw
+  hipLaunchKernelGGL(k_edge_build, grid, dim3(256), 0, st, J);
   return hipGetLastError();
 }
 

@@ -254,9 +254,10 @@ class Server:
         self.h = h
         self._pending = {}
 
-    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None):
-        """fills: list of (pattern, nodes, opts).  Returns (rc, id); `out`
-        (numpy uint8, >= surface bytes) receives the result."""
+    @staticmethod
+    def prepare(fmt, width, height, fills, clear=None, clip=None):
+        """Builds the request descriptor once (it may be submitted many
+        times).  fills: list of (pattern, nodes, opts)."""
         items = (abi.FillItem * max(1, len(fills)))()
         for i, (pat, nodes, opts) in enumerate(fills):
             items[i].pattern = C.pointer(pat)
@@ -269,13 +270,22 @@ class Server:
         rq.clip_mask = clip.ctypes.data if clip is not None else None
         rq.fills = items
         rq.n_fills = len(fills)
+        return (rq, items, fills, clip)  # the descriptor and what it points into
+
+    def submit_prepared(self, prep, out=None):
+        """Returns (rc, id); `out` (numpy uint8, >= surface bytes) receives
+        the result and must stay alive until the request completes."""
         rid = C.c_uint64()
-        rc = lib().z2d_gpu_server_submit(self.h, C.byref(rq),
+        rc = lib().z2d_gpu_server_submit(self.h, C.byref(prep[0]),
                                          out.ctypes.data if out is not None else None,
                                          out.nbytes if out is not None else 0, C.byref(rid))
         if rc == 0:
             self._pending[rid.value] = out  # keeps the result buffer alive
         return rc, rid.value
+
+    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None):
+        """fills: list of (pattern, nodes, opts).  Returns (rc, id)."""
+        return self.submit_prepared(self.prepare(fmt, width, height, fills, clear, clip), out)
 
     def wait(self, rid, timeout_ms=60000.0):
         st = C.c_int32()

@@ -20,15 +20,28 @@ MASK64 = (1 << 64) - 1
 
 
 class SplitMix64:
+    """SplitMix64 (state += 0x9E3779B97F4A7C15, then the two xor-shift
+    multiplies).  Outputs are produced in vectorised blocks; the sequence is
+    the scalar one."""
+
+    BLOCK = 4096
+
     def __init__(self, seed):
         self.s = seed & MASK64
+        self._buf = []
+        self._pos = 0
+
+    def _refill(self):
+        self._buf = splitmix_stream(self.s, self.BLOCK).tolist()
+        self._pos = 0
+        self.s = (self.s + self.BLOCK * 0x9E3779B97F4A7C15) & MASK64
 
     def next(self):
-        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
-        z = self.s
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
-        return z ^ (z >> 31)
+        if self._pos >= len(self._buf):
+            self._refill()
+        v = self._buf[self._pos]
+        self._pos += 1
+        return v
 
     def uniform(self, a=0.0, b=1.0):
         u = (self.next() >> 11) * (1.0 / (1 << 53))
@@ -40,9 +53,7 @@ class SplitMix64:
 
     def u8_array(self, n):
         """n uniform bytes (one u64 draw per 8 bytes, little-endian)."""
-        words = np.empty((n + 7) // 8, dtype=np.uint64)
-        for i in range(len(words)):
-            words[i] = self.next()
+        words = np.array([self.next() for _ in range((n + 7) // 8)], dtype=np.uint64)
         return words.view(np.uint8)[:n].copy()
 
 
