@@ -66,9 +66,11 @@ def count_work(b):
     b.counters(True)
     b.render(0)
     b.wait()
-    pix, edges, _, overflow = b.counters(False)[:4]
+    c = b.counters(False)
+    pix, edges, overflow = c[0], c[1], c[3]
     if overflow:
         raise RuntimeError("bin overflow while counting")
+    b.last_counts = c
     return pix, edges
 
 
@@ -92,7 +94,8 @@ def bench_scene(gpu, ctx, sc, steps, warmup):
     pbytes = sum(paint_bytes(d[0]) for d in sc["draws"])
     alg_bytes = 2 * 4 * pix + 32 * edges + pbytes
     res = dict(pix=pix, edges=edges, wall=wall, render_ms=float(np.mean(render_ms)),
-               pipe_ms=float(np.mean(pipe_ms)), alg_bytes=alg_bytes)
+               pipe_ms=float(np.mean(pipe_ms)), alg_bytes=alg_bytes,
+               strip_entries=int(b.last_counts[2]))
     b.close()
     surf.close()
     return res
@@ -355,7 +358,8 @@ def main():
         "config": {"workload": "scene_3840x2160_10k_paths", "width": 3840, "height": 2160,
                    "paths": args.paths, "seed": "0xC3", "aa": "multisample_4x",
                    "op": "src_over", "pixel_draws_per_step": int(r["pix"]),
-                   "edges_per_step": int(r["edges"])},
+                   "edges_per_step": int(r["edges"]),
+                   "strip_draw_pairs_per_step": r["strip_entries"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_render",
