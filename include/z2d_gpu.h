@@ -46,11 +46,26 @@ int32_t z2d_gpu_fill(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
                      const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
                      uint64_t n_nodes, const z2d_fill_opts *opts);
 
-/* painter.stroke (src/painter.zig:214-344): stroke tessellation runs on the
- * host and feeds the same rasterizer (non-zero rule). */
+/* painter.stroke (src/painter.zig:214-344): stroke tessellation (the
+ * reference's stroke_plotter / dashed_plotter) runs on the host and feeds the
+ * same rasterizer with the non-zero rule.  Errors in the reference's order:
+ * PIXEL_SOURCE_NOT_PREMULTIPLIED, INVALID_MATRIX (the transformation is not
+ * invertible); zero nodes is a no-op; INVALID_STATE.  Hairline strokes
+ * return UNSUPPORTED. */
 int32_t z2d_gpu_stroke(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
                        const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
                        uint64_t n_nodes, const z2d_stroke_opts *opts);
+
+/* stroke_plotter.plot (src/internal/tess/stroke_plotter.zig:40-76) with
+ * painter.stroke's option adjustments (src/painter.zig:236-304) for a
+ * surface of `surface_format`: the stroke's polygon in the AA mode's scaled
+ * space (edges, extents top/bottom/left/right, and the effective AA mode).
+ * Host only, no device work.  Writes at most `capacity` edges; *n_edges is
+ * the full count. */
+int32_t z2d_stroke_polygon(const z2d_path_node *nodes, uint64_t n_nodes,
+                           const z2d_stroke_opts *opts, uint32_t surface_format,
+                           z2d_edge *edges, uint64_t capacity, uint64_t *n_edges,
+                           double extents[4], uint32_t *aa_mode);
 
 /* The rasterizer step of painter.fill/stroke on an existing edge list
  * (direct.run / supersample.run / multisample.run,

@@ -14,7 +14,11 @@ OUT = os.path.join(HERE, "spec")
 PREFIXES = ["001", "002", "003", "004", "005", "006", "007", "013", "016", "023", "024",
             "025", "026", "028", "031", "032", "033", "046", "047", "048", "049", "050",
             "051", "052", "053", "059", "061", "062", "063", "064", "065", "066", "067",
-            "068", "069", "070", "071", "072", "075", "076", "077", "078", "079", "080"]
+            "068", "069", "070", "071", "072", "075", "076", "077", "078", "079", "080",
+            # strokes (host tessellation + device raster)
+            "008", "009", "010", "011", "012", "014", "015", "017", "018", "019", "020",
+            "021", "022", "027", "029", "030", "034", "036", "037", "038", "039", "040",
+            "041", "043", "044", "045", "054", "055", "056", "057", "060", "073"]
 
 
 def main():

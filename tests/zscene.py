@@ -112,6 +112,13 @@ class Path:
         self.nodes.append((abi.LINE_TO, p[0], p[1]))
         self.current = p
 
+    def rel_line_to(self, dx, dy):
+        """Path.relLineTo (src/Path.zig:187-192)."""
+        if self.current is None:
+            raise ValueError("NoCurrentPoint")
+        ux, uy = self.transformation.inverse().user_to_device(*self.current)
+        self.line_to(ux + dx, uy + dy)
+
     def curve_to(self, x1, y1, x2, y2, x3, y3):
         if self.current is None:
             raise ValueError("NoCurrentPoint")
@@ -282,6 +289,9 @@ class Context:
 
     def line_to(self, x, y):
         self.path.line_to(x, y)
+
+    def rel_line_to(self, dx, dy):
+        self.path.rel_line_to(dx, dy)
 
     def curve_to(self, *a):
         self.path.curve_to(*a)

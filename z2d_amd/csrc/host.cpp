@@ -25,6 +25,7 @@
 
 #include "../../include/z2d_gpu.h"
 #include "render.h"
+#include "stroke.h"
 
 using namespace z2d;
 
@@ -814,6 +815,55 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, ymin, ymax);
 }
 
+// Transformation.inverse succeeds (src/Transformation.zig:103-160).
+static bool ctm_invertible(const double m[6]) {
+  if (m[1] == 0 && m[2] == 0) return m[0] != 0 && m[3] != 0;
+  return m[0] * m[3] - m[1] * m[2] != 0;
+}
+
+// painter.stroke's option adjustments (src/painter.zig:236-304) and the
+// stroke plotter: the polygon in the AA mode's scaled space.
+static int32_t stroke_polygon(const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o,
+                              uint32_t format, uint32_t &aa, StrokePolygon &poly) {
+  aa = format == Z2D_FORMAT_ALPHA1 ? (uint32_t)Z2D_AA_NONE : o->anti_aliasing_mode;
+  const double minimum_line_width = 0.00390625;
+  const bool wide = o->line_width >= 2;
+  StrokeParams p;
+  p.cap_mode = wide ? o->line_cap_mode : (uint32_t)Z2D_CAP_BUTT;
+  p.join_mode = wide ? o->line_join_mode : (uint32_t)Z2D_JOIN_MITER;
+  p.miter_limit = wide ? o->miter_limit : 10.0;
+  p.thickness = o->line_width >= minimum_line_width ? o->line_width : minimum_line_width;
+  p.tolerance = std::max(o->tolerance, 0.001);
+  p.scale = aa == Z2D_AA_NONE ? 1.0 : 4.0;
+  for (int i = 0; i < 6; i++) p.ctm[i] = o->transformation[i];
+  p.dashes = o->dashes;
+  p.n_dashes = o->n_dashes;
+  p.dash_offset = o->dash_offset;
+  return stroke_tessellate(nodes, n, p, poly);
+}
+
+// painter.stroke (src/painter.zig:214-344): host tessellation by the
+// stroke plotter, then the polygon is filled non-zero.  Hairline strokes
+// (a separate rasterizer in the reference) are not part of this backend.
+static int32_t batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
+                            const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o) {
+  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  if (!ctm_invertible(o->transformation)) return Z2D_ERR_INVALID_MATRIX;
+  if (n == 0) return Z2D_OK;
+  if (!nodes || o->op >= Z2D_OP_COUNT || o->anti_aliasing_mode > 3 || o->line_cap_mode > 2 ||
+      o->line_join_mode > 2 || (o->n_dashes && !o->dashes))
+    return Z2D_ERR_INVALID_ARGUMENT;
+  if (o->hairline) return Z2D_ERR_UNSUPPORTED;
+  uint32_t aa;
+  StrokePolygon poly;
+  rc = stroke_polygon(nodes, n, o, s->format, aa, poly);
+  if (rc) return rc;
+  return batch_fill_polygon(b, s, pattern, poly.edges.data(), poly.edges.size(), poly.ext, aa,
+                            Z2D_FILL_NON_ZERO, o->op, o->precision, true);
+}
+
 static int32_t batch_prepare(z2d_gpu_batch *b) {
   z2d_gpu_ctx *ctx = b->ctx;
   if (b->prepared) return Z2D_OK;
@@ -1584,21 +1634,61 @@ int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, do
   return Z2D_OK;
 }
 
+struct StrokeArgs {
+  const z2d_pattern_desc *pattern;
+  const z2d_path_node *nodes;
+  uint64_t n;
+  const z2d_stroke_opts *opts;
+};
+
 int32_t z2d_gpu_stroke(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
                        const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
                        uint64_t n_nodes, const z2d_stroke_opts *opts) {
-  (void)ctx; (void)surface; (void)nodes; (void)n_nodes;
-  if (!opts) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!opts || !surface) return Z2D_ERR_INVALID_ARGUMENT;
+  // painter.stroke's checks, in order (src/painter.zig:221-234)
   int32_t rc = check_pattern(pattern);
   if (rc) return rc;
-  return Z2D_ERR_UNSUPPORTED;
+  if (!ctm_invertible(opts->transformation)) return Z2D_ERR_INVALID_MATRIX;
+  if (n_nodes == 0) return Z2D_OK;
+  if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  StrokeArgs a{pattern, nodes, n_nodes, opts};
+  return single_draw(
+      ctx, surface,
+      [](z2d_gpu_batch *b, z2d_gpu_surface *s, const void *p) -> int32_t {
+        const StrokeArgs &f = *static_cast<const StrokeArgs *>(p);
+        return batch_stroke(b, s, f.pattern, f.nodes, f.n, f.opts);
+      },
+      &a);
+}
+
+int32_t z2d_stroke_polygon(const z2d_path_node *nodes, uint64_t n_nodes,
+                           const z2d_stroke_opts *opts, uint32_t surface_format, z2d_edge *edges,
+                           uint64_t capacity, uint64_t *n_edges, double extents[4],
+                           uint32_t *aa_mode) {
+  if (!opts || !n_edges || (n_nodes && !nodes) || surface_format > Z2D_FORMAT_ALPHA1)
+    return Z2D_ERR_INVALID_ARGUMENT;
+  if (!ctm_invertible(opts->transformation)) return Z2D_ERR_INVALID_MATRIX;
+  if (opts->hairline) return Z2D_ERR_UNSUPPORTED;
+  if (opts->n_dashes && !opts->dashes) return Z2D_ERR_INVALID_ARGUMENT;
+  uint32_t aa = 0;
+  StrokePolygon poly;
+  int32_t rc = n_nodes ? stroke_polygon(nodes, n_nodes, opts, surface_format, aa, poly) : 0;
+  if (rc) return rc;
+  *n_edges = poly.edges.size();
+  if (edges)
+    std::memcpy(edges, poly.edges.data(),
+                std::min<uint64_t>(capacity, poly.edges.size()) * sizeof(z2d_edge));
+  if (extents)
+    for (int i = 0; i < 4; i++) extents[i] = poly.ext[i];
+  if (aa_mode) *aa_mode = aa;
+  return Z2D_OK;
 }
 
 int32_t z2d_gpu_batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *surface,
                              const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
                              uint64_t n_nodes, const z2d_stroke_opts *opts) {
-  (void)b; (void)surface; (void)pattern; (void)nodes; (void)n_nodes; (void)opts;
-  return Z2D_ERR_UNSUPPORTED;
+  return batch_stroke(b, surface, pattern, nodes, n_nodes, opts);
 }
 
 }  // extern "C"

@@ -50,6 +50,8 @@ _SIGS = {
     "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_destroy": ([C.c_void_p], None),
     "z2d_gpu_stream_copy": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
+    "z2d_stroke_polygon": ([C.c_void_p, C.c_uint64, P(abi.StrokeOpts), C.c_uint32, C.c_void_p,
+                            C.c_uint64, P(C.c_uint64), P(C.c_double), P(C.c_uint32)], C.c_int32),
     "z2d_gpu_server_create": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)], C.c_int32),
     "z2d_gpu_server_submit": ([C.c_void_p, P(abi.RequestDesc), C.c_void_p, C.c_uint64,
                                P(C.c_uint64)], C.c_int32),
@@ -163,6 +165,10 @@ class Batch:
     def fill(self, surface, pattern, nodes, opts):
         return lib().z2d_gpu_batch_fill(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
                                         len(nodes), C.byref(opts))
+
+    def stroke(self, surface, pattern, nodes, opts):
+        return lib().z2d_gpu_batch_stroke(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
+                                          len(nodes), C.byref(opts))
 
     def fill_polygon(self, surface, pattern, edges, extents, aa, rule, op, precision):
         ext = (C.c_double * 4)(*extents)
@@ -324,3 +330,20 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def stroke_polygon(nodes, opts, surface_format):
+    """Host stroke tessellation (z2d_stroke_polygon): returns (rc, edges,
+    extents, aa_mode) with edges as an EDGE_DTYPE array in scaled space."""
+    import numpy as np
+    n = C.c_uint64()
+    ext = (C.c_double * 4)()
+    aa = C.c_uint32()
+    rc = lib().z2d_stroke_polygon(nodes.ctypes.data, len(nodes), C.byref(opts), surface_format,
+                                  None, 0, C.byref(n), ext, C.byref(aa))
+    if rc:
+        return rc, None, None, None
+    edges = np.zeros(n.value, dtype=abi.EDGE_DTYPE)
+    rc = lib().z2d_stroke_polygon(nodes.ctypes.data, len(nodes), C.byref(opts), surface_format,
+                                  edges.ctypes.data, len(edges), C.byref(n), ext, C.byref(aa))
+    return rc, edges, tuple(ext), aa.value

@@ -10,6 +10,7 @@ library or to any other implementation of the same C ABI.
                    4-512 px boxes, 60% solid / 40% gradient paints,
                    50/50 non-zero / even-odd, src_over, MSAA)
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -171,6 +172,50 @@ def random_paths(rng, n_paths, width, height, max_side=512.0, min_side=4.0, aa=a
         opts = abi.fill_opts(aa, rule, op)
         draws.append((pattern, abi.nodes_array(nodes), opts))
     return draws, keep
+
+
+def stroke_opts(rng, aa=abi.AA_DEFAULT, op="src_over"):
+    """Stroke options of the batched form: width U(0.5, 8), join and cap
+    uniform over the three modes, 25% dashed with 2-6 dash lengths U(1, 20)
+    and offset 0.  Returns (opts, keep)."""
+    o = abi.StrokeOpts()
+    o.anti_aliasing_mode = aa
+    o.line_width = rng.uniform(0.5, 8.0)
+    o.line_join_mode = rng.randint(0, 2)
+    o.line_cap_mode = rng.randint(0, 2)
+    o.op = abi.OP[op]
+    o.precision = abi.PRECISION_INTEGER
+    o.miter_limit = 10.0
+    o.tolerance = 0.1
+    o.transformation[:] = abi.IDENTITY
+    keep = []
+    if rng.uniform() < 0.25:
+        d = (C.c_double * rng.randint(2, 6))(*[])
+        for i in range(len(d)):
+            d[i] = rng.uniform(1.0, 20.0)
+        o.dashes = C.cast(d, C.POINTER(C.c_double))
+        o.n_dashes = len(d)
+        keep.append(d)
+    o.dash_offset = 0.0
+    return o, keep
+
+
+def random_jobs(rng, n_paths, width, height, max_side=512.0, stroke_share=0.3):
+    """The batched form's per-surface path list: each path from the 4K
+    scene's distribution, stroked with probability `stroke_share`.  Returns
+    ([(kind, pattern, nodes, opts)], keep)."""
+    jobs, keep = [], []
+    for _ in range(n_paths):
+        draws, k = random_paths(rng, 1, width, height, max_side=max_side)
+        keep += k
+        pat, nodes, fopts = draws[0]
+        if rng.uniform() < stroke_share:
+            sopts, k2 = stroke_opts(rng)
+            keep += k2
+            jobs.append(("stroke", pat, nodes, sopts))
+        else:
+            jobs.append(("fill", pat, nodes, fopts))
+    return jobs, keep
 
 
 def scene_4k(n_paths=10000, width=3840, height=2160, seed=0xC3):
