@@ -927,7 +927,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
-  const size_t o_eidx = L.take(max_entry_edges * sizeof(uint32_t));
+  const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
   const size_t o_rows = L.take(max_rows * sizeof(RowRec));
@@ -996,7 +996,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
   J.max_entries = (uint32_t)max_entries;
   J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);
-  J.entry_edge_idx = reinterpret_cast<uint32_t *>(D + o_eidx);
+  J.entry_edges = reinterpret_cast<z2d_edge *>(D + o_eidx);
   J.max_entry_edges = (uint32_t)max_entry_edges;
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
   J.count_stats = b->count_stats;

@@ -87,7 +87,7 @@ struct RowRec {
 
 struct BandEntry {
   uint32_t draw;
-  uint32_t edge_off;  // into the entry-edge index array
+  uint32_t edge_off;  // into the entry-edge array
   uint32_t edge_cnt;  // edges of the draw whose y-range meets the band
   uint32_t band;
 };
@@ -117,7 +117,7 @@ struct JobView {
   BandEntry *entries;    // capacity max_entries
   uint32_t max_entries;
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place
-  uint32_t *entry_edge_idx;  // capacity max_entry_edges
+  z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges
   uint32_t max_entry_edges;
   // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;
   // [7] tile-draw entries rasterized; [8] entries on the unpacked path

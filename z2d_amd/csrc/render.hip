@@ -564,7 +564,7 @@ __global__ void k_scan(uint32_t *a, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// k_entry_edges: per (draw, band) entry, the indices of the draw's edges whose
+// k_entry_edges: per (draw, band) entry, copies of the draw's edges whose
 // y-range meets the band's sample rows (top < last mid && bottom >= first mid,
 // the active-edge test of src/internal/raster/multisample.zig and direct.zig
 // taken over the band).  Tiles then walk only those edges.  One wave per
@@ -593,8 +593,9 @@ __global__ void k_entry_edges(JobView J) {
   uint32_t count = 0;
   for (uint32_t k0 = 0; k0 < ne; k0 += 64) {
     bool hit = false;
+    z2d_edge ed{0.0, 0.0, 0.0, 0.0};
     if (k0 + lane < ne) {
-      const z2d_edge ed = edges[k0 + lane];
+      ed = edges[k0 + lane];
       const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
       const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
       hit = top < ys_hi && bot >= ys_lo;
@@ -603,7 +604,7 @@ __global__ void k_entry_edges(JobView J) {
     if (FILL && hit) {
       const uint32_t pos = base + count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
       if (pos < J.max_entry_edges)
-        J.entry_edge_idx[pos] = k0 + lane;
+        J.entry_edges[pos] = ed;
       else
         atomicOr(&J.counters[2], 1u);
     }
@@ -941,8 +942,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           kk[i] = 1;
         }
       } else {
-        const z2d_edge *edges = J.edges + D.node_off;
-        const uint32_t *eidx = J.entry_edge_idx + E.edge_off;
+        const z2d_edge *edges = J.entry_edges + E.edge_off;
         const uint32_t ne = E.edge_off >= J.max_entry_edges
                                 ? 0u
                                 : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
@@ -959,7 +959,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
             int32_t n_sr = 0, lo = 0;
             z2d_edge ed{0.0, 0.0, 0.0, 0.0};
             if (k < ne) {
-              ed = edges[eidx[k]];
+              ed = edges[k];
               const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
               const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
               n_sr = active_rows(top, bot, ys_base, SR, lo);
@@ -1050,7 +1050,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               const double mid = (double)ys + 0.5;
               int32_t w[4] = {0, 0, 0, 0};
               for (uint32_t q = 0; q < ne; q++) {
-                const z2d_edge ed = edges[eidx[q]];
+                const z2d_edge ed = edges[q];
                 const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
                 const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
                 if (!(top < mid && bot >= mid)) continue;
