@@ -634,12 +634,20 @@ struct StripShared {
   // columns per word (column 2i in the low half).  Zero between entries:
   // the coverage phase clears what it reads.
   uint32_t acc[STRIP_SUBROWS][ACC_WORDS];
-  z2d_edge cedge[64];  // staged band edges
   int32_t clo[64];     // first active sample row in the strip
   uint32_t coff[64];   // exclusive offsets of active-row counts
   float soff[STAGED_STOPS];   // the draw's gradient stops, staged
   float4 spre[STAGED_STOPS];
-  uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples of the strip (RGBA8)
+  union {
+    struct {                               // coverage (phase A)
+      z2d_edge cedge[64];                  // staged band edges
+      uint8_t own[64 * STRIP_SUBROWS];     // owning edge of each work item
+    } a;
+    struct {                               // composite
+      uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples (RGBA8)
+      uint8_t plist[ROWS_PER_THREAD * 64];   // covered (row, lane) pairs
+    } c;
+  } u;
   DevPaint paint;      // the draw's paint and gradient, staged
   DevGradient grad;
 };
@@ -972,18 +980,16 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
             const int32_t incl = wave_incl_scan(n_sr);
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
             if (total == 0) continue;  // uniform
-            sh.cedge[lane] = ed;
+            sh.u.a.cedge[lane] = ed;
             sh.clo[lane] = lo;
-            sh.coff[lane] = (uint32_t)(incl - n_sr);
+            const uint32_t my_off = (uint32_t)(incl - n_sr);
+            sh.coff[lane] = my_off;
+            // each edge marks its own items (at most SR per edge)
+            for (int t = 0; t < n_sr; t++) sh.u.a.own[my_off + (uint32_t)t] = (uint8_t)lane;
             wave_lds_sync();
             for (uint32_t it = lane; it < total; it += 64) {
-              // owning edge: the last j with coff[j] <= it
-              uint32_t lj = 0, hj = 64;
-              while (hj - lj > 1) {
-                const uint32_t mj = (lj + hj) >> 1;
-                if (sh.coff[mj] <= it) lj = mj; else hj = mj;
-              }
-              const z2d_edge ce = sh.cedge[lj];
+              const uint32_t lj = sh.u.a.own[it];
+              const z2d_edge ce = sh.u.a.cedge[lj];
               const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
               const double top = ce.y0 < ce.y1 ? ce.y0 : ce.y1;
               const double mid = (double)(ys_base + sr) + 0.5;
@@ -1089,14 +1095,26 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         }
         const bool solid = P.kind == Z2D_PATTERN_OPAQUE;
         if (!solid) {
-          // paint samples of the covered pixels: one evaluation site for all
-          // rows, kept in LDS (each lane reads back only its own)
+          // paint samples of the covered pixels, evaluated densely: the
+          // (row, lane) pairs are listed with a wave scan and taken 64 at a
+          // time by one evaluation site; results go to LDS by pixel
+          const int32_t np = __popc(need);
+          const int32_t incl_p = wave_incl_scan(np);
+          const uint32_t tot_p = (uint32_t)__builtin_amdgcn_readlane(incl_p, 63);
+          uint32_t o = (uint32_t)(incl_p - np);
+#pragma unroll
+          for (int i = 0; i < ROWS_PER_THREAD; i++)
+            if ((need >> i) & 1u) sh.u.c.plist[o++] = (uint8_t)((i << 6) | lane);
+          wave_lds_sync();
 #pragma unroll 1
-          for (int i = 0; i < ROWS_PER_THREAD; i++) {
-            if (!((need >> i) & 1u)) continue;
-            const I4 c = paint_int(sh.paint, sv, J.blue_noise, px, ty0 + i);
-            sh.ppaint[i][lane] = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24);
+          for (uint32_t it = lane; it < tot_p; it += 64) {
+            const uint32_t v = sh.u.c.plist[it];
+            const uint32_t pl = v & 63u, pi = v >> 6;
+            const I4 c = paint_int(sh.paint, sv, J.blue_noise, tx0 + (int32_t)pl,
+                                   ty0 + (int32_t)pi);
+            sh.u.c.ppaint[pi][pl] = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24);
           }
+          wave_lds_sync();
         }
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
@@ -1104,7 +1122,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           n_cov++;
           I4 src = P.px;
           if (!solid) {
-            const uint32_t pv = sh.ppaint[i][lane];
+            const uint32_t pv = sh.u.c.ppaint[i][lane];
             src = I4{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
           }
           pix[i] = composite_src_over(pix[i], S.fmt, kk[i], SC * SC, D, src);
