@@ -648,6 +648,7 @@ struct StripShared {
       uint8_t plist[ROWS_PER_THREAD * 64];   // covered (row, lane) pairs
     } c;
   } u;
+  uint32_t srctab[17];  // solid paint masked by each coverage count (RGBA8)
   DevPaint paint;      // the draw's paint and gradient, staged
   DevGradient grad;
 };
@@ -724,13 +725,12 @@ __device__ __forceinline__ uint32_t composite_cov(uint32_t raw, uint32_t fmt, in
 // composite_cov for the common case, kept inline: src_over in integer
 // precision without a supersample mask, given the paint sample s (the same
 // steps: DST_IN of the paint by the coverage alpha, then SRC_OVER).
-__device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, int k,
-                                                       int kfull, const DevDraw &D, I4 s) {
-  if (k >= kfull && (D.flags & DRAW_FAST_SRC)) return pack(s, fmt);
-  if (k < kfull) {
-    const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
-    s = I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
-  }
+__device__ __forceinline__ I4 mask_coverage(I4 s, int k) {
+  const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
+  return I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
+}
+// s: the paint already masked by the pixel's coverage.
+__device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, I4 s) {
   const I4 d = unpack(raw, fmt);
   const uint32_t sa = s.a;
   const I4 o{u16(s.r + iinvmul(d.r, sa)), u16(s.g + iinvmul(d.g, sa)),
@@ -889,16 +889,19 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;
       if (D.ry1 <= ty0 || D.ry0 >= ty0 + ROWS_PER_THREAD) continue;
       const DevPaint &P = J.paints[D.paint];
+      const uint32_t p_kind = load_uniform(&P.kind);
+      const I4 p_px = load_uniform(&P.px);
+      const DevGradient *const p_grad = load_uniform(&P.gradient);
       StopView sv{nullptr, nullptr};
-      if (P.kind != Z2D_PATTERN_OPAQUE) {
+      if (p_kind != Z2D_PATTERN_OPAQUE) {
         // the paint and its gradient in LDS for the samplers
         wave_lds_sync();
         constexpr uint32_t PW = sizeof(DevPaint) / 4, GW = sizeof(DevGradient) / 4;
         static_assert(PW <= 64 && GW <= 128, "staging sizes");
         if (lane < PW) reinterpret_cast<uint32_t *>(&sh.paint)[lane] =
             reinterpret_cast<const uint32_t *>(&J.paints[D.paint])[lane];
-        if (P.gradient) {
-          const uint32_t *gs = reinterpret_cast<const uint32_t *>(P.gradient);
+        if (p_grad) {
+          const uint32_t *gs = reinterpret_cast<const uint32_t *>(p_grad);
           uint32_t *gd = reinterpret_cast<uint32_t *>(&sh.grad);
           if (lane < GW) gd[lane] = gs[lane];
           if (lane + 64 < GW) gd[lane + 64] = gs[lane + 64];
@@ -906,8 +909,8 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         }
         wave_lds_sync();
       }
-      if (P.gradient) {
-        const DevGradient G = load_uniform(P.gradient);
+      if (p_grad) {
+        const DevGradient G = load_uniform(p_grad);
         if (G.n_stops <= STAGED_STOPS) {
           wave_lds_sync();
           if (lane < G.n_stops) {
@@ -1093,8 +1096,16 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               kk[i] > 0)
             need |= 1u << i;
         }
-        const bool solid = P.kind == Z2D_PATTERN_OPAQUE;
-        if (!solid) {
+        const bool solid = p_kind == Z2D_PATTERN_OPAQUE;
+        const int kfull = SC * SC;
+        if (solid) {
+          // the 17 possible masked sources of a solid paint
+          if (lane <= 16) {
+            const I4 m = (int)lane >= kfull ? p_px : mask_coverage(p_px, (int)lane);
+            sh.srctab[lane] = m.r | (m.g << 8) | (m.b << 16) | (m.a << 24);
+          }
+          wave_lds_sync();
+        } else {
           // paint samples of the covered pixels, evaluated densely: the
           // (row, lane) pairs are listed with a wave scan and taken 64 at a
           // time by one evaluation site; results go to LDS by pixel
@@ -1116,16 +1127,20 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           }
           wave_lds_sync();
         }
+        const bool fast_full = (D.flags & DRAW_FAST_SRC) != 0;
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
           if (!((need >> i) & 1u)) continue;
           n_cov++;
-          I4 src = P.px;
-          if (!solid) {
-            const uint32_t pv = sh.u.c.ppaint[i][lane];
-            src = I4{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+          const int k = kk[i];
+          if (fast_full && k >= kfull) {
+            pix[i] = pack(p_px, S.fmt);  // the solid fast path paints the source
+            continue;
           }
-          pix[i] = composite_src_over(pix[i], S.fmt, kk[i], SC * SC, D, src);
+          const uint32_t pv = solid ? sh.srctab[k] : sh.u.c.ppaint[i][lane];
+          I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+          if (!solid && k < kfull) src = mask_coverage(src, k);
+          pix[i] = composite_src_over(pix[i], S.fmt, src);
         }
       } else {
         // every other operator / precision / mode: one composite instance,
