@@ -32,7 +32,20 @@ __device__ __forceinline__ float zmodf(float x, float y) {
   float a = fmodf(x, y);
   return x < 0.0f ? fmodf(a + y, y) : a;
 }
+// fmod(x, y) for y > 0: |x| < y gives x and y <= |x| < 2y gives x - y with
+// x's sign, exact by Sterbenz's lemma and equal to fmod's exact remainder;
+// other arguments go to fmod.
+__device__ __forceinline__ double fmod_pos(double x, double y) {
+  const double ax = fabs(x);
+  if (ax < y) return x;
+  if (ax < 2.0 * y) return x - copysign(y, x);
+  return fmod(x, y);
+}
 __device__ __forceinline__ double zmodd(double x, double y) {
+  if (y > 0.0) {
+    const double a = fmod_pos(x, y);
+    return x < 0.0 ? fmod_pos(a + y, y) : a;
+  }
   double a = fmod(x, y);
   return x < 0.0 ? fmod(a + y, y) : a;
 }
@@ -519,19 +532,20 @@ __device__ __forceinline__ void user_to_device(const double *m, double &x, doubl
 }
 __device__ __forceinline__ double dclamp01(double v) { return fmax(0.0, fmin(v, 1.0)); }
 
-// Gradient.getOffset at pixel (x, y).
-__device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
+// Gradient.getOffset at pixel (x, y), one function per gradient type.
+__device__ __forceinline__ float linear_offset(const DevGradient &g, int x, int y) {
   double px = (double)x + 0.5, py = (double)y + 0.5;
-  if (g.type == Z2D_GRADIENT_LINEAR) {
-    if (!g.identity) user_to_device(g.m, px, py);
-    if (g.lin_degenerate) return -1.0f;
-    double spx = px - g.geom[0], spy = py - g.geom[1];
-    double dt = 0.0;
-    dt += g.lin_dx * spx;
-    dt += g.lin_dy * spy;
-    return (float)dclamp01(dt * g.lin_inv);
-  }
-  if (g.type == Z2D_GRADIENT_RADIAL) {
+  if (!g.identity) user_to_device(g.m, px, py);
+  if (g.lin_degenerate) return -1.0f;
+  double spx = px - g.geom[0], spy = py - g.geom[1];
+  double dt = 0.0;
+  dt += g.lin_dx * spx;
+  dt += g.lin_dy * spy;
+  return (float)dclamp01(dt * g.lin_inv);
+}
+__device__ __forceinline__ float radial_offset(const DevGradient &g, int x, int y) {
+  double px = (double)x + 0.5, py = (double)y + 0.5;
+  {
     if (g.r1 == 0.0 && g.r2 == 0.0) return -1.0f;
     if (!g.identity) user_to_device(g.m, px, py);
     double pdx = px - g.geom[0], pdy = py - g.geom[1];
@@ -561,10 +575,18 @@ __device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
     }
     return -1.0f;
   }
+}
+__device__ __forceinline__ float conic_offset(const DevGradient &g, int x, int y) {
+  double px = (double)x + 0.5, py = (double)y + 0.5;
   const double two_pi = 3.14159265358979323846 * 2.0;
   if (!g.identity) user_to_device(g.m, px, py);
   double ang = zmodd(atan2(py - g.geom[1], px - g.geom[0]) - g.conic_angle, two_pi);
   return (float)(ang / two_pi);
+}
+__device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
+  if (g.type == Z2D_GRADIENT_LINEAR) return linear_offset(g, x, y);
+  if (g.type == Z2D_GRADIENT_RADIAL) return radial_offset(g, x, y);
+  return conic_offset(g, x, y);
 }
 
 // Stop.List.search.

@@ -725,6 +725,39 @@ __device__ __forceinline__ uint32_t composite_cov(uint32_t raw, uint32_t fmt, in
 // composite_cov for the common case, kept inline: src_over in integer
 // precision without a supersample mask, given the paint sample s (the same
 // steps: DST_IN of the paint by the coverage alpha, then SRC_OVER).
+// Paint samples of a linear or radial (TYPE) gradient pattern interpolated in linear RGB
+// (Gradient.getOffset, Stop.List.search, interpolateEncodeVec of
+// src/gradient.zig) for the pixels listed in sh.u.c.plist; G is wave-uniform
+// and its stops are staged in sh.soff / sh.spre.
+template <uint32_t TYPE>
+__device__ __forceinline__ void dense_gradient(const DevGradient &G, StripShared &sh,
+                                               uint32_t tot_p, uint32_t lane, int32_t tx0,
+                                               int32_t ty0) {
+#pragma unroll 1
+  for (uint32_t it = lane; it < tot_p; it += 64) {
+    const uint32_t v = sh.u.c.plist[it];
+    const uint32_t pl = v & 63u, pi = v >> 6;
+    const int32_t x = tx0 + (int32_t)pl, y = ty0 + (int32_t)pi;
+    float off;
+    if constexpr (TYPE == Z2D_GRADIENT_LINEAR)
+      off = linear_offset(G, x, y);
+    else
+      off = radial_offset(G, x, y);
+    int i0, i1;
+    float t;
+    bool transparent;
+    stops_search(sh.soff, G.n_stops, off, i0, i1, t, transparent);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 A = transparent ? z : sh.spre[i0];
+    const float4 B = transparent ? z : sh.spre[i1];
+    const uint32_t r = f2u8(255.0f * lerpf_(A.x, B.x, t));
+    const uint32_t g = f2u8(255.0f * lerpf_(A.y, B.y, t));
+    const uint32_t b = f2u8(255.0f * lerpf_(A.z, B.z, t));
+    const uint32_t a = f2u8(255.0f * lerpf_(A.w, B.w, t));
+    sh.u.c.ppaint[pi][pl] = r | (g << 8) | (b << 16) | (a << 24);
+  }
+}
+
 __device__ __forceinline__ I4 mask_coverage(I4 s, int k) {
   const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
   return I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
@@ -892,8 +925,20 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       const uint32_t p_kind = load_uniform(&P.kind);
       const I4 p_px = load_uniform(&P.px);
       const DevGradient *const p_grad = load_uniform(&P.gradient);
+      // a linear or radial gradient pattern interpolated in linear RGB with
+      // its stops staged: sampled by dense_gradient, without the paint in LDS
+      // (conic gradients go through paint_int: atan2 inlined here would
+      // spill)
+      uint32_t g_type = Z2D_GRADIENT_CONIC, g_method = 0, g_nstops = 0;
+      if (p_grad) {
+        g_type = load_uniform(&p_grad->type);
+        g_method = load_uniform(&p_grad->method);
+        g_nstops = load_uniform(&p_grad->n_stops);
+      }
+      const bool grad_dense = p_kind == Z2D_PATTERN_GRADIENT && g_type != Z2D_GRADIENT_CONIC &&
+                              g_method == Z2D_INTERP_LINEAR_RGB && g_nstops <= STAGED_STOPS;
       StopView sv{nullptr, nullptr};
-      if (p_kind != Z2D_PATTERN_OPAQUE) {
+      if (p_kind != Z2D_PATTERN_OPAQUE && !grad_dense) {
         // the paint and its gradient in LDS for the samplers
         wave_lds_sync();
         constexpr uint32_t PW = sizeof(DevPaint) / 4, GW = sizeof(DevGradient) / 4;
@@ -1117,13 +1162,21 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           for (int i = 0; i < ROWS_PER_THREAD; i++)
             if ((need >> i) & 1u) sh.u.c.plist[o++] = (uint8_t)((i << 6) | lane);
           wave_lds_sync();
+          if (grad_dense) {
+            const DevGradient G = load_uniform(p_grad);
+            if (G.type == Z2D_GRADIENT_LINEAR)
+              dense_gradient<Z2D_GRADIENT_LINEAR>(G, sh, tot_p, lane, tx0, ty0);
+            else
+              dense_gradient<Z2D_GRADIENT_RADIAL>(G, sh, tot_p, lane, tx0, ty0);
+          } else {
 #pragma unroll 1
-          for (uint32_t it = lane; it < tot_p; it += 64) {
-            const uint32_t v = sh.u.c.plist[it];
-            const uint32_t pl = v & 63u, pi = v >> 6;
-            const I4 c = paint_int(sh.paint, sv, J.blue_noise, tx0 + (int32_t)pl,
-                                   ty0 + (int32_t)pi);
-            sh.u.c.ppaint[pi][pl] = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24);
+            for (uint32_t it = lane; it < tot_p; it += 64) {
+              const uint32_t v = sh.u.c.plist[it];
+              const uint32_t pl = v & 63u, pi = v >> 6;
+              const I4 c = paint_int(sh.paint, sv, J.blue_noise, tx0 + (int32_t)pl,
+                                     ty0 + (int32_t)pi);
+              sh.u.c.ppaint[pi][pl] = c.r | (c.g << 8) | (c.b << 16) | (c.a << 24);
+            }
           }
           wave_lds_sync();
         }
