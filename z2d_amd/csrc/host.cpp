@@ -471,9 +471,11 @@ struct Flattener {
   Pt buf[3];
   int len = 0;
   double tol_sq = 0.01;
-  double ymin = INFINITY, ymax = -INFINITY;
+  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
 
   void track(Pt p) {
+    xmin = std::min(xmin, p.x);
+    xmax = std::max(xmax, p.x);
     ymin = std::min(ymin, p.y);
     ymax = std::max(ymax, p.y);
   }
@@ -559,10 +561,10 @@ struct Flattener {
 };
 
 // Flattens one node set.  Returns a z2d_status; balanced_out reports whether
-// every contour's last point equals its first point.
+// every contour's last point equals its first point; box receives the
+// bounding box of the points and control points (xmin, xmax, ymin, ymax).
 int32_t flatten_nodes(const z2d_path_node *nodes, uint64_t n, double tolerance,
-                      std::vector<FNode> &out, bool &balanced_out, double &ymin,
-                      double &ymax) {
+                      std::vector<FNode> &out, bool &balanced_out, double box[4]) {
   Flattener f;
   f.out = &out;
   f.tol_sq = tolerance * tolerance;
@@ -602,8 +604,10 @@ int32_t flatten_nodes(const z2d_path_node *nodes, uint64_t n, double tolerance,
 done:
   if (f.len > 0 && !peq(f.buf[f.len - 1], f.buf[0])) balanced = false;
   balanced_out = balanced;
-  ymin = f.ymin;
-  ymax = f.ymax;
+  box[0] = f.xmin;
+  box[1] = f.xmax;
+  box[2] = f.ymin;
+  box[3] = f.ymax;
   return Z2D_OK;
 }
 
@@ -646,6 +650,7 @@ struct HostDraw {
   uint32_t surface_slot;  // index into batch surfaces
   uint32_t seq;           // insertion order
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
+  uint32_t tcol_lo, tcol_hi;  // estimated tile-column range (inclusive)
   uint64_t edge_pairs;        // upper bound of (edge, band) pairs
 };
 
@@ -715,8 +720,8 @@ static inline hipStream_t bstream(const z2d_gpu_batch *b) {
 
 static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
                                 const z2d_pattern_desc *pattern, uint32_t aa, uint32_t rule,
-                                uint32_t op, uint32_t precision, HostDraw &hd, double ymin,
-                                double ymax) {
+                                uint32_t op, uint32_t precision, HostDraw &hd,
+                                const double box[4]) {
   (void)pattern;
   hd.surface_slot = b->slot_of(s);
   hd.seq = (uint32_t)b->draws.size();
@@ -732,17 +737,23 @@ static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
   const bool bounded = op_is_bounded(op);
   if (d.mode == MODE_DIRECT && !bounded) d.flags |= DRAW_ROWPROG;
   if (d.flags & DRAW_UNBALANCED) d.flags |= DRAW_ROWPROG;
-  // Band estimate from the input points (pixel rows).
+  // Band and tile-column estimates from the input points (pixels); the
+  // composite region lies within a pixel of the points' box.
   const int32_t nb = (s->h + TILE_H - 1) / TILE_H;
-  if (!bounded || !std::isfinite(ymin) || !std::isfinite(ymax)) {
+  const int32_t nc = (s->w + TILE_W - 1) / TILE_W;
+  const bool finite = std::isfinite(box[0]) && std::isfinite(box[1]) &&
+                      std::isfinite(box[2]) && std::isfinite(box[3]);
+  if (!bounded || !finite) {
     hd.band_lo = 0;
     hd.band_hi = (uint32_t)(nb - 1);
+    hd.tcol_lo = 0;
+    hd.tcol_hi = (uint32_t)(nc - 1);
   } else {
-    double lo = std::floor(ymin) - 2.0, hi = std::ceil(ymax) + 2.0;
-    lo = std::max(0.0, std::min(lo, (double)(s->h - 1)));
-    hi = std::max(0.0, std::min(hi, (double)(s->h - 1)));
-    hd.band_lo = (uint32_t)lo / TILE_H;
-    hd.band_hi = (uint32_t)hi / TILE_H;
+    auto clamped = [](double v, int32_t n) { return std::max(0.0, std::min(v, (double)(n - 1))); };
+    hd.band_lo = (uint32_t)clamped(std::floor(box[2]) - 2.0, s->h) / TILE_H;
+    hd.band_hi = (uint32_t)clamped(std::ceil(box[3]) + 2.0, s->h) / TILE_H;
+    hd.tcol_lo = (uint32_t)clamped(std::floor(box[0]) - 2.0, s->w) / TILE_W;
+    hd.tcol_hi = (uint32_t)clamped(std::ceil(box[1]) + 2.0, s->w) / TILE_W;
   }
   b->draws.push_back(hd);
   b->prepared = false;
@@ -764,9 +775,9 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
   std::memset(&hd, 0, sizeof(hd));
   const size_t before = b->nodes.size();
   bool balanced = true;
-  double ymin, ymax;
+  double box[4];
   double tol = std::max(o->tolerance, 0.001);
-  rc = flatten_nodes(nodes, n, tol, b->nodes, balanced, ymin, ymax);
+  rc = flatten_nodes(nodes, n, tol, b->nodes, balanced, box);
   if (rc) {
     b->nodes.resize(before);
     return rc;
@@ -782,7 +793,7 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
     }
   if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
   return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
-                          o->precision, hd, ymin, ymax);
+                          o->precision, hd, box);
 }
 
 static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
@@ -808,11 +819,12 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
   // extents are in scaled space
   const double sc = (aa == Z2D_AA_NONE || s->format == Z2D_FORMAT_ALPHA1) ? 1.0 : 4.0;
-  double ymin = n_edges ? ext[0] / sc : NAN, ymax = n_edges ? ext[1] / sc : NAN;
+  const double box[4] = {n_edges ? ext[2] / sc : NAN, n_edges ? ext[3] / sc : NAN,
+                         n_edges ? ext[0] / sc : NAN, n_edges ? ext[1] / sc : NAN};
   const uint32_t nbands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
   for (uint64_t i = 0; i < n_edges; i++)
     hd.edge_pairs += band_span(edges[i].y0 / sc, edges[i].y1 / sc, nbands);
-  return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, ymin, ymax);
+  return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, box);
 }
 
 // Transformation.inverse succeeds (src/Transformation.zig:103-160).
@@ -888,11 +900,14 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const uint32_t n_tiles = b->surf_tile_off[ns];
   const uint32_t nd = (uint32_t)b->draws.size();
   uint64_t max_entries = 0, max_rows = 0, scratch = 0, max_entry_edges = 0;
+  uint64_t max_strip_entries = 0;
   std::vector<uint32_t> rowprog_list, rowprog_soff;
   for (uint32_t i = 0; i < nd; i++) {
     HostDraw &h = b->draws[i];
     h.d.surface = h.surface_slot;
     max_entries += h.band_hi - h.band_lo + 1;
+    max_strip_entries += (uint64_t)(h.band_hi - h.band_lo + 1) * STRIPS_PER_TILE *
+                         (h.tcol_hi - h.tcol_lo + 1);
     max_entry_edges += h.edge_pairs;
     if (h.d.flags & DRAW_ROWPROG) {
       const z2d_gpu_surface *s = b->surfaces[h.surface_slot];
@@ -904,7 +919,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     }
   }
   if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull ||
-      max_entry_edges > 0xFFFFFFF0ull)
+      max_entry_edges > 0xFFFFFFF0ull || max_strip_entries > 0xFFFFFFF0ull ||
+      (uint64_t)n_tiles * STRIPS_PER_TILE > 0xFFFFFFF0ull)
     return Z2D_ERR_OUT_OF_MEMORY;
   b->n_rowprog = (uint32_t)rowprog_list.size();
 
@@ -928,6 +944,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
   const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
+  const size_t o_stripcnt = L.take(((size_t)n_tiles * STRIPS_PER_TILE + 1) * sizeof(uint32_t));
+  const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
   const size_t o_rows = L.take(max_rows * sizeof(RowRec));
@@ -998,6 +1016,9 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);
   J.entry_edges = reinterpret_cast<z2d_edge *>(D + o_eidx);
   J.max_entry_edges = (uint32_t)max_entry_edges;
+  J.strip_cnt = reinterpret_cast<uint32_t *>(D + o_stripcnt);
+  J.strip_entries = reinterpret_cast<uint32_t *>(D + o_sentries);
+  J.max_strip_entries = (uint32_t)max_strip_entries;
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
   J.count_stats = b->count_stats;
   J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);

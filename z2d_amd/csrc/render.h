@@ -119,6 +119,11 @@ struct JobView {
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place
   z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges
   uint32_t max_entry_edges;
+  // per strip (tile * STRIPS_PER_TILE + strip): the band entries whose
+  // composite region meets the strip, in draw order
+  uint32_t *strip_cnt;       // n_tiles * STRIPS_PER_TILE + 1 (scanned in place)
+  uint32_t *strip_entries;   // band entry indices, capacity max_strip_entries
+  uint32_t max_strip_entries;
   // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;
   // [7] tile-draw entries rasterized; [8] entries on the unpacked path
   uint32_t *counters;

@@ -564,6 +564,62 @@ __global__ void k_scan(uint32_t *a, uint32_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// k_strip_pass: per strip (ROWS_PER_THREAD rows of one tile), the band entries
+// of its band whose composite region meets the strip, in draw order.  One
+// wave per tile walks the band's entries; FILL=false counts per strip,
+// FILL=true writes after the scan.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void k_strip_pass(JobView J) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (t >= J.n_tiles) return;
+  const TileRef T = J.tiles[t];
+  const uint32_t b = J.surf_band_off[T.surface] + T.ty;
+  const uint32_t e0 = min(J.band_cnt[b], J.max_entries);
+  const uint32_t e1 = min(J.band_cnt[b + 1], J.max_entries);
+  const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * TILE_H;
+  uint32_t cnt[STRIPS_PER_TILE], base[STRIPS_PER_TILE];
+#pragma unroll
+  for (int s = 0; s < STRIPS_PER_TILE; s++) {
+    cnt[s] = 0;
+    base[s] = FILL ? J.strip_cnt[t * STRIPS_PER_TILE + s] : 0u;
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t e = e0; e < e1; e += 64) {
+    uint32_t mask = 0;
+    if (e + lane < e1) {
+      const DevDraw &D = J.draws[J.entries[e + lane].draw];
+      const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
+      if (rx1 > tx0 && rx0 < tx0 + TILE_W) {
+#pragma unroll
+        for (int s = 0; s < STRIPS_PER_TILE; s++) {
+          const int32_t sy0 = ty0 + s * ROWS_PER_THREAD;
+          if (ry1 > sy0 && ry0 < sy0 + ROWS_PER_THREAD) mask |= 1u << s;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < STRIPS_PER_TILE; s++) {
+      const bool hit = (mask >> s) & 1u;
+      const uint64_t m = __ballot(hit);
+      if (FILL && hit) {
+        const uint32_t pos = base[s] + cnt[s] + (uint32_t)__popcll(m & below);
+        if (pos < J.max_strip_entries)
+          J.strip_entries[pos] = e + lane;
+        else
+          atomicOr(&J.counters[2], 1u);
+      }
+      cnt[s] += (uint32_t)__popcll(m);
+    }
+  }
+  if (!FILL && lane == 0) {
+#pragma unroll
+    for (int s = 0; s < STRIPS_PER_TILE; s++) J.strip_cnt[t * STRIPS_PER_TILE + s] = cnt[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_entry_edges: per (draw, band) entry, copies of the draw's edges whose
 // y-range meets the band's sample rows (top < last mid && bottom >= first mid,
 // the active-edge test of src/internal/raster/multisample.zig and direct.zig
@@ -898,9 +954,9 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
     const int32_t tx0 = (int32_t)T.tx * TILE_W;
     const int32_t ty0 = (int32_t)T.ty * TILE_H + (int32_t)strip * ROWS_PER_THREAD;
     if (ty0 >= H) continue;
-    const uint32_t band = load_uniform(&J.surf_band_off[T.surface]) + T.ty;
-    const uint32_t e0 = min(load_uniform(&J.band_cnt[band]), J.max_entries);
-    const uint32_t e1 = min(load_uniform(&J.band_cnt[band + 1]), J.max_entries);
+    const uint32_t sidx = tile_base * STRIPS_PER_TILE + item;
+    const uint32_t e0 = min(load_uniform(&J.strip_cnt[sidx]), J.max_strip_entries);
+    const uint32_t e1 = min(load_uniform(&J.strip_cnt[sidx + 1]), J.max_strip_entries);
     if (e0 == e1 && !S.clip) continue;
     const int32_t px = tx0 + (int32_t)lane;
     const bool col_ok = px < W;
@@ -916,7 +972,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
     for (int i = 0; i < ROWS_PER_THREAD; i++) orig[i] = pix[i];
 
     for (uint32_t e = e0; e < e1; e++) {
-      const BandEntry E = load_uniform(&J.entries[e]);
+      const BandEntry E = load_uniform(&J.entries[load_uniform(&J.strip_entries[e])]);
       const DevDraw D = load_uniform(&J.draws[E.draw]);
       // uniform: the draw's composite region misses this strip
       if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;
@@ -1421,6 +1477,13 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
     hipLaunchKernelGGL(k_entry_edges<false>, egrid, dim3(256), 0, st, J);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.entry_ecnt, J.max_entries);
     hipLaunchKernelGGL(k_entry_edges<true>, egrid, dim3(256), 0, st, J);
+  }
+  if (J.n_tiles) {
+    dim3 tgrid((J.n_tiles + 3) / 4);
+    hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.strip_cnt,
+                       J.n_tiles * (uint32_t)STRIPS_PER_TILE);
+    hipLaunchKernelGGL(k_strip_pass<true>, tgrid, dim3(256), 0, st, J);
   }
   return hipGetLastError();
 }
