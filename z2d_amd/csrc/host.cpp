@@ -899,6 +899,13 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const uint32_t n_bands = surf_band_off[ns];
   const uint32_t n_tiles = b->surf_tile_off[ns];
   const uint32_t nd = (uint32_t)b->draws.size();
+  // band pass slots: each band of a surface with its draws in groups of 64
+  uint64_t n_slots = 0;
+  for (uint32_t i = 0; i < ns; i++) {
+    const uint64_t groups = std::max<uint64_t>(1, (surf_draw_off[i + 1] - surf_draw_off[i] + 63) / 64);
+    n_slots += groups * (surf_band_off[i + 1] - surf_band_off[i]);
+  }
+  if (n_slots > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
   uint64_t max_entries = 0, max_rows = 0, scratch = 0, max_entry_edges = 0;
   uint64_t max_strip_entries = 0;
   std::vector<uint32_t> rowprog_list, rowprog_soff;
@@ -934,12 +941,18 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_sdo = L.take((ns + 1) * sizeof(uint32_t));
   const size_t o_sbo = L.take((ns + 1) * sizeof(uint32_t));
   const size_t o_tiles = L.take((size_t)n_tiles * sizeof(TileRef));
+  const size_t o_slotinfo = L.take((size_t)n_slots * sizeof(uint2));
+  const size_t o_bso = L.take((size_t)n_bands * sizeof(uint32_t));
   const size_t o_rpl = L.take(rowprog_list.size() * sizeof(uint32_t));
   const size_t o_rps = L.take(rowprog_soff.size() * sizeof(uint32_t));
   const size_t o_bn = L.take(sizeof(kBlueNoise));
   const size_t input_bytes = L.off;
   const size_t o_edges = L.take(b->nodes.size() * sizeof(z2d_edge));
   const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
+  const size_t o_slotcnt = L.take(((size_t)n_slots + 1) * sizeof(uint32_t));
+  const uint64_t scan_n = std::max<uint64_t>(
+      std::max<uint64_t>(n_slots, max_entries), (uint64_t)n_tiles * STRIPS_PER_TILE);
+  const size_t o_scantmp = L.take((size_t)(scan_n / 4096 + 2) * sizeof(uint32_t));
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
@@ -987,6 +1000,19 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     for (uint32_t ty = 0; ty < bands; ty++)
       for (uint32_t tx = 0; tx < tcols; tx++) ht[k++] = TileRef{i, (uint16_t)tx, (uint16_t)ty};
   }
+  uint2 *hsi = reinterpret_cast<uint2 *>(H + o_slotinfo);
+  uint32_t *hbso = reinterpret_cast<uint32_t *>(H + o_bso);
+  {
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < ns; i++) {
+      const uint32_t groups =
+          std::max<uint32_t>(1, (surf_draw_off[i + 1] - surf_draw_off[i] + 63) / 64);
+      for (uint32_t bd = surf_band_off[i]; bd < surf_band_off[i + 1]; bd++) {
+        hbso[bd] = k;
+        for (uint32_t g = 0; g < groups; g++) hsi[k++] = make_uint2(bd, i);
+      }
+    }
+  }
   if (!rowprog_list.empty()) {
     std::memcpy(H + o_rpl, rowprog_list.data(), rowprog_list.size() * sizeof(uint32_t));
     std::memcpy(H + o_rps, rowprog_soff.data(), rowprog_soff.size() * sizeof(uint32_t));
@@ -1011,6 +1037,11 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.surf_band_off = reinterpret_cast<const uint32_t *>(D + o_sbo);
   J.n_bands = n_bands;
   J.band_cnt = reinterpret_cast<uint32_t *>(D + o_bandcnt);
+  J.slot_info = reinterpret_cast<const uint2 *>(D + o_slotinfo);
+  J.band_slot_off = reinterpret_cast<const uint32_t *>(D + o_bso);
+  J.n_slots = (uint32_t)n_slots;
+  J.slot_cnt = reinterpret_cast<uint32_t *>(D + o_slotcnt);
+  J.scan_tmp = reinterpret_cast<uint32_t *>(D + o_scantmp);
   J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
   J.max_entries = (uint32_t)max_entries;
   J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);

@@ -113,7 +113,13 @@ struct JobView {
   const uint32_t *surf_band_off;  // n_surfaces + 1
   uint32_t n_bands;
   // binning outputs
-  uint32_t *band_cnt;    // n_bands + 1 (scanned in place)
+  uint32_t *band_cnt;    // n_bands + 1: offsets of the band lists in entries
+  // band pass slots: a band and 64 consecutive draws of its surface
+  const uint2 *slot_info;         // per slot: band, surface
+  const uint32_t *band_slot_off;  // n_bands: the band's first slot
+  uint32_t n_slots;
+  uint32_t *slot_cnt;             // n_slots + 1 (scanned in place)
+  uint32_t *scan_tmp;             // block sums of the scans
   BandEntry *entries;    // capacity max_entries
   uint32_t max_entries;
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place

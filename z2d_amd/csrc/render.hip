@@ -8,15 +8,19 @@
 //   k_rowprog     serial row emulation, only for draws whose coverage depends
 //                 on the reference's sort order (unbalanced contours) or whose
 //                 direct-mode unbounded clears depend on whole rows
-//   k_band_pass / k_scan
+//   k_band_pass / scan
 //                 ordered per-band draw lists (TILE_H-row bins, draw order kept)
-//   k_entry_edges / k_scan
+//   k_entry_edges / scan
 //                 per (draw, band) entry, the draw's edges that meet the band
-//   k_render      one workgroup per TILE_W x TILE_H tile: the tile stays in
-//                 registers while every draw touching its band is rasterized
-//                 (exact f64 intercepts accumulated as winding deltas in LDS,
-//                 wave prefix scans for per-sample winding) and composited in
-//                 draw order; HBM sees one read and one write per tile pixel
+//   k_strip_pass / scan
+//                 per strip (TILE_W x ROWS_PER_THREAD pixels), the band
+//                 entries whose composite region meets it, in draw order
+//   k_render      persistent strip workers, one wave per strip at a time: the
+//                 strip stays in registers while each draw of its list is
+//                 rasterized (exact f64 intercepts accumulated as winding
+//                 deltas in LDS, wave prefix scans for per-sample winding) and
+//                 composited in draw order; HBM sees one read and one write
+//                 per strip pixel
 //   k_composite   SurfaceCompositor.run batches (src/compositor.zig:302-440)
 #include <hip/hip_runtime.h>
 
@@ -480,87 +484,161 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
 // ---------------------------------------------------------------------------
 // Binning: ordered draw lists per band (TILE_H rows of one surface).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t band_surface(const JobView &J, uint32_t b) {
-  uint32_t lo = 0, hi = J.n_surfaces;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (J.surf_band_off[mid] <= b) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
+// One wave per slot: a band and 64 consecutive draws of its surface (every
+// band has at least one slot).  FILL=false counts the draws whose composite
+// rows meet the band; FILL=true writes their entries after the scan of the
+// slot counts, in draw order, and the band offsets band_cnt.
 template <bool FILL>
 __global__ void k_band_pass(JobView J) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (b >= J.n_bands) return;
-  const uint32_t s = band_surface(J, b);
+  const uint32_t slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (slot >= J.n_slots) return;
+  const uint2 si = J.slot_info[slot];  // band, surface
+  const uint32_t b = si.x, s = si.y;
+  const uint32_t c = slot - J.band_slot_off[b];
   const int32_t y0 = (int32_t)(b - J.surf_band_off[s]) * TILE_H, y1 = y0 + TILE_H;
-  const uint32_t d0 = J.surf_draw_off[s], d1 = J.surf_draw_off[s + 1];
-  uint32_t count = 0;
-  const uint32_t base = FILL ? J.band_cnt[b] : 0;
-  for (uint32_t d = d0; d < d1; d += 64) {
-    bool hit = false;
-    if (d + lane < d1) {
-      const int2 r = J.draw_rows[d + lane];
-      hit = r.x < y1 && r.y > y0;
-    }
-    uint64_t m = __ballot(hit);
-    if (FILL && hit) {
-      uint32_t pos = count + __popcll(m & ((1ull << lane) - 1ull));
-      if (base + pos < J.max_entries) {
-        BandEntry e;
-        e.draw = d + lane;
-        e.edge_off = 0;
-        e.edge_cnt = 0;
-        e.band = b;
-        J.entries[base + pos] = e;
-      } else {
-        atomicOr(&J.counters[2], 1u);
-      }
-    }
-    count += __popcll(m);
+  const uint32_t d = J.surf_draw_off[s] + c * 64 + lane, d1 = J.surf_draw_off[s + 1];
+  bool hit = false;
+  if (d < d1) {
+    const int2 r = J.draw_rows[d];
+    hit = r.x < y1 && r.y > y0;
   }
-  if (!FILL && lane == 0) J.band_cnt[b] = count;
+  const uint64_t m = __ballot(hit);
+  if (!FILL) {
+    if (lane == 0) J.slot_cnt[slot] = (uint32_t)__popcll(m);
+    return;
+  }
+  const uint32_t base = J.slot_cnt[slot];
+  if (hit) {
+    const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (pos < J.max_entries) {
+      BandEntry e;
+      e.draw = d;
+      e.edge_off = 0;
+      e.edge_cnt = 0;
+      e.band = b;
+      J.entries[pos] = e;
+    } else {
+      atomicOr(&J.counters[2], 1u);
+    }
+  }
+  if (lane == 0) {
+    if (c == 0) J.band_cnt[b] = base;
+    if (slot == 0) J.band_cnt[J.n_bands] = J.slot_cnt[J.n_slots];
+  }
 }
 
 // Exclusive scan of n u32 values in place; a[n] receives the total.  One
-// workgroup of 1024 lanes walks the array in chunks.
-__global__ void k_scan(uint32_t *a, uint32_t n) {
+// workgroup of 1024 lanes walks the array in chunks of 16 values per lane.
+__global__ void __launch_bounds__(1024) k_scan(uint32_t *a, uint32_t n) {
+  constexpr uint32_t PER = 16;
   __shared__ uint32_t wsum[16];
-  __shared__ uint32_t carry;
+  __shared__ uint32_t chunk_total;
   const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < n; base += 1024) {
-    uint32_t i = base + t;
-    uint32_t v = i < n ? a[i] : 0u;
-    // wave inclusive scan
-    uint32_t x = v;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += 1024 * PER) {
+    const uint32_t i0 = base + t * PER;
+    uint32_t v[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; k++) {
+      v[k] = i0 + k < n ? a[i0 + k] : 0u;
+      sum += v[k];
+    }
+    // wave inclusive scan of the lane sums
+    uint32_t x = sum;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      uint32_t y = __shfl_up(x, off, 64);
+      const uint32_t y = __shfl_up(x, off, 64);
       if (lane >= (uint32_t)off) x += y;
     }
     if (lane == 63) wsum[w] = x;
     __syncthreads();
     if (t < 16) {
-      uint32_t s = wsum[t];
-      uint32_t z = s;
+      const uint32_t s0 = wsum[t];
+      uint32_t z = s0;
       for (int off = 1; off < 16; off <<= 1) {
-        uint32_t y = __shfl_up(z, off, 16);
-        if ((t & 15) >= (uint32_t)off) z += y;
+        const uint32_t y = __shfl_up(z, off, 16);
+        if (t >= (uint32_t)off) z += y;
       }
-      wsum[t] = z - s;  // exclusive wave offsets
+      wsum[t] = z - s0;  // exclusive wave offsets
+      if (t == 15) chunk_total = z;
     }
     __syncthreads();
-    uint32_t c = carry;
-    if (i < n) a[i] = c + wsum[w] + x - v;
-    __syncthreads();
-    if (t == 1023) carry = c + wsum[w] + x;
+    uint32_t run = carry + wsum[w] + x - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; k++) {
+      if (i0 + k < n) a[i0 + k] = run;
+      run += v[k];
+    }
+    carry += chunk_total;
     __syncthreads();
   }
   if (t == 0) a[n] = carry;
+}
+
+// Parallel form of k_scan for long arrays: block sums (k_scan_reduce), their
+// scan (k_scan), then each block's values scanned from its offset
+// (k_scan_apply).  SCAN_BLOCK values per block, 16 per lane.
+constexpr uint32_t SCAN_BLOCK = 4096;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t *a, uint32_t n,
+                                                     uint32_t *bsum) {
+  __shared__ uint32_t ws[4];
+  const uint32_t t = threadIdx.x;
+  const uint32_t i0 = blockIdx.x * SCAN_BLOCK + t * 16;
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) sum += i0 + k < n ? a[i0 + k] : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if ((t & 63) == 0) ws[t >> 6] = sum;
+  __syncthreads();
+  if (t == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// boff: the scanned block sums; the last lane of the last block writes a[n].
+__global__ void __launch_bounds__(256) k_scan_apply(uint32_t *a, uint32_t n,
+                                                    const uint32_t *boff) {
+  __shared__ uint32_t ws[4];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint32_t i0 = blockIdx.x * SCAN_BLOCK + t * 16;
+  uint32_t v[16];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    v[k] = i0 + k < n ? a[i0 + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t x = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  uint32_t run = boff[blockIdx.x] + x - sum;
+  for (uint32_t j = 0; j < w; j++) run += ws[j];
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    if (i0 + k < n) a[i0 + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && t == 255) a[n] = run;
+}
+
+// Exclusive scan of a[0..n) in place with a[n] = total; tmp holds
+// n / SCAN_BLOCK + 2 values.
+static void scan_u32(uint32_t *a, uint32_t n, uint32_t *tmp, hipStream_t st) {
+  const uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  if (nb <= 1) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, a, n);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, st, a, n, tmp);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, tmp, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, a, n, tmp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1468,21 +1546,20 @@ hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n
 
 hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   if (J.n_bands == 0) return hipSuccess;
-  dim3 grid((J.n_bands + 3) / 4);
+  dim3 grid((J.n_slots + 3) / 4);
   hipLaunchKernelGGL(k_band_pass<false>, grid, dim3(256), 0, st, J);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.band_cnt, J.n_bands);
+  scan_u32(J.slot_cnt, J.n_slots, J.scan_tmp, st);
   hipLaunchKernelGGL(k_band_pass<true>, grid, dim3(256), 0, st, J);
   if (J.max_entries) {
     dim3 egrid((J.max_entries + 3) / 4);
     hipLaunchKernelGGL(k_entry_edges<false>, egrid, dim3(256), 0, st, J);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.entry_ecnt, J.max_entries);
+    scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
     hipLaunchKernelGGL(k_entry_edges<true>, egrid, dim3(256), 0, st, J);
   }
   if (J.n_tiles) {
     dim3 tgrid((J.n_tiles + 3) / 4);
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.strip_cnt,
-                       J.n_tiles * (uint32_t)STRIPS_PER_TILE);
+    scan_u32(J.strip_cnt, J.n_tiles * (uint32_t)STRIPS_PER_TILE, J.scan_tmp, st);
     hipLaunchKernelGGL(k_strip_pass<true>, tgrid, dim3(256), 0, st, J);
   }
   return hipGetLastError();
