@@ -909,9 +909,14 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   uint64_t max_entries = 0, max_rows = 0, scratch = 0, max_entry_edges = 0;
   uint64_t max_strip_entries = 0;
   std::vector<uint32_t> rowprog_list, rowprog_soff;
+  std::vector<uint2> egroups;
   for (uint32_t i = 0; i < nd; i++) {
     HostDraw &h = b->draws[i];
     h.d.surface = h.surface_slot;
+    h.d.band_lo = h.band_lo;
+    h.d.band_hi = h.band_hi;
+    h.d.entry_base = (uint32_t)std::min<uint64_t>(max_entries, 0xFFFFFFFFull);
+    for (uint32_t k = 0; k < h.d.node_cnt; k += EDGE_GROUP) egroups.push_back(make_uint2(i, k));
     max_entries += h.band_hi - h.band_lo + 1;
     max_strip_entries += (uint64_t)(h.band_hi - h.band_lo + 1) * STRIPS_PER_TILE *
                          (h.tcol_hi - h.tcol_lo + 1);
@@ -927,7 +932,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   }
   if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull ||
       max_entry_edges > 0xFFFFFFF0ull || max_strip_entries > 0xFFFFFFF0ull ||
-      (uint64_t)n_tiles * STRIPS_PER_TILE > 0xFFFFFFF0ull)
+      (uint64_t)n_tiles * STRIPS_PER_TILE > 0xFFFFFFF0ull || egroups.size() > 0xFFFFFFF0ull ||
+      2 * max_entries + 1 > 0xFFFFFFF0ull)
     return Z2D_ERR_OUT_OF_MEMORY;
   b->n_rowprog = (uint32_t)rowprog_list.size();
 
@@ -942,6 +948,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_sbo = L.take((ns + 1) * sizeof(uint32_t));
   const size_t o_tiles = L.take((size_t)n_tiles * sizeof(TileRef));
   const size_t o_slotinfo = L.take((size_t)n_slots * sizeof(uint2));
+  const size_t o_egroups = L.take(egroups.size() * sizeof(uint2));
   const size_t o_bso = L.take((size_t)n_bands * sizeof(uint32_t));
   const size_t o_rpl = L.take(rowprog_list.size() * sizeof(uint32_t));
   const size_t o_rps = L.take(rowprog_soff.size() * sizeof(uint32_t));
@@ -955,7 +962,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_scantmp = L.take((size_t)(scan_n / 4096 + 2) * sizeof(uint32_t));
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
-  const size_t o_ecnt = L.take((max_entries + 1) * sizeof(uint32_t));
+  const size_t o_blist = L.take(max_entries * sizeof(uint32_t));
+  const size_t o_ecnt = L.take((2 * max_entries + 1) * sizeof(uint32_t));  // + cursors
   const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
   const size_t o_stripcnt = L.take(((size_t)n_tiles * STRIPS_PER_TILE + 1) * sizeof(uint32_t));
   const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
@@ -1000,6 +1008,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     for (uint32_t ty = 0; ty < bands; ty++)
       for (uint32_t tx = 0; tx < tcols; tx++) ht[k++] = TileRef{i, (uint16_t)tx, (uint16_t)ty};
   }
+  if (!egroups.empty())
+    std::memcpy(H + o_egroups, egroups.data(), egroups.size() * sizeof(uint2));
   uint2 *hsi = reinterpret_cast<uint2 *>(H + o_slotinfo);
   uint32_t *hbso = reinterpret_cast<uint32_t *>(H + o_bso);
   {
@@ -1045,6 +1055,10 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
   J.max_entries = (uint32_t)max_entries;
   J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);
+  J.entry_cur = J.entry_ecnt + max_entries + 1;
+  J.band_list = reinterpret_cast<uint32_t *>(D + o_blist);
+  J.egroups = reinterpret_cast<const uint2 *>(D + o_egroups);
+  J.n_egroups = (uint32_t)egroups.size();
   J.entry_edges = reinterpret_cast<z2d_edge *>(D + o_eidx);
   J.max_entry_edges = (uint32_t)max_entry_edges;
   J.strip_cnt = reinterpret_cast<uint32_t *>(D + o_stripcnt);

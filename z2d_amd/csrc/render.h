@@ -27,6 +27,8 @@ constexpr int ACC_WORDS = TILE_W * 2;  // 4 sample columns per pixel, two int16 
 // holds more edges than this cannot overflow them and takes the exact
 // per-thread path instead.
 constexpr uint32_t PACKED_EDGE_LIMIT = 32767;
+// Edge slots of one draw per k_edge_bins work item.
+constexpr uint32_t EDGE_GROUP = 512;
 
 // Flattened path node (move_to / line_to / close_path only; curves are
 // flattened on the host by the reference's Spline algorithm first).
@@ -66,6 +68,9 @@ struct DevDraw {
   uint32_t input_edges;  // nonzero: edges given directly (stroke polygons)
   uint32_t in_edge_off;  // their offset in JobView::in_edges
   double in_ext[4];      // extents when edges are given directly
+  // the host's band estimate [band_lo, band_hi] (bands of the surface) and
+  // the draw's entries: entry_base + (band - band_lo)
+  uint32_t band_lo, band_hi, entry_base;
   // written by k_edge_build
   uint32_t edge_cnt;
   uint32_t flags;
@@ -113,16 +118,20 @@ struct JobView {
   const uint32_t *surf_band_off;  // n_surfaces + 1
   uint32_t n_bands;
   // binning outputs
-  uint32_t *band_cnt;    // n_bands + 1: offsets of the band lists in entries
+  uint32_t *band_cnt;    // n_bands + 1: offsets of the band lists in band_list
+  uint32_t *band_list;   // entry indices, capacity max_entries
   // band pass slots: a band and 64 consecutive draws of its surface
   const uint2 *slot_info;         // per slot: band, surface
   const uint32_t *band_slot_off;  // n_bands: the band's first slot
   uint32_t n_slots;
   uint32_t *slot_cnt;             // n_slots + 1 (scanned in place)
   uint32_t *scan_tmp;             // block sums of the scans
-  BandEntry *entries;    // capacity max_entries
+  BandEntry *entries;    // per draw and estimated band, capacity max_entries
   uint32_t max_entries;
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place
+  uint32_t *entry_cur;       // max_entries: fill cursors (follows entry_ecnt)
+  const uint2 *egroups;      // edge-binning work: draw, first edge slot
+  uint32_t n_egroups;
   z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges
   uint32_t max_entry_edges;
   // per strip (tile * STRIPS_PER_TILE + strip): the band entries whose

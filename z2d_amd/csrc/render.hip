@@ -8,10 +8,11 @@
 //   k_rowprog     serial row emulation, only for draws whose coverage depends
 //                 on the reference's sort order (unbalanced contours) or whose
 //                 direct-mode unbounded clears depend on whole rows
-//   k_band_pass / scan
-//                 ordered per-band draw lists (TILE_H-row bins, draw order kept)
-//   k_entry_edges / scan
+//   k_edge_bins / scan
 //                 per (draw, band) entry, the draw's edges that meet the band
+//   k_band_pass / scan
+//                 ordered per-band lists of entries (TILE_H-row bins, draw
+//                 order kept)
 //   k_strip_pass / scan
 //                 per strip (TILE_W x ROWS_PER_THREAD pixels), the band
 //                 entries whose composite region meets it, in draw order
@@ -486,8 +487,9 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
 // ---------------------------------------------------------------------------
 // One wave per slot: a band and 64 consecutive draws of its surface (every
 // band has at least one slot).  FILL=false counts the draws whose composite
-// rows meet the band; FILL=true writes their entries after the scan of the
-// slot counts, in draw order, and the band offsets band_cnt.
+// rows meet the band; FILL=true writes, after the scan of the slot counts,
+// the band list (the draws' entry indices, in draw order), the band offsets
+// band_cnt and each listed entry's record (its edge list from k_edge_bins).
 template <bool FILL>
 __global__ void k_band_pass(JobView J) {
   const uint32_t lane = threadIdx.x & 63;
@@ -498,10 +500,23 @@ __global__ void k_band_pass(JobView J) {
   const uint32_t c = slot - J.band_slot_off[b];
   const int32_t y0 = (int32_t)(b - J.surf_band_off[s]) * TILE_H, y1 = y0 + TILE_H;
   const uint32_t d = J.surf_draw_off[s] + c * 64 + lane, d1 = J.surf_draw_off[s + 1];
+  const uint32_t lb = b - J.surf_band_off[s];  // band within the surface
   bool hit = false;
+  uint32_t idx = 0;
   if (d < d1) {
     const int2 r = J.draw_rows[d];
     hit = r.x < y1 && r.y > y0;
+    if (hit) {
+      // the draw's entry for this band (entries are owned by draws)
+      const DevDraw &D = J.draws[d];
+      const uint32_t blo = D.band_lo, bhi = D.band_hi;
+      if (lb < blo || lb > bhi) {
+        atomicOr(&J.counters[2], 1u);  // outside the host's band estimate
+        hit = false;
+      } else {
+        idx = D.entry_base + (lb - blo);
+      }
+    }
   }
   const uint64_t m = __ballot(hit);
   if (!FILL) {
@@ -512,12 +527,15 @@ __global__ void k_band_pass(JobView J) {
   if (hit) {
     const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (pos < J.max_entries) {
+      J.band_list[pos] = idx;
+      // only indices below the edge capacity were stored
+      const uint32_t off = J.entry_ecnt[idx], cnt = J.entry_ecnt[idx + 1] - off;
       BandEntry e;
       e.draw = d;
-      e.edge_off = 0;
-      e.edge_cnt = 0;
+      e.edge_off = off;
+      e.edge_cnt = off >= J.max_entry_edges ? 0u : min(cnt, J.max_entry_edges - off);
       e.band = b;
-      J.entries[pos] = e;
+      J.entries[idx] = e;
     } else {
       atomicOr(&J.counters[2], 1u);
     }
@@ -642,8 +660,8 @@ static void scan_u32(uint32_t *a, uint32_t n, uint32_t *tmp, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// k_strip_pass: per strip (ROWS_PER_THREAD rows of one tile), the band entries
-// of its band whose composite region meets the strip, in draw order.  One
+// k_strip_pass: per strip (ROWS_PER_THREAD rows of one tile), the entries of
+// its band list whose composite region meets the strip, in draw order.  One
 // wave per tile walks the band's entries; FILL=false counts per strip,
 // FILL=true writes after the scan.
 // ---------------------------------------------------------------------------
@@ -665,9 +683,10 @@ __global__ void k_strip_pass(JobView J) {
   }
   const uint64_t below = (1ull << lane) - 1ull;
   for (uint32_t e = e0; e < e1; e += 64) {
-    uint32_t mask = 0;
+    uint32_t mask = 0, idx = 0;
     if (e + lane < e1) {
-      const DevDraw &D = J.draws[J.entries[e + lane].draw];
+      idx = J.band_list[e + lane];
+      const DevDraw &D = J.draws[J.entries[idx].draw];
       const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
       if (rx1 > tx0 && rx0 < tx0 + TILE_W) {
 #pragma unroll
@@ -684,7 +703,7 @@ __global__ void k_strip_pass(JobView J) {
       if (FILL && hit) {
         const uint32_t pos = base[s] + cnt[s] + (uint32_t)__popcll(m & below);
         if (pos < J.max_strip_entries)
-          J.strip_entries[pos] = e + lane;
+          J.strip_entries[pos] = idx;
         else
           atomicOr(&J.counters[2], 1u);
       }
@@ -698,59 +717,95 @@ __global__ void k_strip_pass(JobView J) {
 }
 
 // ---------------------------------------------------------------------------
-// k_entry_edges: per (draw, band) entry, copies of the draw's edges whose
+// k_edge_bins: per (draw, band) entry, copies of the draw's edges whose
 // y-range meets the band's sample rows (top < last mid && bottom >= first mid,
 // the active-edge test of src/internal/raster/multisample.zig and direct.zig
-// taken over the band).  Tiles then walk only those edges.  One wave per
-// entry; FILL=false counts, FILL=true writes after the scan.
+// taken over the band); strips then walk only those edges.  One wave per
+// group of EDGE_GROUP edge slots of a draw: each chunk of 64 edges is tested
+// against the bands it can meet.  FILL=false counts per entry (atomics on
+// the zeroed entry_ecnt); FILL=true, after the scan, appends to the entry's
+// list through the entry_cur cursors (the order inside a list is free: the
+// winding deltas of the edges are summed).
 // ---------------------------------------------------------------------------
 template <bool FILL>
-__global__ void k_entry_edges(JobView J) {
+__global__ void k_edge_bins(JobView J) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (e >= J.max_entries) return;
-  const uint32_t total = min(J.band_cnt[J.n_bands], J.max_entries);
-  if (e >= total) {
-    if (!FILL && lane == 0) J.entry_ecnt[e] = 0;
-    return;
-  }
-  BandEntry &E = J.entries[e];
-  const DevDraw &D = J.draws[E.draw];
-  const bool direct_unb = D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED);
+  const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= J.n_egroups) return;
+  const uint2 it = J.egroups[g];  // draw, first edge slot
+  const DevDraw &D = J.draws[it.x];
+  if (D.mode == MODE_DIRECT && (D.flags & DRAW_UNBOUNDED)) return;  // row records instead
+  const int2 rows = J.draw_rows[it.x];
+  if (rows.x >= rows.y) return;  // no entries
+  const uint32_t ne = D.edge_cnt;
+  if (it.y >= ne) return;
+  const uint32_t kend = min(it.y + EDGE_GROUP, ne);
   const int SC = D.mode == MODE_DIRECT ? 1 : 4;
-  const int32_t y0 = (int32_t)(E.band - J.surf_band_off[D.surface]) * TILE_H;
-  const double ys_lo = (double)(y0 * SC) + 0.5;
-  const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+  const int32_t rb_lo = rows.x / TILE_H, rb_hi = (rows.y - 1) / TILE_H;  // entry bands
+  const uint32_t blo = D.band_lo, bhi = D.band_hi, ebase = D.entry_base;
+  const double band_rows = (double)(TILE_H * SC);
   const z2d_edge *edges = J.edges + D.node_off;
-  const uint32_t ne = direct_unb ? 0u : D.edge_cnt;
-  const uint32_t base = FILL ? J.entry_ecnt[e] : 0u;
-  uint32_t count = 0;
-  for (uint32_t k0 = 0; k0 < ne; k0 += 64) {
-    bool hit = false;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t k0 = it.y; k0 < kend; k0 += 64) {
+    const bool valid = k0 + lane < kend;
     z2d_edge ed{0.0, 0.0, 0.0, 0.0};
-    if (k0 + lane < ne) {
+    double top = 0.0, bot = 0.0;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;  // candidate bands, widened by one
+    if (valid) {
       ed = edges[k0 + lane];
-      const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-      const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-      hit = top < ys_hi && bot >= ys_lo;
+      top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
+      bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+      const double flo = fmax(floor(top / band_rows) - 1.0, (double)rb_lo);
+      const double fhi = fmin(floor(bot / band_rows) + 1.0, (double)rb_hi);
+      if (flo <= fhi) {  // (a NaN edge fails the exact test below)
+        lo = (int32_t)flo;
+        hi = (int32_t)fhi;
+      }
     }
-    const uint64_t m = __ballot(hit);
-    if (FILL && hit) {
-      const uint32_t pos = base + count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      if (pos < J.max_entry_edges)
-        J.entry_edges[pos] = ed;
-      else
-        atomicOr(&J.counters[2], 1u);
+    int32_t ulo = lo, uhi = hi;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      ulo = min(ulo, __shfl_xor(ulo, off, 64));
+      uhi = max(uhi, __shfl_xor(uhi, off, 64));
     }
-    count += (uint32_t)__popcll(m);
-  }
-  if (lane == 0) {
-    if (!FILL) {
-      J.entry_ecnt[e] = count;
-    } else {
-      // only indices below the capacity were stored
-      E.edge_off = base;
-      E.edge_cnt = base >= J.max_entry_edges ? 0u : min(count, J.max_entry_edges - base);
+    // bands in windows of 64: lane j holds the count (then the list
+    // position) of band w + j, so each window takes one vector atomic
+    for (int32_t w = ulo; w <= uhi; w += 64) {
+      const int32_t wend = min(uhi, w + 63);
+      uint32_t cnt = 0;
+      for (int32_t bnd = w; bnd <= wend; bnd++) {
+        const int32_t y0 = bnd * TILE_H;
+        const double ys_lo = (double)(y0 * SC) + 0.5;
+        const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+        const bool hit = lo <= bnd && bnd <= hi && top < ys_hi && bot >= ys_lo;
+        const uint32_t n = (uint32_t)__popcll(__ballot(hit));
+        if ((int32_t)lane == bnd - w) cnt = n;
+      }
+      const int32_t bj = w + (int32_t)lane;  // this lane's band
+      const bool has = cnt != 0 && bj <= wend && (uint32_t)bj >= blo && (uint32_t)bj <= bhi;
+      const uint32_t idx = has ? ebase + ((uint32_t)bj - blo) : 0u;
+      if (!FILL) {
+        if (has) atomicAdd(&J.entry_ecnt[idx], cnt);
+        continue;
+      }
+      const uint32_t at = has ? J.entry_ecnt[idx] + atomicAdd(&J.entry_cur[idx], cnt) : 0u;
+      for (int32_t bnd = w; bnd <= wend; bnd++) {
+        if ((uint32_t)bnd < blo || (uint32_t)bnd > bhi) continue;
+        const int32_t y0 = bnd * TILE_H;
+        const double ys_lo = (double)(y0 * SC) + 0.5;
+        const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+        const bool hit = lo <= bnd && bnd <= hi && top < ys_hi && bot >= ys_lo;
+        const uint64_t m = __ballot(hit);
+        if (m == 0) continue;
+        const uint32_t base = (uint32_t)__shfl((int)at, bnd - w, 64);
+        if (hit) {
+          const uint32_t pos = base + (uint32_t)__popcll(m & below);
+          if (pos < J.max_entry_edges)
+            J.entry_edges[pos] = ed;
+          else
+            atomicOr(&J.counters[2], 1u);
+        }
+      }
     }
   }
 }
@@ -1546,16 +1601,19 @@ hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n
 
 hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   if (J.n_bands == 0) return hipSuccess;
+  // entry edge lists: counts, offsets, then the copies
+  const uint64_t ne2 = 2ull * J.max_entries + 1;  // entry_ecnt and entry_cur
+  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((ne2 + 255) / 256)), dim3(256), 0, st,
+                     J.entry_ecnt, ne2, 0u);
+  dim3 ggrid((J.n_egroups + 3) / 4);
+  if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<false>, ggrid, dim3(256), 0, st, J);
+  scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
+  if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<true>, ggrid, dim3(256), 0, st, J);
+  // band lists and entry records
   dim3 grid((J.n_slots + 3) / 4);
   hipLaunchKernelGGL(k_band_pass<false>, grid, dim3(256), 0, st, J);
   scan_u32(J.slot_cnt, J.n_slots, J.scan_tmp, st);
   hipLaunchKernelGGL(k_band_pass<true>, grid, dim3(256), 0, st, J);
-  if (J.max_entries) {
-    dim3 egrid((J.max_entries + 3) / 4);
-    hipLaunchKernelGGL(k_entry_edges<false>, egrid, dim3(256), 0, st, J);
-    scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
-    hipLaunchKernelGGL(k_entry_edges<true>, egrid, dim3(256), 0, st, J);
-  }
   if (J.n_tiles) {
     dim3 tgrid((J.n_tiles + 3) / 4);
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
