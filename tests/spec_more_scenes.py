@@ -1,0 +1,355 @@
+"""More of the reference's acceptance cases (spec/NNN_*.zig), restated as
+recorded Context calls: strokes under transformations (043-045), a gradient
+stroke (059), curved gradient fills (060), draws partly or wholly outside the
+surface (075-077) and a doubled close (078).  The expected outputs are the
+reference's own fixture PNGs (tests/golden/spec/), compared the same way as
+the stroke cases: the multisample mode against the supersampled fixture."""
+import math
+
+from z2d_amd import abi
+from spec_comp_scenes import RGB1
+from spec_scenes import _ctx, _scene
+from zscene import color_rgba_encode
+
+WHITE = abi.Pixel(abi.RGB, 0xFF, 0xFF, 0xFF, 0)
+RED_F3 = abi.Pixel(abi.RGB, 0xF3, 0x00, 0x00, 0)
+GREEN = abi.Pixel(abi.RGB, 0x00, 0xFF, 0x00, 0)
+MAGENTA = abi.Pixel(abi.RGB, 0xFF, 0x00, 0xFF, 0)
+BLACK = abi.Pixel(abi.RGB, 0x00, 0x00, 0x00, 0)
+
+
+def _user_width(c, w):
+    """The spec's line-width block: deviceToUserDistance of (w, w) under the
+    current transformation, the larger component."""
+    ux, uy = c.transformation.device_to_user_distance(w, w)
+    return uy if ux < uy else ux
+
+
+def _rect043(c, x, y, h, w, reset_ctm):
+    saved = c.transformation
+    c.reset_path()
+    c.translate(x, y)
+    c.scale(h, w)
+    c.move_to(0, 0)
+    c.line_to(1, 0)
+    c.line_to(1, 1)
+    c.line_to(0, 1)
+    if reset_ctm:
+        c.set_transformation(saved)
+    c.close_path()
+
+
+def s043(aa):  # spec/043_rect_transforms.zig
+    sfc, c = _ctx(abi.RGB, 400, 400, aa)
+    c.set_source_pixel(WHITE)
+    c.line_width = 5.0
+    c.translate(10, 10)
+    _rect043(c, 0, 0, 50, 100, True)
+    c.fill()
+    _rect043(c, 100, 0, 50, 100, True)
+    c.stroke()
+
+    saved, saved_w = c.transformation, c.line_width
+    _rect043(c, 200, 0, 50, 100, False)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    c.set_transformation(saved)
+    c.line_width = saved_w
+
+    saved, saved_w = c.transformation, c.line_width
+    c.rotate(math.pi / 4.0)
+    _rect043(c, 300, 0, 50, 100, False)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    c.set_transformation(saved)
+    c.line_width = saved_w
+    return _scene("043_rect_transforms", sfc, c)
+
+
+def _line044(c, i, reverse, round_cap):
+    saved, saved_w, saved_src = c.transformation, c.line_width, c.pattern
+    c.cap = abi.CAP_ROUND if round_cap else abi.CAP_SQUARE
+    y_offset = 50.0 + (100.0 if reverse else 0.0) + (200.0 if round_cap else 0.0)
+    if reverse:
+        c.translate(i * 100 + 25, y_offset + 25 * math.sin(math.pi / 6.0 * i))
+        c.rotate(-math.pi / 6.0 * i)
+    else:
+        c.translate(i * 100 + 25, y_offset - 25 * math.sin(math.pi / 6.0 * i))
+        c.rotate(math.pi / 6.0 * i)
+    c.scale(2, 1)
+    if reverse:
+        c.move_to(0, 0)
+        c.line_to(25, 0)
+    else:
+        c.move_to(25, 0)
+        c.line_to(0, 0)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    # the 1-unit guide line in red
+    c.set_source_pixel(RED_F3)
+    c.line_width = 1.0
+    c.stroke()
+    # the spec's deferred restores
+    c.cap = abi.CAP_BUTT
+    c.pattern = saved_src
+    c.line_width = saved_w
+    c.set_transformation(saved)
+    c.reset_path()
+
+
+def s044(aa):  # spec/044_line_transforms.zig
+    sfc, c = _ctx(abi.RGB, 700, 400, aa)
+    c.set_source_pixel(WHITE)
+    c.line_width = 10.0
+    for reverse, round_cap in ((False, False), (True, False), (False, True), (True, True)):
+        for i in range(7):
+            _line044(c, float(i), reverse, round_cap)
+    return _scene("044_line_transforms", sfc, c)
+
+
+def _line045(c, i):
+    saved, saved_w, saved_src = c.transformation, c.line_width, c.pattern
+    x_offset = 200 * math.fmod(i, 4.0) + 100 - 50 * math.cos(math.pi / 6.0 * i)
+    y_offset = 200 * math.floor(i / 4.0) + 75 - 37.5 * math.sin(math.pi / 6.0 * i)
+    c.translate(x_offset, y_offset)
+    c.rotate(math.pi / 6.0 * i)
+    c.scale(2, 1)
+    c.move_to(0, 0)
+    c.line_to(25, 75)
+    c.line_to(50, 0)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    c.set_source_pixel(RED_F3)
+    c.line_width = 1.0
+    c.stroke()
+    c.pattern = saved_src
+    c.line_width = saved_w
+    c.set_transformation(saved)
+    c.reset_path()
+
+
+def s045(aa):  # spec/045_round_join_transforms.zig
+    sfc, c = _ctx(abi.RGB, 800, 600, aa)
+    c.set_source_pixel(WHITE)
+    c.line_width = 20.0
+    c.cap = abi.CAP_ROUND
+    c.join = abi.JOIN_ROUND
+    for i in range(12):
+        _line045(c, float(i))
+    return _scene("045_round_join_transforms", sfc, c)
+
+
+def _rgb_gradient(x0, y0, x1, y1):
+    return abi.Gradient(abi.GRADIENT_LINEAR, [x0, y0, x1, y1],
+                        [(0.0, RGB1(1, 0, 0)), (0.5, RGB1(0, 1, 0)), (1.0, RGB1(0, 0, 1))])
+
+
+def s059(aa):  # spec/059_stroke_star_gradient.zig
+    width = height = 300
+    sfc, c = _ctx(abi.RGB, width, height, aa)
+    c.line_width = 10.0
+    c.join = abi.JOIN_ROUND
+    margin, x_scale, y_scale = 20, 3, 5
+    c.set_source_gradient(_rgb_gradient(0 + margin * 3, height // 2, width - margin * 3,
+                                        height // 2))
+    c.move_to(width // 2, 0 + margin)
+    c.line_to(width - margin * x_scale - 1, height - margin - 1)
+    c.line_to(0 + margin, 0 + margin * y_scale)
+    c.line_to(width - margin - 1, 0 + margin * y_scale)
+    c.line_to(0 + margin * x_scale, height - margin - 1)
+    c.close_path()
+    c.stroke()
+    return _scene("059_stroke_star_gradient", sfc, c)
+
+
+# spec/060_ghostty_logo.zig: four filled outlines (the gradient body, its
+# black inner ring, the gradient face) and two black glyph marks.
+_GHOSTTY = [
+    ("g", (62.186, 97.000), [
+        (58.431, 97.000, 54.746, 95.875, 51.637, 93.800),
+        (48.528, 95.875, 44.836, 97.000, 41.088, 97.000),
+        (37.339, 97.000, 33.647, 95.875, 30.538, 93.800),
+        (27.451, 95.875, 23.878, 96.972, 20.115, 97.000),
+        (20.003, 97.000),
+        (14.897, 97.000, 10.107, 94.968, 6.499, 91.282),
+        (2.948, 87.653, 0.993, 82.878, 0.993, 77.835),
+        (0.993, 41.088),
+        (1.000, 18.983, 18.983, 1.000, 41.088, 1.000),
+        (63.192, 1.000, 81.176, 18.983, 81.176, 41.088),
+        (81.176, 77.849),
+        (81.176, 88.026, 73.298, 96.423, 63.242, 96.972),
+        (62.890, 96.993, 62.538, 97.000, 62.186, 97.000)]),
+    ("k", (62.186, 92.780), [
+        (58.832, 92.780, 55.554, 91.627, 52.945, 89.531),
+        (52.495, 89.165, 52.010, 89.095, 51.686, 89.095),
+        (51.173, 89.095, 50.652, 89.278, 50.223, 89.622),
+        (47.635, 91.662, 44.393, 92.787, 41.088, 92.787),
+        (37.782, 92.787, 34.540, 91.662, 31.952, 89.622),
+        (31.537, 89.292, 31.044, 89.123, 30.538, 89.123),
+        (30.032, 89.123, 29.539, 89.299, 29.125, 89.622),
+        (26.529, 91.669, 23.407, 92.759, 20.094, 92.787),
+        (19.996, 92.787),
+        (16.050, 92.787, 12.330, 91.212, 9.524, 88.343),
+        (6.753, 85.508, 5.219, 81.781, 5.219, 77.849),
+        (5.219, 41.102),
+        (5.219, 21.311, 21.311, 5.220, 41.088, 5.220),
+        (60.864, 5.220, 76.956, 21.311, 76.956, 41.088),
+        (76.956, 77.849),
+        (76.956, 85.782, 70.830, 92.330, 63.009, 92.759),
+        (62.735, 92.773, 62.461, 92.780, 62.186, 92.780)]),
+    ("g", (72.736, 41.088), [
+        (72.736, 77.849),
+        (72.736, 83.476, 68.396, 88.237, 62.777, 88.547),
+        (60.048, 88.694, 57.530, 87.808, 55.582, 86.240),
+        (53.247, 84.362, 49.963, 84.446, 47.607, 86.303),
+        (45.813, 87.717, 43.549, 88.561, 41.080, 88.561),
+        (38.612, 88.561, 36.354, 87.717, 34.561, 86.303),
+        (32.177, 84.425, 28.892, 84.425, 26.508, 86.303),
+        (24.729, 87.703, 22.492, 88.547, 20.059, 88.561),
+        (14.214, 88.603, 9.439, 83.680, 9.439, 77.835),
+        (9.439, 41.088),
+        (9.439, 23.611, 23.610, 9.440, 41.087, 9.440),
+        (58.564, 9.440, 72.736, 23.611, 72.736, 41.088)]),
+    ("k", (34.842, 38.310), [
+        (23.048, 31.502),
+        (21.515, 30.616, 19.546, 31.143, 18.660, 32.676),
+        (17.773, 34.210, 18.301, 36.179, 19.834, 37.065),
+        (26.811, 41.095),
+        (19.834, 45.125),
+        (18.301, 46.011, 17.773, 47.973, 18.660, 49.513),
+        (19.546, 51.047, 21.508, 51.574, 23.048, 50.688),
+        (34.842, 43.880),
+        (36.981, 42.642, 36.981, 39.555, 34.842, 38.317),
+        (34.842, 38.310)]),
+    ("k", (61.547, 37.874), [
+        (46.053, 37.874),
+        (44.281, 37.874, 42.839, 39.309, 42.839, 41.088),
+        (42.839, 42.867, 44.274, 44.302, 46.053, 44.302),
+        (61.547, 44.302),
+        (63.319, 44.302, 64.761, 42.867, 64.761, 41.088),
+        (64.761, 39.309, 63.326, 37.874, 61.547, 37.874)]),
+]
+
+
+def s060(aa):  # spec/060_ghostty_logo.zig
+    sfc, c = _ctx(abi.RGBA, 82, 100, aa)
+    grad = _rgb_gradient(0, 49, 82, 49)
+    for paint, start, segs in _GHOSTTY:
+        if paint == "g":
+            c.set_source_gradient(grad)
+        else:
+            c.set_source_pixel(BLACK)
+        c.reset_path()
+        c.move_to(*start)
+        for seg in segs:
+            if len(seg) == 2:
+                c.line_to(*seg)
+            else:
+                c.curve_to(*seg)
+        c.close_path()
+        c.fill()
+    return _scene("060_ghostty_logo", sfc, c)
+
+
+def _fill_then_stroke(c):
+    c.set_source_pixel(GREEN)
+    c.fill()
+    c.set_source_pixel(MAGENTA)
+    c.stroke()
+
+
+def _grey_ctx(aa, width, height):
+    return _ctx(abi.RGBA, width, height, aa, color_rgba_encode(0.33, 0.33, 0.33))
+
+
+def s075(aa):  # spec/075_oob_draw_corners.zig
+    width = height = 300
+    cx, cy, margin = width // 2, height // 2, 10
+    sfc, c = _grey_ctx(aa, width, height)
+    c.translate(-margin, -margin)
+    c.line_to(cx, cy)  # no current point: a move
+    c.line_to(0 - cx, cy)
+    c.line_to(cx, 0 - cy)
+    c.close_path()
+    _fill_then_stroke(c)
+    for tx, ty, pts in ((margin, -margin, [(cx, cy), (width + cx, cy), (width + cx, 0 - cy),
+                                           (cx, 0 - cy)]),
+                        (-margin, margin, [(cx, cy), (0 - cx, cy), (0 - cx, height + cy),
+                                           (cx, height + cy)]),
+                        (margin, margin, [(cx, cy), (width + cx, cy),
+                                          (width + cx, height + cy), (cx, height + cy)])):
+        c.reset_path()
+        c.set_identity()
+        c.translate(tx, ty)
+        c.move_to(*pts[0])
+        for p in pts[1:]:
+            c.line_to(*p)
+        c.close_path()
+        _fill_then_stroke(c)
+    return _scene("075_oob_draw_corners", sfc, c)
+
+
+def s076(aa):  # spec/076_oob_draw_sides.zig
+    width = height = 300
+    cx, cy, margin = width // 2, height // 2, 10
+    sfc, c = _grey_ctx(aa, width, height)
+    draws = [
+        (0, -margin, [(cx - margin, cy), (cx - margin, 0 - cy), (cx + margin, 0 - cy),
+                      (cx + margin, cy)]),
+        (0, margin, [(cx - margin, cy), (cx - margin, height + cy), (cx + margin, height + cy),
+                     (cx + margin, cy)]),
+        (-margin, 0, [(cx, cy - margin), (0 - cx, cy - margin), (0 - cx, cy + margin),
+                      (cx, cy + margin)]),
+        (margin, 0, [(cx, cy - margin), (width + cx, cy - margin), (width + cx, cy + margin),
+                     (cx, cy + margin)]),
+    ]
+    for k, (tx, ty, pts) in enumerate(draws):
+        if k:
+            c.reset_path()
+            c.set_identity()
+        c.translate(tx, ty)
+        c.move_to(*pts[0])
+        for p in pts[1:]:
+            c.line_to(*p)
+        c.close_path()
+        _fill_then_stroke(c)
+    return _scene("076_oob_draw_sides", sfc, c)
+
+
+def s077(aa):  # spec/077_oob_draw_full_outside.zig
+    width = height = 300
+    sfc, c = _ctx(abi.RGB, width, height, aa)
+    cx = cy = width // 2
+    for x, y in ((-width, -height), (width * 2, -height), (width * 2, height * 2),
+                 (-width, height * 2), (-width, cy), (width * 2, cy), (cx, -height),
+                 (cx, height * 2)):
+        c.reset_path()
+        c.set_identity()
+        c.translate(x, y)
+        c.move_to(x - 20, y - 20)
+        c.line_to(x + 20, y - 20)
+        c.line_to(x + 20, y + 20)
+        c.line_to(x - 20, y + 20)
+        c.close_path()
+        _fill_then_stroke(c)
+    return _scene("077_oob_draw_full_outside", sfc, c)
+
+
+def s078(aa):  # spec/078_double_close.zig
+    sfc, c = _grey_ctx(aa, 300, 300)
+    c.move_to(49, 49)
+    c.line_to(250, 49)
+    c.line_to(250, 250)
+    c.line_to(49, 250)
+    c.close_path()
+    c.close_path()
+    _fill_then_stroke(c)
+    return _scene("078_double_close", sfc, c)
+
+
+# s044 is kept out of the pixel-exact list: its sixth line (rotation pi) lies
+# on a pixel boundary to within 1e-14 px, and the reference puts the red guide
+# line's left half one row up (3 pixels differ in the anti-aliased modes);
+# every other transformed stroke here matches exactly.
+MORE_SCENES = [s043, s045, s059, s060, s075, s076, s077, s078]
+NEAR_SCENES = [s044]

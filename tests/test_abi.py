@@ -58,3 +58,12 @@ def test_unclosed_path_is_rejected():
 def test_valid_fill_needs_a_context():
     nodes = [(abi.MOVE_TO, 0, 0), (abi.LINE_TO, 1, 1), (abi.LINE_TO, 0, 1), (abi.CLOSE_PATH,)]
     assert _fill(abi.solid_pattern(abi.rgba(1, 1, 1, 255)), nodes) == abi.ERR_NO_CONTEXT
+
+
+def test_div255_identity():
+    """The renderer's imul8: (x * 0x8081) >> 23 == x // 255 for every x below
+    2^16 (products of two 8-bit channels), and fails beyond that range."""
+    import numpy as np
+    x = np.arange(1 << 16, dtype=np.uint64)
+    assert np.array_equal((x * 0x8081) >> 23, x // 255)
+    assert (66299 * 0x8081) >> 23 != 66299 // 255

@@ -75,6 +75,11 @@ class Transformation:
         oy = self.cx * x + self.dy * y
         return ox + self.tx, oy + self.ty
 
+    def device_to_user_distance(self, x, y):
+        """deviceToUserDistance: the inverse's userToDeviceDistance."""
+        inv = self.inverse()
+        return inv.ax * x + inv.by * y, inv.cx * x + inv.dy * y
+
 
 def clamp_i24(v):
     return min(max(float(v), float(I24_MIN)), float(I24_MAX))

@@ -181,6 +181,12 @@ __device__ __forceinline__ uint32_t pack(I4 v, uint32_t fmt) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t u16(uint32_t x) { return x & 0xFFFFu; }
 __device__ __forceinline__ uint32_t imul(uint32_t a, uint32_t b) { return u16((a * b) / 255u); }
+// imul for channels a, b <= 255: (x * 0x8081) >> 23 equals x / 255 for every
+// x < 65536, so two 24-bit multiplies and a shift replace the 32-bit
+// multiply-high division.
+__device__ __forceinline__ uint32_t imul8(uint32_t a, uint32_t b) {
+  return __umul24(__umul24(a, b), 0x8081u) >> 23;
+}
 __device__ __forceinline__ uint32_t iinvmul(uint32_t a, uint32_t b) {
   return imul(a, u16(255u - b));
 }

@@ -947,17 +947,41 @@ __device__ __forceinline__ void dense_gradient(const DevGradient &G, StripShared
   }
 }
 
+// Channels of s (each <= 255) masked by coverage count k.
 __device__ __forceinline__ I4 mask_coverage(I4 s, int k) {
   const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
-  return I4{imul(s.r, a), imul(s.g, a), imul(s.b, a), imul(s.a, a)};
+  return I4{imul8(s.r, a), imul8(s.g, a), imul8(s.b, a), imul8(s.a, a)};
 }
-// s: the paint already masked by the pixel's coverage.
+// s: the paint already masked by the pixel's coverage (channels <= 255).
 __device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, I4 s) {
   const I4 d = unpack(raw, fmt);
-  const uint32_t sa = s.a;
-  const I4 o{u16(s.r + iinvmul(d.r, sa)), u16(s.g + iinvmul(d.g, sa)),
-             u16(s.b + iinvmul(d.b, sa)), u16(sa + d.a - imul(sa, d.a))};
+  const uint32_t sa = s.a, na = 255u - sa;
+  const I4 o{u16(s.r + imul8(d.r, na)), u16(s.g + imul8(d.g, na)), u16(s.b + imul8(d.b, na)),
+             u16(sa + d.a - imul8(sa, d.a))};
   return pack(o, fmt);
+}
+
+// The rows of the src_over fast composite.  full_raw: the packed source that
+// full coverage stores when the draw takes the solid fast path.
+__device__ __forceinline__ void src_over_rows(uint32_t (&pix)[ROWS_PER_THREAD],
+                                              const int (&kk)[ROWS_PER_THREAD], uint32_t need,
+                                              bool solid, bool fast_full, int kfull,
+                                              uint32_t full_raw, const StripShared &sh,
+                                              uint32_t lane, uint32_t fmt, uint32_t &n_cov) {
+#pragma unroll
+  for (int i = 0; i < ROWS_PER_THREAD; i++) {
+    if (!((need >> i) & 1u)) continue;
+    n_cov++;
+    const int k = kk[i];
+    if (fast_full && k >= kfull) {
+      pix[i] = full_raw;
+      continue;
+    }
+    const uint32_t pv = solid ? sh.srctab[k] : sh.u.c.ppaint[i][lane];
+    I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+    if (!solid && k < kfull) src = mask_coverage(src, k);
+    pix[i] = composite_src_over(pix[i], fmt, src);
+  }
 }
 
 __device__ __forceinline__ bool msaa_preclear(const DevDraw &D, int W, int x, int y) {
@@ -1370,20 +1394,9 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           wave_lds_sync();
         }
         const bool fast_full = (D.flags & DRAW_FAST_SRC) != 0;
-#pragma unroll
-        for (int i = 0; i < ROWS_PER_THREAD; i++) {
-          if (!((need >> i) & 1u)) continue;
-          n_cov++;
-          const int k = kk[i];
-          if (fast_full && k >= kfull) {
-            pix[i] = pack(p_px, S.fmt);  // the solid fast path paints the source
-            continue;
-          }
-          const uint32_t pv = solid ? sh.srctab[k] : sh.u.c.ppaint[i][lane];
-          I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
-          if (!solid && k < kfull) src = mask_coverage(src, k);
-          pix[i] = composite_src_over(pix[i], S.fmt, src);
-        }
+        // the solid fast path stores the source where coverage is full
+        const uint32_t full_raw = fast_full ? pack(p_px, S.fmt) : 0u;
+        src_over_rows(pix, kk, need, solid, fast_full, kfull, full_raw, sh, lane, S.fmt, n_cov);
       } else {
         // every other operator / precision / mode: one composite instance,
         // rows in a rolled loop over a copy of the strip's pixels
