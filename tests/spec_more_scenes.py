@@ -1,5 +1,6 @@
 """More of the reference's acceptance cases (spec/NNN_*.zig), restated as
-recorded Context calls: strokes under transformations (043-045), a gradient
+recorded Context calls: arcs (035, 042, 066; Path.arc restated in zscene),
+strokes under transformations (043-045), assorted dashes (058), a gradient
 stroke (059), curved gradient fills (060), draws partly or wholly outside the
 surface (075-077) and a doubled close (078).  The expected outputs are the
 reference's own fixture PNGs (tests/golden/spec/), compared the same way as
@@ -347,9 +348,334 @@ def s078(aa):  # spec/078_double_close.zig
     return _scene("078_double_close", sfc, c)
 
 
+def s035(aa):  # spec/035_arc_command.zig
+    sfc, c = _ctx(abi.RGB, 400, 400, aa, abi.Pixel(abi.RGB, 0xFF, 0xFF, 0xFF, 0))
+    blue = abi.Pixel(abi.RGB, 0x00, 0x00, 0xFF, 0)
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xFF, 0x00, 0x00, 0))
+    c.line_width = 5.0
+    c.move_to(200, 200)
+    c.arc_negative(200, 200, 150, math.pi, math.pi * 1.5)
+    c.close_path()
+    c.fill()
+    c.set_source_pixel(blue)
+    c.stroke()
+    c.reset_path()
+    c.move_to(175, 175)
+    c.arc(175, 175, 150, math.pi, math.pi * 1.5)
+    c.close_path()
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xFF, 0xFF, 0x00, 0))
+    c.fill()
+    c.set_source_pixel(blue)
+    c.stroke()
+    return _scene("035_arc_command", sfc, c)
+
+
+def _ellipse042(c, x, y, rx, ry, reset_ctm):
+    saved = c.transformation
+    c.reset_path()
+    c.translate(x + rx / 2, y + ry / 2)
+    c.scale(rx / 2, ry / 2)
+    c.arc(0, 0, 1, 0, 2 * math.pi)
+    if reset_ctm:
+        c.set_transformation(saved)
+    c.close_path()
+
+
+def s042(aa):  # spec/042_arc_ellipses.zig
+    sfc, c = _ctx(abi.RGB, 400, 400, aa)
+    c.set_source_pixel(WHITE)
+    c.line_width = 5.0
+    c.translate(10, 10)
+    _ellipse042(c, 0, 0, 50, 100, True)
+    c.fill()
+    _ellipse042(c, 100, 0, 50, 100, True)
+    c.stroke()
+
+    saved, saved_w = c.transformation, c.line_width
+    _ellipse042(c, 200, 0, 50, 100, False)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    c.set_transformation(saved)
+    c.line_width = saved_w
+
+    saved, saved_w = c.transformation, c.line_width
+    c.rotate(math.pi / 4.0)
+    _ellipse042(c, 300, 0, 50, 100, False)
+    c.line_width = _user_width(c, saved_w)
+    c.stroke()
+    c.set_transformation(saved)
+    c.line_width = saved_w
+    return _scene("042_arc_ellipses", sfc, c)
+
+
+def s066(aa):  # spec/066_conic_pie_gradient.zig
+    sfc, c = _ctx(abi.RGB, 300, 300, aa)
+    stops = [(0.0, RGB1(1, 0, 0)), (1.0 / 3.0, RGB1(1, 0, 0)), (1.0 / 3.0 + 0.005, RGB1(0, 1, 0)),
+             (2.0 / 3.0, RGB1(0, 1, 0)), (2.0 / 3.0 + 0.005, RGB1(0, 0, 1)), (1.0, RGB1(0, 0, 1))]
+    c.set_source_gradient(abi.Gradient(abi.GRADIENT_CONIC, [149, 149, 0], stops))
+    c.arc(149, 149, 100, 0, math.pi * 2)
+    c.close_path()
+    c.fill()
+    return _scene("066_conic_pie_gradient", sfc, c)
+
+
+def s058(aa):  # spec/058_stroke_misc_dashes.zig
+    width, height = 600, 2000
+    sfc, c = _ctx(abi.RGB, width, height, aa)
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xFF, 0xFF, 0xFF, 0))
+    c.line_width = 3
+    c.dashes = [25, 5]
+    c.translate(50, 50)
+    c.move_to(100, 0)
+    c.line_to(200, 200)
+    c.line_to(0, 200)
+    c.close_path()
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(350, 50)
+    c.move_to(0, 0)
+    c.line_to(0, 200)
+    c.line_to(200, 200)
+    c.line_to(200, 0)
+    c.close_path()
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    x_offset = 50
+    y_offset = 350
+    diameter = 200
+    c.translate(x_offset + diameter / 2, y_offset + diameter / 2)
+    c.scale(diameter / 2, diameter / 2)
+    c.arc(0, 0, 1, 0, 2 * math.pi)
+    c.set_identity()
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    x_offset = 350
+    y_offset = 350
+    c.translate(x_offset + diameter / 2, y_offset + diameter / 2)
+    c.scale(diameter / 2, diameter / 2)
+    c.arc(0, 0, 1, 0, 2 * math.pi)
+    c.close_path()
+    c.set_identity()
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xAA, 0xAA, 0xAA, 0))
+    c.fill()
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xFF, 0xFF, 0xFF, 0))
+    c.stroke()
+    c.reset_path()
+    ellipse_line_width = 6
+    c.line_width = ellipse_line_width
+    c.set_identity()
+    x_offset = 100
+    y_offset = 650
+    x_diameter = 100
+    y_diameter = 200
+    c.translate(x_offset + x_diameter / 2, y_offset + y_diameter / 2)
+    c.scale(x_diameter / 2, y_diameter / 2)
+    c.arc(0, 0, 1, 0, 2 * math.pi)
+    c.set_identity()
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    x_offset = 400
+    y_offset = 650
+    c.translate(x_offset + x_diameter / 2, y_offset + y_diameter / 2)
+    c.scale(x_diameter / 2, y_diameter / 2)
+    c.arc(0, 0, 1, 0, 2 * math.pi)
+    c.line_width = _user_width(c, ellipse_line_width)
+    c.stroke()
+    c.set_identity()
+    c.reset_path()
+    c.set_identity()
+    x_offset = 50
+    y_offset = 950
+    c.translate(x_offset, y_offset)
+    c.scale(200, 200)
+    c.move_to(0, 0)
+    c.line_to(0, 1)
+    c.line_to(1, 1)
+    c.line_to(1, 0)
+    c.close_path()
+    c.line_width = _user_width(c, ellipse_line_width)
+    c.stroke()
+    c.set_identity()
+    c.reset_path()
+    c.line_width = 3
+    c.set_identity()
+    x_offset = 350
+    y_offset = 950
+    c.translate(x_offset, y_offset)
+    c.dashes = [28]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    x_offset = 350
+    y_offset = 990
+    c.translate(x_offset, y_offset)
+    c.dashes = [25, 5, 3]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.line_width = 10
+    c.cap = abi.CAP_ROUND
+    c.set_identity()
+    x_offset = 350
+    y_offset = 1030
+    c.translate(x_offset, y_offset)
+    c.dashes = [28]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.cap = abi.CAP_SQUARE
+    c.set_identity()
+    x_offset = 350
+    y_offset = 1070
+    c.translate(x_offset, y_offset)
+    c.dashes = [28]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.cap = abi.CAP_BUTT
+    c.set_identity()
+    x_offset = 350
+    y_offset = 1110
+    c.translate(x_offset, y_offset)
+    c.dashes = [28]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.cap = abi.CAP_ROUND
+    c.set_identity()
+    x_offset = 350
+    y_offset = 1150
+    c.translate(x_offset, y_offset)
+    c.dashes = [0, 20]
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.stroke()
+    c.reset_path()
+    c.cap = abi.CAP_SQUARE
+    c.set_identity()
+    x_offset = 50
+    y_offset = 1250
+    c.translate(x_offset, y_offset)
+    c.dashes = [0, 20]
+    c.move_to(0, 0)
+    c.line_to(0, 200)
+    c.move_to(20, 0)
+    c.line_to(200, 200)
+    c.stroke()
+    c.reset_path()
+    c.cap = abi.CAP_BUTT
+    c.line_width = 3
+    c.set_identity()
+    c.dashes = [25]
+    c.dash_offset = 12.5
+    c.translate(350, 1225)
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.line_to(200, 100)
+    c.line_to(0, 100)
+    c.close_path()
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(350, 1350)
+    c.dash_offset = -12.5
+    c.move_to(0, 0)
+    c.line_to(200, 0)
+    c.line_to(200, 100)
+    c.line_to(0, 100)
+    c.close_path()
+    c.stroke()
+    c.reset_path()
+    c.line_width = 10
+    c.set_identity()
+    c.dashes = []
+    c.dash_offset = 0
+    c.translate(150, 1550)
+    c.move_to(0, 0)
+    c.line_to(300, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1570)
+    c.dashes = [30, 10]
+    c.dash_offset = 0
+    c.move_to(0, 0)
+    c.line_to(300, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1590)
+    c.dash_offset = 30
+    c.move_to(0, 0)
+    c.line_to(300, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1610)
+    c.dash_offset = -30
+    c.move_to(0, 0)
+    c.line_to(300, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1630)
+    c.dash_offset = 10
+    c.move_to(0, 0)
+    c.line_to(300, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_source_pixel(abi.Pixel(abi.RGBA, 127, 0, 0, 127))  # RGBA{255, 0, 0, 127}.multiply()
+    c.set_identity()
+    c.translate(150, 1590)
+    c.dashes = []
+    c.dash_offset = 0
+    c.move_to(0, 0)
+    c.rel_line_to(-30, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1610)
+    c.move_to(0, 0)
+    c.rel_line_to(30, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_identity()
+    c.translate(150, 1630)
+    c.move_to(0, 0)
+    c.rel_line_to(-10, 0)
+    c.stroke()
+    c.reset_path()
+    c.set_source_pixel(abi.Pixel(abi.RGB, 0xFF, 0xFF, 0xFF, 0))
+    c.set_identity()
+    c.translate(200, 1725)
+    c.dash_offset = 0
+    c.dashes = [180, 30]
+    c.move_to(136.737609507049, 188.692435121084)
+    c.line_to(63.2623904929514, 188.692435121084)
+    c.line_to(11.3075648789165, 136.737609507049)
+    c.line_to(11.3075648789165, 63.2623904929514)
+    c.line_to(63.2623904929513, 11.3075648789165)
+    c.line_to(136.737609507049, 11.3075648789165)
+    c.line_to(188.692435121084, 63.2623904929513)
+    c.line_to(188.692435121084, 136.737609507049)
+    c.close_path()
+    c.stroke()
+    return _scene("058_stroke_misc_dashes", sfc, c)
+
+
 # s044 is kept out of the pixel-exact list: its sixth line (rotation pi) lies
 # on a pixel boundary to within 1e-14 px, and the reference puts the red guide
 # line's left half one row up (3 pixels differ in the anti-aliased modes);
 # every other transformed stroke here matches exactly.
-MORE_SCENES = [s043, s045, s059, s060, s075, s076, s077, s078]
+MORE_SCENES = [s035, s042, s043, s045, s058, s059, s060, s066, s075, s076, s077, s078]
 NEAR_SCENES = [s044]

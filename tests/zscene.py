@@ -85,6 +85,89 @@ def clamp_i24(v):
     return min(max(float(v), float(I24_MIN)), float(I24_MAX))
 
 
+_ARC_TABLE = [(math.pi / 1.0, 0.0185185185185185036127),
+              (math.pi / 2.0, 0.000272567143730179811158),
+              (math.pi / 3.0, 2.38647043651461047433e-05),
+              (math.pi / 4.0, 4.2455377443222443279e-06),
+              (math.pi / 5.0, 1.11281001494389081528e-06),
+              (math.pi / 6.0, 3.72662000942734705475e-07),
+              (math.pi / 7.0, 1.47783685574284411325e-07),
+              (math.pi / 8.0, 6.63240432022601149057e-08),
+              (math.pi / 9.0, 3.2715520137536980553e-08),
+              (math.pi / 10.0, 1.73863223499021216974e-08),
+              (math.pi / 11.0, 9.81410988043554039085e-09)]
+
+
+def _arc_max_angle(tolerance):
+    """arc_max_angle_for_tolerance_normalized (src/internal/arc.zig)."""
+    for angle, err in _ARC_TABLE:
+        if err < tolerance:
+            return angle
+    angle = 0.0
+    for i in range(len(_ARC_TABLE), 1000):
+        angle = math.pi / float(i)
+        err = 2.0 / 27.0 * math.pow(math.sin(angle / 4), 6) / math.pow(math.cos(angle / 4), 2)
+        if err <= tolerance:
+            break
+    return angle
+
+
+def _circle_major_axis(m, radius):
+    """transformed_circle_major_axis (src/internal/arc.zig)."""
+    eps = 0.00390625
+    det = m.ax * m.dy - m.by * m.cx
+    if abs(det * det - 1.0) < eps:
+        if abs(m.by) < eps and abs(m.cx) < eps:
+            return radius
+        if abs(m.ax) < eps and abs(m.dy) < eps:
+            return radius
+    i = m.ax * m.ax + m.by * m.by
+    j = m.cx * m.cx + m.dy * m.dy
+    f = 0.5 * (i + j)
+    g = 0.5 * (i - j)
+    h = m.ax * m.cx + m.by * m.dy
+    return radius * math.sqrt(f + math.hypot(g, h))
+
+
+def _arc_segment(path, xc, yc, radius, a, b):
+    rsa, rca = radius * math.sin(a), radius * math.cos(a)
+    rsb, rcb = radius * math.sin(b), radius * math.cos(b)
+    h = 4.0 / 3.0 * math.tan((b - a) / 4.0)
+    path.curve_to(xc + rca - h * rsa, yc + rsa + h * rca, xc + rcb + h * rsb,
+                  yc + rsb - h * rcb, xc + rcb, yc + rsb)
+
+
+def _arc_in_direction(path, xc, yc, radius, amin, amax, forward, ctm, tolerance):
+    """arc_in_direction (src/internal/arc.zig)."""
+    if not (amax * amax >= 0.0) or not (amin * amin >= 0.0):
+        return
+    full = 2 * math.pi * 65536.0
+    if amax - amin > full:
+        amax = math.fmod(amax - amin, 2 * math.pi)
+        amin = math.fmod(amin, 2 * math.pi)
+        amax += amin + full
+    if amax - amin > math.pi:
+        amid = amin + (amax - amin) / 2.0
+        halves = ((amin, amid), (amid, amax)) if forward else ((amid, amax), (amin, amid))
+        for lo, hi in halves:
+            _arc_in_direction(path, xc, yc, radius, lo, hi, forward, ctm, tolerance)
+    elif amax != amin:
+        major = _circle_major_axis(ctm, radius)
+        segments = int(math.ceil(abs(amax - amin) / _arc_max_angle(tolerance / major)))
+        step = (amax - amin) / float(segments)
+        segments -= 1
+        if not forward:
+            amin, amax = amax, amin
+            step = -step
+        path._arc_line_to(xc + radius * math.cos(amin), yc + radius * math.sin(amin))
+        for _ in range(max(0, segments)):
+            _arc_segment(path, xc, yc, radius, amin, amin + step)
+            amin += step
+        _arc_segment(path, xc, yc, radius, amin, amax)
+    else:
+        path._arc_line_to(xc + radius * math.cos(amin), yc + radius * math.sin(amin))
+
+
 class Path:
     """src/Path.zig:116-262, :447-477, :570-572."""
 
@@ -130,6 +213,30 @@ class Path:
         p1, p2, p3 = self._pt(x1, y1), self._pt(x2, y2), self._pt(x3, y3)
         self.nodes.append((abi.CURVE_TO, p1[0], p1[1], p2[0], p2[1], p3[0], p3[1]))
         self.current = p3
+
+    # --- arcs: Path.arc / arcNegative (src/Path.zig:345-443) over
+    # arc_in_direction (src/internal/arc.zig), in user space; points go
+    # through the path's transformation like any line_to / curve_to.
+    def arc(self, xc, yc, radius, angle1, angle2, tolerance=0.1):
+        a2 = float(angle2)
+        while a2 < angle1:
+            a2 += math.pi * 2
+        _arc_in_direction(self, float(xc), float(yc), float(radius), float(angle1), a2, True,
+                          self.transformation, max(tolerance, 0.001))
+
+    def arc_negative(self, xc, yc, radius, angle1, angle2, tolerance=0.1):
+        a2 = float(angle2)
+        while a2 > angle1:
+            a2 -= math.pi * 2
+        _arc_in_direction(self, float(xc), float(yc), float(radius), a2, float(angle1), False,
+                          self.transformation, max(tolerance, 0.001))
+
+    def _arc_line_to(self, x, y):
+        # no-op when the current point (device space) equals the target as
+        # given (user space), as the reference compares them
+        if self.current is not None and self.current[0] == x and self.current[1] == y:
+            return
+        self.line_to(x, y)
 
     def close(self):
         if self.current is None:
@@ -300,6 +407,12 @@ class Context:
 
     def curve_to(self, *a):
         self.path.curve_to(*a)
+
+    def arc(self, xc, yc, radius, a1, a2):
+        self.path.arc(xc, yc, radius, a1, a2, self.tolerance)
+
+    def arc_negative(self, xc, yc, radius, a1, a2):
+        self.path.arc_negative(xc, yc, radius, a1, a2, self.tolerance)
 
     def close_path(self):
         self.path.close()
