@@ -97,3 +97,59 @@ def test_gpu_composite_params(seed):
     got = run_case(gpu_backend(), dst, x, y, ops, prec, keep)
     msg = diff_report(got.data(), want.data())
     assert not msg, f"seed {seed}: {msg}"
+
+
+@pytest.mark.parametrize("op", list(range(28)), ids=abi.OPERATORS)
+@pytest.mark.parametrize("prec", [0, 1], ids=["int", "float"])
+def test_gpu_composite_whole_surface(op, prec):
+    """Whole-surface batches at the origin (the per-operator streaming path):
+    one operator with a surface or pixel source, and {dst_in dst=src
+    src=mask}, {op} with alpha8, 4-byte and pixel masks, over the four 4-byte
+    formats; widths that are and are not multiples of 4."""
+    rng = SplitMix64(0x5EA + op * 13 + prec)
+    fmts32 = [abi.RGBA, abi.ARGB, abi.RGB, abi.XRGB]
+    for case in range(6):
+        dfmt = fmts32[rng.randint(0, 3)]
+        w = 4 * rng.randint(1, 40) + (1 if case == 5 else 0)
+        h = rng.randint(1, 24)
+        dst = rand_surface(rng, dfmt, w, h)
+        keep = []
+
+        def surf(fmt, hh=h):
+            s = rand_surface(rng, fmt, w, hh)
+            sd = s.desc()
+            keep.extend([s, sd])
+            return sd
+
+        def px():
+            return rgba_from_clamped(rng.uniform(), rng.uniform(), rng.uniform(), rng.uniform())
+
+        o = abi.CompOp()
+        o.op = op
+        if case in (0, 5):
+            o.src.kind = abi.PARAM_SURFACE
+            o.src.surface = C.pointer(surf(fmts32[rng.randint(0, 3)], h + rng.randint(0, 3)))
+            ops = [o]
+        elif case == 1:
+            o.src.kind = abi.PARAM_PIXEL
+            o.src.pixel = px()
+            ops = [o]
+        else:
+            m = abi.CompOp()
+            m.op = abi.OP["dst_in"]
+            m.dst.kind = abi.PARAM_SURFACE
+            m.dst.surface = C.pointer(surf(fmts32[rng.randint(0, 3)]))
+            if case == 2:
+                m.src.kind = abi.PARAM_SURFACE
+                m.src.surface = C.pointer(surf(abi.ALPHA8))
+            elif case == 3:
+                m.src.kind = abi.PARAM_SURFACE
+                m.src.surface = C.pointer(surf(fmts32[rng.randint(0, 3)]))
+            else:
+                m.src.kind = abi.PARAM_PIXEL
+                m.src.pixel = px()
+            ops = [m, o]
+        want = run_case(OracleBackend(), dst, 0, 0, ops, prec, keep)
+        got = run_case(gpu_backend(), dst, 0, 0, ops, prec, keep)
+        msg = diff_report(got.data(), want.data())
+        assert not msg, f"case {case} dst fmt {dfmt} {w}x{h}: {msg}"

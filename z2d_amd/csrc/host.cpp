@@ -37,6 +37,7 @@ hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n
 hipError_t z2d_launch_binning(const JobView &J, hipStream_t st);
 hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
                              uint32_t *cursor, uint32_t n_blocks, hipStream_t st);
+hipError_t z2d_launch_composite_quad(const z2d::QuadJob &job, uint64_t n_quads, hipStream_t st);
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -1230,6 +1231,65 @@ static int32_t surface_release(z2d_gpu_ctx *ctx, SurfRef &r, bool copy_back) {
 // ---------------------------------------------------------------------------
 // compositor
 // ---------------------------------------------------------------------------
+// A source of the quad compositor: a pixel, or a surface read at the
+// destination's pixel index (same width, at least as tall, aligned for the
+// vector loads) in a 4-byte format or alpha8.
+static bool quad_src(const DevCompParam &p, const CompJob &job, QuadSrc &o) {
+  o.data = nullptr;
+  o.fmt = 0;
+  o.px = I4{0, 0, 0, 0};
+  if (p.kind == Z2D_PARAM_PIXEL) {
+    o.kind = QUAD_PIXEL;
+    o.px = p.px;
+    return true;
+  }
+  if (p.kind != Z2D_PARAM_SURFACE || p.base != 0 || p.surf.w != job.dst.w ||
+      p.surf.h < job.dst.h)
+    return false;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p.surf.data);
+  if (fbits(p.surf.fmt) == 32 && (a & 15) == 0)
+    o.kind = QUAD_SURF32;
+  else if (p.surf.fmt == Z2D_FORMAT_ALPHA8 && (a & 3) == 0)
+    o.kind = QUAD_ALPHA8;
+  else
+    return false;
+  o.data = p.surf.data;
+  o.fmt = p.surf.fmt;
+  return true;
+}
+
+// The batches the quad compositor takes (on top of the streaming conditions):
+// one operator onto the destination with a surface or pixel source, or
+// {dst_in dst=src src=mask}, {op} (src and mask surfaces or pixels).
+static bool quad_job(const CompJob &job, QuadJob &q) {
+  if ((reinterpret_cast<uintptr_t>(job.dst.data) & 15) ||
+      (reinterpret_cast<uintptr_t>(job.out.data) & 15))
+    return false;
+  q.dst = reinterpret_cast<const uint32_t *>(job.dst.data);
+  q.out = reinterpret_cast<uint32_t *>(job.out.data);
+  q.dst_fmt = job.dst.fmt;
+  q.out_fmt = job.out.fmt;
+  q.precision = job.precision;
+  if (job.n_ops == 1) {
+    const DevCompOp &o = job.ops[0];
+    if (o.dst.kind != Z2D_PARAM_NONE || !quad_src(o.src, job, q.src)) return false;
+    q.op = o.op;
+    q.masked = 0;
+    q.mask = q.src;
+    return true;
+  }
+  if (job.n_ops == 2) {
+    const DevCompOp &m = job.ops[0], &o = job.ops[1];
+    if (m.op != Z2D_OP_DST_IN || o.dst.kind != Z2D_PARAM_NONE || o.src.kind != Z2D_PARAM_NONE)
+      return false;
+    if (!quad_src(m.dst, job, q.src) || !quad_src(m.src, job, q.mask)) return false;
+    q.op = o.op;
+    q.masked = 1;
+    return true;
+  }
+  return false;
+}
+
 static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t dst_x,
                               int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
                               uint32_t precision, z2d_surface_desc *out_d) {
@@ -1387,8 +1447,13 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
             if (q.kind == Z2D_PARAM_SURFACE && (q.base != 0 || q.surf.w != job.dst.w))
               stream = false;
           }
-        e = z2d_launch_composite(reinterpret_cast<const CompJob *>(D + o_job), job, stream,
-                                 ctx->stream);
+        QuadJob qj;
+        if (stream && quad_job(job, qj))
+          e = z2d_launch_composite_quad(qj, (uint64_t)job.width * (uint64_t)job.height / 4,
+                                        ctx->stream);
+        else
+          e = z2d_launch_composite(reinterpret_cast<const CompJob *>(D + o_job), job, stream,
+                                   ctx->stream);
       }
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) rc = fail(ctx, e, "composite launch");

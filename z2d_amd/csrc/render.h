@@ -175,4 +175,27 @@ struct CompJob {
   const uint16_t *blue_noise;
 };
 
+// One source of the quad compositor: a 4-byte-format surface, an alpha8
+// surface or a constant pixel, read at the destination's own pixel index.
+struct QuadSrc {
+  const uint8_t *data;  // surface kinds
+  uint32_t kind;        // QUAD_PIXEL / QUAD_SURF32 / QUAD_ALPHA8
+  uint32_t fmt;         // surface format (QUAD_SURF32)
+  I4 px;                // QUAD_PIXEL
+};
+enum : uint32_t { QUAD_PIXEL = 0, QUAD_SURF32 = 1, QUAD_ALPHA8 = 2 };
+
+// Whole-surface SurfaceCompositor batches of one operator, or of
+// {dst_in dst=src src=mask}, {op}: out = op(dst, src) per pixel, four pixels
+// per lane.
+struct QuadJob {
+  const uint32_t *dst;
+  uint32_t *out;
+  uint32_t dst_fmt, out_fmt;
+  uint32_t op;       // the operator applied to the destination
+  uint32_t masked;   // nonzero: src = dst_in(src, mask) first
+  uint32_t precision;
+  QuadSrc src, mask;
+};
+
 }  // namespace z2d
