@@ -133,8 +133,10 @@ def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3):
     s_mask.upload(mask)
     d_src, d_dst, d_out, d_mask = s_src.desc(), s_dst.desc(), s_out.desc(), s_mask.desc()
     f = gpu.lib().z2d_gpu_composite
-    per_op, per_op_mask = {}, {}
-    tot_bytes = tot_time = 0.0
+    per_op, per_op_mask, dev_op, dev_mask = {}, {}, {}, {}
+    tot_bytes = tot_time = tot_kern = 0.0
+    timing = gpu.lib().z2d_gpu_composite_timing
+    km = C.c_double(0.0)
     for masked in (False, True):
         for name in abi.OPERATORS:
             if masked:
@@ -156,18 +158,28 @@ def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3):
             n = len(arr)
             ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
             t = time.perf_counter()
+            kern = 0.0
             for _ in range(reps):
                 ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
+                ctx.check(timing(ctx.h, C.byref(km)), "composite timing")
+                kern += km.value / 1e3
             dt = (time.perf_counter() - t) / reps
+            kern /= reps
             nbytes = w * h * (13 if masked else 12)
             (per_op_mask if masked else per_op)[name] = round(nbytes / dt / 1e9, 1)
+            (dev_mask if masked else dev_op)[name] = round(nbytes / kern / 1e9, 1)
             tot_bytes += nbytes
             tot_time += dt
+            tot_kern += kern
     for s in (s_src, s_dst, s_out, s_mask):
         s.close()
     return {"GBps_all_passes": round(tot_bytes / tot_time / 1e9, 1), "per_op_GBps": per_op,
             "mask_per_op_GBps": per_op_mask, "passes": 2 * len(abi.OPERATORS),
-            "note": "host-timed per call (upload of the op list, launch, sync)"}
+            "note": "host-timed per call (upload of the op list, launch, sync)",
+            "kernel_GBps_all_passes": round(tot_bytes / tot_kern / 1e9, 1),
+            "kernel_per_op_GBps": dev_op, "kernel_mask_per_op_GBps": dev_mask,
+            "kernel_note": "k_composite_quad device time (HIP events around the launch); "
+                           "12 B/px (13 with the mask) over the 8 TB/s HBM peak"}
 
 
 def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
@@ -322,6 +334,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="distinct requests in the serving pool (cycled at each QPS level)")
+    ap.add_argument("--comp-only", action="store_true",
+                    help="only run the 4096x4096 compositor sweep (one JSON line)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the stream-copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -336,6 +350,9 @@ def main():
         ctx.check(gpu.lib().z2d_gpu_stream_copy(ctx.h, args.copy_probe << 20, 1, C.byref(g)),
                   "stream_copy")
         print(json.dumps({"stream_copy_MiB": args.copy_probe, "GBps": round(g.value, 1)}))
+        return
+    if args.comp_only:
+        print(json.dumps({"compositor_4096": bench_compositor(gpu, ctx)}))
         return
     log("generating scene")
     sc = scenes.scene_4k(n_paths=args.paths)

@@ -124,6 +124,10 @@ int32_t z2d_gpu_batch_prepare(z2d_gpu_batch *b);
  * surfaces; mode 1: one launch per surface.  Asynchronous. */
 int32_t z2d_gpu_batch_render(z2d_gpu_batch *b, uint32_t mode);
 int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b);
+/* Device time of the last z2d_gpu_composite call's kernel, in milliseconds
+ * (HIP events on the context stream around the launch).  A measurement hook
+ * with no counterpart in the reference. */
+int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms);
 /* Device time of the last render: whole pipeline and the tile-render
  * kernel(s) alone, in milliseconds (HIP events on the context stream). */
 int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms);

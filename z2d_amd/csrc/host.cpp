@@ -122,6 +122,7 @@ struct z2d_gpu_ctx {
   DevBuf comp_arena;
   HostBuf comp_stage;
   int n_cus = 256;
+  float last_comp_ms = 0.0f;  // device time of the last composite kernel
 };
 
 struct z2d_gpu_surface {
@@ -1447,15 +1448,23 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
             if (q.kind == Z2D_PARAM_SURFACE && (q.base != 0 || q.surf.w != job.dst.w))
               stream = false;
           }
+        if (!ctx->ev[0]) e = hipEventCreate(&ctx->ev[0]);
+        if (e == hipSuccess && !ctx->ev[1]) e = hipEventCreate(&ctx->ev[1]);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
         QuadJob qj;
-        if (stream && quad_job(job, qj))
+        if (e != hipSuccess) {
+        } else if (stream && quad_job(job, qj))
           e = z2d_launch_composite_quad(qj, (uint64_t)job.width * (uint64_t)job.height / 4,
                                         ctx->stream);
         else
           e = z2d_launch_composite(reinterpret_cast<const CompJob *>(D + o_job), job, stream,
                                    ctx->stream);
       }
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      float ms = 0.0f;
+      if (e == hipSuccess && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
+        ctx->last_comp_ms = ms;
       if (e != hipSuccess) rc = fail(ctx, e, "composite launch");
     }
   }
@@ -1501,6 +1510,8 @@ void z2d_gpu_destroy(z2d_gpu_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->comp_arena.release();
   ctx->comp_stage.release();
+  for (hipEvent_t &ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1592,6 +1603,12 @@ int32_t z2d_gpu_batch_render(z2d_gpu_batch *b, uint32_t mode) {
 int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b) {
   if (!b) return Z2D_ERR_INVALID_ARGUMENT;
   return batch_wait(b);
+}
+
+int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
+  if (!ctx || !kernel_ms) return Z2D_ERR_INVALID_ARGUMENT;
+  *kernel_ms = ctx->last_comp_ms;
+  return Z2D_OK;
 }
 
 int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms) {
