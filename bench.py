@@ -188,7 +188,8 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     distribution; 30% of the paths stroked: width U(0.5, 8), joins and caps
     uniform, 25% dashed).  Strokes are tessellated on the host when the batch
     is built.  One persistent launch vs one launch per surface over the same
-    prepared batch."""
+    prepared batch; the persistent launch also as a replayed hipGraph of the
+    whole pipeline."""
     rng = scenes.SplitMix64(seed)
     b = gpu.Batch(ctx)
     surfs = []
@@ -207,7 +208,9 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     b.prepare()
     pix, _ = count_work(b)
     res = {}
-    for mode, name in ((0, "batched"), (1, "per_surface")):
+    for mode, name, graph in ((0, "batched", False), (0, "batched_graph", True),
+                              (1, "per_surface", False)):
+        b.use_graph(graph)
         for _ in range(2):
             b.render(mode)
             b.wait()
@@ -217,7 +220,9 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
             b.wait()
         dt = (time.perf_counter() - t) / 3
         res[name] = round(pix / dt / 1e6, 1)
+    b.use_graph(False)
     res["speedup"] = round(res["batched"] / res["per_surface"], 2)
+    res["speedup_graph"] = round(res["batched_graph"] / res["per_surface"], 2)
     res["surfaces"] = n_surfaces
     res["pixel_draws"] = int(pix)
     res["stroke_share"] = 0.3

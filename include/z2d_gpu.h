@@ -128,8 +128,14 @@ int32_t z2d_gpu_batch_wait(z2d_gpu_batch *b);
  * (HIP events on the context stream around the launch).  A measurement hook
  * with no counterpart in the reference. */
 int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms);
+/* enable != 0: each render of this batch replays one hipGraph holding the
+ * whole pipeline (captured at the first render after the batch is prepared,
+ * and again when the launch mode or the counter setting changes).  No
+ * counterpart in the reference; results are those of the eager render. */
+int32_t z2d_gpu_batch_use_graph(z2d_gpu_batch *b, uint32_t enable);
 /* Device time of the last render: whole pipeline and the tile-render
- * kernel(s) alone, in milliseconds (HIP events on the context stream). */
+ * kernel(s) alone, in milliseconds (HIP events on the context stream);
+ * render_ms is -1 after a graph replay. */
 int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms);
 /* Number of draws / surfaces / flattened nodes in the batch. */
 int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]);

@@ -707,6 +707,11 @@ struct z2d_gpu_batch {
   float last_pipeline_ms = 0.0f, last_render_ms = 0.0f;
   uint32_t count_stats = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool render_timed = false;  // the last render recorded ev[1] before the tile render
+  // hipGraph replay of the render pipeline (z2d_gpu_batch_use_graph)
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  uint32_t graph_mode = 0, graph_stats = 0;
 
   uint32_t slot_of(z2d_gpu_surface *s) {
     for (uint32_t i = 0; i < surfaces.size(); i++)
@@ -878,10 +883,16 @@ static int32_t batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patt
                             Z2D_FILL_NON_ZERO, o->op, o->precision, true);
 }
 
+static void batch_drop_graph(z2d_gpu_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+}
+
 static int32_t batch_prepare(z2d_gpu_batch *b) {
   z2d_gpu_ctx *ctx = b->ctx;
   if (b->prepared) return Z2D_OK;
   HIP_OK(ctx, hipSetDevice(ctx->device));
+  batch_drop_graph(b);  // its kernel arguments point into the arena
   const uint32_t ns = (uint32_t)b->surfaces.size();
   // Draws grouped by surface, insertion order kept within a surface.
   std::stable_sort(b->draws.begin(), b->draws.end(), [](const HostDraw &a, const HostDraw &c) {
@@ -1082,6 +1093,36 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   return Z2D_OK;
 }
 
+// The render pipeline of a prepared batch on its stream: counters and
+// cursors cleared by fill kernels (no memset nodes, so the sequence can be
+// captured into a graph), edge build, row programs, binning, tile render.
+// mark_render records ev[1] between the binning and the render.
+static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_render) {
+  z2d_gpu_ctx *ctx = b->ctx;
+  JobView &J = b->view;
+  const uint32_t ns = J.n_surfaces;
+  const hipStream_t st = bstream(b);
+  hipError_t e = z2d_launch_fill_u32(J.counters, 16, 0u, st);
+  if (e == hipSuccess) e = z2d_launch_fill_u32(J.cursors, ns + 1, 0u, st);
+  if (e == hipSuccess) e = z2d_launch_edge_build(J, st);
+  if (e == hipSuccess)
+    e = z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,
+                           b->d_scratch_x, b->d_rowprog_soff, st);
+  if (e == hipSuccess) e = z2d_launch_binning(J, st);
+  if (e == hipSuccess && mark_render) e = hipEventRecord(b->ev[1], st);
+  const uint32_t max_blocks = (uint32_t)ctx->n_cus * 8;
+  if (mode == 0) {
+    const uint32_t nb = std::min(J.n_tiles, max_blocks);
+    if (e == hipSuccess) e = z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, st);
+  } else {
+    for (uint32_t s = 0; s < ns && e == hipSuccess; s++) {
+      const uint32_t t0 = b->surf_tile_off[s], tn = b->surf_tile_off[s + 1] - t0;
+      e = z2d_launch_render(J, t0, tn, J.cursors + s, std::min(tn, max_blocks), st);
+    }
+  }
+  return e;
+}
+
 static int32_t batch_render(z2d_gpu_batch *b, uint32_t mode) {
   z2d_gpu_ctx *ctx = b->ctx;
   if (!b->prepared) {
@@ -1092,30 +1133,39 @@ static int32_t batch_render(z2d_gpu_batch *b, uint32_t mode) {
   for (const z2d_gpu_surface *s : b->surfaces) any_clip |= s->clip != nullptr;
   if (b->draws.empty() && !any_clip) return Z2D_OK;
   HIP_OK(ctx, hipSetDevice(ctx->device));
-  JobView &J = b->view;
-  const uint32_t ns = J.n_surfaces;
+  const hipStream_t st = bstream(b);
   for (int i = 0; i < 3; i++)
     if (!b->ev[i]) HIP_OK(ctx, hipEventCreate(&b->ev[i]));
-  HIP_OK(ctx, hipEventRecord(b->ev[0], bstream(b)));
-  HIP_OK(ctx, z2d_launch_fill_u32(J.counters, 16, 0u, bstream(b)));
-  HIP_OK(ctx, z2d_launch_fill_u32(J.cursors, ns + 1, 0u, bstream(b)));
-  HIP_OK(ctx, z2d_launch_edge_build(J, bstream(b)));
-  HIP_OK(ctx, z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,
-                                 b->d_scratch_x, b->d_rowprog_soff, bstream(b)));
-  HIP_OK(ctx, z2d_launch_binning(J, bstream(b)));
-  HIP_OK(ctx, hipEventRecord(b->ev[1], bstream(b)));
-  const uint32_t max_blocks = (uint32_t)ctx->n_cus * 8;
-  if (mode == 0) {
-    uint32_t nb = std::min(J.n_tiles, max_blocks);
-    HIP_OK(ctx, z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, bstream(b)));
-  } else {
-    for (uint32_t s = 0; s < ns; s++) {
-      uint32_t t0 = b->surf_tile_off[s], tn = b->surf_tile_off[s + 1] - t0;
-      uint32_t nb = std::min(tn, max_blocks);
-      HIP_OK(ctx, z2d_launch_render(J, t0, tn, J.cursors + s, nb, bstream(b)));
-    }
+  if (!b->use_graph) {
+    HIP_OK(ctx, hipEventRecord(b->ev[0], st));
+    HIP_OK(ctx, batch_enqueue(b, mode, true));
+    HIP_OK(ctx, hipEventRecord(b->ev[2], st));
+    b->render_timed = true;
+    return Z2D_OK;
   }
-  HIP_OK(ctx, hipEventRecord(b->ev[2], bstream(b)));
+  // graph replay: captured once per prepared batch, launch mode and counter
+  // setting (all of them are kernel arguments)
+  if (b->gexec && (b->graph_mode != mode || b->graph_stats != b->view.count_stats))
+    batch_drop_graph(b);
+  if (!b->gexec) {
+    HIP_OK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = batch_enqueue(b, mode, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->gexec = nullptr;
+      return fail(ctx, e, "batch graph capture");
+    }
+    b->graph_mode = mode;
+    b->graph_stats = b->view.count_stats;
+  }
+  HIP_OK(ctx, hipEventRecord(b->ev[0], st));
+  HIP_OK(ctx, hipGraphLaunch(b->gexec, st));
+  HIP_OK(ctx, hipEventRecord(b->ev[2], st));
+  b->render_timed = false;
   return Z2D_OK;
 }
 
@@ -1125,7 +1175,10 @@ static int32_t batch_wait(z2d_gpu_batch *b) {
   if (b->ev[0] && b->ev[2]) {
     float a = 0.0f, r = 0.0f;
     if (hipEventElapsedTime(&a, b->ev[0], b->ev[2]) == hipSuccess) b->last_pipeline_ms = a;
-    if (hipEventElapsedTime(&r, b->ev[1], b->ev[2]) == hipSuccess) b->last_render_ms = r;
+    if (!b->render_timed)
+      b->last_render_ms = -1.0f;  // a graph replay has no mark before the render
+    else if (hipEventElapsedTime(&r, b->ev[1], b->ev[2]) == hipSuccess)
+      b->last_render_ms = r;
   }
   return Z2D_OK;
 }
@@ -1611,6 +1664,13 @@ int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
   return Z2D_OK;
 }
 
+int32_t z2d_gpu_batch_use_graph(z2d_gpu_batch *b, uint32_t enable) {
+  if (!b) return Z2D_ERR_INVALID_ARGUMENT;
+  b->use_graph = enable != 0;
+  if (!b->use_graph) batch_drop_graph(b);
+  return Z2D_OK;
+}
+
 int32_t z2d_gpu_batch_timing(z2d_gpu_batch *b, double *pipeline_ms, double *render_ms) {
   if (!b) return Z2D_ERR_INVALID_ARGUMENT;
   if (pipeline_ms) *pipeline_ms = b->last_pipeline_ms;
@@ -1659,6 +1719,7 @@ int32_t z2d_gpu_batch_reset(z2d_gpu_batch *b) {
 void z2d_gpu_batch_destroy(z2d_gpu_batch *b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graph(b);
   b->arena.release();
   b->stage.release();
   for (int i = 0; i < 3; i++)

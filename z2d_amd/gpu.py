@@ -45,6 +45,7 @@ _SIGS = {
     "z2d_gpu_batch_render": ([C.c_void_p, C.c_uint32], C.c_int32),
     "z2d_gpu_batch_wait": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_timing": ([C.c_void_p, P(C.c_double), P(C.c_double)], C.c_int32),
+    "z2d_gpu_batch_use_graph": ([C.c_void_p, C.c_uint32], C.c_int32),
     "z2d_gpu_composite_timing": ([C.c_void_p, P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
@@ -185,6 +186,10 @@ class Batch:
 
     def wait(self):
         self.ctx.check(lib().z2d_gpu_batch_wait(self.h), "batch_wait")
+
+    def use_graph(self, enable=True):
+        """Replay the render pipeline as one captured hipGraph."""
+        self.ctx.check(lib().z2d_gpu_batch_use_graph(self.h, 1 if enable else 0), "use_graph")
 
     def timing(self):
         a, r = C.c_double(), C.c_double()
