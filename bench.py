@@ -189,7 +189,7 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     uniform, 25% dashed).  Strokes are tessellated on the host when the batch
     is built.  One persistent launch vs one launch per surface over the same
     prepared batch; the persistent launch also as a replayed hipGraph of the
-    whole pipeline."""
+    whole pipeline.  The band height is picked by the autotuner first."""
     rng = scenes.SplitMix64(seed)
     b = gpu.Batch(ctx)
     surfs = []
@@ -208,6 +208,9 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     b.prepare()
     pix, _ = count_work(b)
     res = {}
+    band_h, tune_ms = b.autotune(0, 2)
+    res["autotuned_band_rows"] = band_h
+    res["autotune_pipeline_ms"] = {str(k): round(v, 3) for k, v in tune_ms.items()}
     for mode, name, graph in ((0, "batched", False), (0, "batched_graph", True),
                               (1, "per_surface", False)):
         b.use_graph(graph)

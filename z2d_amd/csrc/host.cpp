@@ -651,34 +651,38 @@ struct HostDraw {
   DevDraw d;
   uint32_t surface_slot;  // index into batch surfaces
   uint32_t seq;           // insertion order
+  double box[4];          // xmin, xmax, ymin, ymax of the input points (pixels)
+  bool whole;             // unbounded operator or non-finite box: every tile
+  double edge_scale;      // polygon draws: the scale of their edges' space
+  // set when the batch is prepared, for its band height
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
   uint32_t tcol_lo, tcol_hi;  // estimated tile-column range (inclusive)
   uint64_t edge_pairs;        // upper bound of (edge, band) pairs
 };
 
-// Bands [floor(lo / TILE_H), floor(hi / TILE_H)] that an edge between pixel
+// Bands [floor(lo / band_h), floor(hi / band_h)] that an edge between pixel
 // rows a and b can meet, capped at the surface's band count.
-static uint64_t band_span(double a, double b, uint32_t nbands) {
+static uint64_t band_span(double a, double b, uint32_t nbands, uint32_t band_h) {
   const double lo = std::min(a, b), hi = std::max(a, b);
   if (!std::isfinite(lo) || !std::isfinite(hi)) return nbands;
-  const double s = std::floor(hi / TILE_H) - std::floor(lo / TILE_H) + 1.0;
+  const double s = std::floor(hi / band_h) - std::floor(lo / band_h) + 1.0;
   return (uint64_t)std::min(s, (double)nbands);
 }
 
 // Upper bound of the (edge, band) pairs of a flattened draw, following the
 // fill plotter's edge sources: a line joins the previous point to its own,
 // a close joins the last point to the subpath's first.
-static uint64_t node_edge_pairs(const FNode *n, size_t cnt, uint32_t nbands) {
+static uint64_t node_edge_pairs(const FNode *n, size_t cnt, uint32_t nbands, uint32_t band_h) {
   uint64_t total = 0;
   double cur = 0.0, init = 0.0;
   for (size_t i = 0; i < cnt; i++) {
     if (n[i].tag == Z2D_NODE_MOVE_TO) {
       cur = init = n[i].y;
     } else if (n[i].tag == Z2D_NODE_LINE_TO) {
-      total += band_span(cur, n[i].y, nbands);
+      total += band_span(cur, n[i].y, nbands, band_h);
       cur = n[i].y;
     } else if (n[i].tag == Z2D_NODE_CLOSE_PATH) {
-      total += band_span(cur, init, nbands);
+      total += band_span(cur, init, nbands, band_h);
       cur = init;
     }
   }
@@ -706,6 +710,7 @@ struct z2d_gpu_batch {
   std::vector<uint32_t> surf_tile_off;  // per surface slot, + total
   float last_pipeline_ms = 0.0f, last_render_ms = 0.0f;
   uint32_t count_stats = 0;
+  uint32_t band_h = TILE_H;  // rows per band (bin height), a multiple of ROWS_PER_THREAD
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool render_timed = false;  // the last render recorded ev[1] before the tile render
   // hipGraph replay of the render pipeline (z2d_gpu_batch_use_graph)
@@ -744,24 +749,12 @@ static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
   const bool bounded = op_is_bounded(op);
   if (d.mode == MODE_DIRECT && !bounded) d.flags |= DRAW_ROWPROG;
   if (d.flags & DRAW_UNBALANCED) d.flags |= DRAW_ROWPROG;
-  // Band and tile-column estimates from the input points (pixels); the
-  // composite region lies within a pixel of the points' box.
-  const int32_t nb = (s->h + TILE_H - 1) / TILE_H;
-  const int32_t nc = (s->w + TILE_W - 1) / TILE_W;
+  // The points' box gives the band and tile-column estimates when the batch
+  // is prepared; the composite region lies within a pixel of it.
   const bool finite = std::isfinite(box[0]) && std::isfinite(box[1]) &&
                       std::isfinite(box[2]) && std::isfinite(box[3]);
-  if (!bounded || !finite) {
-    hd.band_lo = 0;
-    hd.band_hi = (uint32_t)(nb - 1);
-    hd.tcol_lo = 0;
-    hd.tcol_hi = (uint32_t)(nc - 1);
-  } else {
-    auto clamped = [](double v, int32_t n) { return std::max(0.0, std::min(v, (double)(n - 1))); };
-    hd.band_lo = (uint32_t)clamped(std::floor(box[2]) - 2.0, s->h) / TILE_H;
-    hd.band_hi = (uint32_t)clamped(std::ceil(box[3]) + 2.0, s->h) / TILE_H;
-    hd.tcol_lo = (uint32_t)clamped(std::floor(box[0]) - 2.0, s->w) / TILE_W;
-    hd.tcol_hi = (uint32_t)clamped(std::ceil(box[1]) + 2.0, s->w) / TILE_W;
-  }
+  for (int i = 0; i < 4; i++) hd.box[i] = box[i];
+  hd.whole = !bounded || !finite;
   b->draws.push_back(hd);
   b->prepared = false;
   return Z2D_OK;
@@ -791,8 +784,6 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
   }
   hd.d.node_off = (uint32_t)before;
   hd.d.node_cnt = (uint32_t)(b->nodes.size() - before);
-  hd.edge_pairs = node_edge_pairs(b->nodes.data() + before, hd.d.node_cnt,
-                                  (uint32_t)((s->h + TILE_H - 1) / TILE_H));
   for (size_t i = before + 1; i < b->nodes.size(); i++)
     if (b->nodes[i].tag == Z2D_NODE_LINE_TO && b->nodes[i - 1].tag == Z2D_NODE_CLOSE_PATH) {
       hd.d.flags |= DRAW_SERIAL_PLOT;
@@ -828,9 +819,7 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   const double sc = (aa == Z2D_AA_NONE || s->format == Z2D_FORMAT_ALPHA1) ? 1.0 : 4.0;
   const double box[4] = {n_edges ? ext[2] / sc : NAN, n_edges ? ext[3] / sc : NAN,
                          n_edges ? ext[0] / sc : NAN, n_edges ? ext[1] / sc : NAN};
-  const uint32_t nbands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
-  for (uint64_t i = 0; i < n_edges; i++)
-    hd.edge_pairs += band_span(edges[i].y0 / sc, edges[i].y1 / sc, nbands);
+  hd.edge_scale = sc;
   return batch_add_common(b, s, pattern, aa, rule, op, precision, hd, box);
 }
 
@@ -900,11 +889,13 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   });
   std::vector<uint32_t> surf_draw_off(ns + 1, 0), surf_band_off(ns + 1, 0);
   b->surf_tile_off.assign(ns + 1, 0);
+  const uint32_t band_h = b->band_h;
+  const uint32_t spt = band_h / ROWS_PER_THREAD;  // strips per tile
   for (const HostDraw &h : b->draws) surf_draw_off[h.surface_slot + 1]++;
   for (uint32_t i = 0; i < ns; i++) {
     surf_draw_off[i + 1] += surf_draw_off[i];
     const z2d_gpu_surface *s = b->surfaces[i];
-    uint32_t bands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
+    uint32_t bands = (uint32_t)((s->h + band_h - 1) / band_h);
     uint32_t tcols = (uint32_t)((s->w + TILE_W - 1) / TILE_W);
     surf_band_off[i + 1] = surf_band_off[i] + bands;
     b->surf_tile_off[i + 1] = b->surf_tile_off[i] + bands * tcols;
@@ -926,12 +917,41 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   for (uint32_t i = 0; i < nd; i++) {
     HostDraw &h = b->draws[i];
     h.d.surface = h.surface_slot;
+    {
+      // band / tile-column estimates and the (edge, band) bound for band_h
+      const z2d_gpu_surface *s = b->surfaces[h.surface_slot];
+      const uint32_t bh = b->band_h;
+      const uint32_t nb = (uint32_t)((s->h + bh - 1) / bh);
+      const uint32_t nc = (uint32_t)((s->w + TILE_W - 1) / TILE_W);
+      if (h.whole) {
+        h.band_lo = 0;
+        h.band_hi = nb - 1;
+        h.tcol_lo = 0;
+        h.tcol_hi = nc - 1;
+      } else {
+        auto clamped = [](double v, int32_t n) {
+          return std::max(0.0, std::min(v, (double)(n - 1)));
+        };
+        h.band_lo = (uint32_t)clamped(std::floor(h.box[2]) - 2.0, s->h) / bh;
+        h.band_hi = (uint32_t)clamped(std::ceil(h.box[3]) + 2.0, s->h) / bh;
+        h.tcol_lo = (uint32_t)clamped(std::floor(h.box[0]) - 2.0, s->w) / TILE_W;
+        h.tcol_hi = (uint32_t)clamped(std::ceil(h.box[1]) + 2.0, s->w) / TILE_W;
+      }
+      if (h.d.input_edges) {
+        h.edge_pairs = 0;
+        const z2d_edge *ed = b->in_edges.data() + h.d.in_edge_off;
+        for (uint32_t k = 0; k < h.d.node_cnt; k++)
+          h.edge_pairs += band_span(ed[k].y0 / h.edge_scale, ed[k].y1 / h.edge_scale, nb, bh);
+      } else {
+        h.edge_pairs = node_edge_pairs(b->nodes.data() + h.d.node_off, h.d.node_cnt, nb, bh);
+      }
+    }
     h.d.band_lo = h.band_lo;
     h.d.band_hi = h.band_hi;
     h.d.entry_base = (uint32_t)std::min<uint64_t>(max_entries, 0xFFFFFFFFull);
     for (uint32_t k = 0; k < h.d.node_cnt; k += EDGE_GROUP) egroups.push_back(make_uint2(i, k));
     max_entries += h.band_hi - h.band_lo + 1;
-    max_strip_entries += (uint64_t)(h.band_hi - h.band_lo + 1) * STRIPS_PER_TILE *
+    max_strip_entries += (uint64_t)(h.band_hi - h.band_lo + 1) * spt *
                          (h.tcol_hi - h.tcol_lo + 1);
     max_entry_edges += h.edge_pairs;
     if (h.d.flags & DRAW_ROWPROG) {
@@ -945,7 +965,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   }
   if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull ||
       max_entry_edges > 0xFFFFFFF0ull || max_strip_entries > 0xFFFFFFF0ull ||
-      (uint64_t)n_tiles * STRIPS_PER_TILE > 0xFFFFFFF0ull || egroups.size() > 0xFFFFFFF0ull ||
+      (uint64_t)n_tiles * spt > 0xFFFFFFF0ull || egroups.size() > 0xFFFFFFF0ull ||
       2 * max_entries + 1 > 0xFFFFFFF0ull)
     return Z2D_ERR_OUT_OF_MEMORY;
   b->n_rowprog = (uint32_t)rowprog_list.size();
@@ -971,14 +991,14 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
   const size_t o_slotcnt = L.take(((size_t)n_slots + 1) * sizeof(uint32_t));
   const uint64_t scan_n = std::max<uint64_t>(
-      std::max<uint64_t>(n_slots, max_entries), (uint64_t)n_tiles * STRIPS_PER_TILE);
+      std::max<uint64_t>(n_slots, max_entries), (uint64_t)n_tiles * spt);
   const size_t o_scantmp = L.take((size_t)(scan_n / 4096 + 2) * sizeof(uint32_t));
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_blist = L.take(max_entries * sizeof(uint32_t));
   const size_t o_ecnt = L.take((2 * max_entries + 1) * sizeof(uint32_t));  // + cursors
   const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
-  const size_t o_stripcnt = L.take(((size_t)n_tiles * STRIPS_PER_TILE + 1) * sizeof(uint32_t));
+  const size_t o_stripcnt = L.take(((size_t)n_tiles * spt + 1) * sizeof(uint32_t));
   const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
@@ -1015,7 +1035,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   TileRef *ht = reinterpret_cast<TileRef *>(H + o_tiles);
   for (uint32_t i = 0; i < ns; i++) {
     const z2d_gpu_surface *s = b->surfaces[i];
-    const uint32_t bands = (uint32_t)((s->h + TILE_H - 1) / TILE_H);
+    const uint32_t bands = (uint32_t)((s->h + band_h - 1) / band_h);
     const uint32_t tcols = (uint32_t)((s->w + TILE_W - 1) / TILE_W);
     uint32_t k = b->surf_tile_off[i];
     for (uint32_t ty = 0; ty < bands; ty++)
@@ -1074,6 +1094,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.n_egroups = (uint32_t)egroups.size();
   J.entry_edges = reinterpret_cast<z2d_edge *>(D + o_eidx);
   J.max_entry_edges = (uint32_t)max_entry_edges;
+  J.band_h = band_h;
+  J.strips_per_tile = spt;
   J.strip_cnt = reinterpret_cast<uint32_t *>(D + o_stripcnt);
   J.strip_entries = reinterpret_cast<uint32_t *>(D + o_sentries);
   J.max_strip_entries = (uint32_t)max_strip_entries;
@@ -1662,6 +1684,75 @@ int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
   if (!ctx || !kernel_ms) return Z2D_ERR_INVALID_ARGUMENT;
   *kernel_ms = ctx->last_comp_ms;
   return Z2D_OK;
+}
+
+int32_t z2d_gpu_batch_set_band_height(z2d_gpu_batch *b, uint32_t rows) {
+  if (!b || rows < (uint32_t)ROWS_PER_THREAD || rows > (uint32_t)MAX_BAND_H ||
+      rows % ROWS_PER_THREAD)
+    return Z2D_ERR_INVALID_ARGUMENT;
+  if (rows != b->band_h) {
+    b->band_h = rows;
+    b->prepared = false;
+  }
+  return Z2D_OK;
+}
+
+// The autotuner: renders the batch once per candidate band height (a warm-up
+// and `reps` timed pipelines each), keeps the fastest and prepares the batch
+// for it.  The surfaces are saved before and restored after every candidate,
+// so the batch's targets end as they started.
+int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
+                               uint32_t *band_h, double ms_out[3]) {
+  if (!b || reps == 0 || mode > 1) return Z2D_ERR_INVALID_ARGUMENT;
+  z2d_gpu_ctx *ctx = b->ctx;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  const uint32_t cand[3] = {8, 16, 32};
+  size_t total = 0;
+  for (const z2d_gpu_surface *s : b->surfaces) total += (s->nbytes + 255) & ~(size_t)255;
+  DevBuf saved;
+  hipError_t e = saved.ensure(total ? total : 1);
+  if (e != hipSuccess) return fail(ctx, e, "autotune save buffer");
+  const hipStream_t st = bstream(b);
+  auto copy_all = [&](bool save) {
+    size_t off = 0;
+    hipError_t r = hipSuccess;
+    for (const z2d_gpu_surface *s : b->surfaces) {
+      if (r == hipSuccess)
+        r = save ? hipMemcpyAsync(saved.p + off, s->data, s->nbytes, hipMemcpyDeviceToDevice, st)
+                 : hipMemcpyAsync(s->data, saved.p + off, s->nbytes, hipMemcpyDeviceToDevice, st);
+      off += (s->nbytes + 255) & ~(size_t)255;
+    }
+    return r;
+  };
+  e = copy_all(true);
+  double best = 1e30;
+  uint32_t best_h = b->band_h;
+  int32_t rc = Z2D_OK;
+  for (int c = 0; c < 3 && e == hipSuccess && rc == Z2D_OK; c++) {
+    b->band_h = cand[c];
+    b->prepared = false;
+    double sum = 0.0;
+    for (uint32_t r = 0; r <= reps && rc == Z2D_OK; r++) {
+      rc = batch_render(b, mode);
+      if (rc == Z2D_OK) rc = batch_wait(b);
+      if (r > 0) sum += b->last_pipeline_ms;
+    }
+    if (rc == Z2D_OK) e = copy_all(false);
+    const double ms = sum / reps;
+    if (ms_out) ms_out[c] = ms;
+    if (rc == Z2D_OK && ms < best) {
+      best = ms;
+      best_h = cand[c];
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  saved.release();
+  if (e != hipSuccess) return fail(ctx, e, "autotune");
+  if (rc) return rc;
+  b->band_h = best_h;
+  b->prepared = false;
+  if (band_h) *band_h = best_h;
+  return batch_prepare(b);
 }
 
 int32_t z2d_gpu_batch_use_graph(z2d_gpu_batch *b, uint32_t enable) {

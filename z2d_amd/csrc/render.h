@@ -11,16 +11,18 @@
 
 namespace z2d {
 
-// Tile shape of the renderer: TILE_W pixels x TILE_H rows per band entry;
-// workgroups of 256 lanes (4 wave64s) hold four independent strip workers.
+// Tile shape of the renderer: TILE_W pixels x band_h rows per band entry
+// (JobView::band_h, TILE_H unless tuned; a multiple of ROWS_PER_THREAD up to
+// MAX_BAND_H); workgroups of 256 lanes (4 wave64s) hold four independent
+// strip workers.
 constexpr int TILE_W = 64;
 constexpr int TILE_H = 16;
+constexpr int MAX_BAND_H = 32;
 constexpr int RENDER_THREADS = 256;
-constexpr int ROWS_PER_THREAD = TILE_H / 4;
-constexpr int MAX_SUBROWS = TILE_H * 4;
 // Each wave renders one strip of ROWS_PER_THREAD rows of a tile on its own
 // (no workgroup barriers): lane = column, one register per strip row.
-constexpr int STRIPS_PER_TILE = TILE_H / ROWS_PER_THREAD;
+constexpr int ROWS_PER_THREAD = 4;
+constexpr int MAX_STRIPS_PER_TILE = MAX_BAND_H / ROWS_PER_THREAD;
 constexpr int STRIP_SUBROWS = ROWS_PER_THREAD * 4;
 constexpr int ACC_WORDS = TILE_W * 2;  // 4 sample columns per pixel, two int16 per word
 // Winding deltas are accumulated as int16 pairs in LDS; an entry whose band
@@ -134,9 +136,11 @@ struct JobView {
   uint32_t n_egroups;
   z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges
   uint32_t max_entry_edges;
-  // per strip (tile * STRIPS_PER_TILE + strip): the band entries whose
+  uint32_t band_h;           // rows per band
+  uint32_t strips_per_tile;  // band_h / ROWS_PER_THREAD
+  // per strip (tile * strips_per_tile + strip): the band entries whose
   // composite region meets the strip, in draw order
-  uint32_t *strip_cnt;       // n_tiles * STRIPS_PER_TILE + 1 (scanned in place)
+  uint32_t *strip_cnt;       // n_tiles * strips_per_tile + 1 (scanned in place)
   uint32_t *strip_entries;   // band entry indices, capacity max_strip_entries
   uint32_t max_strip_entries;
   // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;

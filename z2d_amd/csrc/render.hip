@@ -11,7 +11,7 @@
 //   k_edge_bins / scan
 //                 per (draw, band) entry, the draw's edges that meet the band
 //   k_band_pass / scan
-//                 ordered per-band lists of entries (TILE_H-row bins, draw
+//                 ordered per-band lists of entries (band_h-row bins, draw
 //                 order kept)
 //   k_strip_pass / scan
 //                 per strip (TILE_W x ROWS_PER_THREAD pixels), the band
@@ -483,7 +483,7 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
 }
 
 // ---------------------------------------------------------------------------
-// Binning: ordered draw lists per band (TILE_H rows of one surface).
+// Binning: ordered draw lists per band (band_h rows of one surface).
 // ---------------------------------------------------------------------------
 // One wave per slot: a band and 64 consecutive draws of its surface (every
 // band has at least one slot).  FILL=false counts the draws whose composite
@@ -498,7 +498,8 @@ __global__ void k_band_pass(JobView J) {
   const uint2 si = J.slot_info[slot];  // band, surface
   const uint32_t b = si.x, s = si.y;
   const uint32_t c = slot - J.band_slot_off[b];
-  const int32_t y0 = (int32_t)(b - J.surf_band_off[s]) * TILE_H, y1 = y0 + TILE_H;
+  const int32_t y0 = (int32_t)(b - J.surf_band_off[s]) * (int32_t)J.band_h,
+                y1 = y0 + (int32_t)J.band_h;
   const uint32_t d = J.surf_draw_off[s] + c * 64 + lane, d1 = J.surf_draw_off[s + 1];
   const uint32_t lb = b - J.surf_band_off[s];  // band within the surface
   bool hit = false;
@@ -674,12 +675,13 @@ __global__ void k_strip_pass(JobView J) {
   const uint32_t b = J.surf_band_off[T.surface] + T.ty;
   const uint32_t e0 = min(J.band_cnt[b], J.max_entries);
   const uint32_t e1 = min(J.band_cnt[b + 1], J.max_entries);
-  const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * TILE_H;
-  uint32_t cnt[STRIPS_PER_TILE], base[STRIPS_PER_TILE];
+  const uint32_t spt = J.strips_per_tile;
+  const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * (int32_t)J.band_h;
+  uint32_t cnt[MAX_STRIPS_PER_TILE], base[MAX_STRIPS_PER_TILE];
 #pragma unroll
-  for (int s = 0; s < STRIPS_PER_TILE; s++) {
+  for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
     cnt[s] = 0;
-    base[s] = FILL ? J.strip_cnt[t * STRIPS_PER_TILE + s] : 0u;
+    base[s] = (FILL && (uint32_t)s < spt) ? J.strip_cnt[t * spt + s] : 0u;
   }
   const uint64_t below = (1ull << lane) - 1ull;
   for (uint32_t e = e0; e < e1; e += 64) {
@@ -690,14 +692,15 @@ __global__ void k_strip_pass(JobView J) {
       const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
       if (rx1 > tx0 && rx0 < tx0 + TILE_W) {
 #pragma unroll
-        for (int s = 0; s < STRIPS_PER_TILE; s++) {
+        for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
           const int32_t sy0 = ty0 + s * ROWS_PER_THREAD;
-          if (ry1 > sy0 && ry0 < sy0 + ROWS_PER_THREAD) mask |= 1u << s;
+          if ((uint32_t)s < spt && ry1 > sy0 && ry0 < sy0 + ROWS_PER_THREAD) mask |= 1u << s;
         }
       }
     }
 #pragma unroll
-    for (int s = 0; s < STRIPS_PER_TILE; s++) {
+    for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
+      if ((uint32_t)s >= spt) break;  // uniform
       const bool hit = (mask >> s) & 1u;
       const uint64_t m = __ballot(hit);
       if (FILL && hit) {
@@ -712,7 +715,8 @@ __global__ void k_strip_pass(JobView J) {
   }
   if (!FILL && lane == 0) {
 #pragma unroll
-    for (int s = 0; s < STRIPS_PER_TILE; s++) J.strip_cnt[t * STRIPS_PER_TILE + s] = cnt[s];
+    for (int s = 0; s < MAX_STRIPS_PER_TILE; s++)
+      if ((uint32_t)s < spt) J.strip_cnt[t * spt + s] = cnt[s];
   }
 }
 
@@ -741,9 +745,10 @@ __global__ void k_edge_bins(JobView J) {
   if (it.y >= ne) return;
   const uint32_t kend = min(it.y + EDGE_GROUP, ne);
   const int SC = D.mode == MODE_DIRECT ? 1 : 4;
-  const int32_t rb_lo = rows.x / TILE_H, rb_hi = (rows.y - 1) / TILE_H;  // entry bands
+  const int32_t bh = (int32_t)J.band_h;
+  const int32_t rb_lo = rows.x / bh, rb_hi = (rows.y - 1) / bh;  // entry bands
   const uint32_t blo = D.band_lo, bhi = D.band_hi, ebase = D.entry_base;
-  const double band_rows = (double)(TILE_H * SC);
+  const double band_rows = (double)(bh * SC);
   const z2d_edge *edges = J.edges + D.node_off;
   const uint64_t below = (1ull << lane) - 1ull;
   for (uint32_t k0 = it.y; k0 < kend; k0 += 64) {
@@ -774,9 +779,9 @@ __global__ void k_edge_bins(JobView J) {
       const int32_t wend = min(uhi, w + 63);
       uint32_t cnt = 0;
       for (int32_t bnd = w; bnd <= wend; bnd++) {
-        const int32_t y0 = bnd * TILE_H;
+        const int32_t y0 = bnd * bh;
         const double ys_lo = (double)(y0 * SC) + 0.5;
-        const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+        const double ys_hi = (double)((y0 + bh) * SC - 1) + 0.5;
         const bool hit = lo <= bnd && bnd <= hi && top < ys_hi && bot >= ys_lo;
         const uint32_t n = (uint32_t)__popcll(__ballot(hit));
         if ((int32_t)lane == bnd - w) cnt = n;
@@ -791,9 +796,9 @@ __global__ void k_edge_bins(JobView J) {
       const uint32_t at = has ? J.entry_ecnt[idx] + atomicAdd(&J.entry_cur[idx], cnt) : 0u;
       for (int32_t bnd = w; bnd <= wend; bnd++) {
         if ((uint32_t)bnd < blo || (uint32_t)bnd > bhi) continue;
-        const int32_t y0 = bnd * TILE_H;
+        const int32_t y0 = bnd * bh;
         const double ys_lo = (double)(y0 * SC) + 0.5;
-        const double ys_hi = (double)((y0 + TILE_H) * SC - 1) + 0.5;
+        const double ys_hi = (double)((y0 + bh) * SC - 1) + 0.5;
         const bool hit = lo <= bnd && bnd <= hi && top < ys_hi && bot >= ys_lo;
         const uint64_t m = __ballot(hit);
         if (m == 0) continue;
@@ -1077,7 +1082,8 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
   __shared__ StripShared shw[RENDER_THREADS / 64];
   const uint32_t lane = threadIdx.x & 63;
   StripShared &sh = shw[threadIdx.x >> 6];
-  const uint32_t n_items = tile_count * STRIPS_PER_TILE;
+  const uint32_t spt = J.strips_per_tile;
+  const uint32_t n_items = tile_count * spt;
 
   // The accumulator starts at zero and every entry leaves it at zero.
   for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
@@ -1104,14 +1110,14 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       }
       break;
     }
-    const TileRef T = load_uniform(&J.tiles[tile_base + item / STRIPS_PER_TILE]);
-    const uint32_t strip = item % STRIPS_PER_TILE;
+    const TileRef T = load_uniform(&J.tiles[tile_base + item / spt]);
+    const uint32_t strip = item % spt;
     const DevSurface S = load_uniform(&J.surfaces[T.surface]);
     const int32_t W = S.w, H = S.h;
     const int32_t tx0 = (int32_t)T.tx * TILE_W;
-    const int32_t ty0 = (int32_t)T.ty * TILE_H + (int32_t)strip * ROWS_PER_THREAD;
+    const int32_t ty0 = (int32_t)T.ty * (int32_t)J.band_h + (int32_t)strip * ROWS_PER_THREAD;
     if (ty0 >= H) continue;
-    const uint32_t sidx = tile_base * STRIPS_PER_TILE + item;
+    const uint32_t sidx = tile_base * spt + item;
     const uint32_t e0 = min(load_uniform(&J.strip_cnt[sidx]), J.max_strip_entries);
     const uint32_t e1 = min(load_uniform(&J.strip_cnt[sidx + 1]), J.max_strip_entries);
     if (e0 == e1 && !S.clip) continue;
@@ -1724,7 +1730,7 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   if (J.n_tiles) {
     dim3 tgrid((J.n_tiles + 3) / 4);
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
-    scan_u32(J.strip_cnt, J.n_tiles * (uint32_t)STRIPS_PER_TILE, J.scan_tmp, st);
+    scan_u32(J.strip_cnt, J.n_tiles * J.strips_per_tile, J.scan_tmp, st);
     hipLaunchKernelGGL(k_strip_pass<true>, tgrid, dim3(256), 0, st, J);
   }
   return hipGetLastError();

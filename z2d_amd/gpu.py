@@ -46,6 +46,9 @@ _SIGS = {
     "z2d_gpu_batch_wait": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_timing": ([C.c_void_p, P(C.c_double), P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_use_graph": ([C.c_void_p, C.c_uint32], C.c_int32),
+    "z2d_gpu_batch_set_band_height": ([C.c_void_p, C.c_uint32], C.c_int32),
+    "z2d_gpu_batch_autotune": ([C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_uint32),
+                                P(C.c_double)], C.c_int32),
     "z2d_gpu_composite_timing": ([C.c_void_p, P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
@@ -186,6 +189,18 @@ class Batch:
 
     def wait(self):
         self.ctx.check(lib().z2d_gpu_batch_wait(self.h), "batch_wait")
+
+    def set_band_height(self, rows):
+        self.ctx.check(lib().z2d_gpu_batch_set_band_height(self.h, rows), "set_band_height")
+
+    def autotune(self, mode=0, reps=2):
+        """Pick the band height (8, 16, 32) with the fastest pipeline; returns
+        (band_h, {candidate: mean ms})."""
+        h = C.c_uint32()
+        ms = (C.c_double * 3)()
+        self.ctx.check(lib().z2d_gpu_batch_autotune(self.h, mode, reps, C.byref(h), ms),
+                       "autotune")
+        return h.value, dict(zip((8, 16, 32), list(ms)))
 
     def use_graph(self, enable=True):
         """Replay the render pipeline as one captured hipGraph."""
