@@ -234,21 +234,23 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
 
 
 def serving_requests(gpu, n, seed=0xC5):
-    """Request mix of the serving form: one surface job each, side
-    log-uniform 16-1024 px, 80% RGBA8 / 20% alpha8 cleared to 0, 1-50 paths
-    of the 4K scene's distribution, half with an alpha8 clip mask."""
+    """Request mix of the serving form: one surface job of the batched
+    form's distribution each (side log-uniform 16-1024 px, 80% RGBA8 / 20%
+    alpha8 cleared to 0, 1-50 paths of the 4K scene's distribution, 30% of
+    them stroked), half with an alpha8 clip mask."""
     rng = scenes.SplitMix64(seed)
     reqs = []
     for _ in range(n):
         side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
         fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
-        draws, keep = scenes.random_paths(rng, rng.randint(1, 50), side, side,
-                                          max_side=min(512.0, side))
+        jobs, keep = scenes.random_jobs(rng, rng.randint(1, 50), side, side,
+                                        max_side=min(512.0, side))
         clip = None
         if rng.uniform() < 0.5:
             clip = scenes.splitmix_stream(rng.next(), (side * side + 7) // 8).view(np.uint8)
             clip = clip[: side * side].copy()
-        prep = gpu.Server.prepare(fmt, side, side, draws, clear=abi.pixel(fmt), clip=clip)
+        prep = gpu.Server.prepare(fmt, side, side, [], clear=abi.pixel(fmt), clip=clip,
+                                  draws=jobs)
         reqs.append(dict(prep=prep, keep=keep,
                          out=np.empty(abi.surface_nbytes(fmt, side, side), np.uint8)))
     return reqs

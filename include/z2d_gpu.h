@@ -177,6 +177,17 @@ typedef struct z2d_fill_item {
   z2d_fill_opts opts;
 } z2d_fill_item;
 
+/* A fill or a stroke of a request (painter.fill / painter.stroke). */
+enum { Z2D_DRAW_FILL = 0, Z2D_DRAW_STROKE = 1 };
+typedef struct z2d_draw_item {
+  uint32_t kind;                   /* Z2D_DRAW_FILL or Z2D_DRAW_STROKE */
+  const z2d_pattern_desc *pattern;
+  const z2d_path_node *nodes;
+  uint64_t n_nodes;
+  z2d_fill_opts fill;              /* kind == Z2D_DRAW_FILL */
+  const z2d_stroke_opts *stroke;   /* kind == Z2D_DRAW_STROKE (its dashes are copied) */
+} z2d_draw_item;
+
 typedef struct z2d_request_desc {
   uint32_t format;
   int32_t width;
@@ -185,6 +196,8 @@ typedef struct z2d_request_desc {
   const uint8_t *clip_mask;   /* optional alpha8 width*height mask */
   const z2d_fill_item *fills;
   uint64_t n_fills;
+  const z2d_draw_item *draws; /* fills and strokes, drawn in order after `fills` */
+  uint64_t n_draws;
 } z2d_request_desc;
 
 int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch,

@@ -158,3 +158,61 @@ def test_server_error_status():
         assert not diff_report(out, oracle_render(good))
     finally:
         srv.close()
+
+
+def test_server_fills_and_strokes():
+    """Requests of the batched form's distribution (30% of the paths
+    stroked, dashes included) against the CPU path: the host tessellator's
+    polygons and the fills rasterized by the oracle, in draw order, then the
+    clip mask; also alone vs in the mix."""
+    from stroke_check import StrokeCheckBackend
+    rng = scenes.SplitMix64(0x57C)
+    reqs = []
+    for i in range(24):
+        fmt = [abi.RGBA, abi.ALPHA8, abi.RGB, abi.ALPHA4][rng.randint(0, 3)]
+        w, h = rng.randint(8, 260), rng.randint(8, 260)
+        jobs, keep = scenes.random_jobs(rng, rng.randint(0, 14), w, h,
+                                        max_side=min(200.0, max(w, h)))
+        clip = rng.u8_array(w * h) if rng.uniform() < 0.4 else None
+        reqs.append(dict(fmt=fmt, w=w, h=h, clear=_clear_pixel(rng, fmt), jobs=jobs, keep=keep,
+                         clip=clip))
+
+    def cpu(rq):
+        s = Surface(rq["fmt"], rq["w"], rq["h"], init_px=rq["clear"])
+        be = StrokeCheckBackend()
+        for kind, pat, nodes, opts in rq["jobs"]:
+            rc = (be.stroke if kind == "stroke" else be.fill)(s, pat, nodes, opts)
+            assert rc == 0, rc
+        if rq["clip"] is not None:
+            m = Surface(abi.ALPHA8, rq["w"], rq["h"])
+            m.buf[: m.nbytes] = rq["clip"]
+            md = m.desc()
+            op = abi.CompOp()
+            op.op = abi.OP["dst_in"]
+            op.src.kind = abi.PARAM_SURFACE
+            op.src.surface = C.pointer(md)
+            be.composite(s, 0, 0, [op], abi.PRECISION_INTEGER)
+        return np.asarray(s.data(), np.uint8)[: s.nbytes].copy()
+
+    srv = _server(256 << 20, max_batch=8)
+    try:
+        subs = []
+        for rq in reqs:
+            out = np.zeros(abi.surface_nbytes(rq["fmt"], rq["w"], rq["h"]), np.uint8)
+            rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], [], clear=rq["clear"],
+                                 clip=rq["clip"], out=out, draws=rq["jobs"])
+            assert rc == 0, rc
+            subs.append((rid, out))
+        for i, ((rid, out), rq) in enumerate(zip(subs, reqs)):
+            assert srv.wait(rid) == 0, f"request {i}"
+            msg = diff_report(out, cpu(rq))
+            assert not msg, f"request {i} fmt={rq['fmt']} {rq['w']}x{rq['h']}: {msg}"
+        # the last request alone
+        rq = reqs[-1]
+        alone = np.zeros_like(subs[-1][1])
+        rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], [], clear=rq["clear"],
+                             clip=rq["clip"], out=alone, draws=rq["jobs"])
+        assert rc == 0 and srv.wait(rid) == 0
+        assert not diff_report(alone, subs[-1][1])
+    finally:
+        srv.close()

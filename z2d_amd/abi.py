@@ -128,10 +128,18 @@ class FillItem(C.Structure):
                 ("n_nodes", C.c_uint64), ("opts", FillOpts)]
 
 
+DRAW_FILL, DRAW_STROKE = 0, 1
+
+
+class DrawItem(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("pattern", C.POINTER(PatternDesc)), ("nodes", C.c_void_p),
+                ("n_nodes", C.c_uint64), ("fill", FillOpts), ("stroke", C.POINTER(StrokeOpts))]
+
+
 class RequestDesc(C.Structure):
     _fields_ = [("format", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32),
                 ("clear", Pixel), ("clip_mask", C.c_void_p), ("fills", C.POINTER(FillItem)),
-                ("n_fills", C.c_uint64)]
+                ("n_fills", C.c_uint64), ("draws", C.POINTER(DrawItem)), ("n_draws", C.c_uint64)]
 
 
 IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)

@@ -2049,12 +2049,17 @@ struct HbmPool {
   }
 };
 
+// One draw of a request, copied at submit: a fill (opts) or a stroke (sopts
+// with its own copy of the dash array).
 struct SrvFill {
+  uint32_t kind = Z2D_DRAW_FILL;
   z2d_pattern_desc pat;
   z2d_gradient_desc grad;
   std::vector<z2d_stop> stops;
   std::vector<z2d_path_node> nodes;
   z2d_fill_opts opts;
+  z2d_stroke_opts sopts;
+  std::vector<double> dashes;
 };
 
 struct SrvRequest {
@@ -2171,8 +2176,11 @@ struct z2d_gpu_server {
       // a clipped surface takes part in the render even with no fills
       if (r->mask_dev) sl.batch.slot_of(&r->surf);
       for (auto &f : r->fills) {
-        int32_t rc = batch_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(), f->nodes.size(),
-                                &f->opts);
+        int32_t rc = f->kind == Z2D_DRAW_STROKE
+                         ? batch_stroke(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
+                                        f->nodes.size(), &f->sopts)
+                         : batch_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
+                                      f->nodes.size(), &f->opts);
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
       }
     }
@@ -2300,7 +2308,8 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   if (!s || !rq || !id) return Z2D_ERR_INVALID_ARGUMENT;
   if (rq->width < 1) return Z2D_ERR_INVALID_WIDTH;
   if (rq->height < 1) return Z2D_ERR_INVALID_HEIGHT;
-  if (rq->format > Z2D_FORMAT_ALPHA1 || (rq->n_fills && !rq->fills))
+  if (rq->format > Z2D_FORMAT_ALPHA1 || (rq->n_fills && !rq->fills) ||
+      (rq->n_draws && !rq->draws))
     return Z2D_ERR_INVALID_ARGUMENT;
   auto r = std::make_unique<SrvRequest>();
   r->format = rq->format;
@@ -2330,6 +2339,26 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
     copy_pattern(*f, it.pattern);
     f->nodes.assign(it.nodes, it.nodes + it.n_nodes);
     f->opts = it.opts;
+    r->fills.push_back(std::move(f));
+  }
+  for (uint64_t i = 0; i < rq->n_draws; i++) {
+    const z2d_draw_item &it = rq->draws[i];
+    if (it.kind > Z2D_DRAW_STROKE || (it.kind == Z2D_DRAW_STROKE && !it.stroke) ||
+        (it.n_nodes && !it.nodes))
+      return Z2D_ERR_INVALID_ARGUMENT;
+    int32_t rc = check_pattern(it.pattern);
+    if (rc) return rc;
+    auto f = std::make_unique<SrvFill>();
+    copy_pattern(*f, it.pattern);
+    f->kind = it.kind;
+    f->nodes.assign(it.nodes, it.nodes + it.n_nodes);
+    f->opts = it.fill;
+    if (it.kind == Z2D_DRAW_STROKE) {
+      f->sopts = *it.stroke;
+      if (f->sopts.n_dashes && !f->sopts.dashes) return Z2D_ERR_INVALID_ARGUMENT;
+      f->dashes.assign(f->sopts.dashes, f->sopts.dashes + f->sopts.n_dashes);
+      f->sopts.dashes = f->dashes.data();
+    }
     r->fills.push_back(std::move(f));
   }
   std::lock_guard<std::mutex> lk(s->mu);

@@ -282,22 +282,38 @@ class Server:
         self._pending = {}
 
     @staticmethod
-    def prepare(fmt, width, height, fills, clear=None, clip=None):
+    def prepare(fmt, width, height, fills, clear=None, clip=None, draws=None):
         """Builds the request descriptor once (it may be submitted many
-        times).  fills: list of (pattern, nodes, opts)."""
+        times).  fills: list of (pattern, nodes, opts); draws (after the
+        fills, in order): list of ("fill" | "stroke", pattern, nodes, opts)."""
         items = (abi.FillItem * max(1, len(fills)))()
         for i, (pat, nodes, opts) in enumerate(fills):
             items[i].pattern = C.pointer(pat)
             items[i].nodes = nodes.ctypes.data
             items[i].n_nodes = len(nodes)
             items[i].opts = opts
+        draws = draws or []
+        ditems = (abi.DrawItem * max(1, len(draws)))()
+        for i, (kind, pat, nodes, opts) in enumerate(draws):
+            ditems[i].pattern = C.pointer(pat)
+            ditems[i].nodes = nodes.ctypes.data
+            ditems[i].n_nodes = len(nodes)
+            if kind == "stroke":
+                ditems[i].kind = abi.DRAW_STROKE
+                ditems[i].stroke = C.pointer(opts)
+            else:
+                ditems[i].kind = abi.DRAW_FILL
+                ditems[i].fill = opts
         rq = abi.RequestDesc()
         rq.format, rq.width, rq.height = fmt, width, height
         rq.clear = clear if clear is not None else abi.Pixel()
         rq.clip_mask = clip.ctypes.data if clip is not None else None
         rq.fills = items
         rq.n_fills = len(fills)
-        return (rq, items, fills, clip)  # the descriptor and what it points into
+        rq.draws = ditems
+        rq.n_draws = len(draws)
+        # the descriptor and what it points into
+        return (rq, items, fills, clip, ditems, draws)
 
     def submit_prepared(self, prep, out=None):
         """Returns (rc, id); `out` (numpy uint8, >= surface bytes) receives
@@ -310,9 +326,11 @@ class Server:
             self._pending[rid.value] = out  # keeps the result buffer alive
         return rc, rid.value
 
-    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None):
-        """fills: list of (pattern, nodes, opts).  Returns (rc, id)."""
-        return self.submit_prepared(self.prepare(fmt, width, height, fills, clear, clip), out)
+    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None, draws=None):
+        """fills: list of (pattern, nodes, opts); draws: list of
+        ("fill" | "stroke", pattern, nodes, opts).  Returns (rc, id)."""
+        return self.submit_prepared(
+            self.prepare(fmt, width, height, fills, clear, clip, draws), out)
 
     def wait(self, rid, timeout_ms=60000.0):
         st = C.c_int32()
