@@ -760,38 +760,62 @@ static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
   return Z2D_OK;
 }
 
-static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
-                          const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o) {
-  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
-  // painter.fill checks, in order (src/painter.zig:73-82)
-  int32_t rc = check_pattern(pattern);
-  if (rc) return rc;
+// The host geometry of one fill: painter.fill's checks after the pattern's
+// (src/painter.zig:73-82, in order), then the flattened nodes.  `empty`: zero
+// nodes, nothing to draw.
+struct FlatDraw {
+  std::vector<FNode> nodes;
+  double box[4] = {0.0, 0.0, 0.0, 0.0};
+  bool balanced = true, serial = false, empty = false;
+};
+
+static int32_t flatten_draw(const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o,
+                            FlatDraw &f) {
+  f.nodes.clear();
+  f.empty = n == 0;
   if (n == 0) return Z2D_OK;
   if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
   if (!is_closed_node_set(nodes, n)) return Z2D_ERR_PATH_NOT_CLOSED;
   if (o->op >= Z2D_OP_COUNT || o->fill_rule > 1 || o->anti_aliasing_mode > 3)
     return Z2D_ERR_INVALID_ARGUMENT;
+  const double tol = std::max(o->tolerance, 0.001);
+  const int32_t rc = flatten_nodes(nodes, n, tol, f.nodes, f.balanced, f.box);
+  if (rc) return rc;
+  f.serial = false;
+  for (size_t i = 1; i < f.nodes.size(); i++)
+    if (f.nodes[i].tag == Z2D_NODE_LINE_TO && f.nodes[i - 1].tag == Z2D_NODE_CLOSE_PATH) {
+      f.serial = true;
+      break;
+    }
+  return Z2D_OK;
+}
+
+// Appends a flattened fill to the batch.
+static int32_t batch_add_flat(z2d_gpu_batch *b, z2d_gpu_surface *s,
+                              const z2d_pattern_desc *pattern, const FlatDraw &f,
+                              const z2d_fill_opts *o) {
+  if (f.empty) return Z2D_OK;
   HostDraw hd;
   std::memset(&hd, 0, sizeof(hd));
   const size_t before = b->nodes.size();
-  bool balanced = true;
-  double box[4];
-  double tol = std::max(o->tolerance, 0.001);
-  rc = flatten_nodes(nodes, n, tol, b->nodes, balanced, box);
-  if (rc) {
-    b->nodes.resize(before);
-    return rc;
-  }
+  b->nodes.insert(b->nodes.end(), f.nodes.begin(), f.nodes.end());
   hd.d.node_off = (uint32_t)before;
-  hd.d.node_cnt = (uint32_t)(b->nodes.size() - before);
-  for (size_t i = before + 1; i < b->nodes.size(); i++)
-    if (b->nodes[i].tag == Z2D_NODE_LINE_TO && b->nodes[i - 1].tag == Z2D_NODE_CLOSE_PATH) {
-      hd.d.flags |= DRAW_SERIAL_PLOT;
-      break;
-    }
-  if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
+  hd.d.node_cnt = (uint32_t)f.nodes.size();
+  if (f.serial) hd.d.flags |= DRAW_SERIAL_PLOT;
+  if (!f.balanced) hd.d.flags |= DRAW_UNBALANCED;
   return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
-                          o->precision, hd, box);
+                          o->precision, hd, f.box);
+}
+
+static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
+                          const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o) {
+  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
+  if (rc) return rc;
+  FlatDraw f;
+  rc = flatten_draw(nodes, n, o, f);
+  if (rc) return rc;
+  return batch_add_flat(b, s, pattern, f, o);
 }
 
 static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
@@ -853,23 +877,44 @@ static int32_t stroke_polygon(const z2d_path_node *nodes, uint64_t n, const z2d_
 // painter.stroke (src/painter.zig:214-344): host tessellation by the
 // stroke plotter, then the polygon is filled non-zero.  Hairline strokes
 // (a separate rasterizer in the reference) are not part of this backend.
-static int32_t batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
-                            const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o) {
-  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
-  int32_t rc = check_pattern(pattern);
-  if (rc) return rc;
+// The host geometry of one stroke: painter.stroke's checks after the
+// pattern's (src/painter.zig:221-236, in order), then the stroke polygon and
+// the effective AA mode.  `empty`: zero nodes.
+struct StrokeDraw {
+  StrokePolygon poly;
+  uint32_t aa = 0;
+  bool empty = false;
+};
+
+static int32_t stroke_draw(const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o,
+                           uint32_t format, StrokeDraw &d) {
   if (!ctm_invertible(o->transformation)) return Z2D_ERR_INVALID_MATRIX;
+  d.empty = n == 0;
   if (n == 0) return Z2D_OK;
   if (!nodes || o->op >= Z2D_OP_COUNT || o->anti_aliasing_mode > 3 || o->line_cap_mode > 2 ||
       o->line_join_mode > 2 || (o->n_dashes && !o->dashes))
     return Z2D_ERR_INVALID_ARGUMENT;
   if (o->hairline) return Z2D_ERR_UNSUPPORTED;
-  uint32_t aa;
-  StrokePolygon poly;
-  rc = stroke_polygon(nodes, n, o, s->format, aa, poly);
+  return stroke_polygon(nodes, n, o, format, d.aa, d.poly);
+}
+
+static int32_t batch_add_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s,
+                                const z2d_pattern_desc *pattern, const StrokeDraw &d,
+                                const z2d_stroke_opts *o) {
+  if (d.empty) return Z2D_OK;
+  return batch_fill_polygon(b, s, pattern, d.poly.edges.data(), d.poly.edges.size(),
+                            d.poly.ext, d.aa, Z2D_FILL_NON_ZERO, o->op, o->precision, true);
+}
+
+static int32_t batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
+                            const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o) {
+  if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
+  int32_t rc = check_pattern(pattern);
   if (rc) return rc;
-  return batch_fill_polygon(b, s, pattern, poly.edges.data(), poly.edges.size(), poly.ext, aa,
-                            Z2D_FILL_NON_ZERO, o->op, o->precision, true);
+  StrokeDraw d;
+  rc = stroke_draw(nodes, n, o, s->format, d);
+  if (rc) return rc;
+  return batch_add_stroke(b, s, pattern, d, o);
 }
 
 static void batch_drop_graph(z2d_gpu_batch *b) {
@@ -2060,6 +2105,10 @@ struct SrvFill {
   z2d_fill_opts opts;
   z2d_stroke_opts sopts;
   std::vector<double> dashes;
+  // host geometry, computed by the server's preparation threads
+  int32_t geo_rc = Z2D_OK;
+  FlatDraw flat;
+  StrokeDraw stroke;
 };
 
 struct SrvRequest {
@@ -2107,6 +2156,33 @@ struct z2d_gpu_server {
   SrvSlot slots[2];
   std::thread worker;
   std::string err;
+  // preparation threads: the host geometry of submitted requests (flattened
+  // fills, stroke polygons) is built in parallel before admission
+  std::vector<std::thread> preppers;
+  std::deque<SrvRequest *> prep_q;
+  std::condition_variable cv_prep;
+  bool stop_prep = false;
+
+  void prep_loop() {
+    for (;;) {
+      SrvRequest *r = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_prep.wait(lk, [&] { return stop_prep || !prep_q.empty(); });
+        if (prep_q.empty()) return;  // stopping and drained
+        r = prep_q.front();
+        prep_q.pop_front();
+      }
+      for (auto &f : r->fills)
+        f->geo_rc = f->kind == Z2D_DRAW_STROKE
+                        ? stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
+                                      f->stroke)
+                        : flatten_draw(f->nodes.data(), f->nodes.size(), &f->opts, f->flat);
+      std::lock_guard<std::mutex> lk(mu);
+      queue.push_back(r);
+      cv_work.notify_all();
+    }
+  }
 
   void finalize(SrvSlot &sl) {
     uint64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2176,11 +2252,11 @@ struct z2d_gpu_server {
       // a clipped surface takes part in the render even with no fills
       if (r->mask_dev) sl.batch.slot_of(&r->surf);
       for (auto &f : r->fills) {
-        int32_t rc = f->kind == Z2D_DRAW_STROKE
-                         ? batch_stroke(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
-                                        f->nodes.size(), &f->sopts)
-                         : batch_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
-                                      f->nodes.size(), &f->opts);
+        int32_t rc = f->geo_rc;
+        if (rc == Z2D_OK)
+          rc = f->kind == Z2D_DRAW_STROKE
+                   ? batch_add_stroke(&sl.batch, &r->surf, &f->pat, f->stroke, &f->sopts)
+                   : batch_add_flat(&sl.batch, &r->surf, &f->pat, f->flat, &f->opts);
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
       }
     }
@@ -2299,6 +2375,9 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
     return fail(ctx, e, "server create");
   }
   s->worker = std::thread([s] { s->run(); });
+  const unsigned hc = std::thread::hardware_concurrency();
+  const unsigned np = std::max(2u, std::min(8u, hc ? hc / 2 : 2u));
+  for (unsigned i = 0; i < np; i++) s->preppers.emplace_back([s] { s->prep_loop(); });
   *out = s;
   return Z2D_OK;
 }
@@ -2370,9 +2449,9 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   *id = r->id;
   SrvRequest *raw = r.release();
   s->reqs[raw->id] = raw;
-  s->queue.push_back(raw);
+  s->prep_q.push_back(raw);
   s->n_submitted++;
-  s->cv_work.notify_all();
+  s->cv_prep.notify_one();
   return Z2D_OK;
 }
 
@@ -2433,12 +2512,19 @@ int32_t z2d_gpu_server_stats(z2d_gpu_server *s, uint64_t out[8]) {
   out[4] = s->pixel_draws;
   out[5] = s->used;
   out[6] = s->peak;
-  out[7] = s->queue.size();
+  out[7] = s->queue.size() + s->prep_q.size();
   return Z2D_OK;
 }
 
 void z2d_gpu_server_destroy(z2d_gpu_server *s) {
   if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_prep = true;
+    s->cv_prep.notify_all();
+  }
+  for (std::thread &t : s->preppers)
+    if (t.joinable()) t.join();
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
