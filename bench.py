@@ -208,9 +208,10 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     b.prepare()
     pix, _ = count_work(b)
     res = {}
-    band_h, tune_ms = b.autotune(0, 2)
-    res["autotuned_band_rows"] = band_h
-    res["autotune_pipeline_ms"] = {str(k): round(v, 3) for k, v in tune_ms.items()}
+    band_h, longest, tune_ms = b.autotune(0, 2)
+    res["autotuned"] = {"band_rows": band_h, "queue": "longest_first" if longest else "natural"}
+    res["autotune_pipeline_ms"] = {f"{k[0]}{'_lpt' if k[1] else ''}": round(v, 3)
+                                   for k, v in tune_ms.items()}
     for mode, name, graph in ((0, "batched", False), (0, "batched_graph", True),
                               (1, "per_surface", False)):
         b.use_graph(graph)

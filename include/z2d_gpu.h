@@ -132,13 +132,19 @@ int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms);
  * from 4 to 32, 16 by default.  The output does not depend on it.  No
  * counterpart in the reference. */
 int32_t z2d_gpu_batch_set_band_height(z2d_gpu_batch *b, uint32_t rows);
-/* Autotuner: renders the batch with band heights 8, 16 and 32 (one warm-up
- * and `reps` timed pipelines each, launch mode `mode`), keeps the fastest
- * (*band_h, per-candidate mean pipeline ms in ms_out, either may be null) and
- * prepares the batch for it.  The batch's surfaces are restored after every
- * candidate, so they end as they started.  No counterpart in the reference. */
+/* Queue order of the single-launch render: 0 natural (tile by tile), 1
+ * longest strip lists first (a counting sort on the device after binning).
+ * The output does not depend on it.  No counterpart in the reference. */
+int32_t z2d_gpu_batch_set_queue_order(z2d_gpu_batch *b, uint32_t longest_first);
+/* Autotuner: renders the batch with band heights 8, 16 and 32, each in the
+ * natural and the longest-first queue order (one warm-up and `reps` timed
+ * pipelines per candidate, launch mode `mode`), keeps the fastest (*band_h,
+ * *longest_first, the candidates' mean pipeline ms in ms_out: natural 8, 16,
+ * 32 then longest-first 8, 16, 32; any may be null) and prepares the batch
+ * for it.  The batch's surfaces are restored after every candidate, so they
+ * end as they started.  No counterpart in the reference. */
 int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
-                               uint32_t *band_h, double ms_out[3]);
+                               uint32_t *band_h, uint32_t *longest_first, double ms_out[6]);
 /* enable != 0: each render of this batch replays one hipGraph holding the
  * whole pipeline (captured at the first render after the batch is prepared,
  * and again when the launch mode or the counter setting changes).  No

@@ -1,6 +1,7 @@
-"""Bin height and the autotuner: the output of a batch does not depend on the
-band height (4 to 32 rows), and the autotuner leaves the batch's surfaces as
-they were, picks one of its candidates, and renders the same result after."""
+"""Bin height, queue order and the autotuner: the output of a batch does not
+depend on the band height (4 to 32 rows) or the strip queue order, and the
+autotuner leaves the batch's surfaces as they were, picks one of its
+candidates, and renders the same result after."""
 import numpy as np
 import pytest
 
@@ -20,26 +21,27 @@ def _contents(surfs):
 
 
 @pytest.mark.parametrize("mode", [0, 1], ids=["persistent", "per_surface"])
-def test_band_height_does_not_change_output(mode):
+def test_band_height_and_order_do_not_change_output(mode):
     from z2d_amd import gpu
     ctx = gpu_backend().ctx
     results = {}
-    for rows in (16, 4, 8, 12, 32):
+    for rows, order in ((16, 0), (4, 0), (8, 1), (12, 0), (32, 1), (16, 1)):
         b, spec, keep = _build(gpu, ctx, 0x7B + mode)
         surfs = _surfaces(gpu, ctx, b, spec)
         try:
             b.set_band_height(rows)
+            b.set_queue_order(order)
             b.render(mode)
             b.wait()
-            results[rows] = _contents(surfs)
+            results[(rows, order)] = _contents(surfs)
         finally:
             for s in surfs:
                 s.close()
             b.close()
-    for rows, got in results.items():
-        for i, (g, w) in enumerate(zip(got, results[16])):
+    for key, got in results.items():
+        for i, (g, w) in enumerate(zip(got, results[(16, 0)])):
             msg = diff_report(g, w)
-            assert not msg, f"band height {rows} surface {i}: {msg}"
+            assert not msg, f"band height / order {key} surface {i}: {msg}"
 
 
 def test_autotune_keeps_surfaces_and_output():
@@ -51,8 +53,8 @@ def test_autotune_keeps_surfaces_and_output():
     ref = _surfaces(gpu, ctx, ref_b, spec)
     try:
         before = _contents(surfs)
-        rows, ms = b.autotune(0, 1)
-        assert rows in (8, 16, 32) and all(v > 0 for v in ms.values())
+        rows, longest, ms = b.autotune(0, 1)
+        assert rows in (8, 16, 32) and longest in (0, 1) and all(v > 0 for v in ms.values())
         for i, (g, w) in enumerate(zip(_contents(surfs), before)):
             assert not diff_report(g, w), f"surface {i} changed by the autotuner"
         b.render(0)

@@ -711,6 +711,7 @@ struct z2d_gpu_batch {
   float last_pipeline_ms = 0.0f, last_render_ms = 0.0f;
   uint32_t count_stats = 0;
   uint32_t band_h = TILE_H;  // rows per band (bin height), a multiple of ROWS_PER_THREAD
+  uint32_t longest_first = 0;  // strip queue order of the single-launch render
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool render_timed = false;  // the last render recorded ev[1] before the tile render
   // hipGraph replay of the render pipeline (z2d_gpu_batch_use_graph)
@@ -1044,6 +1045,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_ecnt = L.take((2 * max_entries + 1) * sizeof(uint32_t));  // + cursors
   const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
   const size_t o_stripcnt = L.take(((size_t)n_tiles * spt + 1) * sizeof(uint32_t));
+  const size_t o_order = L.take(b->longest_first ? (size_t)n_tiles * spt * sizeof(uint32_t) : 0);
+  const size_t o_ohist = L.take(b->longest_first ? (ORDER_BUCKETS + 1) * sizeof(uint32_t) : 0);
   const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
@@ -1141,6 +1144,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.max_entry_edges = (uint32_t)max_entry_edges;
   J.band_h = band_h;
   J.strips_per_tile = spt;
+  J.strip_order = b->longest_first ? reinterpret_cast<uint32_t *>(D + o_order) : nullptr;
+  J.order_hist = b->longest_first ? reinterpret_cast<uint32_t *>(D + o_ohist) : nullptr;
   J.strip_cnt = reinterpret_cast<uint32_t *>(D + o_stripcnt);
   J.strip_entries = reinterpret_cast<uint32_t *>(D + o_sentries);
   J.max_strip_entries = (uint32_t)max_strip_entries;
@@ -1731,6 +1736,15 @@ int32_t z2d_gpu_composite_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
   return Z2D_OK;
 }
 
+int32_t z2d_gpu_batch_set_queue_order(z2d_gpu_batch *b, uint32_t longest_first) {
+  if (!b || longest_first > 1) return Z2D_ERR_INVALID_ARGUMENT;
+  if (longest_first != b->longest_first) {
+    b->longest_first = longest_first;
+    b->prepared = false;
+  }
+  return Z2D_OK;
+}
+
 int32_t z2d_gpu_batch_set_band_height(z2d_gpu_batch *b, uint32_t rows) {
   if (!b || rows < (uint32_t)ROWS_PER_THREAD || rows > (uint32_t)MAX_BAND_H ||
       rows % ROWS_PER_THREAD)
@@ -1742,12 +1756,13 @@ int32_t z2d_gpu_batch_set_band_height(z2d_gpu_batch *b, uint32_t rows) {
   return Z2D_OK;
 }
 
-// The autotuner: renders the batch once per candidate band height (a warm-up
-// and `reps` timed pipelines each), keeps the fastest and prepares the batch
-// for it.  The surfaces are saved before and restored after every candidate,
-// so the batch's targets end as they started.
+// The autotuner: renders the batch once per candidate (band height 8, 16, 32
+// with the natural queue order, then each with the longest-first order; a
+// warm-up and `reps` timed pipelines each), keeps the fastest and prepares
+// the batch for it.  The surfaces are saved before and restored after every
+// candidate, so the batch's targets end as they started.
 int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
-                               uint32_t *band_h, double ms_out[3]) {
+                               uint32_t *band_h, uint32_t *longest_first, double ms_out[6]) {
   if (!b || reps == 0 || mode > 1) return Z2D_ERR_INVALID_ARGUMENT;
   z2d_gpu_ctx *ctx = b->ctx;
   HIP_OK(ctx, hipSetDevice(ctx->device));
@@ -1771,10 +1786,11 @@ int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
   };
   e = copy_all(true);
   double best = 1e30;
-  uint32_t best_h = b->band_h;
+  uint32_t best_h = b->band_h, best_o = b->longest_first;
   int32_t rc = Z2D_OK;
-  for (int c = 0; c < 3 && e == hipSuccess && rc == Z2D_OK; c++) {
-    b->band_h = cand[c];
+  for (int c = 0; c < 6 && e == hipSuccess && rc == Z2D_OK; c++) {
+    b->band_h = cand[c % 3];
+    b->longest_first = c >= 3 ? 1u : 0u;
     b->prepared = false;
     double sum = 0.0;
     for (uint32_t r = 0; r <= reps && rc == Z2D_OK; r++) {
@@ -1787,7 +1803,8 @@ int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
     if (ms_out) ms_out[c] = ms;
     if (rc == Z2D_OK && ms < best) {
       best = ms;
-      best_h = cand[c];
+      best_h = cand[c % 3];
+      best_o = b->longest_first;
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1795,8 +1812,10 @@ int32_t z2d_gpu_batch_autotune(z2d_gpu_batch *b, uint32_t mode, uint32_t reps,
   if (e != hipSuccess) return fail(ctx, e, "autotune");
   if (rc) return rc;
   b->band_h = best_h;
+  b->longest_first = best_o;
   b->prepared = false;
   if (band_h) *band_h = best_h;
+  if (longest_first) *longest_first = best_o;
   return batch_prepare(b);
 }
 

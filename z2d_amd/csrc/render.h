@@ -29,6 +29,9 @@ constexpr int ACC_WORDS = TILE_W * 2;  // 4 sample columns per pixel, two int16 
 // holds more edges than this cannot overflow them and takes the exact
 // per-thread path instead.
 constexpr uint32_t PACKED_EDGE_LIMIT = 32767;
+// Length classes of the longest-first strip order (lists longer than this
+// share the first class).
+constexpr uint32_t ORDER_BUCKETS = 1024;
 // Edge slots of one draw per k_edge_bins work item.
 constexpr uint32_t EDGE_GROUP = 512;
 
@@ -138,6 +141,11 @@ struct JobView {
   uint32_t max_entry_edges;
   uint32_t band_h;           // rows per band
   uint32_t strips_per_tile;  // band_h / ROWS_PER_THREAD
+  // longest-first queue order of the single-launch render (null: natural
+  // order): strips by decreasing list length, from a counting sort over
+  // ORDER_BUCKETS lengths
+  uint32_t *strip_order;     // n_tiles * strips_per_tile
+  uint32_t *order_hist;      // ORDER_BUCKETS + 1
   // per strip (tile * strips_per_tile + strip): the band entries whose
   // composite region meets the strip, in draw order
   uint32_t *strip_cnt;       // n_tiles * strips_per_tile + 1 (scanned in place)

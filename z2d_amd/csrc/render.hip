@@ -721,6 +721,27 @@ __global__ void k_strip_pass(JobView J) {
 }
 
 // ---------------------------------------------------------------------------
+// Longest-first strip order: a counting sort of the strips by list length
+// (descending), so that the render's queue hands out the long strips first
+// and the short ones fill the tail.  Order within a length class is free.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t order_bucket(const JobView &J, uint32_t i) {
+  const uint32_t n = J.strip_cnt[i + 1] - J.strip_cnt[i];
+  return ORDER_BUCKETS - 1u - min(n, ORDER_BUCKETS - 1u);
+}
+__global__ void k_order_hist(JobView J, uint32_t n_strips) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_strips) atomicAdd(&J.order_hist[order_bucket(J, i)], 1u);
+}
+// after the scan of order_hist: each strip takes the next slot of its class
+__global__ void k_order_scatter(JobView J, uint32_t n_strips) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_strips) return;
+  const uint32_t pos = atomicAdd(&J.order_hist[order_bucket(J, i)], 1u);
+  if (pos < n_strips) J.strip_order[pos] = i;
+}
+
+// ---------------------------------------------------------------------------
 // k_edge_bins: per (draw, band) entry, copies of the draw's edges whose
 // y-range meets the band's sample rows (top < last mid && bottom >= first mid,
 // the active-edge test of src/internal/raster/multisample.zig and direct.zig
@@ -1076,9 +1097,10 @@ __device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, in
 // Persistent strip workers: each wave takes (tile, strip) items from the
 // queue and renders every draw of the tile's band over its ROWS_PER_THREAD x
 // TILE_W pixels, which stay in registers from the first draw to the last.
+// ordered: take the strips in J.strip_order (a launch over all tiles).
 __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_t tile_base,
                                                            uint32_t tile_count,
-                                                           uint32_t *cursor) {
+                                                           uint32_t *cursor, uint32_t ordered) {
   __shared__ StripShared shw[RENDER_THREADS / 64];
   const uint32_t lane = threadIdx.x & 63;
   StripShared &sh = shw[threadIdx.x >> 6];
@@ -1110,14 +1132,14 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       }
       break;
     }
-    const TileRef T = load_uniform(&J.tiles[tile_base + item / spt]);
-    const uint32_t strip = item % spt;
+    const uint32_t sidx = ordered ? load_uniform(&J.strip_order[item]) : tile_base * spt + item;
+    const TileRef T = load_uniform(&J.tiles[sidx / spt]);
+    const uint32_t strip = sidx % spt;
     const DevSurface S = load_uniform(&J.surfaces[T.surface]);
     const int32_t W = S.w, H = S.h;
     const int32_t tx0 = (int32_t)T.tx * TILE_W;
     const int32_t ty0 = (int32_t)T.ty * (int32_t)J.band_h + (int32_t)strip * ROWS_PER_THREAD;
     if (ty0 >= H) continue;
-    const uint32_t sidx = tile_base * spt + item;
     const uint32_t e0 = min(load_uniform(&J.strip_cnt[sidx]), J.max_strip_entries);
     const uint32_t e1 = min(load_uniform(&J.strip_cnt[sidx + 1]), J.max_strip_entries);
     if (e0 == e1 && !S.clip) continue;
@@ -1732,6 +1754,15 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
     scan_u32(J.strip_cnt, J.n_tiles * J.strips_per_tile, J.scan_tmp, st);
     hipLaunchKernelGGL(k_strip_pass<true>, tgrid, dim3(256), 0, st, J);
+    if (J.strip_order) {
+      const uint32_t ns = J.n_tiles * J.strips_per_tile;
+      const dim3 og((ns + 255) / 256);
+      hipLaunchKernelGGL(k_fill_u32, dim3((ORDER_BUCKETS + 1 + 255) / 256), dim3(256), 0, st,
+                         J.order_hist, (uint64_t)ORDER_BUCKETS + 1, 0u);
+      hipLaunchKernelGGL(k_order_hist, og, dim3(256), 0, st, J, ns);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, J.order_hist, ORDER_BUCKETS);
+      hipLaunchKernelGGL(k_order_scatter, og, dim3(256), 0, st, J, ns);
+    }
   }
   return hipGetLastError();
 }
@@ -1739,8 +1770,9 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
 hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
                              uint32_t *cursor, uint32_t n_blocks, hipStream_t st) {
   if (tile_count == 0) return hipSuccess;
+  const uint32_t ordered = (J.strip_order && tile_base == 0 && tile_count == J.n_tiles) ? 1u : 0u;
   hipLaunchKernelGGL(k_render, dim3(n_blocks), dim3(RENDER_THREADS), 0, st, J, tile_base,
-                     tile_count, cursor);
+                     tile_count, cursor, ordered);
   return hipGetLastError();
 }
 

@@ -47,8 +47,9 @@ _SIGS = {
     "z2d_gpu_batch_timing": ([C.c_void_p, P(C.c_double), P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_use_graph": ([C.c_void_p, C.c_uint32], C.c_int32),
     "z2d_gpu_batch_set_band_height": ([C.c_void_p, C.c_uint32], C.c_int32),
+    "z2d_gpu_batch_set_queue_order": ([C.c_void_p, C.c_uint32], C.c_int32),
     "z2d_gpu_batch_autotune": ([C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_uint32),
-                                P(C.c_double)], C.c_int32),
+                                P(C.c_uint32), P(C.c_double)], C.c_int32),
     "z2d_gpu_composite_timing": ([C.c_void_p, P(C.c_double)], C.c_int32),
     "z2d_gpu_batch_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
@@ -193,14 +194,20 @@ class Batch:
     def set_band_height(self, rows):
         self.ctx.check(lib().z2d_gpu_batch_set_band_height(self.h, rows), "set_band_height")
 
+    def set_queue_order(self, longest_first):
+        self.ctx.check(lib().z2d_gpu_batch_set_queue_order(self.h, 1 if longest_first else 0),
+                       "set_queue_order")
+
     def autotune(self, mode=0, reps=2):
-        """Pick the band height (8, 16, 32) with the fastest pipeline; returns
-        (band_h, {candidate: mean ms})."""
-        h = C.c_uint32()
-        ms = (C.c_double * 3)()
-        self.ctx.check(lib().z2d_gpu_batch_autotune(self.h, mode, reps, C.byref(h), ms),
-                       "autotune")
-        return h.value, dict(zip((8, 16, 32), list(ms)))
+        """Pick the band height (8, 16, 32) and queue order (natural,
+        longest-first) with the fastest pipeline; returns (band_h,
+        longest_first, {(band_h, longest_first): mean ms})."""
+        h, o = C.c_uint32(), C.c_uint32()
+        ms = (C.c_double * 6)()
+        self.ctx.check(lib().z2d_gpu_batch_autotune(self.h, mode, reps, C.byref(h), C.byref(o),
+                                                    ms), "autotune")
+        keys = [(8, 0), (16, 0), (32, 0), (8, 1), (16, 1), (32, 1)]
+        return h.value, o.value, dict(zip(keys, list(ms)))
 
     def use_graph(self, enable=True):
         """Replay the render pipeline as one captured hipGraph."""
