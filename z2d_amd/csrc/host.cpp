@@ -38,6 +38,8 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st);
 hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
                              uint32_t *cursor, uint32_t n_blocks, hipStream_t st);
 hipError_t z2d_launch_composite_quad(const z2d::QuadJob &job, uint64_t n_quads, hipStream_t st);
+hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs, uint32_t n_jobs,
+                              z2d::FNode *out, z2d::FlatResult *res, bool emit, hipStream_t st);
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -468,151 +470,6 @@ struct Pt {
 };
 inline bool peq(Pt a, Pt b) { return a.x == b.x && a.y == b.y; }
 
-struct Flattener {
-  std::vector<FNode> *out;
-  Pt buf[3];
-  int len = 0;
-  double tol_sq = 0.01;
-  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-
-  void track(Pt p) {
-    xmin = std::min(xmin, p.x);
-    xmax = std::max(xmax, p.x);
-    ymin = std::min(ymin, p.y);
-    ymax = std::max(ymax, p.y);
-  }
-  void emit(uint32_t tag, Pt p) {
-    FNode n;
-    n.x = p.x;
-    n.y = p.y;
-    n.tag = tag;
-    n.pad = 0;
-    out->push_back(n);
-  }
-  void add(Pt p) {
-    if (len < 3) {
-      buf[len++] = p;
-    } else {
-      buf[1] = buf[2];
-      buf[2] = p;
-    }
-  }
-  // line_to through the point buffer (dedupe against the last point).
-  void line_to(Pt p) {
-    emit(Z2D_NODE_LINE_TO, p);
-    track(p);
-    if (!peq(buf[len - 1], p)) add(p);
-  }
-  struct Knots {
-    Pt a, b, c, d;
-  };
-  static Pt half(Pt a, Pt b) { return Pt{a.x + ((b.x - a.x) / 2.0), a.y + ((b.y - a.y) / 2.0)}; }
-  static double err_sq(const Knots &k) {
-    double bx = k.b.x - k.a.x, by = k.b.y - k.a.y, cx = k.c.x - k.a.x, cy = k.c.y - k.a.y;
-    if (k.a.x != k.d.x || k.a.y != k.d.y) {
-      double dx = k.d.x - k.a.x, dy = k.d.y - k.a.y;
-      double dd = dx * dx + dy * dy;
-      double bd = bx * dx + by * dy;
-      if (bd >= dd) {
-        bx -= dx;
-        by -= dy;
-      } else {
-        bx -= bd / dd * dx;
-        by -= bd / dd * dy;
-      }
-      double cd = cx * dx + cy * dy;
-      if (cd >= dd) {
-        cx -= dx;
-        cy -= dy;
-      } else {
-        cx -= cd / dd * dx;
-        cy -= cd / dd * dy;
-      }
-    }
-    double be = bx * bx + by * by, ce = cx * cx + cy * cy;
-    return be > ce ? be : ce;
-  }
-  static Knots split(Knots &k) {
-    Pt ab = half(k.a, k.b), bc = half(k.b, k.c), cd = half(k.c, k.d);
-    Pt abbc = half(ab, bc), bccd = half(bc, cd), fin = half(abbc, bccd);
-    Knots r{fin, bccd, cd, k.d};
-    k.b = ab;
-    k.c = abbc;
-    k.d = fin;
-    return r;
-  }
-  void decompose_into(Knots &s1, Pt start, int depth) {
-    if (err_sq(s1) < tol_sq || depth > 64) {
-      if (!peq(s1.a, start)) line_to(s1.a);
-      return;
-    }
-    Knots s2 = split(s1);
-    decompose_into(s1, start, depth + 1);
-    decompose_into(s2, start, depth + 1);
-  }
-  void curve(Pt b, Pt c, Pt d) {
-    Pt a = buf[len - 1];
-    if (peq(a, b) && peq(c, d)) {
-      line_to(d);
-      return;
-    }
-    Knots s1{a, b, c, d};
-    decompose_into(s1, a, 0);
-    line_to(d);
-  }
-};
-
-// Flattens one node set.  Returns a z2d_status; balanced_out reports whether
-// every contour's last point equals its first point; box receives the
-// bounding box of the points and control points (xmin, xmax, ymin, ymax).
-int32_t flatten_nodes(const z2d_path_node *nodes, uint64_t n, double tolerance,
-                      std::vector<FNode> &out, bool &balanced_out, double box[4]) {
-  Flattener f;
-  f.out = &out;
-  f.tol_sq = tolerance * tolerance;
-  bool balanced = true;
-  for (uint64_t i = 0; i < n; i++) {
-    const z2d_path_node &nd = nodes[i];
-    Pt p{nd.v[0], nd.v[1]};
-    switch (nd.tag) {
-      case Z2D_NODE_MOVE_TO:
-        if (i == n - 1) {
-          f.emit(Z2D_NODE_MOVE_TO, p);
-          goto done;
-        }
-        if (f.len > 0 && !peq(f.buf[f.len - 1], f.buf[0])) balanced = false;
-        f.len = 0;
-        f.add(p);
-        f.emit(Z2D_NODE_MOVE_TO, p);
-        f.track(p);
-        break;
-      case Z2D_NODE_LINE_TO:
-        if (f.len == 0) return Z2D_ERR_INVALID_STATE;
-        f.line_to(p);
-        break;
-      case Z2D_NODE_CURVE_TO:
-        if (f.len == 0) return Z2D_ERR_INVALID_STATE;
-        f.track(Pt{nd.v[0], nd.v[1]});
-        f.track(Pt{nd.v[2], nd.v[3]});
-        f.curve(Pt{nd.v[0], nd.v[1]}, Pt{nd.v[2], nd.v[3]}, Pt{nd.v[4], nd.v[5]});
-        break;
-      case Z2D_NODE_CLOSE_PATH:
-        f.emit(Z2D_NODE_CLOSE_PATH, Pt{0, 0});
-        if (f.len >= 3 && !peq(f.buf[f.len - 1], f.buf[0])) f.add(f.buf[0]);
-        break;
-      default: return Z2D_ERR_INVALID_ARGUMENT;
-    }
-  }
-done:
-  if (f.len > 0 && !peq(f.buf[f.len - 1], f.buf[0])) balanced = false;
-  balanced_out = balanced;
-  box[0] = f.xmin;
-  box[1] = f.xmax;
-  box[2] = f.ymin;
-  box[3] = f.ymax;
-  return Z2D_OK;
-}
-
 // PathNode.isClosedNodeSet (src/internal/path_nodes.zig:23-36).
 bool is_closed_node_set(const z2d_path_node *nodes, uint64_t n) {
   if (n == 0) return false;
@@ -652,8 +509,11 @@ struct HostDraw {
   uint32_t surface_slot;  // index into batch surfaces
   uint32_t seq;           // insertion order
   double box[4];          // xmin, xmax, ymin, ymax of the input points (pixels)
+  bool bounded;           // bounded operator
   bool whole;             // unbounded operator or non-finite box: every tile
   double edge_scale;      // polygon draws: the scale of their edges' space
+  uint32_t raw_off, raw_cnt;  // fill draws: their path nodes in raw_nodes
+  double tol_sq;              // fill draws: squared flattening tolerance
   // set when the batch is prepared, for its band height
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
   uint32_t tcol_lo, tcol_hi;  // estimated tile-column range (inclusive)
@@ -669,32 +529,15 @@ static uint64_t band_span(double a, double b, uint32_t nbands, uint32_t band_h) 
   return (uint64_t)std::min(s, (double)nbands);
 }
 
-// Upper bound of the (edge, band) pairs of a flattened draw, following the
-// fill plotter's edge sources: a line joins the previous point to its own,
-// a close joins the last point to the subpath's first.
-static uint64_t node_edge_pairs(const FNode *n, size_t cnt, uint32_t nbands, uint32_t band_h) {
-  uint64_t total = 0;
-  double cur = 0.0, init = 0.0;
-  for (size_t i = 0; i < cnt; i++) {
-    if (n[i].tag == Z2D_NODE_MOVE_TO) {
-      cur = init = n[i].y;
-    } else if (n[i].tag == Z2D_NODE_LINE_TO) {
-      total += band_span(cur, n[i].y, nbands, band_h);
-      cur = n[i].y;
-    } else if (n[i].tag == Z2D_NODE_CLOSE_PATH) {
-      total += band_span(cur, init, nbands, band_h);
-      cur = init;
-    }
-  }
-  return total;
-}
-
 struct z2d_gpu_batch {
   z2d_gpu_ctx *ctx = nullptr;
   hipStream_t stream = nullptr;  // null: the context stream
   std::vector<z2d_gpu_surface *> surfaces;
-  std::vector<FNode> nodes;
+  std::vector<z2d_path_node> raw_nodes;  // fill draws' path nodes
   std::vector<z2d_edge> in_edges;
+  uint64_t n_nodes = 0;  // flattened nodes and edge slots of the prepared batch
+  DevBuf flat_buf;       // device flattening inputs and results
+  HostBuf flat_stage;
   std::vector<HostDraw> draws;
   PaintTable paints;
   bool prepared = false;
@@ -755,57 +598,58 @@ static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
   const bool finite = std::isfinite(box[0]) && std::isfinite(box[1]) &&
                       std::isfinite(box[2]) && std::isfinite(box[3]);
   for (int i = 0; i < 4; i++) hd.box[i] = box[i];
+  hd.bounded = bounded;
   hd.whole = !bounded || !finite;
   b->draws.push_back(hd);
   b->prepared = false;
   return Z2D_OK;
 }
 
-// The host geometry of one fill: painter.fill's checks after the pattern's
-// (src/painter.zig:73-82, in order), then the flattened nodes.  `empty`: zero
-// nodes, nothing to draw.
-struct FlatDraw {
-  std::vector<FNode> nodes;
-  double box[4] = {0.0, 0.0, 0.0, 0.0};
-  bool balanced = true, serial = false, empty = false;
-};
-
-static int32_t flatten_draw(const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o,
-                            FlatDraw &f) {
-  f.nodes.clear();
-  f.empty = n == 0;
+// painter.fill's checks after the pattern's (src/painter.zig:73-82, in order)
+// and the node walk's own errors, which depend on the tags alone: a line or
+// curve before the first move is InvalidState (fill_plotter.zig:52-71), an
+// unknown tag an invalid argument.  Flattening happens on the device.
+static int32_t fill_checks(const z2d_path_node *nodes, uint64_t n, const z2d_fill_opts *o) {
   if (n == 0) return Z2D_OK;
   if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
   if (!is_closed_node_set(nodes, n)) return Z2D_ERR_PATH_NOT_CLOSED;
   if (o->op >= Z2D_OP_COUNT || o->fill_rule > 1 || o->anti_aliasing_mode > 3)
     return Z2D_ERR_INVALID_ARGUMENT;
-  const double tol = std::max(o->tolerance, 0.001);
-  const int32_t rc = flatten_nodes(nodes, n, tol, f.nodes, f.balanced, f.box);
-  if (rc) return rc;
-  f.serial = false;
-  for (size_t i = 1; i < f.nodes.size(); i++)
-    if (f.nodes[i].tag == Z2D_NODE_LINE_TO && f.nodes[i - 1].tag == Z2D_NODE_CLOSE_PATH) {
-      f.serial = true;
-      break;
+  bool moved = false;
+  for (uint64_t i = 0; i < n; i++) {
+    switch (nodes[i].tag) {
+      case Z2D_NODE_MOVE_TO:
+        if (i == n - 1) return Z2D_OK;  // a final move ends the walk
+        moved = true;
+        break;
+      case Z2D_NODE_LINE_TO:
+      case Z2D_NODE_CURVE_TO:
+        if (!moved) return Z2D_ERR_INVALID_STATE;
+        break;
+      case Z2D_NODE_CLOSE_PATH: break;
+      default: return Z2D_ERR_INVALID_ARGUMENT;
     }
+  }
   return Z2D_OK;
 }
 
-// Appends a flattened fill to the batch.
-static int32_t batch_add_flat(z2d_gpu_batch *b, z2d_gpu_surface *s,
-                              const z2d_pattern_desc *pattern, const FlatDraw &f,
-                              const z2d_fill_opts *o) {
-  if (f.empty) return Z2D_OK;
+// Appends a checked fill: its path nodes go to the device flattener when the
+// batch is prepared.
+static int32_t batch_add_fill(z2d_gpu_batch *b, z2d_gpu_surface *s,
+                              const z2d_pattern_desc *pattern, const z2d_path_node *nodes,
+                              uint64_t n, const z2d_fill_opts *o) {
+  if (n == 0) return Z2D_OK;
+  if (b->raw_nodes.size() + n > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
   HostDraw hd;
   std::memset(&hd, 0, sizeof(hd));
-  const size_t before = b->nodes.size();
-  b->nodes.insert(b->nodes.end(), f.nodes.begin(), f.nodes.end());
-  hd.d.node_off = (uint32_t)before;
-  hd.d.node_cnt = (uint32_t)f.nodes.size();
-  if (f.serial) hd.d.flags |= DRAW_SERIAL_PLOT;
-  if (!f.balanced) hd.d.flags |= DRAW_UNBALANCED;
+  hd.raw_off = (uint32_t)b->raw_nodes.size();
+  hd.raw_cnt = (uint32_t)n;
+  const double tol = std::max(o->tolerance, 0.001);
+  hd.tol_sq = tol * tol;
+  b->raw_nodes.insert(b->raw_nodes.end(), nodes, nodes + n);
+  const double box[4] = {NAN, NAN, NAN, NAN};  // from the device flattener
   return batch_add_common(b, s, pattern, o->anti_aliasing_mode, o->fill_rule, o->op,
-                          o->precision, hd, f.box);
+                          o->precision, hd, box);
 }
 
 static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
@@ -813,10 +657,9 @@ static int32_t batch_fill(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_patter
   if (!b || !s || !o) return Z2D_ERR_INVALID_ARGUMENT;
   int32_t rc = check_pattern(pattern);
   if (rc) return rc;
-  FlatDraw f;
-  rc = flatten_draw(nodes, n, o, f);
+  rc = fill_checks(nodes, n, o);
   if (rc) return rc;
-  return batch_add_flat(b, s, pattern, f, o);
+  return batch_add_fill(b, s, pattern, nodes, n, o);
 }
 
 static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
@@ -830,10 +673,8 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   if (op >= Z2D_OP_COUNT || rule > 1 || aa > 3) return Z2D_ERR_INVALID_ARGUMENT;
   HostDraw hd;
   std::memset(&hd, 0, sizeof(hd));
-  // Edge slots are reserved in the node range (one slot per edge).
-  const size_t before = b->nodes.size();
-  b->nodes.resize(before + n_edges);
-  hd.d.node_off = (uint32_t)before;
+  // one edge slot per edge (offsets assigned when the batch is prepared)
+  if (n_edges > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
   hd.d.node_cnt = (uint32_t)n_edges;
   hd.d.input_edges = 1;
   hd.d.in_edge_off = (uint32_t)b->in_edges.size();
@@ -937,6 +778,67 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   b->surf_tile_off.assign(ns + 1, 0);
   const uint32_t band_h = b->band_h;
   const uint32_t spt = band_h / ROWS_PER_THREAD;  // strips per tile
+  const hipStream_t st = bstream(b);
+
+  // ---- the fill draws' flattening, count pass: node counts, balance and
+  // line-after-close flags, boxes and (edge, band) bounds come back here
+  std::vector<FlatJob> jobs;
+  std::vector<uint32_t> job_draw;
+  for (uint32_t i = 0; i < (uint32_t)b->draws.size(); i++) {
+    const HostDraw &h = b->draws[i];
+    if (h.d.input_edges) continue;
+    const z2d_gpu_surface *s = b->surfaces[h.surface_slot];
+    FlatJob fj;
+    std::memset(&fj, 0, sizeof(fj));
+    fj.raw_off = h.raw_off;
+    fj.raw_cnt = h.raw_cnt;
+    fj.nbands = (uint32_t)((s->h + band_h - 1) / band_h);
+    fj.band_h = band_h;
+    fj.tol_sq = h.tol_sq;
+    jobs.push_back(fj);
+    job_draw.push_back(i);
+  }
+  size_t fo_jobs = 0, fo_res = 0;
+  if (!jobs.empty()) {
+    Layout FL;
+    FL.take(b->raw_nodes.size() * sizeof(z2d_path_node));
+    fo_jobs = FL.take(jobs.size() * sizeof(FlatJob));
+    fo_res = FL.take(jobs.size() * sizeof(FlatResult));
+    hipError_t e = b->flat_buf.ensure(FL.off);
+    if (e == hipSuccess) e = b->flat_stage.ensure(FL.off);
+    if (e != hipSuccess) return fail(ctx, e, "flatten buffers");
+    uint8_t *FH = b->flat_stage.p, *FD = b->flat_buf.p;
+    std::memcpy(FH, b->raw_nodes.data(), b->raw_nodes.size() * sizeof(z2d_path_node));
+    std::memcpy(FH + fo_jobs, jobs.data(), jobs.size() * sizeof(FlatJob));
+    HIP_OK(ctx, hipMemcpyAsync(FD, FH, fo_res, hipMemcpyHostToDevice, st));
+    HIP_OK(ctx, z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(FD),
+                                   reinterpret_cast<const FlatJob *>(FD + fo_jobs),
+                                   (uint32_t)jobs.size(), nullptr,
+                                   reinterpret_cast<FlatResult *>(FD + fo_res), false, st));
+    HIP_OK(ctx, hipMemcpyAsync(FH + fo_res, FD + fo_res, jobs.size() * sizeof(FlatResult),
+                               hipMemcpyDeviceToHost, st));
+    HIP_OK(ctx, hipStreamSynchronize(st));
+    const FlatResult *res = reinterpret_cast<const FlatResult *>(FH + fo_res);
+    for (size_t k = 0; k < jobs.size(); k++) {
+      HostDraw &h = b->draws[job_draw[k]];
+      const FlatResult &r = res[k];
+      if (r.flags & FLAT_RUNAWAY) return Z2D_ERR_INVALID_ARGUMENT;  // non-finite curve
+      h.d.node_cnt = r.n_nodes;
+      h.edge_pairs = r.edge_pairs;
+      if (r.flags & FLAT_UNBALANCED) h.d.flags |= DRAW_UNBALANCED | DRAW_ROWPROG;
+      if (r.flags & FLAT_SERIAL) h.d.flags |= DRAW_SERIAL_PLOT;
+      for (int c = 0; c < 4; c++) h.box[c] = r.box[c];
+      h.whole = !h.bounded || !(std::isfinite(r.box[0]) && std::isfinite(r.box[1]) &&
+                                std::isfinite(r.box[2]) && std::isfinite(r.box[3]));
+    }
+  }
+  // node ranges: flattened nodes of fills, one edge slot per edge of polygons
+  uint64_t total_nodes = 0;
+  for (HostDraw &h : b->draws) {
+    h.d.node_off = (uint32_t)std::min<uint64_t>(total_nodes, 0xFFFFFFFFull);
+    total_nodes += h.d.node_cnt;
+  }
+  b->n_nodes = total_nodes;
   for (const HostDraw &h : b->draws) surf_draw_off[h.surface_slot + 1]++;
   for (uint32_t i = 0; i < ns; i++) {
     surf_draw_off[i + 1] += surf_draw_off[i];
@@ -983,13 +885,11 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
         h.tcol_lo = (uint32_t)clamped(std::floor(h.box[0]) - 2.0, s->w) / TILE_W;
         h.tcol_hi = (uint32_t)clamped(std::ceil(h.box[1]) + 2.0, s->w) / TILE_W;
       }
-      if (h.d.input_edges) {
+      if (h.d.input_edges) {  // (fills: from the flattener's count pass)
         h.edge_pairs = 0;
         const z2d_edge *ed = b->in_edges.data() + h.d.in_edge_off;
         for (uint32_t k = 0; k < h.d.node_cnt; k++)
           h.edge_pairs += band_span(ed[k].y0 / h.edge_scale, ed[k].y1 / h.edge_scale, nb, bh);
-      } else {
-        h.edge_pairs = node_edge_pairs(b->nodes.data() + h.d.node_off, h.d.node_cnt, nb, bh);
       }
     }
     h.d.band_lo = h.band_lo;
@@ -1009,7 +909,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
       scratch += h.d.node_cnt + 1;
     }
   }
-  if (max_entries > 0xFFFFFFF0ull || b->nodes.size() > 0xFFFFFFF0ull ||
+  if (max_entries > 0xFFFFFFF0ull || total_nodes > 0xFFFFFFF0ull ||
       max_entry_edges > 0xFFFFFFF0ull || max_strip_entries > 0xFFFFFFF0ull ||
       (uint64_t)n_tiles * spt > 0xFFFFFFF0ull || egroups.size() > 0xFFFFFFF0ull ||
       2 * max_entries + 1 > 0xFFFFFFF0ull)
@@ -1018,7 +918,6 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
 
   // ---- arena layout
   Layout L;
-  const size_t o_nodes = L.take(b->nodes.size() * sizeof(FNode));
   const size_t o_inedges = L.take(b->in_edges.size() * sizeof(z2d_edge));
   const size_t o_draws = L.take(nd * sizeof(DevDraw));
   const size_t o_surfs = L.take(ns * sizeof(DevSurface));
@@ -1033,7 +932,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_rps = L.take(rowprog_soff.size() * sizeof(uint32_t));
   const size_t o_bn = L.take(sizeof(kBlueNoise));
   const size_t input_bytes = L.off;
-  const size_t o_edges = L.take(b->nodes.size() * sizeof(z2d_edge));
+  const size_t o_nodes = L.take(total_nodes * sizeof(FNode));  // written by k_flatten
+  const size_t o_edges = L.take(total_nodes * sizeof(z2d_edge));
   const size_t o_bandcnt = L.take(((size_t)n_bands + 1) * sizeof(uint32_t));
   const size_t o_slotcnt = L.take(((size_t)n_slots + 1) * sizeof(uint32_t));
   const uint64_t scan_n = std::max<uint64_t>(
@@ -1061,7 +961,6 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   uint8_t *H = b->stage.p;
   uint8_t *D = b->arena.p;
 
-  if (!b->nodes.empty()) std::memcpy(H + o_nodes, b->nodes.data(), b->nodes.size() * sizeof(FNode));
   if (!b->in_edges.empty())
     std::memcpy(H + o_inedges, b->in_edges.data(), b->in_edges.size() * sizeof(z2d_edge));
   DevDraw *hdraws = reinterpret_cast<DevDraw *>(H + o_draws);
@@ -1115,6 +1014,17 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
 
   JobView &J = b->view;
   J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
+  if (!jobs.empty()) {
+    // emit pass: the flattened nodes at their draws' offsets
+    FlatJob *hj = reinterpret_cast<FlatJob *>(b->flat_stage.p + fo_jobs);
+    for (size_t k = 0; k < jobs.size(); k++) hj[k].node_off = b->draws[job_draw[k]].d.node_off;
+    HIP_OK(ctx, hipMemcpyAsync(b->flat_buf.p + fo_jobs, hj, jobs.size() * sizeof(FlatJob),
+                               hipMemcpyHostToDevice, st));
+    HIP_OK(ctx, z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(b->flat_buf.p),
+                                   reinterpret_cast<const FlatJob *>(b->flat_buf.p + fo_jobs),
+                                   (uint32_t)jobs.size(), reinterpret_cast<FNode *>(D + o_nodes),
+                                   nullptr, true, st));
+  }
   J.in_edges = reinterpret_cast<const z2d_edge *>(D + o_inedges);
   J.edges = reinterpret_cast<z2d_edge *>(D + o_edges);
   J.draws = reinterpret_cast<DevDraw *>(D + o_draws);
@@ -1257,7 +1167,8 @@ static int32_t batch_wait(z2d_gpu_batch *b) {
 
 static void batch_clear(z2d_gpu_batch *b) {
   b->surfaces.clear();
-  b->nodes.clear();
+  b->raw_nodes.clear();
+  b->n_nodes = 0;
   b->in_edges.clear();
   b->draws.clear();
   b->paints.clear();
@@ -1837,7 +1748,7 @@ int32_t z2d_gpu_batch_stats(z2d_gpu_batch *b, uint64_t out[4]) {
   if (!b || !out) return Z2D_ERR_INVALID_ARGUMENT;
   out[0] = b->draws.size();
   out[1] = b->surfaces.size();
-  out[2] = b->nodes.size();
+  out[2] = b->n_nodes;
   out[3] = b->surf_tile_off.empty() ? 0 : b->surf_tile_off.back();
   return Z2D_OK;
 }
@@ -2124,9 +2035,9 @@ struct SrvFill {
   z2d_fill_opts opts;
   z2d_stroke_opts sopts;
   std::vector<double> dashes;
-  // host geometry, computed by the server's preparation threads
+  // host work, done by the server's preparation threads: a fill's checks
+  // (its flattening runs on the device), a stroke's polygon
   int32_t geo_rc = Z2D_OK;
-  FlatDraw flat;
   StrokeDraw stroke;
 };
 
@@ -2196,7 +2107,7 @@ struct z2d_gpu_server {
         f->geo_rc = f->kind == Z2D_DRAW_STROKE
                         ? stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
                                       f->stroke)
-                        : flatten_draw(f->nodes.data(), f->nodes.size(), &f->opts, f->flat);
+                        : fill_checks(f->nodes.data(), f->nodes.size(), &f->opts);
       std::lock_guard<std::mutex> lk(mu);
       queue.push_back(r);
       cv_work.notify_all();
@@ -2275,7 +2186,8 @@ struct z2d_gpu_server {
         if (rc == Z2D_OK)
           rc = f->kind == Z2D_DRAW_STROKE
                    ? batch_add_stroke(&sl.batch, &r->surf, &f->pat, f->stroke, &f->sopts)
-                   : batch_add_flat(&sl.batch, &r->surf, &f->pat, f->flat, &f->opts);
+                   : batch_add_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
+                                    f->nodes.size(), &f->opts);
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
       }
     }

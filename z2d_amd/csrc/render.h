@@ -42,6 +42,25 @@ struct FNode {
   uint32_t tag, pad;
 };
 
+// Bézier flattening on the device (k_flatten): one job per fill draw.
+struct FlatJob {
+  uint32_t raw_off, raw_cnt;  // the draw's path nodes
+  uint32_t node_off;          // where its flattened nodes go (emit pass)
+  uint32_t nbands, band_h;    // for the (edge, band) bound (count pass)
+  uint32_t pad;
+  double tol_sq;              // squared tolerance (>= 0.001^2)
+};
+enum : uint32_t {
+  FLAT_UNBALANCED = 1u << 0,  // a contour does not end where it starts
+  FLAT_SERIAL = 1u << 1,      // a line follows a close
+  FLAT_RUNAWAY = 1u << 2,     // a curve exceeded the subdivision visit limit
+};
+struct FlatResult {
+  uint32_t n_nodes, flags;
+  uint64_t edge_pairs;  // upper bound of (edge, band) pairs of the flattened draw
+  double box[4];        // xmin, xmax, ymin, ymax of the tracked points
+};
+
 struct DevSurface {
   uint8_t *data;
   int32_t w, h;

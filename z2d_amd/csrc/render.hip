@@ -483,6 +483,213 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
 }
 
 // ---------------------------------------------------------------------------
+// k_flatten: the fill plotter's node walk with Spline decomposition
+// (src/internal/tess/Spline.zig:37-165, fill_plotter.zig:52-118) on the
+// device, one thread per fill draw.  The count pass (EMIT=false) returns the
+// number of flattened nodes, the contour-balance and line-after-close flags,
+// the (edge, band) bound and the bounding box of the tracked points; the emit
+// pass writes the nodes.  The recursive subdivision is walked depth-first with
+// the knots of each visited piece recomputed from the curve by its path of
+// halvings, which reproduces the recursion's values exactly.
+// ---------------------------------------------------------------------------
+struct FPt {
+  double x, y;
+};
+__device__ __forceinline__ bool fpeq(FPt a, FPt b) { return a.x == b.x && a.y == b.y; }
+__device__ __forceinline__ FPt fhalf(FPt a, FPt b) {
+  return FPt{a.x + ((b.x - a.x) / 2.0), a.y + ((b.y - a.y) / 2.0)};
+}
+struct FKnots {
+  FPt a, b, c, d;
+};
+// Spline.Knots.errorSq (src/internal/tess/Spline.zig:83-123)
+__device__ __forceinline__ double knots_err_sq(const FKnots &k) {
+  double bx = k.b.x - k.a.x, by = k.b.y - k.a.y, cx = k.c.x - k.a.x, cy = k.c.y - k.a.y;
+  if (k.a.x != k.d.x || k.a.y != k.d.y) {
+    const double dx = k.d.x - k.a.x, dy = k.d.y - k.a.y;
+    const double dd = dx * dx + dy * dy;
+    const double bd = bx * dx + by * dy;
+    if (bd >= dd) {
+      bx -= dx;
+      by -= dy;
+    } else {
+      bx -= bd / dd * dx;
+      by -= bd / dd * dy;
+    }
+    const double cd = cx * dx + cy * dy;
+    if (cd >= dd) {
+      cx -= dx;
+      cy -= dy;
+    } else {
+      cx -= cd / dd * dx;
+      cy -= cd / dd * dy;
+    }
+  }
+  const double be = bx * bx + by * by, ce = cx * cx + cy * cy;
+  return be > ce ? be : ce;
+}
+// deCasteljau at the midpoint: the first (right = false) or second half
+__device__ __forceinline__ FKnots knots_half(const FKnots &k, bool right) {
+  const FPt ab = fhalf(k.a, k.b), bc = fhalf(k.b, k.c), cd = fhalf(k.c, k.d);
+  const FPt abbc = fhalf(ab, bc), bccd = fhalf(bc, cd), fin = fhalf(abbc, bccd);
+  return right ? FKnots{fin, bccd, cd, k.d} : FKnots{k.a, ab, abbc, fin};
+}
+
+constexpr uint32_t FLAT_MAX_VISITS = 1u << 20;  // per curve
+
+__device__ __forceinline__ double flat_band_span(double a, double b, uint32_t nbands,
+                                                 uint32_t band_h) {
+  const double lo = fmin(a, b), hi = fmax(a, b);
+  if (!isfinite(lo) || !isfinite(hi)) return (double)nbands;
+  const double s = floor(hi / band_h) - floor(lo / band_h) + 1.0;
+  return fmin(s, (double)nbands);
+}
+
+template <bool EMIT>
+struct DevFlattener {
+  FNode *out;
+  uint32_t n = 0, flags = 0;
+  uint64_t pairs = 0;
+  uint32_t nbands, band_h;
+  double cur_y = 0.0, init_y = 0.0;
+  uint32_t last_tag = 0xFFFFFFFFu;
+  FPt buf[3];
+  int len = 0;
+  double box[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+
+  __device__ void track(FPt p) {
+    box[0] = fmin(box[0], p.x);
+    box[1] = fmax(box[1], p.x);
+    box[2] = fmin(box[2], p.y);
+    box[3] = fmax(box[3], p.y);
+  }
+  __device__ void emit(uint32_t tag, FPt p) {
+    if (EMIT) out[n] = FNode{p.x, p.y, tag, 0u};
+    n++;
+    // the (edge, band) bound over the emitted stream, as the host computes it
+    if (tag == Z2D_NODE_MOVE_TO) {
+      cur_y = init_y = p.y;
+    } else if (tag == Z2D_NODE_LINE_TO) {
+      pairs += (uint64_t)flat_band_span(cur_y, p.y, nbands, band_h);
+      cur_y = p.y;
+      if (last_tag == Z2D_NODE_CLOSE_PATH) flags |= FLAT_SERIAL;
+    } else if (tag == Z2D_NODE_CLOSE_PATH) {
+      pairs += (uint64_t)flat_band_span(cur_y, init_y, nbands, band_h);
+      cur_y = init_y;
+    }
+    last_tag = tag;
+  }
+  __device__ void add(FPt p) {
+    if (len < 3) {
+      buf[len++] = p;
+    } else {
+      buf[1] = buf[2];
+      buf[2] = p;
+    }
+  }
+  __device__ void line_to(FPt p) {
+    emit(Z2D_NODE_LINE_TO, p);
+    track(p);
+    if (!fpeq(buf[len - 1], p)) add(p);
+  }
+  // Spline.decompose: depth-first over the halvings, leaves emit their first
+  // point unless it is the curve's start; pieces deeper than 64 halvings are
+  // leaves.
+  __device__ void curve(FPt b, FPt c, FPt d, double tol_sq) {
+    const FPt a = buf[len - 1];
+    if (fpeq(a, b) && fpeq(c, d)) {
+      line_to(d);
+      return;
+    }
+    const FKnots root{a, b, c, d};
+    uint32_t depth = 0;
+    uint64_t path = 0;  // bit (depth - 1 - i): the half taken at level i
+    uint32_t visits = 0;
+    for (;;) {
+      FKnots k = root;
+      for (uint32_t i = 0; i < depth; i++) k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
+      bool leaf = knots_err_sq(k) < tol_sq || depth > 64;
+      if (++visits > FLAT_MAX_VISITS) {
+        flags |= FLAT_RUNAWAY;
+        leaf = true;
+      }
+      if (!leaf && depth == 64) {
+        // both halves are leaves (depth 65): their first points
+        if (!fpeq(k.a, a)) line_to(k.a);
+        const FKnots r = knots_half(k, true);
+        if (!fpeq(r.a, a)) line_to(r.a);
+        leaf = true;
+      } else if (leaf) {
+        if (!fpeq(k.a, a)) line_to(k.a);
+      }
+      if (leaf) {
+        while (depth > 0 && (path & 1u)) {
+          path >>= 1;
+          depth--;
+        }
+        if (depth == 0 || (flags & FLAT_RUNAWAY)) break;
+        path |= 1u;  // the second half of the same parent
+      } else {
+        depth++;
+        path <<= 1;
+      }
+    }
+    line_to(d);
+  }
+};
+
+template <bool EMIT>
+__global__ void k_flatten(const z2d_path_node *__restrict__ raw, const FlatJob *__restrict__ jobs,
+                          uint32_t n_jobs, FNode *out, FlatResult *res) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  const FlatJob J = jobs[j];
+  DevFlattener<EMIT> f;
+  f.out = out + J.node_off;
+  f.nbands = J.nbands;
+  f.band_h = J.band_h;
+  const z2d_path_node *nodes = raw + J.raw_off;
+  const uint32_t n = J.raw_cnt;
+  bool balanced = true;
+  // the node walk of the host's checks (tags and first move already
+  // validated): fill_plotter.plot with PointBuffer(1, 3)
+  for (uint32_t i = 0; i < n; i++) {
+    const z2d_path_node nd = nodes[i];
+    const FPt p{nd.v[0], nd.v[1]};
+    if (nd.tag == Z2D_NODE_MOVE_TO) {
+      if (i == n - 1) {
+        f.emit(Z2D_NODE_MOVE_TO, p);
+        break;
+      }
+      if (f.len > 0 && !fpeq(f.buf[f.len - 1], f.buf[0])) balanced = false;
+      f.len = 0;
+      f.add(p);
+      f.emit(Z2D_NODE_MOVE_TO, p);
+      f.track(p);
+    } else if (nd.tag == Z2D_NODE_LINE_TO) {
+      f.line_to(p);
+    } else if (nd.tag == Z2D_NODE_CURVE_TO) {
+      const FPt b{nd.v[0], nd.v[1]}, c{nd.v[2], nd.v[3]}, d{nd.v[4], nd.v[5]};
+      f.track(b);
+      f.track(c);
+      f.curve(b, c, d, J.tol_sq);
+    } else {  // close
+      f.emit(Z2D_NODE_CLOSE_PATH, FPt{0.0, 0.0});
+      if (f.len >= 3 && !fpeq(f.buf[f.len - 1], f.buf[0])) f.add(f.buf[0]);
+    }
+  }
+  if (f.len > 0 && !fpeq(f.buf[f.len - 1], f.buf[0])) balanced = false;
+  if (!EMIT) {
+    FlatResult r;
+    r.n_nodes = f.n;
+    r.flags = f.flags | (balanced ? 0u : FLAT_UNBALANCED);
+    r.edge_pairs = f.pairs;
+    for (int i = 0; i < 4; i++) r.box[i] = f.box[i];
+    res[j] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Binning: ordered draw lists per band (band_h rows of one surface).
 // ---------------------------------------------------------------------------
 // One wave per slot: a band and 64 consecutive draws of its surface (every
@@ -1764,6 +1971,17 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
       hipLaunchKernelGGL(k_order_scatter, og, dim3(256), 0, st, J, ns);
     }
   }
+  return hipGetLastError();
+}
+
+hipError_t z2d_launch_flatten(const z2d_path_node *raw, const FlatJob *jobs, uint32_t n_jobs,
+                              FNode *out, FlatResult *res, bool emit, hipStream_t st) {
+  if (n_jobs == 0) return hipSuccess;
+  const dim3 grid((n_jobs + 63) / 64);
+  if (emit)
+    hipLaunchKernelGGL(k_flatten<true>, grid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
+  else
+    hipLaunchKernelGGL(k_flatten<false>, grid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
   return hipGetLastError();
 }
 
