@@ -88,6 +88,25 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
                           uint32_t precision, z2d_surface_desc *out);
 
+/* ---- PNG export ------------------------------------------------------------
+ * writeToPNGFile (src/export_png.zig:56-84), into memory (the caller writes
+ * the file).  Same IHDR, gAMA and IEND chunks (:100-145, :422-446) and the
+ * same scanline stream (:153-360: filter byte 0, demultiplied RGBA,
+ * MSB-first packed alpha, optional sRGB re-encode); the zlib stream is this
+ * library's own device deflate, so a decoder yields the reference's bytes
+ * while the file is not byte-identical.  `surface` is a host or device
+ * descriptor; `profile` a z2d_png_profile.  The reference panics on an
+ * empty surface; here INVALID_WIDTH / INVALID_HEIGHT.  An `out_cap` below
+ * the encoded size gives INVALID_ARGUMENT with *out_len = the size needed;
+ * z2d_gpu_png_bound returns a size that always suffices. */
+int32_t z2d_gpu_png_bound(uint32_t format, int32_t width, int32_t height, uint32_t profile,
+                          uint64_t *bound);
+int32_t z2d_gpu_png_encode(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, uint32_t profile,
+                           void *out, uint64_t out_cap, uint64_t *out_len);
+/* Device time of the last z2d_gpu_png_encode's kernels, in ms (no reference
+ * counterpart). */
+int32_t z2d_gpu_png_timing(z2d_gpu_ctx *ctx, double *kernel_ms);
+
 /* ---- device-resident surfaces ----------------------------------------------
  * Surface.init / initBuffer (src/surface.zig:97-184) for surfaces that stay
  * in HBM across calls. */

@@ -41,6 +41,16 @@ enum z2d_format {
   Z2D_FORMAT_ALPHA1 = 7,
 };
 
+/* WriteToPNGFileOptions.color_profile (src/export_png.zig:33-41): NONE is the
+ * reference's null (no gAMA chunk, no re-encode); LINEAR writes gAMA 100000;
+ * SRGB writes gAMA 45454 and re-encodes the color channels with pow(c, 1/2.2)
+ * (src/export_png.zig:125-145, :313-360). */
+enum z2d_png_profile {
+  Z2D_PNG_PROFILE_NONE = 0,
+  Z2D_PNG_PROFILE_LINEAR = 1,
+  Z2D_PNG_PROFILE_SRGB = 2,
+};
+
 enum z2d_operator {
   Z2D_OP_CLEAR = 0,
   Z2D_OP_SRC,

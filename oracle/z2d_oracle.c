@@ -2185,3 +2185,83 @@ uint64_t zo_count_coverage(const z2d_edge *edges, uint64_t n_edges, const double
   free(buf);
   return c;
 }
+
+/* ---------------------------------------------------------------------------
+ * PNG scanline stream (src/export_png.zig:153-305, encodeRGBAVec :313-360).
+ * Rows are walked in chunks of the reference's default vector_length (16,
+ * build.zig:141-145); with 16 a multiple of 8/bits the packed formats come
+ * out as continuous MSB-first rows padded to a byte at the row end.
+ * ------------------------------------------------------------------------- */
+#define ZO_PNG_VL 16
+
+/* encodeRGBAVec for one pixel: demultiply (pixel_vector.RGBA16.demultiply,
+ * src/internal/pixel_vector.zig:27-49) when the format has alpha, then the
+ * sRGB profile's decodeRGBAVecRaw -> applyGammaVec -> encodeRGBAVecRaw
+ * (src/internal/color_vector.zig:210-224, :266-284). */
+static void png_encode_px(uint32_t c[4], int use_alpha, uint32_t profile) {
+  if (use_alpha) {
+    for (int k = 0; k < 3; k++) {
+      uint32_t den = c[3] > 1 ? c[3] : 1;
+      c[k] = c[3] == 0 ? 0 : (uint16_t)(c[k] * 255u) / den;
+    }
+  }
+  if (profile == Z2D_PNG_PROFILE_SRGB) {
+    const float ig = 1.0f / kGammaSRGB;
+    for (int k = 0; k < 4; k++) {
+      float v = (float)c[k] / 255.0f;
+      if (k < 3) v = zo_powf(v, ig);
+      c[k] = (uint32_t)roundf(255.0f * v);
+    }
+  }
+}
+
+uint64_t zo_png_scanlines(const z2d_surface_desc *s, uint32_t profile, uint8_t *out) {
+  const int32_t w = s->width, h = s->height;
+  int bits = 32, chans = 0;
+  switch (s->format) {
+    case Z2D_FORMAT_XRGB: case Z2D_FORMAT_RGB: chans = 3; break;
+    case Z2D_FORMAT_ARGB: case Z2D_FORMAT_RGBA: chans = 4; break;
+    case Z2D_FORMAT_ALPHA8: bits = 8; break;
+    case Z2D_FORMAT_ALPHA4: bits = 4; break;
+    case Z2D_FORMAT_ALPHA2: bits = 2; break;
+    default: bits = 1; break;
+  }
+  const uint64_t row_bytes = chans ? (uint64_t)w * chans : ((uint64_t)w * bits + 7) / 8;
+  const uint64_t total = (uint64_t)h * (1 + row_bytes);
+  if (!out) return total;
+  uint8_t *o = out;
+  for (int32_t y = 0; y < h; y++) {
+    *o++ = 0; /* filter type 0 */
+    for (int32_t x = 0; x < w; x += ZO_PNG_VL) {
+      const int32_t n = w - x < ZO_PNG_VL ? w - x : ZO_PNG_VL;
+      if (chans) {
+        for (int32_t i = 0; i < n; i++) {
+          const uint8_t *p = s->data + ((uint64_t)y * w + x + i) * 4;
+          uint32_t c[4];
+          if (s->format == Z2D_FORMAT_RGBA || s->format == Z2D_FORMAT_RGB) {
+            c[0] = p[0]; c[1] = p[1]; c[2] = p[2]; c[3] = p[3];
+          } else { /* argb / xrgb bytes are b, g, r, a */
+            c[0] = p[2]; c[1] = p[1]; c[2] = p[0]; c[3] = p[3];
+          }
+          if (chans == 3) c[3] = 255;
+          png_encode_px(c, chans == 4, profile);
+          for (int k = 0; k < chans; k++) *o++ = (uint8_t)c[k];
+        }
+      } else if (bits == 8) {
+        memcpy(o, s->data + (uint64_t)y * w + x, (size_t)n);
+        o += n;
+      } else {
+        const int scale = 8 / bits;
+        const uint64_t nb = ((uint64_t)n * bits + 7) / 8;
+        memset(o, 0, nb);
+        for (int32_t i = 0; i < n; i++) {
+          uint32_t v = packed_get(s->data, (uint64_t)y * w + x + i, bits);
+          int sh = 8 - bits - (i % scale) * bits;
+          o[i / scale] |= (uint8_t)(v << sh);
+        }
+        o += nb;
+      }
+    }
+  }
+  return total;
+}

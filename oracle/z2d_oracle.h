@@ -66,6 +66,13 @@ void zo_sample_pattern_int(const z2d_pattern_desc *p, int32_t x, int32_t y,
 /* std.math.pow(f32) as used for gamma (src/color.zig:336-354). */
 float zo_powf(float x, float y);
 
+/* writePNGIDATStream's scanline stream (src/export_png.zig:153-305) with
+ * encodeRGBAVec (:313-360): for each row a filter byte 0, then rgb/xrgb as
+ * r,g,b; argb/rgba demultiplied r,g,b,a; alpha8 as is; alpha4/2/1 packed
+ * MSB-first with each row starting on a byte.  The bytes the reference hands
+ * its zlib compressor.  Returns the stream length; `out` may be NULL. */
+uint64_t zo_png_scanlines(const z2d_surface_desc *s, uint32_t profile, uint8_t *out);
+
 /* Polygon.inBox (src/internal/tess/Polygon.zig:142-201). */
 int zo_in_box(const double extents[4], double scale, int32_t w, int32_t h);
 

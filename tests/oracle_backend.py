@@ -50,6 +50,8 @@ def lib():
         L.zo_count_coverage.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double), C.c_uint32,
                                         C.c_int32, C.c_int32]
         L.zo_count_coverage.restype = C.c_uint64
+        L.zo_png_scanlines.argtypes = [P(abi.SurfaceDesc), C.c_uint32, C.c_void_p]
+        L.zo_png_scanlines.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -83,3 +85,14 @@ def plot_fill(nodes, scale, tolerance, cap=1 << 20):
     if n < 0:
         raise RuntimeError(f"plot_fill error {-n}")
     return edges[:n].copy(), tuple(ext)
+
+
+def png_scanlines(fmt, width, height, data, profile=abi.PNG_PROFILE_NONE):
+    """The oracle's PNG scanline stream (zo_png_scanlines) for a surface
+    buffer: the bytes the reference hands its zlib compressor."""
+    buf = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8))
+    d = abi.SurfaceDesc(fmt, width, height, 0, buf.ctypes.data, buf.nbytes)
+    n = lib().zo_png_scanlines(C.byref(d), profile, None)
+    out = np.zeros(n, np.uint8)
+    lib().zo_png_scanlines(C.byref(d), profile, out.ctypes.data)
+    return out.tobytes()

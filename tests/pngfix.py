@@ -9,6 +9,30 @@ import numpy as np
 from z2d_amd import abi
 
 
+def png_chunks(data):
+    """(type, payload, crc_ok) for each chunk of a PNG file's bytes."""
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    out = []
+    i = 8
+    while i < len(data):
+        (n,) = struct.unpack(">I", data[i:i + 4])
+        t = data[i + 4:i + 8]
+        d = data[i + 8:i + 8 + n]
+        (crc,) = struct.unpack(">I", data[i + 8 + n:i + 12 + n])
+        out.append((t, d, zlib.crc32(t + d) & 0xFFFFFFFF == crc))
+        i += 12 + n
+    assert i == len(data)
+    return out
+
+
+def png_stream(data):
+    """The decompressed IDAT stream (filter bytes + scanlines) of a PNG
+    file's bytes or path."""
+    if isinstance(data, str):
+        data = open(data, "rb").read()
+    return zlib.decompress(b"".join(d for t, d, _ in png_chunks(data) if t == b"IDAT"))
+
+
 def decode_png(path):
     """Returns (color_type, bit_depth, width, height, rows) where rows is a
     list of unfiltered scanline byte strings."""
@@ -121,3 +145,44 @@ def export_surface(fmt, w, h, data):
         o[..., 3] = a
         return (o & 0xFF).astype(np.uint8)
     return ch[..., 3].astype(np.uint8)
+
+
+def surface_from_fixture(path):
+    """The surface buffer a fixture was exported from, recovered from the
+    file: color type 2 -> rgb, 6 -> rgba (re-premultiplied: c*255//a is
+    strictly increasing in c for a <= 255, so the premultiplied value is the
+    unique c that exports to the stored byte), gray -> alpha8/4/2/1 at the
+    file's depth.  Returns (format, width, height, bytes)."""
+    ctype, depth, w, h, rows = decode_png(path)
+    n = w * h
+    if ctype == 2:
+        px = np.frombuffer(b"".join(rows), np.uint8).reshape(h, w, 3)
+        out = np.zeros((h, w, 4), np.uint8)
+        out[..., :3] = px
+        return abi.RGB, w, h, out.tobytes()
+    if ctype == 6:
+        px = np.frombuffer(b"".join(rows), np.uint8).reshape(h, w, 4).astype(np.int64)
+        a = px[..., 3]
+        out = np.zeros((h, w, 4), np.uint8)
+        out[..., 3] = a
+        c = np.arange(256, dtype=np.int64)
+        for av in np.unique(a):
+            if av == 0:
+                continue
+            exp = c * 255 // av          # what each premultiplied c exports to
+            inv = np.full(256, -1, np.int64)
+            inv[exp[exp < 256]] = c[exp < 256]
+            sel = a == av
+            for k in range(3):
+                v = inv[px[..., k][sel]]
+                assert (v >= 0).all(), "stored byte no premultiplied value exports to"
+                out[..., k][sel] = v
+        return abi.RGBA, w, h, out.tobytes()
+    vals = png_pixels(path).reshape(-1).astype(np.uint32)
+    if depth == 8:
+        return abi.ALPHA8, w, h, vals.astype(np.uint8).tobytes()
+    fmt = {4: abi.ALPHA4, 2: abi.ALPHA2, 1: abi.ALPHA1}[depth]
+    buf = np.zeros((n * depth + 7) // 8, np.uint8)
+    idx = np.arange(n, dtype=np.int64) * depth
+    np.bitwise_or.at(buf, idx >> 3, (vals << (idx & 7).astype(np.uint32)).astype(np.uint8))
+    return fmt, w, h, buf.tobytes()

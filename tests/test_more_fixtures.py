@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 from z2d_amd import abi
-from pngfix import export_surface, png_pixels
+from oracle_backend import png_scanlines
+from pngfix import export_surface, png_pixels, png_stream
 from spec_more_scenes import MORE_SCENES
 from stroke_check import StrokeCheckBackend
 from test_stroke_fixtures import stroke_fixture
@@ -24,6 +25,8 @@ def test_more_matches_fixture(builder, aa):
     assert got.shape == want.shape
     diff = np.argwhere(got != want)
     assert diff.size == 0, f"{len(diff)} mismatches, first at {diff[:3].tolist()}"
+    # the export's scanline stream, byte for byte (filter bytes, packing, demultiply)
+    assert png_scanlines(out.format, out.width, out.height, out.data()) == png_stream(stroke_fixture(scene.name, aa))
 
 
 @pytest.mark.parametrize("aa", AA, ids=["none", "ssaa", "msaa"])

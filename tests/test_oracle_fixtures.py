@@ -9,8 +9,8 @@ import numpy as np
 import pytest
 
 from z2d_amd import abi
-from oracle_backend import OracleBackend
-from pngfix import export_surface, png_pixels
+from oracle_backend import OracleBackend, png_scanlines
+from pngfix import export_surface, png_pixels, png_stream
 from spec_scenes import AA_SUFFIX, PATH_SCENES
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "spec")
@@ -34,3 +34,5 @@ def test_oracle_matches_fixture(builder, aa):
     assert got.shape == want.shape
     diff = np.argwhere(got != want)
     assert diff.size == 0, f"{len(diff)} mismatches, first at {diff[:3].tolist()}"
+    # the export's scanline stream, byte for byte (filter bytes, packing, demultiply)
+    assert png_scanlines(out.format, out.width, out.height, out.data()) == png_stream(fixture_path(scene.name, aa))

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from z2d_amd import abi
-from pngfix import export_surface, png_pixels
+from oracle_backend import png_scanlines
+from pngfix import export_surface, png_pixels, png_stream
 from spec_scenes import AA_SUFFIX
 from spec_stroke_scenes import STROKE_SCENES
 from stroke_check import StrokeCheckBackend
@@ -32,3 +33,5 @@ def test_stroke_matches_fixture(builder, aa):
     assert got.shape == want.shape
     diff = np.argwhere(got != want)
     assert diff.size == 0, f"{len(diff)} mismatches, first at {diff[:3].tolist()}"
+    # the export's scanline stream, byte for byte (filter bytes, packing, demultiply)
+    assert png_scanlines(out.format, out.width, out.height, out.data()) == png_stream(stroke_fixture(scene.name, aa))

@@ -36,6 +36,8 @@ COLOR_LINEAR_RGB, COLOR_SRGB, COLOR_HSL = 0, 1, 2
 DITHER_NONE, DITHER_BAYER, DITHER_BLUE_NOISE = 0, 1, 2
 DITHER_SRC_PIXEL, DITHER_SRC_COLOR, DITHER_SRC_GRADIENT = 0, 1, 2
 PATTERN_OPAQUE, PATTERN_GRADIENT, PATTERN_DITHER = 0, 1, 2
+# WriteToPNGFileOptions.color_profile (src/export_png.zig:33-41): none = null
+PNG_PROFILE_NONE, PNG_PROFILE_LINEAR, PNG_PROFILE_SRGB = 0, 1, 2
 PARAM_NONE, PARAM_DITHER, PARAM_GRADIENT, PARAM_PIXEL, PARAM_SURFACE = range(5)
 
 # status codes (include/z2d_types.h)

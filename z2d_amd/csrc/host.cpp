@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/z2d_gpu.h"
+#include "png.h"
 #include "render.h"
 #include "stroke.h"
 
@@ -45,6 +46,7 @@ hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob 
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st);
+hipError_t z2d_launch_png(const z2d::PngJob &J, hipStream_t st);  // png.hip
 
 static const uint16_t kBlueNoise[4096] = {
 #include "blue_noise_64x64.inc"
@@ -125,6 +127,11 @@ struct z2d_gpu_ctx {
   HostBuf comp_stage;
   int n_cus = 256;
   float last_comp_ms = 0.0f;  // device time of the last composite kernel
+  DevBuf png_work;            // scanline stream, segment staging, offsets
+  DevBuf png_out;             // the encoded file
+  HostBuf png_meta;
+  hipEvent_t png_ev[2] = {nullptr, nullptr};
+  float last_png_ms = 0.0f;   // device time of the last PNG encode
 };
 
 struct z2d_gpu_surface {
@@ -1546,7 +1553,12 @@ void z2d_gpu_destroy(z2d_gpu_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->comp_arena.release();
   ctx->comp_stage.release();
+  ctx->png_work.release();
+  ctx->png_out.release();
+  ctx->png_meta.release();
   for (hipEvent_t &ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t &ev : ctx->png_ev)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1874,6 +1886,158 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           const z2d_comp_op *ops, uint64_t n_ops, uint32_t precision,
                           z2d_surface_desc *out) {
   return composite_impl(ctx, dst, dst_x, dst_y, ops, n_ops, precision, out);
+}
+
+// ---------------------------------------------------------------------------
+// PNG export (src/export_png.zig:56-360)
+// ---------------------------------------------------------------------------
+struct PngShape {
+  uint64_t row_bytes = 0, raw_len = 0, n_seg = 0, max_zlib = 0, max_idat = 0, bound = 0;
+  uint32_t head_len = 0, stream_bpp = 1;
+};
+
+static int32_t png_shape(uint32_t format, int32_t w, int32_t h, uint32_t profile, PngShape &o) {
+  if (w < 1) return Z2D_ERR_INVALID_WIDTH;
+  if (h < 1) return Z2D_ERR_INVALID_HEIGHT;
+  if (format > Z2D_FORMAT_ALPHA1 || profile > Z2D_PNG_PROFILE_SRGB) return Z2D_ERR_INVALID_ARGUMENT;
+  const uint64_t W = (uint64_t)w;
+  switch (format) {
+    case Z2D_FORMAT_XRGB:
+    case Z2D_FORMAT_RGB:
+      o.row_bytes = W * 3;
+      o.stream_bpp = 3;
+      break;
+    case Z2D_FORMAT_ARGB:
+    case Z2D_FORMAT_RGBA:
+      o.row_bytes = W * 4;
+      o.stream_bpp = 4;
+      break;
+    case Z2D_FORMAT_ALPHA8: o.row_bytes = W; break;
+    case Z2D_FORMAT_ALPHA4: o.row_bytes = (W * 4 + 7) / 8; break;
+    case Z2D_FORMAT_ALPHA2: o.row_bytes = (W * 2 + 7) / 8; break;
+    default: o.row_bytes = (W + 7) / 8; break;
+  }
+  o.raw_len = (uint64_t)h * (o.row_bytes + 1);
+  o.n_seg = (o.raw_len + PNG_SEG - 1) / PNG_SEG;
+  if (o.n_seg > 0x7FFFFFFFull) return Z2D_ERR_INVALID_ARGUMENT;
+  o.head_len = 8 + 25 + (profile != Z2D_PNG_PROFILE_NONE ? 16 : 0);
+  o.max_zlib = 2 + o.raw_len + 5 * o.n_seg + 4;  // every segment stored
+  o.max_idat = (o.max_zlib + PNG_IDAT - 1) / PNG_IDAT;
+  o.bound = o.head_len + o.max_zlib + 12 * o.max_idat + 12;
+  return Z2D_OK;
+}
+
+// LZ77 candidate distances for a scanline stream with pixels of `bpp` bytes
+// and rows of `row` bytes (filter byte included): the previous pixel, the
+// pixel above and its neighbours, then longer pixel and row periods (dither
+// tiles repeat every 8 and 64 pixels).
+static uint32_t png_candidates(uint64_t bpp, uint64_t row, uint32_t *out) {
+  const uint64_t want[PNG_MAX_CAND] = {bpp, row, row - bpp, row + bpp, 2 * bpp, 2 * row,
+                                       8 * bpp, 64 * bpp};
+  uint32_t n = 0;
+  for (uint64_t d : want) {
+    if (d < 1 || d > 32768) continue;
+    bool dup = false;
+    for (uint32_t i = 0; i < n; i++) dup |= out[i] == d;
+    if (!dup) out[n++] = (uint32_t)d;
+  }
+  return n;
+}
+
+int32_t z2d_gpu_png_bound(uint32_t format, int32_t width, int32_t height, uint32_t profile,
+                          uint64_t *bound) {
+  if (!bound) return Z2D_ERR_INVALID_ARGUMENT;
+  PngShape sh;
+  int32_t rc = png_shape(format, width, height, profile, sh);
+  if (rc) return rc;
+  *bound = sh.bound;
+  return Z2D_OK;
+}
+
+int32_t z2d_gpu_png_encode(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, uint32_t profile,
+                           void *out, uint64_t out_cap, uint64_t *out_len) {
+  if (!ctx) return Z2D_ERR_NO_CONTEXT;
+  if (!surface || !out_len || (!out && out_cap)) return Z2D_ERR_INVALID_ARGUMENT;
+  PngShape sh;
+  int32_t rc = png_shape(surface->format, surface->width, surface->height, profile, sh);
+  if (rc) return rc;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  SurfRef ref;
+  rc = surface_from_desc(ctx, surface, ref);
+  if (rc) return rc;
+  Layout L;
+  const size_t o_raw = L.take(sh.raw_len + PNG_RAW_PAD);
+  const size_t o_seg = L.take(sh.n_seg * PNG_SEG_CAP);
+  const size_t o_len = L.take(sh.n_seg * 4);
+  const size_t o_sum = L.take(sh.n_seg * 8);
+  const size_t o_off = L.take((sh.n_seg + 1) * 8);
+  const size_t o_meta = L.take(32);
+  hipError_t e = ctx->png_work.ensure(L.off);
+  if (e == hipSuccess) e = ctx->png_out.ensure(sh.bound);
+  if (e == hipSuccess) e = ctx->png_meta.ensure(32);
+  for (int i = 0; i < 2 && e == hipSuccess; i++)
+    if (!ctx->png_ev[i]) e = hipEventCreate(&ctx->png_ev[i]);
+  if (e != hipSuccess) {
+    surface_release(ctx, ref, false);
+    return fail(ctx, e, "png buffers");
+  }
+  uint8_t *D = ctx->png_work.p;
+  PngJob J{};
+  J.surface = ref.s->data;
+  J.format = surface->format;
+  J.profile = profile;
+  J.width = surface->width;
+  J.height = surface->height;
+  J.row_bytes = sh.row_bytes;
+  J.raw_len = sh.raw_len;
+  J.raw = D + o_raw;
+  J.n_seg = (uint32_t)sh.n_seg;
+  J.n_cand = png_candidates(sh.stream_bpp, sh.row_bytes + 1, J.cand);
+  J.seg = D + o_seg;
+  J.seg_len = reinterpret_cast<uint32_t *>(D + o_len);
+  J.seg_sum = reinterpret_cast<uint32_t *>(D + o_sum);
+  J.seg_off = reinterpret_cast<uint64_t *>(D + o_off);
+  J.meta = reinterpret_cast<uint64_t *>(D + o_meta);
+  J.out = ctx->png_out.p;
+  J.out_cap = ctx->png_out.cap;
+  J.head_len = sh.head_len;
+  J.max_idat = (uint32_t)sh.max_idat;
+  e = hipEventRecord(ctx->png_ev[0], ctx->stream);
+  if (e == hipSuccess) e = z2d_launch_png(J, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->png_ev[1], ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->png_meta.p, J.meta, 32, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    surface_release(ctx, ref, false);
+    return fail(ctx, e, "png encode");
+  }
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->png_ev[0], ctx->png_ev[1]) == hipSuccess) ctx->last_png_ms = ms;
+  const uint64_t *meta = reinterpret_cast<const uint64_t *>(ctx->png_meta.p);
+  const uint64_t file_len = meta[0];
+  *out_len = file_len;
+  if (meta[3] || file_len > sh.bound) {  // cannot happen: the buffer holds the stored bound
+    surface_release(ctx, ref, false);
+    ctx->err = "png encode: output exceeds its bound";
+    return Z2D_ERR_DEVICE;
+  }
+  if (file_len > out_cap) {
+    surface_release(ctx, ref, false);
+    ctx->err = "png encode: out_cap smaller than the encoded size (returned in out_len)";
+    return Z2D_ERR_INVALID_ARGUMENT;
+  }
+  e = hipMemcpyAsync(out, ctx->png_out.p, file_len, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  rc = surface_release(ctx, ref, false);
+  if (e != hipSuccess) return fail(ctx, e, "png download");
+  return rc;
+}
+
+int32_t z2d_gpu_png_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
+  if (!ctx || !kernel_ms) return Z2D_ERR_INVALID_ARGUMENT;
+  *kernel_ms = ctx->last_png_ms;
+  return Z2D_OK;
 }
 
 int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {

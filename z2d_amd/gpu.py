@@ -55,6 +55,11 @@ _SIGS = {
     "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_destroy": ([C.c_void_p], None),
+    "z2d_gpu_png_bound": ([C.c_uint32, C.c_int32, C.c_int32, C.c_uint32, P(C.c_uint64)],
+                          C.c_int32),
+    "z2d_gpu_png_encode": ([C.c_void_p, P(abi.SurfaceDesc), C.c_uint32, C.c_void_p, C.c_uint64,
+                           P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_png_timing": ([C.c_void_p, P(C.c_double)], C.c_int32),
     "z2d_gpu_stream_copy": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
     "z2d_stroke_polygon": ([C.c_void_p, C.c_uint64, P(abi.StrokeOpts), C.c_uint32, C.c_void_p,
                             C.c_uint64, P(C.c_uint64), P(C.c_double), P(C.c_uint32)], C.c_int32),
@@ -270,6 +275,31 @@ class GpuBackend:
         od = out.desc() if out is not None else None
         return lib().z2d_gpu_composite(self.ctx.h, C.byref(d), dst_x, dst_y, arr, len(ops),
                                        precision, C.byref(od) if od is not None else None)
+
+
+def png_bound(fmt, width, height, profile=abi.PNG_PROFILE_NONE):
+    n = C.c_uint64()
+    rc = lib().z2d_gpu_png_bound(fmt, width, height, profile, C.byref(n))
+    if rc != 0:
+        raise GpuError(f"z2d_gpu_png_bound returned {rc}")
+    return n.value
+
+
+def png_encode(ctx, desc, profile=abi.PNG_PROFILE_NONE):
+    """writeToPNGFile into memory (z2d_gpu_png_encode): the PNG file's bytes
+    for a host or device surface descriptor."""
+    cap = png_bound(desc.format, desc.width, desc.height, profile)
+    buf = C.create_string_buffer(cap)
+    n = C.c_uint64()
+    ctx.check(lib().z2d_gpu_png_encode(ctx.h, C.byref(desc), profile, buf, cap, C.byref(n)),
+              "z2d_gpu_png_encode")
+    return buf.raw[:n.value]
+
+
+def png_timing(ctx):
+    ms = C.c_double()
+    ctx.check(lib().z2d_gpu_png_timing(ctx.h, C.byref(ms)), "z2d_gpu_png_timing")
+    return ms.value
 
 
 class Server:
