@@ -5,8 +5,9 @@ Headline workload (the configuration the metric is quoted on): one
 Beziers, 3-64 segments, log-uniform 4-512 px boxes, 60% solid / 40%
 gradient paints, 50/50 non-zero / even-odd, src_over, 4x multisample AA),
 inputs resident in HBM.  One step = one pass of the hot path over the whole
-scene: edge build from the flattened nodes, binning, and the tile
-rasterize + composite of all 10,000 paths in order.
+scene: device flattening of the raw path nodes (Bezier subdivision), edge
+build, binning, and the tile rasterize + composite of all 10,000 paths in
+order.
 
   value          pixel-draws per second (Mpix/s): pixels with nonzero
                  coverage summed over all paths, counted on the device
@@ -18,8 +19,9 @@ rasterize + composite of all 10,000 paths in order.
                  on one host core
   extra          512x512 star latency (p50/p99), 4096x4096 compositor
                  passes (GB/s), batched uneven surfaces vs one launch per
-                 surface, and the request server under open-loop Poisson
-                 arrivals (Mpix/s, p50/p99 latency per QPS level)
+                 surface, the request server under open-loop Poisson
+                 arrivals (Mpix/s, p50/p99 latency per QPS level), and the
+                 PNG export of the rendered scene
 """
 import argparse
 import ctypes as C
@@ -315,6 +317,47 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     return out
 
 
+def bench_png(gpu, ctx, sc, reps=10):
+    """PNG export of the rendered 3840x2160 scene (z2d_gpu_png_encode from a
+    device surface): device time of the encode kernels, the end-to-end call
+    with the file copied to host memory, and the file size.  Its CPU
+    baseline is the oracle's scanline stream plus zlib level 6 on one core
+    (the reference compresses with std.compress.flate at its default
+    level)."""
+    import zlib
+    surf, b = build_batch(gpu, ctx, sc)
+    b.render(0)
+    b.wait()
+    b.close()
+    d = surf.desc()
+    png = gpu.png_encode(ctx, d)  # warm-up: buffers, code objects
+    dev, wall = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        png = gpu.png_encode(ctx, d)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev.append(gpu.png_timing(ctx))
+    raw_len = sc["height"] * (1 + 4 * sc["width"])
+    host = np.zeros(sc["width"] * sc["height"] * 4, np.uint8)
+    surf.download(host)
+    surf.close()
+    res = dict(device_ms=round(float(np.median(dev)), 4), call_ms=round(float(np.median(wall)), 4),
+               stream_bytes=raw_len, file_bytes=len(png),
+               stream_GBps=round(raw_len / (np.median(dev) * 1e-3) / 1e9, 1))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_backend import png_scanlines  # the checker, timed as the baseline
+    t = time.perf_counter()
+    stream = png_scanlines(sc["format"], sc["width"], sc["height"], host.tobytes())
+    t1 = time.perf_counter()
+    z = zlib.compress(stream, 6)
+    t2 = time.perf_counter()
+    res["cpu_baseline"] = dict(ms=round((t2 - t) * 1e3, 1), scanlines_ms=round((t1 - t) * 1e3, 1),
+                               zlib6_ms=round((t2 - t1) * 1e3, 1), file_bytes=len(z) + 57,
+                               cores=1, kind="port",
+                               sample="the same surface: C oracle scanlines + zlib level 6")
+    return res
+
+
 def cpu_baseline(gpu, ctx, sc, n_paths):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_backend import OracleBackend  # the checker, timed as the baseline
@@ -347,6 +390,8 @@ def main():
                     help="distinct requests in the serving pool (cycled at each QPS level)")
     ap.add_argument("--comp-only", action="store_true",
                     help="only run the 4096x4096 compositor sweep (one JSON line)")
+    ap.add_argument("--png-only", action="store_true",
+                    help="only run the PNG export of the rendered scene (one JSON line)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the stream-copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -367,6 +412,9 @@ def main():
         return
     log("generating scene")
     sc = scenes.scene_4k(n_paths=args.paths)
+    if args.png_only:
+        print(json.dumps({"png_export_3840x2160": bench_png(gpu, ctx, sc)}))
+        return
     log("rendering scene")
     r = bench_scene(gpu, ctx, sc, args.steps, args.warmup)
     log(f"scene: {r['render_ms']:.3f} ms render, {r['pipe_ms']:.3f} ms pipeline")
@@ -418,6 +466,7 @@ def main():
             "compositor_4096_GBps": bench_compositor(gpu, ctx),
             "uneven_fill_batch_Mpix_s": bench_uneven_batch(gpu, ctx),
             "serving": bench_serving(gpu, ctx, serving_requests(gpu, args.serve_requests)),
+            "png_export_3840x2160": bench_png(gpu, ctx, sc),
         }
     print(json.dumps(line), flush=True)
 

@@ -2,7 +2,9 @@
 fill plotter (src/internal/tess/fill_plotter.zig:21-97) on node lists that
 exercise its state: repeated points, closes with fewer than three points,
 runs of closes, a line right after a close (no move), a final move, and
-subpaths crossing the 64-node chunk boundary."""
+subpaths crossing the 64-node chunk boundary; with curves, the device
+flattening's per-node form and its per-draw fallback (a curve after a close,
+a close before the first move, short closed contours)."""
 import math
 
 import numpy as np
@@ -14,7 +16,7 @@ from oracle_backend import OracleBackend
 
 pytestmark = pytest.mark.gpu
 
-M, L, Z = abi.MOVE_TO, abi.LINE_TO, abi.CLOSE_PATH
+M, L, Z, C = abi.MOVE_TO, abi.LINE_TO, abi.CLOSE_PATH, abi.CURVE_TO
 
 
 def _cases():
@@ -27,6 +29,38 @@ def _cases():
     yield "final_move", [(M, *a), (L, *b), (L, *c), (Z,), (M, *a), (M, *d)]
     yield "horizontal", [(M, 3.0, 10.0), (L, 50.0, 10.0), (L, 50.0, 40.0), (L, 3.0, 10.0),
                          (Z,), (M, 3.0, 10.0)]
+    # curves: after a close (no move), as the only segment of a closed
+    # contour, degenerate (a == b and c == d), right before a close, and
+    # returning to the contour's start
+    yield "curve_after_close", [(M, *a), (L, *b), (L, *c), (Z,), (C, *d, *e, 20.0, 50.0),
+                                (L, *b), (Z,)]
+    yield "curve_only_contour", [(M, *a), (C, *b, *c, *d), (Z,), (M, *e), (C, 40.0, 60.0,
+                                 10.0, 60.0, *e), (Z,)]
+    yield "degenerate_curve", [(M, *a), (C, *a, *c, *c), (L, *d), (Z,), (M, *b),
+                               (C, *b, *b, *b), (L, *e), (L, *c), (Z,)]
+    yield "curve_before_close", [(M, *a), (L, *b), (C, *c, *d, *e), (Z,)]
+    yield "curve_to_start", [(M, *a), (L, *b), (C, *c, *d, *a), (Z,), (M, *c), (C, *d, *e, *c),
+                             (Z,)]
+    yield "close_first", [(Z,), (M, *a), (C, *b, *c, *d), (L, *e), (Z,)]
+    yield "double_close_curve", [(M, *a), (C, *b, *c, *d), (L, *e), (Z,), (Z,), (C, *b, *c, *e),
+                                 (Z,)]
+    rng = scenes.SplitMix64(0xC0FE)
+    nodes = []
+    for k in range(150):  # mixed lines and curves across several 64-node chunks
+        pt = (rng.uniform(2, 62), rng.uniform(2, 62))
+        if k % 37 == 0:
+            if k:
+                nodes.append((Z,))
+            nodes.append((M, *pt))
+        elif rng.uniform() < 0.4:
+            nodes.append((C, rng.uniform(2, 62), rng.uniform(2, 62), rng.uniform(2, 62),
+                          rng.uniform(2, 62), *pt))
+        else:
+            nodes.append((L, *pt))
+        if k % 37 == 30:
+            nodes.append((Z,))  # a close inside the contour, lines/curves after it
+    nodes.append((Z,))
+    yield "mixed_curves_chunks", nodes
     # long subpaths across chunk boundaries, with repeats and closes near them
     rng = scenes.SplitMix64(0x9107)
     for n_sub, n_pts in ((1, 200), (5, 63), (7, 64), (3, 130)):

@@ -40,7 +40,9 @@ hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile
                              uint32_t *cursor, uint32_t n_blocks, hipStream_t st);
 hipError_t z2d_launch_composite_quad(const z2d::QuadJob &job, uint64_t n_quads, hipStream_t st);
 hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs, uint32_t n_jobs,
-                              z2d::FNode *out, z2d::FlatResult *res, bool emit, hipStream_t st);
+                              z2d::FNode *out, z2d::FlatResult *res, uint32_t *rel,
+                              const uint2 *curves, uint32_t n_curves, z2d::FlatTask *tasks,
+                              uint32_t *toff, z2d::FlatCurve *cv, bool emit, hipStream_t st);
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -545,6 +547,14 @@ struct z2d_gpu_batch {
   uint64_t n_nodes = 0;  // flattened nodes and edge slots of the prepared batch
   DevBuf flat_buf;       // device flattening inputs and results
   HostBuf flat_stage;
+  uint32_t n_flat_jobs = 0;  // draws flattened on the device at every render
+  const FlatJob *d_flat_jobs = nullptr;
+  FNode *d_flat_out = nullptr;
+  uint32_t *d_flat_rel = nullptr;
+  const uint2 *d_flat_curves = nullptr;
+  uint32_t n_flat_curves = 0;
+  FlatTask *d_flat_tasks = nullptr;
+  uint32_t *d_flat_toff = nullptr;
   std::vector<HostDraw> draws;
   PaintTable paints;
   bool prepared = false;
@@ -791,6 +801,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   // line-after-close flags, boxes and (edge, band) bounds come back here
   std::vector<FlatJob> jobs;
   std::vector<uint32_t> job_draw;
+  std::vector<uint2> curve_list;  // (job, raw node) of every curve, in draw and node order
   for (uint32_t i = 0; i < (uint32_t)b->draws.size(); i++) {
     const HostDraw &h = b->draws[i];
     if (h.d.input_edges) continue;
@@ -802,26 +813,43 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     fj.nbands = (uint32_t)((s->h + band_h - 1) / band_h);
     fj.band_h = band_h;
     fj.tol_sq = h.tol_sq;
+    fj.curve_base = (uint32_t)curve_list.size();
+    for (uint32_t k = 0; k < h.raw_cnt; k++)
+      if (b->raw_nodes[h.raw_off + k].tag == Z2D_NODE_CURVE_TO)
+        curve_list.push_back(make_uint2((uint32_t)jobs.size(), h.raw_off + k));
     jobs.push_back(fj);
     job_draw.push_back(i);
   }
-  size_t fo_jobs = 0, fo_res = 0;
+  size_t fo_jobs = 0, fo_res = 0, fo_rel = 0, fo_curves = 0, fo_tasks = 0, fo_toff = 0,
+         fo_cv = 0;
+  const uint32_t n_curves = (uint32_t)curve_list.size();
   if (!jobs.empty()) {
     Layout FL;
     FL.take(b->raw_nodes.size() * sizeof(z2d_path_node));
+    fo_curves = FL.take(curve_list.size() * sizeof(uint2));
     fo_jobs = FL.take(jobs.size() * sizeof(FlatJob));
     fo_res = FL.take(jobs.size() * sizeof(FlatResult));
+    fo_rel = FL.take(b->raw_nodes.size() * sizeof(uint32_t));  // per raw node output offsets
+    fo_tasks = FL.take((size_t)n_curves * FLAT_TASKS * sizeof(FlatTask));
+    fo_toff = FL.take((size_t)n_curves * FLAT_TASKS * sizeof(uint32_t));
+    fo_cv = FL.take((size_t)n_curves * sizeof(FlatCurve));
     hipError_t e = b->flat_buf.ensure(FL.off);
     if (e == hipSuccess) e = b->flat_stage.ensure(FL.off);
     if (e != hipSuccess) return fail(ctx, e, "flatten buffers");
     uint8_t *FH = b->flat_stage.p, *FD = b->flat_buf.p;
     std::memcpy(FH, b->raw_nodes.data(), b->raw_nodes.size() * sizeof(z2d_path_node));
     std::memcpy(FH + fo_jobs, jobs.data(), jobs.size() * sizeof(FlatJob));
+    if (n_curves) std::memcpy(FH + fo_curves, curve_list.data(), curve_list.size() * sizeof(uint2));
     HIP_OK(ctx, hipMemcpyAsync(FD, FH, fo_res, hipMemcpyHostToDevice, st));
     HIP_OK(ctx, z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(FD),
                                    reinterpret_cast<const FlatJob *>(FD + fo_jobs),
                                    (uint32_t)jobs.size(), nullptr,
-                                   reinterpret_cast<FlatResult *>(FD + fo_res), false, st));
+                                   reinterpret_cast<FlatResult *>(FD + fo_res),
+                                   reinterpret_cast<uint32_t *>(FD + fo_rel),
+                                   reinterpret_cast<const uint2 *>(FD + fo_curves), n_curves,
+                                   reinterpret_cast<FlatTask *>(FD + fo_tasks),
+                                   reinterpret_cast<uint32_t *>(FD + fo_toff),
+                                   reinterpret_cast<FlatCurve *>(FD + fo_cv), false, st));
     HIP_OK(ctx, hipMemcpyAsync(FH + fo_res, FD + fo_res, jobs.size() * sizeof(FlatResult),
                                hipMemcpyDeviceToHost, st));
     HIP_OK(ctx, hipStreamSynchronize(st));
@@ -834,6 +862,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
       h.edge_pairs = r.edge_pairs;
       if (r.flags & FLAT_UNBALANCED) h.d.flags |= DRAW_UNBALANCED | DRAW_ROWPROG;
       if (r.flags & FLAT_SERIAL) h.d.flags |= DRAW_SERIAL_PLOT;
+      jobs[k].serial = (r.flags & FLAT_NONSIMPLE) ? 1u : 0u;
       for (int c = 0; c < 4; c++) h.box[c] = r.box[c];
       h.whole = !h.bounded || !(std::isfinite(r.box[0]) && std::isfinite(r.box[1]) &&
                                 std::isfinite(r.box[2]) && std::isfinite(r.box[3]));
@@ -1021,16 +1050,25 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
 
   JobView &J = b->view;
   J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
+  b->n_flat_jobs = 0;
   if (!jobs.empty()) {
-    // emit pass: the flattened nodes at their draws' offsets
+    // the emit pass runs at the start of every render (batch_enqueue): the
+    // flattened nodes at their draws' offsets, from the resident raw nodes
     FlatJob *hj = reinterpret_cast<FlatJob *>(b->flat_stage.p + fo_jobs);
-    for (size_t k = 0; k < jobs.size(); k++) hj[k].node_off = b->draws[job_draw[k]].d.node_off;
+    for (size_t k = 0; k < jobs.size(); k++) {
+      hj[k].node_off = b->draws[job_draw[k]].d.node_off;
+      hj[k].serial = jobs[k].serial;
+    }
     HIP_OK(ctx, hipMemcpyAsync(b->flat_buf.p + fo_jobs, hj, jobs.size() * sizeof(FlatJob),
                                hipMemcpyHostToDevice, st));
-    HIP_OK(ctx, z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(b->flat_buf.p),
-                                   reinterpret_cast<const FlatJob *>(b->flat_buf.p + fo_jobs),
-                                   (uint32_t)jobs.size(), reinterpret_cast<FNode *>(D + o_nodes),
-                                   nullptr, true, st));
+    b->n_flat_jobs = (uint32_t)jobs.size();
+    b->d_flat_jobs = reinterpret_cast<const FlatJob *>(b->flat_buf.p + fo_jobs);
+    b->d_flat_out = reinterpret_cast<FNode *>(D + o_nodes);
+    b->d_flat_rel = reinterpret_cast<uint32_t *>(b->flat_buf.p + fo_rel);
+    b->d_flat_curves = reinterpret_cast<const uint2 *>(b->flat_buf.p + fo_curves);
+    b->n_flat_curves = n_curves;
+    b->d_flat_tasks = reinterpret_cast<FlatTask *>(b->flat_buf.p + fo_tasks);
+    b->d_flat_toff = reinterpret_cast<uint32_t *>(b->flat_buf.p + fo_toff);
   }
   J.in_edges = reinterpret_cast<const z2d_edge *>(D + o_inedges);
   J.edges = reinterpret_cast<z2d_edge *>(D + o_edges);
@@ -1091,7 +1129,13 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
   JobView &J = b->view;
   const uint32_t ns = J.n_surfaces;
   const hipStream_t st = bstream(b);
-  hipError_t e = z2d_launch_fill_u32(J.counters, 16, 0u, st);
+  hipError_t e = hipSuccess;
+  if (b->n_flat_jobs)
+    e = z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(b->flat_buf.p),
+                           b->d_flat_jobs, b->n_flat_jobs, b->d_flat_out, nullptr,
+                           b->d_flat_rel, b->d_flat_curves, b->n_flat_curves, b->d_flat_tasks,
+                           b->d_flat_toff, nullptr, true, st);
+  if (e == hipSuccess) e = z2d_launch_fill_u32(J.counters, 16, 0u, st);
   if (e == hipSuccess) e = z2d_launch_fill_u32(J.cursors, ns + 1, 0u, st);
   if (e == hipSuccess) e = z2d_launch_edge_build(J, st);
   if (e == hipSuccess)

@@ -47,13 +47,36 @@ struct FlatJob {
   uint32_t raw_off, raw_cnt;  // the draw's path nodes
   uint32_t node_off;          // where its flattened nodes go (emit pass)
   uint32_t nbands, band_h;    // for the (edge, band) bound (count pass)
-  uint32_t pad;
+  uint32_t serial;            // flattened by the per-draw walk (see FLAT_NONSIMPLE)
   double tol_sq;              // squared tolerance (>= 0.001^2)
+  uint32_t curve_base;        // index of the draw's first curve in the curve list
+  uint32_t pad;
+};
+// Curves are subdivided as FLAT_TASKS independent pieces: the subtrees of the
+// Spline recursion at depth FLAT_TASK_DEPTH, in order (a leaf above that
+// depth belongs to the leftmost piece under it).
+constexpr uint32_t FLAT_TASK_DEPTH = 4;
+constexpr uint32_t FLAT_TASKS = 1u << FLAT_TASK_DEPTH;
+struct FlatTask {  // count pass, per curve piece
+  uint32_t cnt, flags;
+  uint64_t pairs;   // (edge, band) bound between the piece's own points
+  double fy, ly;    // its first and last point's y
+  double box[4];
+};
+struct FlatCurve {  // count pass, per curve
+  uint32_t cnt, flags;
+  uint64_t pairs;
+  double box[4];
 };
 enum : uint32_t {
   FLAT_UNBALANCED = 1u << 0,  // a contour does not end where it starts
   FLAT_SERIAL = 1u << 1,      // a line follows a close
   FLAT_RUNAWAY = 1u << 2,     // a curve exceeded the subdivision visit limit
+  // the draw's point-buffer state is not a function of each node and its
+  // predecessor (a close followed by anything but a move, a close before
+  // the first move, or a contour with fewer than 3 distinct points before
+  // its close): it is flattened by the per-draw walk instead of per node
+  FLAT_NONSIMPLE = 1u << 3,
 };
 struct FlatResult {
   uint32_t n_nodes, flags;

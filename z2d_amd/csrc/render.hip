@@ -483,14 +483,23 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
 }
 
 // ---------------------------------------------------------------------------
-// k_flatten: the fill plotter's node walk with Spline decomposition
+// Flattening: the fill plotter's node walk with Spline decomposition
 // (src/internal/tess/Spline.zig:37-165, fill_plotter.zig:52-118) on the
-// device, one thread per fill draw.  The count pass (EMIT=false) returns the
+// device.  A curve starts at its predecessor's end point, so its subdivision
+// is independent of the rest of the draw: k_curve_tasks splits each curve's
+// recursion into FLAT_TASKS subtrees (one thread each) and k_curve_combine
+// turns their counts into offsets and the curve's bounds.  k_flatten_wave
+// takes one wave per fill draw and one lane per raw node: wave scans give the
+// output offsets and the contour state.  Draws whose point-buffer state needs
+// the whole walk (FLAT_NONSIMPLE) go through k_flatten_serial, one thread per
+// draw.  The count pass (EMIT=false, once per prepared batch) returns the
 // number of flattened nodes, the contour-balance and line-after-close flags,
-// the (edge, band) bound and the bounding box of the tracked points; the emit
-// pass writes the nodes.  The recursive subdivision is walked depth-first with
-// the knots of each visited piece recomputed from the curve by its path of
-// halvings, which reproduces the recursion's values exactly.
+// the (edge, band) bound and the bounding box of the tracked points, and
+// stores each raw node's output offset; the emit pass (every render) writes
+// the nodes.  The recursion is walked depth-first with the knots of a piece
+// halved from its parent when descending and recomputed from the curve by its
+// path of halvings when moving to a sibling, which reproduces the recursion's
+// values exactly.
 // ---------------------------------------------------------------------------
 struct FPt {
   double x, y;
@@ -545,6 +554,57 @@ __device__ __forceinline__ double flat_band_span(double a, double b, uint32_t nb
   return fmin(s, (double)nbands);
 }
 
+// Spline.decompose from start point a: depth-first over the halvings, leaves
+// give their first point unless it is the curve's start, then d; pieces
+// deeper than 64 halvings are leaves.  `pt` receives each line_to point.
+template <class F>
+__device__ __forceinline__ void spline_points(FPt a, FPt b, FPt c, FPt d, double tol_sq,
+                                              uint32_t &flags, F &&pt) {
+  if (fpeq(a, b) && fpeq(c, d)) {
+    pt(d);
+    return;
+  }
+  const FKnots root{a, b, c, d};
+  uint32_t depth = 0;
+  uint64_t path = 0;  // bit (depth - 1 - i): the half taken at level i
+  uint32_t visits = 0;
+  FKnots k = root;  // the visited piece: a first half from its parent when
+                    // descending, else recomputed from the root by its path
+  for (;;) {
+    bool leaf = knots_err_sq(k) < tol_sq || depth > 64;
+    bool runaway = false;
+    if (++visits > FLAT_MAX_VISITS) {
+      flags |= FLAT_RUNAWAY;
+      runaway = true;
+      leaf = true;
+    }
+    if (!leaf && depth == 64) {
+      // both halves are leaves (depth 65): their first points
+      if (!fpeq(k.a, a)) pt(k.a);
+      const FKnots r = knots_half(k, true);
+      if (!fpeq(r.a, a)) pt(r.a);
+      leaf = true;
+    } else if (leaf) {
+      if (!fpeq(k.a, a)) pt(k.a);
+    }
+    if (leaf) {
+      while (depth > 0 && (path & 1u)) {
+        path >>= 1;
+        depth--;
+      }
+      if (depth == 0 || runaway) break;
+      path |= 1u;  // the second half of the same parent
+      k = root;
+      for (uint32_t i = 0; i < depth; i++) k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
+    } else {
+      depth++;
+      path <<= 1;
+      k = knots_half(k, false);
+    }
+  }
+  pt(d);
+}
+
 template <bool EMIT>
 struct DevFlattener {
   FNode *out;
@@ -592,58 +652,21 @@ struct DevFlattener {
     track(p);
     if (!fpeq(buf[len - 1], p)) add(p);
   }
-  // Spline.decompose: depth-first over the halvings, leaves emit their first
-  // point unless it is the curve's start; pieces deeper than 64 halvings are
-  // leaves.
   __device__ void curve(FPt b, FPt c, FPt d, double tol_sq) {
-    const FPt a = buf[len - 1];
-    if (fpeq(a, b) && fpeq(c, d)) {
-      line_to(d);
-      return;
-    }
-    const FKnots root{a, b, c, d};
-    uint32_t depth = 0;
-    uint64_t path = 0;  // bit (depth - 1 - i): the half taken at level i
-    uint32_t visits = 0;
-    for (;;) {
-      FKnots k = root;
-      for (uint32_t i = 0; i < depth; i++) k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
-      bool leaf = knots_err_sq(k) < tol_sq || depth > 64;
-      if (++visits > FLAT_MAX_VISITS) {
-        flags |= FLAT_RUNAWAY;
-        leaf = true;
-      }
-      if (!leaf && depth == 64) {
-        // both halves are leaves (depth 65): their first points
-        if (!fpeq(k.a, a)) line_to(k.a);
-        const FKnots r = knots_half(k, true);
-        if (!fpeq(r.a, a)) line_to(r.a);
-        leaf = true;
-      } else if (leaf) {
-        if (!fpeq(k.a, a)) line_to(k.a);
-      }
-      if (leaf) {
-        while (depth > 0 && (path & 1u)) {
-          path >>= 1;
-          depth--;
-        }
-        if (depth == 0 || (flags & FLAT_RUNAWAY)) break;
-        path |= 1u;  // the second half of the same parent
-      } else {
-        depth++;
-        path <<= 1;
-      }
-    }
-    line_to(d);
+    spline_points(buf[len - 1], b, c, d, tol_sq, flags, [&](FPt p) { line_to(p); });
   }
 };
 
+// The per-draw walk, for the draws k_flatten_wave marked FLAT_NONSIMPLE
+// (count pass) or the host marked serial (emit pass).
 template <bool EMIT>
-__global__ void k_flatten(const z2d_path_node *__restrict__ raw, const FlatJob *__restrict__ jobs,
-                          uint32_t n_jobs, FNode *out, FlatResult *res) {
+__global__ void k_flatten_serial(const z2d_path_node *__restrict__ raw,
+                                 const FlatJob *__restrict__ jobs, uint32_t n_jobs, FNode *out,
+                                 FlatResult *res) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_jobs) return;
   const FlatJob J = jobs[j];
+  if (EMIT ? !J.serial : !(res[j].flags & FLAT_NONSIMPLE)) return;
   DevFlattener<EMIT> f;
   f.out = out + J.node_off;
   f.nbands = J.nbands;
@@ -682,11 +705,324 @@ __global__ void k_flatten(const z2d_path_node *__restrict__ raw, const FlatJob *
   if (!EMIT) {
     FlatResult r;
     r.n_nodes = f.n;
-    r.flags = f.flags | (balanced ? 0u : FLAT_UNBALANCED);
+    r.flags = f.flags | (balanced ? 0u : FLAT_UNBALANCED) | FLAT_NONSIMPLE;
     r.edge_pairs = f.pairs;
     for (int i = 0; i < 4; i++) r.box[i] = f.box[i];
     res[j] = r;
   }
+}
+
+__device__ __forceinline__ void box_track(double *box, FPt p) {
+  box[0] = fmin(box[0], p.x);
+  box[1] = fmax(box[1], p.x);
+  box[2] = fmin(box[2], p.y);
+  box[3] = fmax(box[3], p.y);
+}
+
+// One wave per draw, one lane per raw node (64 at a time).  For the draws it
+// keeps (every close is the last node or followed by a move, comes after a
+// move, and ends a contour of at least 3 distinct points), the point buffer
+// before a line or curve ends at its predecessor's end point, a close always
+// returns the contour to its start, and a contour without a close is
+// balanced exactly when it ends at its move point.  The distinct-point count
+// is taken over the end points (a curve counted by its end point only), a
+// lower bound of the buffer's length, so a draw kept here has at least 3.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_flatten_wave(const z2d_path_node *__restrict__ raw,
+                                                      const FlatJob *__restrict__ jobs,
+                                                      uint32_t n_jobs, FNode *out,
+                                                      FlatResult *res, uint32_t *rel,
+                                                      const FlatCurve *__restrict__ cv) {
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= n_jobs) return;
+  const FlatJob J = jobs[j];
+  if (EMIT && J.serial) return;
+  const z2d_path_node *nodes = raw + J.raw_off;
+  uint32_t *rl = rel + J.raw_off;
+  FNode *o = out + J.node_off;
+  const uint32_t n = J.raw_cnt;
+  const uint64_t below = (1ull << lane) - 1ull;
+  // carried from the previous 64 nodes (wave-uniform)
+  uint32_t c_tag = NONE, c_S = 0, c_out = 0, c_mvS = 0, c_cur = J.curve_base;
+  FPt c_pt{0.0, 0.0}, c_mv{0.0, 0.0};
+  bool c_mv_ok = false;
+  // per lane, reduced at the end
+  uint64_t pairs = 0;
+  uint32_t flags = 0;
+  double box[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t i = base + lane;
+    const bool act = i < n;
+    z2d_path_node nd;
+    uint32_t tag = NONE;
+    FPt p{0.0, 0.0};
+    if (act) {
+      nd = nodes[i];
+      tag = nd.tag;
+      if (tag == Z2D_NODE_MOVE_TO || tag == Z2D_NODE_LINE_TO) p = FPt{nd.v[0], nd.v[1]};
+      else if (tag == Z2D_NODE_CURVE_TO) p = FPt{nd.v[4], nd.v[5]};
+    }
+    // the predecessor's tag and end point
+    uint32_t ptag = __shfl_up(tag, 1, 64);
+    FPt pp{__shfl_up(p.x, 1, 64), __shfl_up(p.y, 1, 64)};
+    if (lane == 0) {
+      ptag = c_tag;
+      pp = c_pt;
+    }
+    const uint32_t ntag = (act && i + 1 < n) ? nodes[i + 1].tag : NONE;
+    // S: running count of lines/curves whose end point differs from the
+    // previous end point
+    uint32_t S = ((tag == Z2D_NODE_LINE_TO || tag == Z2D_NODE_CURVE_TO) && !fpeq(p, pp)) ? 1u : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(S, off, 64);
+      if (lane >= (uint32_t)off) S += y;
+    }
+    S += c_S;
+    // the contour's move: the last move at or before this node, and the last
+    // one before it
+    const uint64_t mv_mask = __ballot(tag == Z2D_NODE_MOVE_TO);
+    const uint64_t at = mv_mask & (below | (1ull << lane)), before = mv_mask & below;
+    const int src_at = at ? 63 - __builtin_clzll(at) : 0;
+    const int src_bf = before ? 63 - __builtin_clzll(before) : 0;
+    FPt mv{__shfl(p.x, src_at, 64), __shfl(p.y, src_at, 64)};
+    uint32_t mvS = __shfl(S, src_at, 64);
+    FPt mvb{__shfl(p.x, src_bf, 64), __shfl(p.y, src_bf, 64)};
+    bool mv_ok = true;
+    if (!at) {
+      mv = c_mv;
+      mvS = c_mvS;
+      mv_ok = c_mv_ok;
+    }
+    if (!before) mvb = c_mv;
+    uint32_t cnt = 0;
+    if (tag == Z2D_NODE_MOVE_TO) {
+      cnt = 1;
+      if (i != n - 1) box_track(box, p);
+      // the previous contour ends at the predecessor
+      if ((ptag == Z2D_NODE_LINE_TO || ptag == Z2D_NODE_CURVE_TO) && !fpeq(pp, mvb))
+        flags |= FLAT_UNBALANCED;
+    } else if (tag == Z2D_NODE_LINE_TO) {
+      cnt = 1;
+      box_track(box, p);
+      pairs += (uint64_t)flat_band_span(pp.y, p.y, J.nbands, J.band_h);
+    } else if (tag == Z2D_NODE_CLOSE_PATH) {
+      cnt = 1;
+      pairs += (uint64_t)flat_band_span(pp.y, mv.y, J.nbands, J.band_h);
+      if (!mv_ok || (ntag != NONE && ntag != Z2D_NODE_MOVE_TO) || 1u + S - mvS < 3u)
+        flags |= FLAT_NONSIMPLE;
+    }
+    if ((tag == Z2D_NODE_LINE_TO || tag == Z2D_NODE_CURVE_TO) && i == n - 1 && !fpeq(p, mv))
+      flags |= FLAT_UNBALANCED;
+    // output offsets from the node counts (a curve's from k_curve_combine),
+    // stored; the emit pass leaves curves to k_curve_tasks
+    const uint64_t cm = __ballot(tag == Z2D_NODE_CURVE_TO);
+    if (!EMIT && tag == Z2D_NODE_CURVE_TO) {
+      const FlatCurve &r = cv[c_cur + (uint32_t)__popcll(cm & below)];
+      cnt = r.cnt;
+      flags |= r.flags;
+      pairs += r.pairs;
+      box[0] = fmin(box[0], r.box[0]);
+      box[1] = fmax(box[1], r.box[1]);
+      box[2] = fmin(box[2], r.box[2]);
+      box[3] = fmax(box[3], r.box[3]);
+    } else if (EMIT && act && tag != Z2D_NODE_CURVE_TO) {
+      o[rl[i]] = FNode{tag == Z2D_NODE_CLOSE_PATH ? 0.0 : p.x,
+                       tag == Z2D_NODE_CLOSE_PATH ? 0.0 : p.y, tag, 0u};
+    }
+    c_cur += (uint32_t)__popcll(cm);
+    if (!EMIT) {
+      uint32_t x = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+      }
+      if (act) rl[i] = c_out + x - cnt;
+      c_out += __shfl(x, 63, 64);
+    }
+    c_tag = __shfl(tag, 63, 64);
+    c_pt = FPt{__shfl(p.x, 63, 64), __shfl(p.y, 63, 64)};
+    c_S = __shfl(S, 63, 64);
+    if (mv_mask) {
+      const int last = 63 - __builtin_clzll(mv_mask);
+      c_mv = FPt{__shfl(p.x, last, 64), __shfl(p.y, last, 64)};
+      c_mvS = __shfl(S, last, 64);
+      c_mv_ok = true;
+    }
+  }
+  if (!EMIT) {
+    for (int off = 32; off > 0; off >>= 1) {
+      pairs += __shfl_xor(pairs, off, 64);
+      flags |= __shfl_xor(flags, off, 64);
+      box[0] = fmin(box[0], __shfl_xor(box[0], off, 64));
+      box[1] = fmax(box[1], __shfl_xor(box[1], off, 64));
+      box[2] = fmin(box[2], __shfl_xor(box[2], off, 64));
+      box[3] = fmax(box[3], __shfl_xor(box[3], off, 64));
+    }
+    if (lane == 0) {
+      FlatResult r;
+      r.n_nodes = c_out;
+      r.flags = flags;
+      r.edge_pairs = pairs;
+      for (int q = 0; q < 4; q++) r.box[q] = box[q];
+      res[j] = r;
+    }
+  }
+}
+
+// One piece of a curve's Spline walk (task t of FLAT_TASKS): the subtree at
+// depth FLAT_TASK_DEPTH along t's bits, depth-first, or the leaf above that
+// depth when t is the leftmost piece under it.  Concatenated in t order, the
+// pieces give spline_points' leaf points (without the final d).
+template <class F>
+__device__ __forceinline__ void spline_task(FPt a, FPt b, FPt c, FPt d, double tol_sq, uint32_t t,
+                                            uint32_t &flags, F &&pt) {
+  const FKnots root{a, b, c, d};
+  FKnots k = root;
+  for (uint32_t l = 0; l < FLAT_TASK_DEPTH; l++) {
+    if (knots_err_sq(k) < tol_sq) {
+      if ((t & ((1u << (FLAT_TASK_DEPTH - l)) - 1u)) == 0u && !fpeq(k.a, a)) pt(k.a);
+      return;
+    }
+    k = knots_half(k, (t >> (FLAT_TASK_DEPTH - 1 - l)) & 1u);
+  }
+  uint32_t depth = FLAT_TASK_DEPTH;
+  uint64_t path = t;
+  uint32_t visits = 0;
+  for (;;) {
+    bool leaf = knots_err_sq(k) < tol_sq || depth > 64;
+    bool runaway = false;
+    if (++visits > FLAT_MAX_VISITS) {
+      flags |= FLAT_RUNAWAY;
+      runaway = true;
+      leaf = true;
+    }
+    if (!leaf && depth == 64) {
+      if (!fpeq(k.a, a)) pt(k.a);
+      const FKnots r = knots_half(k, true);
+      if (!fpeq(r.a, a)) pt(r.a);
+      leaf = true;
+    } else if (leaf) {
+      if (!fpeq(k.a, a)) pt(k.a);
+    }
+    if (leaf) {
+      while (depth > FLAT_TASK_DEPTH && (path & 1u)) {
+        path >>= 1;
+        depth--;
+      }
+      if (depth == FLAT_TASK_DEPTH || runaway) break;
+      path |= 1u;
+      k = root;
+      for (uint32_t i = 0; i < depth; i++) k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
+    } else {
+      depth++;
+      path <<= 1;
+      k = knots_half(k, false);
+    }
+  }
+}
+
+__device__ __forceinline__ void curve_points(const z2d_path_node *raw, uint32_t i, FPt &a, FPt &b,
+                                             FPt &c, FPt &d) {
+  // a curve of a per-node draw follows a move, line or curve
+  const z2d_path_node nd = raw[i], pv = raw[i - 1];
+  a = pv.tag == Z2D_NODE_CURVE_TO ? FPt{pv.v[4], pv.v[5]} : FPt{pv.v[0], pv.v[1]};
+  b = FPt{nd.v[0], nd.v[1]};
+  c = FPt{nd.v[2], nd.v[3]};
+  d = FPt{nd.v[4], nd.v[5]};
+}
+
+// One thread per curve piece.  Count pass: the piece's point count, its
+// first and last y, the (edge, band) bound between its points and its box.
+// Emit pass: its points at the offset k_curve_combine gave it; the last
+// piece adds the curve's end point.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_curve_tasks(const z2d_path_node *__restrict__ raw,
+                                                     const FlatJob *__restrict__ jobs,
+                                                     const uint2 *__restrict__ curves,
+                                                     uint32_t n_curves, FlatTask *tasks,
+                                                     const uint32_t *__restrict__ toff,
+                                                     const uint32_t *__restrict__ rel,
+                                                     FNode *out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t ci = g / FLAT_TASKS, t = g % FLAT_TASKS;
+  if (ci >= n_curves) return;
+  const uint2 e = curves[ci];
+  const FlatJob &J = jobs[e.x];
+  if (EMIT && J.serial) return;
+  FPt a, b, c, d;
+  curve_points(raw, e.y, a, b, c, d);
+  const bool degenerate = fpeq(a, b) && fpeq(c, d);  // the curve is one line to d
+  uint32_t flags = 0;
+  if (EMIT) {
+    FNode *o = out + J.node_off + rel[e.y] + toff[g];
+    uint32_t k = 0;
+    if (!degenerate)
+      spline_task(a, b, c, d, J.tol_sq, t, flags,
+                  [&](FPt q) { o[k++] = FNode{q.x, q.y, Z2D_NODE_LINE_TO, 0u}; });
+    if (t == FLAT_TASKS - 1) o[k] = FNode{d.x, d.y, Z2D_NODE_LINE_TO, 0u};
+  } else {
+    FlatTask r;
+    r.cnt = 0;
+    r.pairs = 0;
+    r.fy = r.ly = 0.0;
+    r.box[0] = r.box[2] = INFINITY;
+    r.box[1] = r.box[3] = -INFINITY;
+    if (!degenerate)
+      spline_task(a, b, c, d, J.tol_sq, t, flags, [&](FPt q) {
+        if (r.cnt) r.pairs += (uint64_t)flat_band_span(r.ly, q.y, J.nbands, J.band_h);
+        else r.fy = q.y;
+        r.ly = q.y;
+        r.cnt++;
+        box_track(r.box, q);
+      });
+    r.flags = flags;
+    tasks[g] = r;
+  }
+}
+
+// Count pass, one thread per curve: piece offsets, and the curve's point
+// count (pieces + its end point), (edge, band) bound from its start point
+// along all its points, box (control points and points) and flags.
+__global__ void __launch_bounds__(256) k_curve_combine(const z2d_path_node *__restrict__ raw,
+                                                       const FlatJob *__restrict__ jobs,
+                                                       const uint2 *__restrict__ curves,
+                                                       uint32_t n_curves,
+                                                       const FlatTask *__restrict__ tasks,
+                                                       uint32_t *toff, FlatCurve *cv) {
+  const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= n_curves) return;
+  const uint2 e = curves[ci];
+  const FlatJob &J = jobs[e.x];
+  FPt a, b, c, d;
+  curve_points(raw, e.y, a, b, c, d);
+  FlatCurve r;
+  r.cnt = 0;
+  r.flags = 0;
+  r.pairs = 0;
+  r.box[0] = r.box[2] = INFINITY;
+  r.box[1] = r.box[3] = -INFINITY;
+  box_track(r.box, b);
+  box_track(r.box, c);
+  double cy = a.y;
+  for (uint32_t t = 0; t < FLAT_TASKS; t++) {
+    const FlatTask &T = tasks[ci * FLAT_TASKS + t];
+    toff[ci * FLAT_TASKS + t] = r.cnt;
+    r.flags |= T.flags;
+    if (!T.cnt) continue;
+    r.pairs += (uint64_t)flat_band_span(cy, T.fy, J.nbands, J.band_h) + T.pairs;
+    cy = T.ly;
+    r.cnt += T.cnt;
+    r.box[0] = fmin(r.box[0], T.box[0]);
+    r.box[1] = fmax(r.box[1], T.box[1]);
+    r.box[2] = fmin(r.box[2], T.box[2]);
+    r.box[3] = fmax(r.box[3], T.box[3]);
+  }
+  r.pairs += (uint64_t)flat_band_span(cy, d.y, J.nbands, J.band_h);
+  box_track(r.box, d);
+  r.cnt += 1;
+  cv[ci] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1974,14 +2310,37 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Count pass (emit = false): the curve pieces, their combination per curve,
+// the per-node pass for every draw, then the per-draw walk for the draws it
+// marked FLAT_NONSIMPLE.  Emit pass: each draw by the form its
+// FlatJob.serial names, curves by pieces.
 hipError_t z2d_launch_flatten(const z2d_path_node *raw, const FlatJob *jobs, uint32_t n_jobs,
-                              FNode *out, FlatResult *res, bool emit, hipStream_t st) {
+                              FNode *out, FlatResult *res, uint32_t *rel, const uint2 *curves,
+                              uint32_t n_curves, FlatTask *tasks, uint32_t *toff, FlatCurve *cv,
+                              bool emit, hipStream_t st) {
   if (n_jobs == 0) return hipSuccess;
-  const dim3 grid((n_jobs + 63) / 64);
-  if (emit)
-    hipLaunchKernelGGL(k_flatten<true>, grid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
-  else
-    hipLaunchKernelGGL(k_flatten<false>, grid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
+  const dim3 wgrid((n_jobs + 3) / 4), sgrid((n_jobs + 63) / 64);
+  const dim3 tgrid((uint32_t)(((uint64_t)n_curves * FLAT_TASKS + 255) / 256)),
+      cgrid((n_curves + 255) / 256);
+  if (emit) {
+    hipLaunchKernelGGL(k_flatten_wave<true>, wgrid, dim3(256), 0, st, raw, jobs, n_jobs, out, res,
+                       rel, cv);
+    if (n_curves)
+      hipLaunchKernelGGL(k_curve_tasks<true>, tgrid, dim3(256), 0, st, raw, jobs, curves, n_curves,
+                         tasks, toff, rel, out);
+    hipLaunchKernelGGL(k_flatten_serial<true>, sgrid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
+  } else {
+    if (n_curves) {
+      hipLaunchKernelGGL(k_curve_tasks<false>, tgrid, dim3(256), 0, st, raw, jobs, curves,
+                         n_curves, tasks, toff, rel, out);
+      hipLaunchKernelGGL(k_curve_combine, cgrid, dim3(256), 0, st, raw, jobs, curves, n_curves,
+                         tasks, toff, cv);
+    }
+    hipLaunchKernelGGL(k_flatten_wave<false>, wgrid, dim3(256), 0, st, raw, jobs, n_jobs, out, res,
+                       rel, cv);
+    hipLaunchKernelGGL(k_flatten_serial<false>, sgrid, dim3(64), 0, st, raw, jobs, n_jobs, out,
+                       res);
+  }
   return hipGetLastError();
 }
 
