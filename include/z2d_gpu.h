@@ -88,6 +88,37 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
                           uint32_t precision, z2d_surface_desc *out);
 
+/* ---- text ---------------------------------------------------------------------
+ * Font.loadBuffer (src/Font.zig:28-45): validates the font (magic, table
+ * checksums, required tables, cmap subtable, loca format) and keeps a copy of
+ * the bytes.  A font may be shared between threads (its glyph cache is
+ * locked). */
+typedef struct z2d_font z2d_font;
+int32_t z2d_font_load(const void *data, uint64_t len, z2d_font **out);
+void z2d_font_destroy(z2d_font *font);
+/* Font.Meta (src/Font.zig:173-260): units per em, advance width max, number
+ * of long hmetrics, flags (bit 0 lsb_is_at_x_zero, bit 1 long loca, bit 2
+ * full-range cmap). */
+int32_t z2d_font_info(z2d_font *font, uint32_t out[4]);
+/* Glyph.init (src/internal/Glyph.zig): glyph index, advance, lsb and glyf
+ * offset (-1 = no outline) for a codepoint. */
+int32_t z2d_font_glyph(z2d_font *font, uint32_t codepoint, int32_t out[4]);
+/* The path text.show builds (src/text.zig:73-196): glyph outlines laid out
+ * with advances and kerning, in device space.  Host only; *n_out is the node
+ * count (INVALID_ARGUMENT when cap is smaller). */
+int32_t z2d_text_path(z2d_font *font, const char *utf8, uint64_t len, double x, double y,
+                      const z2d_text_opts *opts, z2d_path_node *out, uint64_t cap,
+                      uint64_t *n_out);
+/* text.show (src/text.zig:73-196): that path filled with painter.fill's
+ * checks and raster (z2d_gpu_fill). */
+int32_t z2d_gpu_text_show(z2d_gpu_ctx *ctx, z2d_surface_desc *surface,
+                          const z2d_pattern_desc *pattern, z2d_font *font, const char *utf8,
+                          uint64_t len, double x, double y, const z2d_text_opts *opts);
+/* text.show into a batch: one fill draw per call, rendered with the batch. */
+int32_t z2d_gpu_batch_text(z2d_gpu_batch *b, z2d_gpu_surface *surface,
+                           const z2d_pattern_desc *pattern, z2d_font *font, const char *utf8,
+                           uint64_t len, double x, double y, const z2d_text_opts *opts);
+
 /* ---- PNG export ------------------------------------------------------------
  * writeToPNGFile (src/export_png.zig:56-84), into memory (the caller writes
  * the file).  Same IHDR, gAMA and IEND chunks (:100-145, :422-446) and the

@@ -252,6 +252,15 @@ typedef struct z2d_fill_opts {
   double tolerance;
 } z2d_fill_opts;
 
+/* text.ShowTextOptions (src/text.zig:32-46): the path's transformation
+ * {ax, by, cx, dy, tx, ty} (Transformation.zig field order), the font size in
+ * user units (em size), and the fill options. */
+typedef struct z2d_text_opts {
+  double transformation[6];
+  double size;
+  z2d_fill_opts fill;
+} z2d_text_opts;
+
 /* painter.StrokeOptions (src/painter.zig:145-198). */
 typedef struct z2d_stroke_opts {
   uint32_t anti_aliasing_mode;
@@ -304,6 +313,13 @@ enum z2d_status {
   Z2D_ERR_BUDGET_EXCEEDED = -3,
   Z2D_ERR_NO_CONTEXT = -4,
   Z2D_ERR_UNSUPPORTED = -5,
+  /* text.ShowTextError's UTF-8 errors (src/text.zig:48-67) */
+  Z2D_ERR_INVALID_TEXT = -6,
+  /* Font.LoadBufferError, Font.FileError and Glyph.Outline.InitError
+   * (src/Font.zig, src/internal/Glyph.zig): not a font, checksum mismatch,
+   * missing table, no usable cmap, bad loca format, a read past the end, a
+   * malformed glyph */
+  Z2D_ERR_INVALID_FONT = -7,
 };
 
 #ifdef __cplusplus

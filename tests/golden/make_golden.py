@@ -1,7 +1,9 @@
 """Copies the reference's own acceptance fixtures (spec/files/*.png, data
 produced by the reference's `zig build spec -Dupdate=true`) that cover the
-fill/composite hot path into tests/golden/spec/, with a SHA-256 manifest.
-Reads /root/reference; run once where the reference is mounted."""
+fill/composite hot path into tests/golden/spec/, with a SHA-256 manifest,
+and the test fonts the reference's text tests load (spec/test-fonts,
+src/internal/test-fonts) into tests/golden/fonts/.  Reads /root/reference;
+run once where the reference is mounted."""
 import glob
 import hashlib
 import os
@@ -20,7 +22,9 @@ PREFIXES = ["001", "002", "003", "004", "005", "006", "007", "013", "016", "023"
             "021", "022", "027", "029", "030", "034", "036", "037", "038", "039", "040",
             "041", "043", "044", "045", "054", "055", "056", "057", "060", "073",
             # arcs (Path.arc restated on the test side)
-            "035", "042", "058"]
+            "035", "042", "058",
+            # text (the library's font parser and layout)
+            "074"]
 
 
 def main():
@@ -37,6 +41,12 @@ def main():
             lines.append(f"{h}  {os.path.basename(f)}")
     with open(os.path.join(OUT, "SHA256SUMS"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
+    fonts = os.path.join(HERE, "fonts")
+    os.makedirs(fonts, exist_ok=True)
+    for rel in ("spec/test-fonts/Inter-Regular.ttf", "src/internal/test-fonts/Inter-Regular.subset.ttf",
+                "src/internal/test-fonts/Inter-SemiBold.subset.ttf",
+                "src/internal/test-fonts/AnnotationMono-Regular.ttf"):
+        shutil.copyfile(os.path.join(REF, rel), os.path.join(fonts, os.path.basename(rel)))
     print(f"copied {len(lines)} fixtures")
 
 

@@ -241,3 +241,17 @@ def fill_opts(aa=AA_DEFAULT, rule=NON_ZERO, op="src_over", precision=PRECISION_I
     o.precision = precision
     o.tolerance = tolerance
     return o
+
+
+class TextOpts(C.Structure):
+    """text.ShowTextOptions: transformation {ax, by, cx, dy, tx, ty}, size, fill options."""
+    _fields_ = [("transformation", C.c_double * 6), ("size", C.c_double), ("fill", FillOpts)]
+
+
+def text_opts(size=16.0, transformation=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), fill=None):
+    o = TextOpts()
+    for i, v in enumerate(transformation):
+        o.transformation[i] = v
+    o.size = size
+    o.fill = fill if fill is not None else fill_opts()
+    return o

@@ -5,6 +5,8 @@ on the product path."""
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +57,17 @@ _SIGS = {
     "z2d_gpu_batch_counters": ([C.c_void_p, C.c_uint32, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_batch_reset": ([C.c_void_p], C.c_int32),
     "z2d_gpu_batch_destroy": ([C.c_void_p], None),
+    "z2d_font_load": ([C.c_void_p, C.c_uint64, P(C.c_void_p)], C.c_int32),
+    "z2d_font_destroy": ([C.c_void_p], None),
+    "z2d_font_info": ([C.c_void_p, P(C.c_uint32)], C.c_int32),
+    "z2d_font_glyph": ([C.c_void_p, C.c_uint32, P(C.c_int32)], C.c_int32),
+    "z2d_text_path": ([C.c_void_p, C.c_char_p, C.c_uint64, C.c_double, C.c_double,
+                       P(abi.TextOpts), C.c_void_p, C.c_uint64, P(C.c_uint64)], C.c_int32),
+    "z2d_gpu_text_show": ([C.c_void_p, P(abi.SurfaceDesc), P(abi.PatternDesc), C.c_void_p,
+                           C.c_char_p, C.c_uint64, C.c_double, C.c_double, P(abi.TextOpts)],
+                          C.c_int32),
+    "z2d_gpu_batch_text": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p, C.c_char_p,
+                            C.c_uint64, C.c_double, C.c_double, P(abi.TextOpts)], C.c_int32),
     "z2d_gpu_png_bound": ([C.c_uint32, C.c_int32, C.c_int32, C.c_uint32, P(C.c_uint64)],
                           C.c_int32),
     "z2d_gpu_png_encode": ([C.c_void_p, P(abi.SurfaceDesc), C.c_uint32, C.c_void_p, C.c_uint64,
@@ -177,6 +190,11 @@ class Batch:
         return lib().z2d_gpu_batch_fill(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
                                         len(nodes), C.byref(opts))
 
+    def text(self, surface, pattern, font, text, x, y, opts):
+        b = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+        return lib().z2d_gpu_batch_text(self.h, surface.h, C.byref(pattern), font.h, b, len(b),
+                                        x, y, C.byref(opts))
+
     def stroke(self, surface, pattern, nodes, opts):
         return lib().z2d_gpu_batch_stroke(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
                                           len(nodes), C.byref(opts))
@@ -275,6 +293,65 @@ class GpuBackend:
         od = out.desc() if out is not None else None
         return lib().z2d_gpu_composite(self.ctx.h, C.byref(d), dst_x, dst_y, arr, len(ops),
                                        precision, C.byref(od) if od is not None else None)
+
+
+class Font:
+    """Font.loadBuffer (z2d_font_load): a validated copy of a TrueType font."""
+
+    def __init__(self, data):
+        self.data = bytes(data)
+        h = C.c_void_p()
+        rc = lib().z2d_font_load(self.data, len(self.data), C.byref(h))
+        if rc != 0:
+            raise GpuError(f"z2d_font_load returned {rc}")
+        self.h = h
+
+    def info(self):
+        out = (C.c_uint32 * 4)()
+        rc = lib().z2d_font_info(self.h, out)
+        if rc != 0:
+            raise GpuError(f"z2d_font_info returned {rc}")
+        return dict(units_per_em=out[0], advance_width_max=out[1], number_of_hmetrics=out[2],
+                    lsb_is_at_x_zero=bool(out[3] & 1), loca_long=bool(out[3] & 2),
+                    cmap_full=bool(out[3] & 4))
+
+    def glyph(self, codepoint):
+        out = (C.c_int32 * 4)()
+        rc = lib().z2d_font_glyph(self.h, codepoint, out)
+        if rc != 0:
+            raise GpuError(f"z2d_font_glyph returned {rc}")
+        return dict(index=out[0], advance=out[1], lsb=out[2],
+                    offset=None if out[3] < 0 else out[3])
+
+    def text_path(self, text, x, y, opts):
+        """The node array text.show hands painter.fill (z2d_text_path)."""
+        b = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+        n = C.c_uint64()
+        rc = lib().z2d_text_path(self.h, b, len(b), x, y, C.byref(opts), None, 0, C.byref(n))
+        if rc not in (0, -1):
+            return rc, None
+        arr = np.zeros(n.value, dtype=abi.NODE_DTYPE)
+        rc = lib().z2d_text_path(self.h, b, len(b), x, y, C.byref(opts), arr.ctypes.data,
+                                 n.value, C.byref(n))
+        return rc, arr
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().z2d_font_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def text_show(ctx, desc, pattern, font, text, x, y, opts):
+    """text.show on the device (z2d_gpu_text_show)."""
+    b = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+    return lib().z2d_gpu_text_show(ctx.h, C.byref(desc), C.byref(pattern), font.h, b, len(b),
+                                   x, y, C.byref(opts))
 
 
 def png_bound(fmt, width, height, profile=abi.PNG_PROFILE_NONE):
