@@ -21,7 +21,7 @@ order.
                  passes (GB/s), batched uneven surfaces vs one launch per
                  surface, the request server under open-loop Poisson
                  arrivals (Mpix/s, p50/p99 latency per QPS level), and the
-                 PNG export of the rendered scene
+                 PNG export of the rendered scene, and a page of text
 """
 import argparse
 import ctypes as C
@@ -358,6 +358,71 @@ def bench_png(gpu, ctx, sc, reps=10):
     return res
 
 
+TEXT_WORDS = ("The quick brown fox jumps over the lázy dog; façade naïve Ærøskøbing "
+              "0123456789 AVATAR Wavy typography kerning ligatures").split()
+
+
+def bench_text(gpu, ctx, lines=100, reps=5):
+    """A 3840x2160 page of text (Inter Regular, size 18, one line per draw,
+    ~100 characters each) laid out on the host (z2d_gpu_batch_text) and
+    rendered by one batch launch; the oracle rasterizing the same text
+    paths on one core is its CPU baseline."""
+    font_path = os.path.join(ROOT, "tests", "golden", "fonts", "Inter-Regular.ttf")
+    font = gpu.Font(open(font_path, "rb").read())
+    rng = scenes.SplitMix64(0x7E47)
+    texts = []
+    for _ in range(lines):
+        words = []
+        while sum(len(w) + 1 for w in words) < 100:
+            words.append(TEXT_WORDS[int(rng.uniform() * len(TEXT_WORDS)) % len(TEXT_WORDS)])
+        texts.append(" ".join(words))
+    opts = abi.text_opts(18.0, fill=abi.fill_opts(abi.AA_DEFAULT))
+    pat = abi.solid_pattern(abi.rgba(20, 20, 20, 255))
+    surf = gpu.DeviceSurface(ctx, abi.RGBA, 3840, 2160)
+    surf.upload(np.full(3840 * 2160 * 4, 255, np.uint8))
+    b = gpu.Batch(ctx)
+    t0 = time.perf_counter()
+    for i, t in enumerate(texts):
+        assert b.text(surf, pat, font, t, 10.0, 20.0 + 21.0 * i, opts) == 0
+    t1 = time.perf_counter()
+    b.prepare()
+    t2 = time.perf_counter()
+    b.render(0)
+    b.wait()
+    b.counters(True)
+    b.render(0)
+    b.wait()
+    pix = b.counters(False)[0]
+    pipe = []
+    for _ in range(reps):
+        b.render(0)
+        b.wait()
+        pipe.append(b.timing()[0])
+    glyphs = sum(len(t.replace(" ", "")) for t in texts)
+    nodes = 0
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_backend import OracleBackend  # the checker, timed as the baseline
+    from zscene import Surface
+    ref = Surface(abi.RGBA, 3840, 2160)
+    ref.buf[: ref.nbytes] = 255
+    paths = [font.text_path(t, 10.0, 20.0 + 21.0 * i, opts)[1] for i, t in enumerate(texts)]
+    nodes = sum(len(p) for p in paths)
+    ob = OracleBackend()
+    t3 = time.perf_counter()
+    for p in paths[:20]:
+        ob.fill(ref, pat, p, opts.fill)
+    cpu_s = (time.perf_counter() - t3) * lines / 20
+    b.close()
+    surf.close()
+    ms = float(np.median(pipe))
+    return dict(lines=lines, glyphs=glyphs, path_nodes=nodes, pixel_draws=int(pix),
+                host_layout_ms=round((t1 - t0) * 1e3, 2), prepare_ms=round((t2 - t1) * 1e3, 2),
+                device_pipeline_ms=round(ms, 4), glyphs_per_s=round(glyphs / (ms * 1e-3), 0),
+                cpu_baseline=dict(ms=round(cpu_s * 1e3, 1), cores=1, kind="port",
+                                  sample="oracle fill of the first 20 of the 100 line paths, "
+                                         "scaled to 100"))
+
+
 def cpu_baseline(gpu, ctx, sc, n_paths):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_backend import OracleBackend  # the checker, timed as the baseline
@@ -390,6 +455,8 @@ def main():
                     help="distinct requests in the serving pool (cycled at each QPS level)")
     ap.add_argument("--comp-only", action="store_true",
                     help="only run the 4096x4096 compositor sweep (one JSON line)")
+    ap.add_argument("--text-only", action="store_true",
+                    help="only run the text page (one JSON line)")
     ap.add_argument("--png-only", action="store_true",
                     help="only run the PNG export of the rendered scene (one JSON line)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -409,6 +476,9 @@ def main():
         return
     if args.comp_only:
         print(json.dumps({"compositor_4096": bench_compositor(gpu, ctx)}))
+        return
+    if args.text_only:
+        print(json.dumps({"text_page_3840x2160": bench_text(gpu, ctx)}))
         return
     log("generating scene")
     sc = scenes.scene_4k(n_paths=args.paths)
@@ -467,6 +537,7 @@ def main():
             "uneven_fill_batch_Mpix_s": bench_uneven_batch(gpu, ctx),
             "serving": bench_serving(gpu, ctx, serving_requests(gpu, args.serve_requests)),
             "png_export_3840x2160": bench_png(gpu, ctx, sc),
+            "text_page_3840x2160": bench_text(gpu, ctx),
         }
     print(json.dumps(line), flush=True)
 
