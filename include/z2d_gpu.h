@@ -254,6 +254,13 @@ typedef struct z2d_request_desc {
   uint64_t n_fills;
   const z2d_draw_item *draws; /* fills and strokes, drawn in order after `fills` */
   uint64_t n_draws;
+  /* the result in `out`: 0 = the surface's bytes; 1 + z2d_png_profile = a
+   * PNG file (writeToPNGFile, src/export_png.zig:56-84), encoded on the
+   * device after the render.  A file larger than out_len completes with
+   * INVALID_ARGUMENT. */
+  uint32_t encode;
+  uint32_t reserved;
+  uint64_t *out_written;      /* optional: bytes written to `out` (or needed) */
 } z2d_request_desc;
 
 int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t max_batch,

@@ -216,3 +216,60 @@ def test_server_fills_and_strokes():
         assert not diff_report(alone, subs[-1][1])
     finally:
         srv.close()
+
+
+def test_server_png_responses():
+    """Requests that ask for a PNG file (encode = 1 + profile): the file
+    decodes to the oracle's scanline stream of the same request's raw result
+    (byte for byte; the sRGB profile within 1 LSB), raw and encoded requests
+    mixed in one server; a buffer smaller than the file completes with
+    INVALID_ARGUMENT and reports the size needed."""
+    from oracle_backend import png_scanlines
+    from pngfix import png_chunks, png_stream
+    rng = scenes.SplitMix64(0x96)
+    reqs = []
+    for i in range(18):
+        fmt = [abi.RGBA, abi.RGB, abi.ALPHA8, abi.ALPHA1, abi.ARGB][rng.randint(0, 4)]
+        w, h = rng.randint(8, 300), rng.randint(8, 200)
+        jobs, keep = scenes.random_jobs(rng, rng.randint(0, 10), w, h,
+                                        max_side=min(180.0, max(w, h)))
+        prof = [abi.PNG_PROFILE_NONE, abi.PNG_PROFILE_LINEAR, abi.PNG_PROFILE_SRGB][i % 3]
+        reqs.append(dict(fmt=fmt, w=w, h=h, clear=_clear_pixel(rng, fmt), jobs=jobs, keep=keep,
+                         prof=prof))
+    srv = _server(256 << 20, max_batch=6)
+    try:
+        subs = []
+        for rq in reqs:
+            raw = np.zeros(abi.surface_nbytes(rq["fmt"], rq["w"], rq["h"]), np.uint8)
+            png = np.zeros(gpu_png_bound(rq), np.uint8)
+            r1 = srv.submit(rq["fmt"], rq["w"], rq["h"], [], clear=rq["clear"], out=raw,
+                            draws=rq["jobs"])
+            r2 = srv.submit(rq["fmt"], rq["w"], rq["h"], [], clear=rq["clear"], out=png,
+                            draws=rq["jobs"], encode=1 + rq["prof"])
+            assert r1[0] == 0 and r2[0] == 0
+            subs.append((r1[1], raw, r2[1], png))
+        for i, ((rid1, raw, rid2, png), rq) in enumerate(zip(subs, reqs)):
+            assert srv.wait(rid1) == 0 and srv.wait(rid2) == 0, f"request {i}"
+            n = srv.written(rid2)
+            data = png[:n].tobytes()
+            assert all(ok for _, _, ok in png_chunks(data))
+            want = np.frombuffer(png_scanlines(rq["fmt"], rq["w"], rq["h"], raw.tobytes(),
+                                               rq["prof"]), np.uint8).astype(np.int32)
+            got = np.frombuffer(png_stream(data), np.uint8).astype(np.int32)
+            tol = 1 if rq["prof"] == abi.PNG_PROFILE_SRGB and rq["fmt"] <= abi.RGBA else 0
+            assert got.shape == want.shape and np.abs(got - want).max(initial=0) <= tol, i
+        # a buffer too small for the file
+        rq = reqs[0]
+        small = np.zeros(16, np.uint8)
+        rc, rid = srv.submit(rq["fmt"], rq["w"], rq["h"], [], clear=rq["clear"], out=small,
+                             draws=rq["jobs"], encode=1)
+        assert rc == 0
+        assert srv.wait(rid) == -1  # INVALID_ARGUMENT
+        assert srv.written(rid) > 16
+    finally:
+        srv.close()
+
+
+def gpu_png_bound(rq):
+    from z2d_amd import gpu
+    return gpu.png_bound(rq["fmt"], rq["w"], rq["h"], rq["prof"])

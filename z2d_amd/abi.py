@@ -141,7 +141,9 @@ class DrawItem(C.Structure):
 class RequestDesc(C.Structure):
     _fields_ = [("format", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32),
                 ("clear", Pixel), ("clip_mask", C.c_void_p), ("fills", C.POINTER(FillItem)),
-                ("n_fills", C.c_uint64), ("draws", C.POINTER(DrawItem)), ("n_draws", C.c_uint64)]
+                ("n_fills", C.c_uint64), ("draws", C.POINTER(DrawItem)), ("n_draws", C.c_uint64),
+                ("encode", C.c_uint32), ("reserved", C.c_uint32),
+                ("out_written", C.POINTER(C.c_uint64))]
 
 
 IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)

@@ -1988,6 +1988,44 @@ static uint32_t png_candidates(uint64_t bpp, uint64_t row, uint32_t *out) {
   return n;
 }
 
+// Enqueues the encode of a device surface on `st`: work buffers from `work`
+// (grown if needed), the file at `out` (capacity `out_cap` >= the bound) and
+// meta[0..3] (file bytes, zlib bytes, Adler-32, overflow flag) at `meta`.
+static hipError_t png_enqueue(const z2d_gpu_surface &s, uint32_t profile, const PngShape &sh,
+                              DevBuf &work, uint8_t *out, uint64_t out_cap, uint64_t *meta,
+                              hipStream_t st) {
+  Layout L;
+  const size_t o_raw = L.take(sh.raw_len + PNG_RAW_PAD);
+  const size_t o_seg = L.take(sh.n_seg * PNG_SEG_CAP);
+  const size_t o_len = L.take(sh.n_seg * 4);
+  const size_t o_sum = L.take(sh.n_seg * 8);
+  const size_t o_off = L.take((sh.n_seg + 1) * 8);
+  hipError_t e = work.ensure(L.off);
+  if (e != hipSuccess) return e;
+  uint8_t *D = work.p;
+  PngJob J{};
+  J.surface = s.data;
+  J.format = s.format;
+  J.profile = profile;
+  J.width = s.w;
+  J.height = s.h;
+  J.row_bytes = sh.row_bytes;
+  J.raw_len = sh.raw_len;
+  J.raw = D + o_raw;
+  J.n_seg = (uint32_t)sh.n_seg;
+  J.n_cand = png_candidates(sh.stream_bpp, sh.row_bytes + 1, J.cand);
+  J.seg = D + o_seg;
+  J.seg_len = reinterpret_cast<uint32_t *>(D + o_len);
+  J.seg_sum = reinterpret_cast<uint32_t *>(D + o_sum);
+  J.seg_off = reinterpret_cast<uint64_t *>(D + o_off);
+  J.meta = meta;
+  J.out = out;
+  J.out_cap = out_cap;
+  J.head_len = sh.head_len;
+  J.max_idat = (uint32_t)sh.max_idat;
+  return z2d_launch_png(J, st);
+}
+
 int32_t z2d_gpu_png_bound(uint32_t format, int32_t width, int32_t height, uint32_t profile,
                           uint64_t *bound) {
   if (!bound) return Z2D_ERR_INVALID_ARGUMENT;
@@ -2009,15 +2047,7 @@ int32_t z2d_gpu_png_encode(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, uint32_t
   SurfRef ref;
   rc = surface_from_desc(ctx, surface, ref);
   if (rc) return rc;
-  Layout L;
-  const size_t o_raw = L.take(sh.raw_len + PNG_RAW_PAD);
-  const size_t o_seg = L.take(sh.n_seg * PNG_SEG_CAP);
-  const size_t o_len = L.take(sh.n_seg * 4);
-  const size_t o_sum = L.take(sh.n_seg * 8);
-  const size_t o_off = L.take((sh.n_seg + 1) * 8);
-  const size_t o_meta = L.take(32);
-  hipError_t e = ctx->png_work.ensure(L.off);
-  if (e == hipSuccess) e = ctx->png_out.ensure(sh.bound);
+  hipError_t e = ctx->png_out.ensure(sh.bound + 256);
   if (e == hipSuccess) e = ctx->png_meta.ensure(32);
   for (int i = 0; i < 2 && e == hipSuccess; i++)
     if (!ctx->png_ev[i]) e = hipEventCreate(&ctx->png_ev[i]);
@@ -2025,32 +2055,14 @@ int32_t z2d_gpu_png_encode(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, uint32_t
     surface_release(ctx, ref, false);
     return fail(ctx, e, "png buffers");
   }
-  uint8_t *D = ctx->png_work.p;
-  PngJob J{};
-  J.surface = ref.s->data;
-  J.format = surface->format;
-  J.profile = profile;
-  J.width = surface->width;
-  J.height = surface->height;
-  J.row_bytes = sh.row_bytes;
-  J.raw_len = sh.raw_len;
-  J.raw = D + o_raw;
-  J.n_seg = (uint32_t)sh.n_seg;
-  J.n_cand = png_candidates(sh.stream_bpp, sh.row_bytes + 1, J.cand);
-  J.seg = D + o_seg;
-  J.seg_len = reinterpret_cast<uint32_t *>(D + o_len);
-  J.seg_sum = reinterpret_cast<uint32_t *>(D + o_sum);
-  J.seg_off = reinterpret_cast<uint64_t *>(D + o_off);
-  J.meta = reinterpret_cast<uint64_t *>(D + o_meta);
-  J.out = ctx->png_out.p;
-  J.out_cap = ctx->png_out.cap;
-  J.head_len = sh.head_len;
-  J.max_idat = (uint32_t)sh.max_idat;
+  uint64_t *dmeta = reinterpret_cast<uint64_t *>(ctx->png_out.p);  // meta, then the file
+  uint8_t *dfile = ctx->png_out.p + 256;
   e = hipEventRecord(ctx->png_ev[0], ctx->stream);
-  if (e == hipSuccess) e = z2d_launch_png(J, ctx->stream);
+  if (e == hipSuccess)
+    e = png_enqueue(*ref.s, profile, sh, ctx->png_work, dfile, sh.bound, dmeta, ctx->stream);
   if (e == hipSuccess) e = hipEventRecord(ctx->png_ev[1], ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(ctx->png_meta.p, J.meta, 32, hipMemcpyDeviceToHost, ctx->stream);
+    e = hipMemcpyAsync(ctx->png_meta.p, dmeta, 32, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     surface_release(ctx, ref, false);
@@ -2071,7 +2083,7 @@ int32_t z2d_gpu_png_encode(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, uint32_t
     ctx->err = "png encode: out_cap smaller than the encoded size (returned in out_len)";
     return Z2D_ERR_INVALID_ARGUMENT;
   }
-  e = hipMemcpyAsync(out, ctx->png_out.p, file_len, hipMemcpyDeviceToHost, ctx->stream);
+  e = hipMemcpyAsync(out, dfile, file_len, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   rc = surface_release(ctx, ref, false);
   if (e != hipSuccess) return fail(ctx, e, "png download");
@@ -2258,6 +2270,9 @@ struct SrvRequest {
   std::vector<std::unique_ptr<SrvFill>> fills;
   void *out = nullptr;
   uint64_t out_len = 0;
+  uint32_t encode = 0;  // 0 raw bytes, else 1 + PNG profile
+  uint64_t *out_written = nullptr;
+  PngShape png;
   size_t surf_bytes = 0, mask_bytes = 0;
   z2d_gpu_surface surf;
   uint8_t *mask_dev = nullptr;
@@ -2272,6 +2287,8 @@ struct SrvSlot {
   std::vector<SrvRequest *> reqs;
   HostBuf result_stage;
   std::vector<size_t> stage_off;
+  DevBuf png_work, png_files;  // PNG encodes of the batch: work, then meta + file
+  std::vector<size_t> png_off;
   bool busy = false;
 };
 
@@ -2328,9 +2345,23 @@ struct z2d_gpu_server {
     for (size_t i = 0; i < sl.reqs.size(); i++) {
       SrvRequest *r = sl.reqs[i];
       if (cnt[3] && r->status == Z2D_OK) r->status = Z2D_ERR_DEVICE;  // bin overflow
-      if (r->out && r->status == Z2D_OK)
-        std::memcpy(r->out, sl.result_stage.p + sl.stage_off[i],
-                    std::min<uint64_t>(r->out_len, r->surf.nbytes));
+      const uint8_t *res = sl.result_stage.p + sl.stage_off[i];
+      if (r->encode && r->status == Z2D_OK) {
+        const uint64_t *meta = reinterpret_cast<const uint64_t *>(res);
+        const uint64_t flen = meta[0];
+        if (meta[3] || flen > r->png.bound) {
+          r->status = Z2D_ERR_DEVICE;
+        } else {
+          if (r->out_written) *r->out_written = flen;
+          if (r->out) {
+            if (flen <= r->out_len) std::memcpy(r->out, res + 256, flen);
+            else r->status = Z2D_ERR_INVALID_ARGUMENT;
+          }
+        }
+      } else if (r->status == Z2D_OK) {
+        if (r->out) std::memcpy(r->out, res, std::min<uint64_t>(r->out_len, r->surf.nbytes));
+        if (r->out_written) *r->out_written = r->out ? std::min<uint64_t>(r->out_len, r->surf.nbytes) : 0;
+      }
     }
     std::lock_guard<std::mutex> lk(mu);
     for (SrvRequest *r : sl.reqs) {
@@ -2356,13 +2387,18 @@ struct z2d_gpu_server {
     sl.batch.ctx = ctx;
     sl.batch.stream = st;
     sl.batch.count_stats = 1;
-    size_t stage = 0;
+    size_t stage = 0, files = 0;
     sl.stage_off.clear();
+    sl.png_off.clear();
     for (SrvRequest *r : take) {
       sl.stage_off.push_back(stage);
-      stage += (r->surf.nbytes + 255) & ~(size_t)255;
+      sl.png_off.push_back(files);
+      const size_t res = r->encode ? 256 + r->png.bound : r->surf.nbytes;
+      stage += (res + 255) & ~(size_t)255;
+      if (r->encode) files += (256 + r->png.bound + 255) & ~(size_t)255;
     }
     hipError_t e = sl.result_stage.ensure(stage);
+    if (e == hipSuccess && files) e = sl.png_files.ensure(files);
     for (SrvRequest *r : take) {
       if (e != hipSuccess) break;
       // Surface.initPixel: every pixel = T.fromPixel(clear)
@@ -2402,8 +2438,19 @@ struct z2d_gpu_server {
     int32_t rc = e == hipSuccess ? batch_prepare(&sl.batch) : Z2D_ERR_DEVICE;
     if (rc == Z2D_OK) rc = batch_render(&sl.batch, 0);
     for (size_t i = 0; i < take.size() && rc == Z2D_OK; i++) {
-      hipError_t ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], take[i]->surf.data,
-                                     take[i]->surf.nbytes, hipMemcpyDeviceToHost, st);
+      SrvRequest *r = take[i];
+      hipError_t ce;
+      if (r->encode) {  // the PNG file (meta, then the file at +256) instead of the surface
+        uint8_t *region = sl.png_files.p + sl.png_off[i];
+        ce = png_enqueue(r->surf, r->encode - 1, r->png, sl.png_work, region + 256, r->png.bound,
+                         reinterpret_cast<uint64_t *>(region), st);
+        if (ce == hipSuccess)
+          ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], region, 256 + r->png.bound,
+                              hipMemcpyDeviceToHost, st);
+      } else {
+        ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], r->surf.data, r->surf.nbytes,
+                            hipMemcpyDeviceToHost, st);
+      }
       if (ce != hipSuccess) rc = Z2D_ERR_DEVICE;
     }
     if (rc != Z2D_OK)
@@ -2536,6 +2583,13 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   r->clear = rq->clear;
   r->out = out;
   r->out_len = out_len;
+  r->encode = rq->encode;
+  r->out_written = rq->out_written;
+  if (rq->encode > 1 + Z2D_PNG_PROFILE_SRGB) return Z2D_ERR_INVALID_ARGUMENT;
+  if (rq->encode) {
+    int32_t prc = png_shape(rq->format, rq->width, rq->height, rq->encode - 1, r->png);
+    if (prc) return prc;
+  }
   r->surf.ctx = s->ctx;
   r->surf.format = rq->format;
   r->surf.w = rq->width;
@@ -2543,7 +2597,7 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   r->surf.nbytes = surface_bytes(rq->format, rq->width, rq->height);
   r->surf.owned = false;
   r->surf_bytes = padded_bytes(r->surf.nbytes);
-  if (out && out_len < r->surf.nbytes) return Z2D_ERR_INVALID_ARGUMENT;
+  if (!rq->encode && out && out_len < r->surf.nbytes) return Z2D_ERR_INVALID_ARGUMENT;
   if (rq->clip_mask) {
     size_t n = (size_t)rq->width * (size_t)rq->height;
     r->mask.assign(rq->clip_mask, rq->clip_mask + n);
@@ -2677,6 +2731,8 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
     for (int i = 0; i < 3; i++)
       if (sl.batch.ev[i]) (void)hipEventDestroy(sl.batch.ev[i]);
     sl.result_stage.release();
+    sl.png_work.release();
+    sl.png_files.release();
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }

@@ -394,12 +394,15 @@ class Server:
                   "server_create")
         self.h = h
         self._pending = {}
+        self._written = {}
 
     @staticmethod
-    def prepare(fmt, width, height, fills, clear=None, clip=None, draws=None):
+    def prepare(fmt, width, height, fills, clear=None, clip=None, draws=None, encode=0):
         """Builds the request descriptor once (it may be submitted many
         times).  fills: list of (pattern, nodes, opts); draws (after the
-        fills, in order): list of ("fill" | "stroke", pattern, nodes, opts)."""
+        fills, in order): list of ("fill" | "stroke", pattern, nodes, opts);
+        encode: 0 = surface bytes, 1 + PNG profile = a PNG file (its length
+        lands in prep[-1], a c_uint64)."""
         items = (abi.FillItem * max(1, len(fills)))()
         for i, (pat, nodes, opts) in enumerate(fills):
             items[i].pattern = C.pointer(pat)
@@ -426,31 +429,47 @@ class Server:
         rq.n_fills = len(fills)
         rq.draws = ditems
         rq.n_draws = len(draws)
+        rq.encode = encode
+        written = C.c_uint64()
+        rq.out_written = C.pointer(written)
         # the descriptor and what it points into
-        return (rq, items, fills, clip, ditems, draws)
+        return (rq, items, fills, clip, ditems, draws, written)
 
     def submit_prepared(self, prep, out=None):
         """Returns (rc, id); `out` (numpy uint8, >= surface bytes) receives
         the result and must stay alive until the request completes."""
         rid = C.c_uint64()
+        w = C.c_uint64()
+        prep[0].out_written = C.pointer(w)  # copied at submit: one counter per request
         rc = lib().z2d_gpu_server_submit(self.h, C.byref(prep[0]),
                                          out.ctypes.data if out is not None else None,
                                          out.nbytes if out is not None else 0, C.byref(rid))
         if rc == 0:
-            self._pending[rid.value] = out  # keeps the result buffer alive
+            self._pending[rid.value] = (out, w)  # keeps the result buffer alive
         return rc, rid.value
 
-    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None, draws=None):
+    def _done(self, rid):
+        entry = self._pending.pop(rid, None)
+        if entry is not None:
+            self._written[rid] = entry[1].value
+
+    def written(self, rid):
+        """Bytes written to the completed request's `out` (the PNG file's
+        length for an encoded request, or the size it needed)."""
+        return self._written.pop(rid, None)
+
+    def submit(self, fmt, width, height, fills, clear=None, clip=None, out=None, draws=None,
+               encode=0):
         """fills: list of (pattern, nodes, opts); draws: list of
         ("fill" | "stroke", pattern, nodes, opts).  Returns (rc, id)."""
         return self.submit_prepared(
-            self.prepare(fmt, width, height, fills, clear, clip, draws), out)
+            self.prepare(fmt, width, height, fills, clear, clip, draws, encode), out)
 
     def wait(self, rid, timeout_ms=60000.0):
         st = C.c_int32()
         rc = lib().z2d_gpu_server_wait(self.h, rid, C.byref(st), timeout_ms)
         if rc == 1:
-            self._pending.pop(rid, None)
+            self._done(rid)
             return st.value
         if rc == 0:
             raise TimeoutError(f"request {rid} not done after {timeout_ms} ms")
@@ -461,7 +480,7 @@ class Server:
         rid, st = C.c_uint64(), C.c_int32()
         rc = lib().z2d_gpu_server_wait_any(self.h, C.byref(rid), C.byref(st), timeout_ms)
         if rc == 1:
-            self._pending.pop(rid.value, None)
+            self._done(rid.value)
             return rid.value, st.value
         if rc == 0:
             return None
