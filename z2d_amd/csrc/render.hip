@@ -1793,6 +1793,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         // crossing adds its direction at its sample column (crossings left of
         // the strip at column 0, right of it nowhere); work items are the
         // (edge, active sample row) pairs, compacted by a wave scan.
+        uint32_t inner = 0;  // a crossing at a column past 0 (per lane)
         if (packed) {
           for (uint32_t c0 = 0; c0 < ne; c0 += 64) {
             const uint32_t k = c0 + lane;
@@ -1829,6 +1830,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               const int32_t xr = x - xs_base;
               if (xr >= cols) continue;
               const int32_t col = xr < 0 ? 0 : xr;
+              inner |= (uint32_t)col;
               const int32_t dv = eo ? 1 : (ce.y0 < ce.y1 ? -1 : 1);
               atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? ((uint32_t)dv << 16) : (uint32_t)dv);
             }
@@ -1840,7 +1842,23 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         // the composite in draw order.
         const RowRec *R = (D.flags & DRAW_ROWPROG) ? J.rows + D.row_off : nullptr;
         const int c_lo = (int)lane * SC;
-        if (packed && SC == 4) {
+        // every delta at column 0: each sample row's winding is the same
+        // across the strip, so a pixel row's coverage is 4 samples per
+        // sample row inside (no prefix sums)
+        const bool uniform = packed && SC == 4 && !R && __ballot(inner != 0) == 0;
+        if (uniform) {
+          uint32_t w0 = 0;  // lane < 16: sample row `lane`'s column-0 word
+          if (lane < (uint32_t)SR) {
+            w0 = sh.acc[lane][0];
+            sh.acc[lane][0] = 0u;
+          }
+          int32_t lo, hi;
+          unpack_pair(w0, lo, hi);
+          const uint64_t in_rows = __ballot(lane < (uint32_t)SR && wind_in(lo, eo));
+#pragma unroll
+          for (int i = 0; i < ROWS_PER_THREAD; i++)
+            kk[i] = 4 * __popcll((in_rows >> (4 * i)) & 0xFull);
+        } else if (packed && SC == 4) {
           if (eo) {
             if (R) coverage_msaa<true, true>(sh, lane, xs_base, ys_base, R, D, kk);
             else coverage_msaa<true, false>(sh, lane, xs_base, ys_base, R, D, kk);
