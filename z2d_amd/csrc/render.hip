@@ -1945,6 +1945,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         }
         const bool solid = p_kind == Z2D_PATTERN_OPAQUE;
         const int kfull = SC * SC;
+        if (__ballot(need != 0) == 0) continue;  // nothing covered in this strip (uniform)
         if (solid) {
           // the 17 possible masked sources of a solid paint
           if (lane <= 16) {
