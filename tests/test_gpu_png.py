@@ -145,10 +145,15 @@ def test_random_surfaces(fmt, kind):
         assert png_stream(png) == png_scanlines(fmt, w, h, data)
 
 
+SEG = 4096  # stream bytes per deflate segment (PNG_SEG)
+
+
 def test_segment_boundaries():
-    # alpha8 streams of exactly 2 segments (rows of 16384 bytes), one byte
-    # more and one byte less; rgb rows wider than the 32 KiB window
-    for w, h in [(16383, 2), (16383, 1), (16384, 2), (16382, 2), (12000, 4)]:
+    # alpha8 streams that end exactly on a segment (rows of SEG and 4 SEG
+    # bytes with their filter byte), one byte past it and one byte short;
+    # rgb rows wider than the 32 KiB window
+    for w, h in [(SEG - 1, 2), (SEG - 1, 1), (SEG, 2), (SEG - 2, 2), (4 * SEG - 1, 3),
+                 (4 * SEG, 1), (12000, 4)]:
         for kind in ("flat", "blocks", "noise"):
             fmt = abi.ALPHA8 if w != 12000 else abi.RGB
             data = random_surface(fmt, w, h, kind, w + h)

@@ -22,7 +22,7 @@
 
 namespace z2d {
 
-constexpr uint32_t PNG_SEG = 16384;              // stream bytes per deflate segment
+constexpr uint32_t PNG_SEG = 4096;               // stream bytes per deflate segment
 constexpr uint32_t PNG_SEG_CAP = PNG_SEG + 128;  // staging bytes per segment (>= stored size)
 constexpr uint32_t PNG_IDAT = 16384;             // zlib bytes per IDAT chunk
 constexpr uint32_t PNG_MAX_CAND = 8;             // LZ77 candidate distances
