@@ -378,16 +378,34 @@ __global__ void __launch_bounds__(1024) k_png_layout(PngJob J) {
   pb[t] = B;
   pn[t] = N;
   __syncthreads();
-  if (t == 0) {
-    uint64_t run = 0;
-    uint32_t a = 1, b = 0;
-    for (uint32_t u = 0; u < 1024; u++) {
-      const uint64_t v = part[u];
-      part[u] = run;
-      run += v;
-      b = (uint32_t)((b + (uint64_t)pn[u] * a + pb[u]) % kAdlerMod);
-      a = (a + pa[u]) % kAdlerMod;
+  // inclusive scan of the byte counts, and the ordered combination of the
+  // Adler sums (X then Y: A = Ax + Ay, B = Bx + Ny * Ax + By, N = Nx + Ny),
+  // both by doubling steps over the 1024 partials
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    uint64_t v = 0;
+    uint32_t xa = 0, xb = 0, xn = 0;
+    const bool has = t >= d;
+    if (has) {
+      v = part[t - d];
+      xa = pa[t - d];
+      xb = pb[t - d];
+      xn = pn[t - d];
     }
+    __syncthreads();
+    if (has) {
+      part[t] += v;
+      pb[t] = (uint32_t)((xb + (uint64_t)pn[t] * xa + pb[t]) % kAdlerMod);
+      pa[t] = (xa + pa[t]) % kAdlerMod;
+      pn[t] = (xn + pn[t]) % kAdlerMod;
+    }
+    __syncthreads();
+  }
+  if (t == 1023) {
+    const uint64_t run = part[1023];
+    // the Adler state starts at (a, b) = (1, 0): the stream's N bytes add N
+    // times the initial a to b
+    const uint32_t a = (1u + pa[1023]) % kAdlerMod;
+    const uint32_t b = (uint32_t)(((uint64_t)pb[1023] + (J.raw_len % kAdlerMod)) % kAdlerMod);
     const uint64_t zlib = 2 + run + 4;
     const uint64_t nidat = (zlib + PNG_IDAT - 1) / PNG_IDAT;
     const uint64_t file = J.head_len + zlib + 12 * nidat + 12;
@@ -398,7 +416,7 @@ __global__ void __launch_bounds__(1024) k_png_layout(PngJob J) {
     J.meta[3] = file > J.out_cap ? 1 : 0;
   }
   __syncthreads();
-  uint64_t run = part[t];
+  uint64_t run = part[t] - sum;  // exclusive
   for (uint32_t i = i0; i < i1; i++) {
     J.seg_off[i] = run;
     run += J.seg_len[i];
@@ -434,10 +452,11 @@ __global__ void __launch_bounds__(256) k_png_copy(PngJob J) {
   }
 }
 
-// CRC-32 polynomial arithmetic for combining the CRCs of pieces (the
-// GF(2) method published with zlib's crc32_combine): multiply modulo the
-// reflected polynomial, and x^(n * 2^k) by squaring.
-__device__ uint32_t multmodp(uint32_t a, uint32_t b) {
+// CRC-32 polynomial arithmetic for combining the CRCs of pieces (the GF(2)
+// method published with zlib's crc32_combine): multiply modulo the reflected
+// polynomial, and x^(8n) from a table of x^(2^k).  Both tables are built at
+// compile time.
+__host__ __device__ constexpr uint32_t multmodp(uint32_t a, uint32_t b) {
   uint32_t m = 1u << 31, p = 0;
   for (;;) {
     if (a & m) {
@@ -449,12 +468,31 @@ __device__ uint32_t multmodp(uint32_t a, uint32_t b) {
   }
   return p;
 }
-// x^(8n) mod P: x2n[k] holds x^(2^k).
-__device__ uint32_t x8nmodp(const uint32_t *x2n, uint64_t n) {
+struct CrcTables {
+  uint32_t tab[256];  // byte-at-a-time table
+  uint32_t x2n[32];   // x^(2^k) mod P
+  constexpr CrcTables() : tab(), x2n() {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
+      tab[i] = c;
+    }
+    uint32_t p = 1u << 30;  // x^1
+    x2n[0] = p;
+    for (int k = 1; k < 32; k++) {
+      p = multmodp(p, p);
+      x2n[k] = p;
+    }
+  }
+};
+__constant__ CrcTables kCrc = CrcTables();
+
+// x^(8n) mod P: appending n zero bytes to a CRC register multiplies it by this.
+__device__ uint32_t x8nmodp(uint64_t n) {
   uint32_t p = 1u << 31;  // x^0
   uint32_t k = 3;
   while (n) {
-    if (n & 1u) p = multmodp(x2n[k & 31u], p);
+    if (n & 1u) p = multmodp(kCrc.x2n[k & 31u], p);
     n >>= 1;
     k++;
   }
@@ -462,7 +500,15 @@ __device__ uint32_t x8nmodp(const uint32_t *x2n, uint64_t n) {
 }
 
 __device__ uint32_t crc_update(const uint32_t *tab, uint32_t c, const uint8_t *p, uint32_t n) {
-  for (uint32_t i = 0; i < n; i++) c = tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8);
+  uint32_t i = 0;
+  for (; i + 16 <= n; i += 16) {  // 16 independent loads in flight, then the table steps
+    uint8_t b[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) b[k] = p[i + k];
+#pragma unroll
+    for (int k = 0; k < 16; k++) c = tab[(c ^ b[k]) & 0xFFu] ^ (c >> 8);
+  }
+  for (; i < n; i++) c = tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8);
   return c;
 }
 
@@ -484,24 +530,16 @@ __device__ uint32_t put_chunk(const uint32_t *tab, uint8_t *o, const char *type,
   return 12 + n;
 }
 
-// k_png_crc: block b < IDAT count frames chunk b (one wave: 64 pieces, each
-// lane's CRC, combined in order); the last block writes the signature,
-// IHDR, gAMA and IEND (src/export_png.zig:86-145, :422-446).
+// k_png_crc: block b < IDAT count frames chunk b (one wave: each lane's
+// piece CRC register, started at zero, is shifted to the chunk's end and the
+// 64 terms XOR together, the type bytes' register shifted likewise); the last
+// block writes the signature, IHDR, gAMA and IEND (src/export_png.zig:86-145,
+// :422-446).
 __global__ void __launch_bounds__(64) k_png_crc(PngJob J) {
   __shared__ uint32_t tab[256];
-  __shared__ uint32_t x2n[32];
-  __shared__ uint32_t pc[64], plen[64];
   if (J.meta[3]) return;
   const uint32_t lane = threadIdx.x;
-  for (uint32_t i = lane; i < 256; i += 64) {
-    uint32_t c = i;
-    for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
-    tab[i] = c;
-  }
-  if (lane == 0) {
-    x2n[0] = 1u << 30;  // x^1
-    for (int k = 1; k < 32; k++) x2n[k] = multmodp(x2n[k - 1], x2n[k - 1]);
-  }
+  for (uint32_t i = lane; i < 256; i += 64) tab[i] = kCrc.tab[i];
   __syncthreads();
   const uint64_t file = J.meta[0], zlib = J.meta[1];
   const uint64_t nidat = (zlib + PNG_IDAT - 1) / PNG_IDAT;
@@ -536,21 +574,16 @@ __global__ void __launch_bounds__(64) k_png_crc(PngJob J) {
   uint8_t *o = J.out + co;
   const uint32_t P = (len + 63u) / 64u;
   const uint32_t lo = min(lane * P, len), hi = min(lo + P, len);
-  pc[lane] = crc_update(tab, 0xFFFFFFFFu, o + 8 + lo, hi - lo) ^ 0xFFFFFFFFu;
-  plen[lane] = hi - lo;
-  __syncthreads();
+  uint32_t term = 0;
+  if (hi > lo) term = multmodp(x8nmodp(len - hi), crc_update(tab, 0u, o + 8 + lo, hi - lo));
+  for (int off = 32; off > 0; off >>= 1) term ^= __shfl_xor(term, off, 64);
   if (lane == 0) {
     const uint8_t type[4] = {'I', 'D', 'A', 'T'};
-    uint32_t acc = crc_update(tab, 0xFFFFFFFFu, type, 4) ^ 0xFFFFFFFFu;
-    const uint32_t opP = x8nmodp(x2n, P);
-    for (uint32_t u = 0; u < 64; u++) {
-      if (!plen[u]) continue;
-      const uint32_t op = plen[u] == P ? opP : x8nmodp(x2n, plen[u]);
-      acc = multmodp(op, acc) ^ pc[u];
-    }
+    const uint32_t head = crc_update(tab, 0xFFFFFFFFu, type, 4);
+    const uint32_t crc = ~(multmodp(x8nmodp(len), head) ^ term);
     put_be32(o, len);
     for (int i = 0; i < 4; i++) o[4 + i] = type[i];
-    put_be32(o + 8 + len, acc);
+    put_be32(o + 8 + len, crc);
   }
 }
 
