@@ -142,6 +142,10 @@ struct BandEntry {
   uint32_t edge_off;  // into the entry-edge array
   uint32_t edge_cnt;  // edges of the draw whose y-range meets the band
   uint32_t band;
+  // pixel columns of its edges' crossings: a tile wholly beside them has no
+  // coverage ([INT32_MIN, INT32_MAX] for draws where that may still change
+  // pixels)
+  int32_t px_lo, px_hi;
 };
 
 struct TileRef {
@@ -177,6 +181,8 @@ struct JobView {
   uint32_t max_entries;
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place
   uint32_t *entry_cur;       // max_entries: fill cursors (follows entry_ecnt)
+  int32_t *entry_xlo;        // max_entries: sample-column range of the entry's edges
+  int32_t *entry_xhi;
   const uint2 *egroups;      // edge-binning work: draw, first edge slot
   uint32_t n_egroups;
   z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges

@@ -1079,6 +1079,26 @@ __global__ void k_band_pass(JobView J) {
       e.edge_off = off;
       e.edge_cnt = off >= J.max_entry_edges ? 0u : min(cnt, J.max_entry_edges - off);
       e.band = b;
+      // A tile wholly right of every crossing of the entry's edges, or wholly
+      // left of them, sees winding 0 (contours return to their start), so no
+      // coverage; with a bounded operator outside SSAA (which composites its
+      // whole mask box) nothing changes there.  Sample columns to pixels,
+      // rounded outward.
+      e.px_lo = INT32_MIN;
+      e.px_hi = INT32_MAX;
+      const DevDraw &Dd = J.draws[d];
+      if (Dd.mode != MODE_SSAA &&
+          !(Dd.flags & (DRAW_UNBOUNDED | DRAW_UNBALANCED | DRAW_ROWPROG | DRAW_SERIAL_PLOT))) {
+        const int32_t sc = Dd.mode == MODE_DIRECT ? 1 : 4;
+        const int32_t xl = J.entry_xlo[idx], xh = J.entry_xhi[idx];
+        if (xl > xh) {  // no edges in the band
+          e.px_lo = INT32_MAX;
+          e.px_hi = INT32_MIN;
+        } else if (xl != INT32_MIN && xh != INT32_MAX) {
+          e.px_lo = (int32_t)floor((double)xl / sc);
+          e.px_hi = (int32_t)floor((double)xh / sc);
+        }
+      }
       J.entries[idx] = e;
     } else {
       atomicOr(&J.counters[2], 1u);
@@ -1231,9 +1251,12 @@ __global__ void k_strip_pass(JobView J) {
     uint32_t mask = 0, idx = 0;
     if (e + lane < e1) {
       idx = J.band_list[e + lane];
-      const DevDraw &D = J.draws[J.entries[idx].draw];
+      const BandEntry &E = J.entries[idx];
+      const DevDraw &D = J.draws[E.draw];
       const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
-      if (rx1 > tx0 && rx0 < tx0 + TILE_W) {
+      const bool beside = (int64_t)E.px_lo >= (int64_t)tx0 + TILE_W + 1 ||
+                          (int64_t)E.px_hi <= (int64_t)tx0 - 2;  // see k_band_pass
+      if (!beside && rx1 > tx0 && rx0 < tx0 + TILE_W) {
 #pragma unroll
         for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
           const int32_t sy0 = ty0 + s * ROWS_PER_THREAD;
@@ -1320,10 +1343,22 @@ __global__ void k_edge_bins(JobView J) {
     z2d_edge ed{0.0, 0.0, 0.0, 0.0};
     double top = 0.0, bot = 0.0;
     int32_t lo = INT32_MAX, hi = INT32_MIN;  // candidate bands, widened by one
+    int32_t xlo = INT32_MAX, xhi = INT32_MIN;  // the edge's sample columns (fill pass)
     if (valid) {
       ed = edges[k0 + lane];
       top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
       bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
+      if (FILL) {
+        const double xb = ed.x_start + ed.x_inc * (bot - top);
+        const double a = fmin(ed.x_start, xb), c = fmax(ed.x_start, xb);
+        if (a >= -2.0e9 && c <= 2.0e9) {
+          xlo = (int32_t)floor(a);
+          xhi = (int32_t)floor(c);
+        } else {  // huge or non-finite: the whole row
+          xlo = INT32_MIN;
+          xhi = INT32_MAX;
+        }
+      }
       const double flo = fmax(floor(top / band_rows) - 1.0, (double)rb_lo);
       const double fhi = fmin(floor(bot / band_rows) + 1.0, (double)rb_hi);
       if (flo <= fhi) {  // (a NaN edge fails the exact test below)
@@ -1367,6 +1402,18 @@ __global__ void k_edge_bins(JobView J) {
         const uint64_t m = __ballot(hit);
         if (m == 0) continue;
         const uint32_t base = (uint32_t)__shfl((int)at, bnd - w, 64);
+        // the entry's column range, widened by this wave's edges in the band
+        int32_t rlo = hit ? xlo : INT32_MAX, rhi = hit ? xhi : INT32_MIN;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          rlo = min(rlo, __shfl_xor(rlo, off, 64));
+          rhi = max(rhi, __shfl_xor(rhi, off, 64));
+        }
+        if (lane == 0) {
+          const uint32_t eidx = ebase + ((uint32_t)bnd - blo);
+          atomicMin(&J.entry_xlo[eidx], rlo);
+          atomicMax(&J.entry_xhi[eidx], rhi);
+        }
         if (hit) {
           const uint32_t pos = base + (uint32_t)__popcll(m & below);
           if (pos < J.max_entry_edges)
@@ -1705,6 +1752,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       // uniform: the draw's composite region misses this strip
       if (D.rx1 <= tx0 || D.rx0 >= tx0 + TILE_W) continue;
       if (D.ry1 <= ty0 || D.ry0 >= ty0 + ROWS_PER_THREAD) continue;
+
       const DevPaint &P = J.paints[D.paint];
       const uint32_t p_kind = load_uniform(&P.kind);
       const I4 p_px = load_uniform(&P.px);
@@ -2302,6 +2350,14 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   const uint64_t ne2 = 2ull * J.max_entries + 1;  // entry_ecnt and entry_cur
   hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((ne2 + 255) / 256)), dim3(256), 0, st,
                      J.entry_ecnt, ne2, 0u);
+  // entry column ranges: empty until k_edge_bins<true> widens them
+  const uint64_t nx = J.max_entries;
+  if (nx) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nx + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<uint32_t *>(J.entry_xlo), nx, 0x7FFFFFFFu);
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nx + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<uint32_t *>(J.entry_xhi), nx, 0x80000000u);
+  }
   dim3 ggrid((J.n_egroups + 3) / 4);
   if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<false>, ggrid, dim3(256), 0, st, J);
   scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
