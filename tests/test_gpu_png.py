@@ -11,7 +11,8 @@ the oracle.
 - Random surfaces of every format and ragged sizes (1x1, rows not a whole
   byte, streams that end exactly on a deflate segment or one byte past it,
   rows longer than the 32 KiB window) against the oracle's stream; noise
-  (stored blocks) and flat content (fixed-Huffman matches).
+  (stored blocks), flat content (matches) and mixed content (blocks with
+  codes built per segment).
 - argb/rgba and xrgb/rgb surfaces of the same picture give identical files
   (the reference's own test, src/export_png.zig:448-491).
 """

@@ -6,9 +6,10 @@
 //                  the bytes the reference feeds its zlib compressor
 //   k_png_deflate  one wave per PNG_SEG-byte segment of the stream: greedy
 //                  LZ77 over a few candidate distances (pixel and row
-//                  neighbours) coded with the fixed Huffman table, or a
-//                  stored block when that is smaller; each segment ends on a
-//                  byte (empty stored block) so segments concatenate
+//                  neighbours) coded with Huffman codes built for the
+//                  segment, the fixed codes, or a stored block, whichever is
+//                  smallest; each segment ends on a byte (empty stored block)
+//                  so segments concatenate
 //   k_png_layout   segment offsets (exclusive scan), Adler-32 of the stream
 //                  from per-segment sums, zlib and file lengths
 //   k_png_copy     segments into the file, framed as IDAT chunks of
@@ -23,7 +24,7 @@
 namespace z2d {
 
 constexpr uint32_t PNG_SEG = 4096;               // stream bytes per deflate segment
-constexpr uint32_t PNG_SEG_CAP = PNG_SEG + 128;  // staging bytes per segment (>= stored size)
+constexpr uint32_t PNG_SEG_CAP = 5504;  // staging bytes per segment: >= 4 * (PNG_SEG / 3 + 2) token words, >= stored size
 constexpr uint32_t PNG_IDAT = 16384;             // zlib bytes per IDAT chunk
 constexpr uint32_t PNG_MAX_CAND = 8;             // LZ77 candidate distances
 constexpr uint32_t PNG_RAW_PAD = PNG_SEG + 256;  // readable bytes past the stream's end

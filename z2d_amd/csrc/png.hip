@@ -100,13 +100,16 @@ __global__ void __launch_bounds__(256) k_png_rows(PngJob J) {
 // Matching: for each candidate distance d a bitmap marks the positions whose
 // byte differs from the byte d earlier (positions past the segment end, and
 // those with no byte d earlier, count as differing).  A position where some
-// candidate repeats three bytes is "good".  The wave then parses greedily:
-// literal runs up to the next good position are coded 64 bytes at a time
-// (one lane each, offsets by a wave prefix sum); at a good position each
-// candidate lane measures its run from its bitmap and the longest (earliest
-// candidate on ties) becomes a match of up to 258 bytes.  Codes go into an
-// LDS bit buffer with atomic ORs.  The fixed-Huffman form is dropped for a
-// stored block as soon as it grows past the stored size.
+// candidate repeats three bytes is "good".  The wave parses greedily: literal
+// runs up to the next good position, and at a good position each candidate
+// lane measures its run from its bitmap; the longest (earliest candidate on
+// ties) becomes a match of up to 258 bytes.  The parse is kept as match
+// tokens and symbol counts; the block is then coded with Huffman codes built
+// for the segment (RFC 1951 3.2.7: a bitonic sort of the counts, one lane's
+// two-queue merge, lengths up to 15 / 7 bits, else the fixed codes), with
+// the fixed codes, or stored, whichever is smallest.  Codes go into an LDS
+// bit buffer with atomic ORs; literal runs are coded 64 bytes at a time (one
+// lane each, offsets by a wave prefix sum).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void put_bits(uint32_t *w, uint32_t pos, uint32_t v, uint32_t nb) {
   const uint32_t k = pos >> 5, sh = pos & 31u;
@@ -114,68 +117,268 @@ __device__ __forceinline__ void put_bits(uint32_t *w, uint32_t pos, uint32_t v, 
   if (sh + nb > 32u) atomicOr(&w[k + 1], v >> (32u - sh));
 }
 
-// Fixed literal/length code (RFC 1951 3.2.6), bit-reversed for LSB-first
-// packing; returns the bit count.
-__device__ __forceinline__ uint32_t lit_code(uint32_t b, uint32_t &code) {
-  if (b < 144u) {
-    code = __builtin_bitreverse32(0x30u + b) >> 24;
-    return 8u;
-  }
-  code = __builtin_bitreverse32(0x190u + b - 144u) >> 23;
-  return 9u;
+// Length symbol (257..285) of a match length 3..258 and its extra bits.
+__device__ __forceinline__ uint32_t len_sym(uint32_t len, uint32_t &ext, uint32_t &val) {
+  ext = 0;
+  val = 0;
+  if (len == 258u) return 285u;
+  const uint32_t v = len - 3u;
+  if (v < 8u) return 257u + v;
+  const uint32_t e = 31u - __builtin_clz(v) - 2u;
+  ext = e;
+  val = v & ((1u << e) - 1u);
+  return 257u + 4u * (e + 1u) + ((v >> e) & 3u);
+}
+// Distance symbol (0..29) of a distance 1..32768 and its extra bits.
+__device__ __forceinline__ uint32_t dist_sym(uint32_t dist, uint32_t &ext, uint32_t &val) {
+  ext = 0;
+  val = 0;
+  const uint32_t dv = dist - 1u;
+  if (dv < 4u) return dv;
+  const uint32_t e = 31u - __builtin_clz(dv) - 1u;
+  ext = e;
+  val = dv & ((1u << e) - 1u);
+  return 2u * (e + 1u) + ((dv >> e) & 1u);
 }
 
-// A match of `len` (3..258) at distance `dist` (1..32768) as one bit string.
-__device__ __forceinline__ uint32_t match_code(uint32_t len, uint32_t dist, uint32_t &bits) {
-  uint32_t sym, lext = 0, lval = 0;
-  if (len == 258u) {
-    sym = 285u;
-  } else {
-    const uint32_t v = len - 3u;
-    if (v < 8u) {
-      sym = 257u + v;
-    } else {
-      const uint32_t e = 31u - __builtin_clz(v) - 2u;
-      sym = 257u + 4u * (e + 1u) + ((v >> e) & 3u);
-      lext = e;
-      lval = v & ((1u << e) - 1u);
+constexpr uint32_t kMaxTok = PNG_SEG / 3 + 2;  // matches in a segment
+static_assert(4 * kMaxTok <= PNG_SEG_CAP, "token words must fit the segment's staging bytes");
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+struct DeflateShared {
+  union {
+    struct {  // matching, until the parse ends
+      uint32_t mm[PNG_MAX_CAND][kMatchWords];
+      uint32_t good[kMatchWords];
+    };
+    struct {  // code construction, after it
+      uint32_t key[512];  // sort keys: count << 9 | symbol
+      uint16_t hw[576];   // two-queue weights (a segment's counts sum to <= PNG_SEG + 1)
+      uint16_t hpar[576];
+      uint8_t hdep[576];
+    };
+    uint8_t cl[320];           // literal/length then distance code lengths, for the run coding
+    uint32_t outw[kOutWords];  // the coded block, last
+  };
+  uint32_t lfreq[288], dfreq[32], cfreq[20];
+  uint32_t lcode[288], dcode[32], ccode[20];  // bit-reversed code | length << 16
+  uint8_t llen[288], dlen[32], clen[20];
+  uint32_t rle[320];  // code-length symbols: symbol | extra << 8 | extra bits << 16
+  uint32_t litw[kSegWords], mstw[kSegWords];  // literal positions, match starts
+  uint32_t n_rle;
+};
+
+// Ascending bitonic sort of 64 * R keys held in registers, element
+// lane * R + r in v[r]; strides below R swap within a lane, larger ones
+// exchange with lane ^ (stride / R).
+template <int R>
+__device__ __forceinline__ void wave_sort(uint32_t (&v)[R], uint32_t lane) {
+#pragma unroll
+  for (uint32_t size = 2; size <= 64u * R; size <<= 1) {
+#pragma unroll
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride < (uint32_t)R) {
+#pragma unroll
+        for (uint32_t r = 0; r < (uint32_t)R; r++) {
+          if (r & stride) continue;
+          const uint32_t e = lane * R + r;
+          const bool up = (e & size) == 0;
+          const uint32_t a = v[r], b = v[r ^ stride];
+          const bool sw = (a > b) == up;
+          v[r] = sw ? b : a;
+          v[r ^ stride] = sw ? a : b;
+        }
+      } else {
+#pragma unroll
+        for (uint32_t r = 0; r < (uint32_t)R; r++) {
+          const uint32_t e = lane * R + r;
+          const bool up = (e & size) == 0, low = (e & stride) == 0;
+          const uint32_t a = v[r], b = (uint32_t)__shfl_xor((int)a, (int)(stride / R), 64);
+          v[r] = (low == up) ? min(a, b) : max(a, b);
+        }
+      }
     }
   }
-  uint32_t lcode, lnb;
-  if (sym <= 279u) {
-    lcode = __builtin_bitreverse32(sym - 256u) >> 25;
-    lnb = 7u;
-  } else {
-    lcode = __builtin_bitreverse32(0xC0u + sym - 280u) >> 24;
-    lnb = 8u;
-  }
-  uint32_t dc, dext = 0, dval = 0;
-  const uint32_t dv = dist - 1u;
-  if (dv < 4u) {
-    dc = dv;
-  } else {
-    const uint32_t e = 31u - __builtin_clz(dv) - 1u;
-    dc = 2u * (e + 1u) + ((dv >> e) & 1u);
-    dext = e;
-    dval = dv & ((1u << e) - 1u);
-  }
-  const uint32_t dcode = __builtin_bitreverse32(dc) >> 27;
-  uint32_t v = lcode;
-  uint32_t n = lnb;
-  v |= lval << n;
-  n += lext;
-  v |= dcode << n;
-  n += 5u;
-  v |= dval << n;
-  n += dext;
-  bits = n;  // <= 8 + 5 + 5 + 13 = 31
-  return v;
 }
 
-__global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
-  __shared__ uint32_t mm[PNG_MAX_CAND][kMatchWords];
-  __shared__ uint32_t good[kMatchWords];
-  __shared__ uint32_t outw[kOutWords];
+// Code lengths for `nsym` counts (64 * R >= nsym), limited to `maxlen`;
+// returns false (lengths unusable) when a code would be longer.  Keys
+// count << 9 | symbol are sorted on the wave, one lane merges the two queues
+// (leaves, then internal nodes in order of creation), and depths follow by
+// pointer jumping up the parent links.
+template <int R>
+__device__ bool huff_lengths(DeflateShared &S, const uint32_t *freq, uint32_t nsym,
+                             uint32_t maxlen, uint8_t *lens, uint32_t lane) {
+  uint32_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t i = lane * R + r;
+    v[r] = (i < nsym && freq[i]) ? (freq[i] << 9 | i) : kNoKey;
+  }
+  wave_sort<R>(v, lane);
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    S.key[lane * R + r] = v[r];
+    cnt += v[r] != kNoKey;
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  const uint32_t m = cnt;
+  for (uint32_t i = lane; i < nsym; i += 64) lens[i] = 0;
+  __syncthreads();
+  if (m == 0) return true;
+  if (m == 1) {
+    if (lane == 0) lens[S.key[0] & 511u] = 1;
+    __syncthreads();
+    return true;
+  }
+  const uint32_t root = 2 * m - 2;
+  for (uint32_t k = lane; k < m; k += 64) S.hw[k] = (uint16_t)(S.key[k] >> 9);
+  __syncthreads();
+  if (lane == 0) {
+    uint32_t i1 = 0, i2 = m, nxt = m;
+    uint32_t w1 = S.hw[0], w2 = 0;
+    while (nxt <= root) {
+      uint32_t sum = 0;
+      for (int t = 0; t < 2; t++) {
+        uint32_t pick;
+        if (i1 < m && (i2 >= nxt || w1 <= w2)) {
+          pick = i1++;
+          sum += w1;
+          w1 = i1 < m ? S.hw[i1] : 0u;
+        } else {
+          pick = i2++;
+          sum += w2;
+          w2 = i2 < nxt ? S.hw[i2] : 0u;
+        }
+        S.hpar[pick] = (uint16_t)nxt;
+      }
+      S.hw[nxt] = (uint16_t)sum;
+      if (i2 == nxt) w2 = sum;  // the internal queue was empty
+      nxt++;
+    }
+    S.hpar[root] = (uint16_t)root;
+  }
+  __syncthreads();
+  // depth by pointer jumping: after j rounds hpar[k] is the 2^j-th ancestor
+  // (or the root) and hdep[k] the distance to it; five rounds settle every
+  // depth up to 32, and a node not yet at the root is deeper than that
+  constexpr int NR = 9;  // nodes per lane (2 * 286 - 1 <= 64 * 9)
+#pragma unroll
+  for (int j = 0; j < NR; j++) {
+    const uint32_t k = lane + 64u * j;
+    if (k <= root) S.hdep[k] = k == root ? 0 : 1;
+  }
+  __syncthreads();
+  for (int round = 0; round < 5; round++) {
+    uint32_t na[NR], nd[NR];
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t k = lane + 64u * j;
+      if (k <= root) {
+        const uint32_t a = S.hpar[k];
+        na[j] = S.hpar[a];
+        nd[j] = (uint32_t)S.hdep[k] + S.hdep[a];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+      const uint32_t k = lane + 64u * j;
+      if (k <= root) {
+        S.hpar[k] = (uint16_t)na[j];
+        S.hdep[k] = (uint8_t)min(nd[j], 255u);
+      }
+    }
+    __syncthreads();
+  }
+  bool ok = true;
+  for (uint32_t k = lane; k < m; k += 64) {
+    const uint32_t d = S.hdep[k];
+    if (S.hpar[k] != root || d > maxlen) ok = false;
+    lens[S.key[k] & 511u] = (uint8_t)d;
+  }
+  __syncthreads();
+  return __ballot(!ok) == 0;
+}
+
+// Canonical codes from lengths (RFC 1951 3.2.2), bit-reversed for LSB-first
+// output, packed as code | length << 16: a symbol's code is the first code of
+// its length plus the number of symbols of that length before it.
+__device__ void huff_codes(const uint8_t *lens, uint32_t nsym, uint32_t *codes, uint32_t lane) {
+  uint32_t bl[16];
+#pragma unroll
+  for (int L = 0; L < 16; L++) bl[L] = 0;
+  for (uint32_t c0 = 0; c0 < nsym; c0 += 64) {
+    const uint32_t i = c0 + lane;
+    const uint32_t l = i < nsym ? lens[i] : 0u;
+#pragma unroll
+    for (int L = 1; L < 16; L++) bl[L] += (uint32_t)__popcll(__ballot(l == (uint32_t)L));
+  }
+  uint32_t next[16];
+  uint32_t code = 0;
+  next[0] = 0;
+#pragma unroll
+  for (int L = 1; L < 16; L++) {
+    code = (code + (L > 1 ? bl[L - 1] : 0u)) << 1;
+    next[L] = code;
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t c0 = 0; c0 < nsym; c0 += 64) {
+    const uint32_t i = c0 + lane;
+    const uint32_t l = i < nsym ? lens[i] : 0u;
+    uint32_t out = 0;
+#pragma unroll
+    for (int L = 1; L < 16; L++) {
+      const uint64_t msk = __ballot(l == (uint32_t)L);
+      if (l == (uint32_t)L) {
+        const uint32_t c = next[L] + (uint32_t)__popcll(msk & below);
+        out = (__builtin_bitreverse32(c) >> (32 - L)) | ((uint32_t)L << 16);
+      }
+      next[L] += (uint32_t)__popcll(msk);
+    }
+    if (i < nsym) codes[i] = out;
+  }
+  __syncthreads();
+}
+
+// Code-length symbols for a run of L lengths v (RFC 1951 3.2.7: 16 repeats
+// the previous length 3-6 times, 17 / 18 give 3-10 / 11-138 zeros), as
+// symbol | extra << 8 | extra bits << 16 from out[k]; returns their count.
+// With out null it only counts.
+__device__ uint32_t rle_run(uint32_t v, uint32_t L, uint32_t *out, uint32_t k) {
+  uint32_t c = 0;
+  auto put = [&](uint32_t x) {
+    if (out) out[k + c] = x;
+    c++;
+  };
+  if (v == 0) {
+    while (L >= 11) {
+      const uint32_t r = min(L, 138u);
+      put(18u | ((r - 11u) << 8) | (7u << 16));
+      L -= r;
+    }
+    if (L >= 3) {
+      put(17u | ((L - 3u) << 8) | (3u << 16));
+      L = 0;
+    }
+    for (; L > 0; L--) put(0u);
+  } else {
+    put(v);
+    L--;
+    while (L >= 3) {
+      const uint32_t r = min(L, 6u);
+      put(16u | ((r - 3u) << 8) | (2u << 16));
+      L -= r;
+    }
+    for (; L > 0; L--) put(v);
+  }
+  return c;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_png_deflate(PngJob J) {
+  __shared__ DeflateShared S;
   const uint32_t lane = threadIdx.x;
   const uint32_t s = blockIdx.x;
   const uint64_t base = (uint64_t)s * PNG_SEG;
@@ -183,7 +386,17 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
   const bool final_seg = s + 1 == J.n_seg;
   const uint32_t nw = (n >> 5) + 2;  // bitmap words written (<= kMatchWords)
   const uint32_t nc = J.n_cand;
-  for (uint32_t w = lane; w < kOutWords; w += 64) outw[w] = 0;
+  uint8_t *dst = J.seg + (uint64_t)s * PNG_SEG_CAP;
+  // match tokens (pos << 11 | (len - 3) << 3 | candidate) wait in the
+  // segment's staging bytes until the block is coded over them
+  uint32_t *tok = reinterpret_cast<uint32_t *>(dst);
+  for (uint32_t i = lane; i < 288; i += 64) S.lfreq[i] = i == 256 ? 1u : 0u;  // one end of block
+  for (uint32_t w = lane; w < kSegWords; w += 64) {
+    S.litw[w] = 0;
+    S.mstw[w] = 0;
+  }
+  if (lane < 32) S.dfreq[lane] = 0;
+  if (lane < 20) S.cfreq[lane] = 0;
 
   // bitmaps and the Adler-32 partial sums of the segment
   uint64_t s1 = 0, s2 = 0;
@@ -226,7 +439,7 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
           if (gp < d || J.raw[gp - d] != b) m |= 1u << k;
         }
       }
-      mm[c][w] = m | ~valid;
+      S.mm[c][w] = m | ~valid;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -241,25 +454,21 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
   for (uint32_t w = lane; w + 1 < nw; w += 64) {
     uint32_t gm = 0;
     for (uint32_t c = 0; c < nc; c++) {
-      const uint32_t m = mm[c][w], m1 = mm[c][w + 1];
+      const uint32_t m = S.mm[c][w], m1 = S.mm[c][w + 1];
       gm |= ~(m | ((m >> 1) | (m1 << 31)) | ((m >> 2) | (m1 << 30)));
     }
-    good[w] = gm;
+    S.good[w] = gm;
   }
   __syncthreads();
 
-  // greedy parse and fixed-Huffman coding
-  const uint32_t limit = (n + 5u) * 8u;  // bits of the stored form
+  // greedy parse into match tokens and the literal / match-start bitmaps
   const uint32_t ngw = (n + 31u) >> 5;
-  if (lane == 0) outw[0] = (final_seg ? 1u : 0u) | (1u << 1);  // BFINAL, BTYPE=01
-  uint32_t bp = 3;
-  uint32_t p = 0;
-  bool over = false;
+  uint32_t p = 0, nt = 0;
   while (p < n) {
     uint32_t q = n;
     for (uint32_t wb = p >> 5; wb < ngw; wb += 64) {
       const uint32_t wi = wb + lane;
-      uint32_t g = wi < ngw ? good[wi] : 0u;
+      uint32_t g = wi < ngw ? S.good[wi] : 0u;
       if (wi == (p >> 5)) g &= ~0u << (p & 31u);
       const uint64_t bal = __ballot(g != 0u);
       if (bal) {
@@ -270,32 +479,18 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
       }
     }
     if (q > p) {  // literal run [p, q)
-      for (uint32_t i0 = p; i0 < q; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        uint32_t code = 0, nb = 0;
-        if (i < q) nb = lit_code(J.raw[base + i], code);
-        uint32_t x = nb;
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t y = __shfl_up(x, off, 64);
-          if (lane >= (uint32_t)off) x += y;
-        }
-        const uint32_t tot = __shfl(x, 63, 64);
-        if (nb) put_bits(outw, bp + x - nb, code, nb);
-        bp += tot;
-        if (bp > limit) {
-          over = true;
-          break;
-        }
+      for (uint32_t wi = (p >> 5) + lane; wi <= ((q - 1u) >> 5); wi += 64) {
+        const uint32_t lo = max(p, wi * 32u) - wi * 32u, hi = min(q, wi * 32u + 32u) - wi * 32u;
+        atomicOr(&S.litw[wi], (hi >= 32u ? ~0u : ((1u << hi) - 1u)) & (~0u << lo));
       }
-      if (over) break;
       p = q;
       continue;
     }
     uint32_t len = 0;
     if (lane < nc) {
       uint32_t w = p >> 5;
-      uint32_t m = mm[lane][w] & (~0u << (p & 31u));
-      while (m == 0u && w * 32u + 32u - p < 258u) m = mm[lane][++w];
+      uint32_t m = S.mm[lane][w] & (~0u << (p & 31u));
+      while (m == 0u && w * 32u + 32u - p < 258u) m = S.mm[lane][++w];
       const uint32_t e = m ? w * 32u + (uint32_t)__builtin_ctz(m) : w * 32u + 32u;
       len = min(e - p, 258u);
     }
@@ -303,43 +498,149 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
     for (int off = 32; off > 0; off >>= 1) key = max(key, (uint32_t)__shfl_xor(key, off, 64));
     const uint32_t blen = key >> 4, c = 15u - (key & 15u);
     if (blen < 3u) {  // not reached for a good position; kept as a literal
-      uint32_t code;
-      const uint32_t nb = lit_code(J.raw[base + p], code);
-      if (lane == 0) put_bits(outw, bp, code, nb);
-      bp += nb;
+      if (lane == 0) atomicOr(&S.litw[p >> 5], 1u << (p & 31u));
       p += 1;
-    } else {
-      uint32_t nb;
-      const uint32_t v = match_code(blen, J.cand[c], nb);
-      if (lane == 0) put_bits(outw, bp, v, nb);
-      bp += nb;
-      p += blen;
+      continue;
     }
-    if (bp > limit) {
-      over = true;
-      break;
+    if (lane == 0) {
+      tok[nt] = (p << 11) | ((blen - 3u) << 3) | c;
+      atomicOr(&S.mstw[p >> 5], 1u << (p & 31u));
     }
-  }
-  if (!over) {
-    bp += 7;  // end of block (code 0000000)
-    if (!final_seg) {
-      bp += 3;  // empty stored block: BFINAL=0, BTYPE=00, then LEN=0, NLEN=0xFFFF on a byte
-      bp = (bp + 7u) & ~7u;
-      if (lane == 0) put_bits(outw, bp, 0xFFFF0000u, 32);
-      bp += 32;
-    } else {
-      bp = (bp + 7u) & ~7u;
-    }
-    if (bp / 8u >= n + 5u) over = true;
+    nt++;
+    p += blen;
   }
   __syncthreads();
-  uint8_t *dst = J.seg + (uint64_t)s * PNG_SEG_CAP;
-  if (!over) {
-    const uint32_t nbytes = bp / 8u;
-    uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-    for (uint32_t w = lane; w < (nbytes + 3u) / 4u; w += 64) d32[w] = outw[w];
-    if (lane == 0) J.seg_len[s] = nbytes;
-  } else {
+
+  // From here each lane owns the 64 positions [64 lane, 64 lane + 64): its
+  // literals (bytes as 16 words), its match starts (tokens tb.. in order).
+  const uint64_t LB = S.litw[2 * lane] | (uint64_t)S.litw[2 * lane + 1] << 32;
+  const uint64_t MB = S.mstw[2 * lane] | (uint64_t)S.mstw[2 * lane + 1] << 32;
+  const uint32_t *r32 = reinterpret_cast<const uint32_t *>(J.raw + base) + lane * 16u;
+  uint32_t tb = (uint32_t)__popcll(MB);
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(tb, off, 64);
+    if (lane >= (uint32_t)off) tb += y;
+  }
+  tb -= (uint32_t)__popcll(MB);
+  // symbol counts and extra bits
+  uint32_t extra = 0;
+  {
+    uint32_t rw[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) rw[j] = r32[j];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t lb = (uint32_t)(LB >> (4 * j)) & 15u;
+      for (uint32_t b = 0; b < 4; b++)
+        if (lb >> b & 1u) atomicAdd(&S.lfreq[(rw[j] >> (8u * b)) & 0xFFu], 1u);
+    }
+    uint64_t m = MB;
+    for (uint32_t t = tb; m; m &= m - 1ull, t++) {
+      const uint32_t tk = tok[t];
+      uint32_t le, lv, de, dv;
+      const uint32_t ls = len_sym(((tk >> 3) & 0xFFu) + 3u, le, lv);
+      const uint32_t ds = dist_sym(J.cand[tk & 7u], de, dv);
+      atomicAdd(&S.lfreq[ls], 1u);
+      atomicAdd(&S.dfreq[ds], 1u);
+      extra += le + de;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) extra += __shfl_xor(extra, off, 64);
+  __syncthreads();
+
+  // block sizes: fixed codes, the segment's own codes, stored
+  uint64_t fixed_bits = 0;
+  for (uint32_t i = lane; i < 286; i += 64) {
+    const uint32_t fl = i < 144 ? 8u : i < 256 ? 9u : i < 280 ? 7u : 8u;
+    fixed_bits += (uint64_t)S.lfreq[i] * fl;
+  }
+  if (lane < 30) fixed_bits += (uint64_t)S.dfreq[lane] * 5u;
+  for (int off = 32; off > 0; off >>= 1) fixed_bits += __shfl_xor(fixed_bits, off, 64);
+  fixed_bits += 3 + extra;
+  uint64_t dyn_bits = ~0ull;
+  uint32_t hlit = 257, hdist = 1, hclen = 4;
+  const bool ok_l = huff_lengths<8>(S, S.lfreq, 286, 15, S.llen, lane);
+  const bool ok_d = huff_lengths<1>(S, S.dfreq, 30, 15, S.dlen, lane);
+  if (ok_l && ok_d) {
+    for (uint32_t i = lane; i < 286; i += 64)
+      if (S.llen[i]) hlit = max(hlit, i + 1);
+    if (lane < 30 && S.dlen[lane]) hdist = lane + 1;
+    for (int off = 32; off > 0; off >>= 1) {
+      hlit = max(hlit, (uint32_t)__shfl_xor((int)hlit, off, 64));
+      hdist = max(hdist, (uint32_t)__shfl_xor((int)hdist, off, 64));
+    }
+    // the code lengths, run-length coded: runs of equal lengths over the
+    // literal/length then distance lengths, each run coded by the lane that
+    // holds its first position, in position order (a prefix sum per chunk)
+    const uint32_t total = hlit + hdist;
+    for (uint32_t i = lane; i < 320; i += 64)
+      S.cl[i] = (uint8_t)(i < hlit ? S.llen[i] : i < total ? S.dlen[i - hlit] : 0xFFu);
+    __syncthreads();
+    uint64_t st[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const uint32_t i = 64u * c + lane;
+      st[c] = __ballot(i < total && (i == 0 || S.cl[i] != S.cl[i - 1]));
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+      if (total >> 6 == (uint32_t)c) st[c] |= 1ull << (total & 63u);  // end of the last run
+    uint32_t nr = 0;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const uint32_t i = 64u * c + lane;
+      uint32_t L = 0;
+      if (st[c] >> lane & 1ull && i < total) {
+        uint32_t nxt = 0;
+        bool found = false;
+#pragma unroll
+        for (int c2 = c; c2 < 5; c2++) {
+          const uint64_t mk = c2 == c ? (lane == 63 ? 0ull : st[c2] & (~0ull << (lane + 1))) : st[c2];
+          if (!found && mk) {
+            nxt = 64u * c2 + (uint32_t)__builtin_ctzll(mk);
+            found = true;
+          }
+        }
+        L = nxt - i;
+      }
+      const uint32_t v = i < 320 ? S.cl[i] : 0u;
+      const uint32_t cnt = L ? rle_run(v, L, nullptr, 0) : 0u;
+      uint32_t x = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+      }
+      if (L) rle_run(v, L, S.rle, nr + x - cnt);
+      nr += (uint32_t)__shfl((int)x, 63, 64);
+    }
+    if (lane == 0) S.n_rle = nr;
+    __syncthreads();
+    for (uint32_t k = lane; k < S.n_rle; k += 64) atomicAdd(&S.cfreq[S.rle[k] & 0xFFu], 1u);
+    __syncthreads();
+    if (huff_lengths<1>(S, S.cfreq, 19, 7, S.clen, lane)) {
+      constexpr uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+      const uint32_t used = lane < 19 && S.clen[order[lane]] ? lane + 1 : 0u;
+      hclen = max(4u, used);
+      uint64_t bits = 0;
+      for (uint32_t k = lane; k < S.n_rle; k += 64)
+        bits += S.clen[S.rle[k] & 0xFFu] + ((S.rle[k] >> 16) & 0xFFu);
+      for (uint32_t i = lane; i < 286; i += 64) bits += (uint64_t)S.lfreq[i] * S.llen[i];
+      if (lane < 30) bits += (uint64_t)S.dfreq[lane] * S.dlen[lane];
+      for (int off = 32; off > 0; off >>= 1) {
+        bits += __shfl_xor(bits, off, 64);
+        hclen = max(hclen, (uint32_t)__shfl_xor((int)hclen, off, 64));
+      }
+      dyn_bits = bits + 3 + 14 + 3ull * hclen + extra;
+    }
+  }
+  // 0 stored, 1 fixed, 2 own codes; the sync flush or final padding is the
+  // same for the coded forms
+  uint32_t mode = fixed_bits <= dyn_bits ? 1u : 2u;
+  const uint64_t coded = mode == 1u ? fixed_bits : dyn_bits;
+  const uint64_t coded_bytes = (coded + 7 + (final_seg ? 0 : 3 + 7 + 32)) / 8;
+  if (coded_bytes >= (uint64_t)n + 5u) mode = 0;
+
+  if (mode == 0) {
     if (lane == 0) {
       dst[0] = final_seg ? 1 : 0;  // BFINAL, BTYPE=00
       dst[1] = (uint8_t)(n & 0xFFu);
@@ -349,7 +650,117 @@ __global__ void __launch_bounds__(64) k_png_deflate(PngJob J) {
       J.seg_len[s] = n + 5u;
     }
     for (uint32_t i = lane; i < n; i += 64) dst[5 + i] = J.raw[base + i];
+    return;
   }
+  for (uint32_t w = lane; w < kOutWords; w += 64) S.outw[w] = 0;
+  __syncthreads();
+  // code tables
+  if (mode == 1u) {
+    for (uint32_t i = lane; i < 288; i += 64)
+      S.llen[i] = (uint8_t)(i < 144 ? 8 : i < 256 ? 9 : i < 280 ? 7 : 8);
+    if (lane < 30) S.dlen[lane] = 5;
+    __syncthreads();
+    huff_codes(S.llen, 288, S.lcode, lane);
+    huff_codes(S.dlen, 30, S.dcode, lane);
+  } else {
+    huff_codes(S.llen, 286, S.lcode, lane);
+    huff_codes(S.dlen, 30, S.dcode, lane);
+    huff_codes(S.clen, 19, S.ccode, lane);
+  }
+  uint32_t bp = 3;
+  if (lane == 0) put_bits(S.outw, 0, (final_seg ? 1u : 0u) | (mode << 1), 3);
+  if (mode == 2u) {
+    constexpr uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+    if (lane == 0) put_bits(S.outw, bp, (hlit - 257u) | (hdist - 1u) << 5 | (hclen - 4u) << 10, 14);
+    if (lane < hclen) put_bits(S.outw, bp + 14 + 3 * lane, S.clen[order[lane]], 3);
+    bp += 14 + 3 * hclen;
+    for (uint32_t k0 = 0; k0 < S.n_rle; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      uint32_t r = 0, cc = 0, nb = 0;
+      if (k < S.n_rle) {
+        r = S.rle[k];
+        cc = S.ccode[r & 0xFFu];
+        nb = (cc >> 16) + ((r >> 16) & 0xFFu);
+      }
+      uint32_t x = nb;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+      }
+      if (nb) put_bits(S.outw, bp + x - nb, (cc & 0xFFFFu) | (((r >> 8) & 0xFFu) << (cc >> 16)), nb);
+      bp += (uint32_t)__shfl((int)x, 63, 64);
+    }
+  }
+  // each lane's bits, their offsets (a wave prefix sum), then each lane
+  // codes its positions in order
+  {
+    uint32_t rw[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) rw[j] = r32[j];
+    uint32_t nb = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t lb = (uint32_t)(LB >> (4 * j)) & 15u;
+      for (uint32_t b = 0; b < 4; b++)
+        if (lb >> b & 1u) nb += S.lcode[(rw[j] >> (8u * b)) & 0xFFu] >> 16;
+    }
+    uint64_t m = MB;
+    for (uint32_t t = tb; m; m &= m - 1ull, t++) {
+      const uint32_t tk = tok[t];
+      uint32_t le, lv, de, dv;
+      const uint32_t ls = len_sym(((tk >> 3) & 0xFFu) + 3u, le, lv);
+      const uint32_t ds = dist_sym(J.cand[tk & 7u], de, dv);
+      nb += (S.lcode[ls] >> 16) + le + (S.dcode[ds] >> 16) + de;
+    }
+    uint32_t x = nb;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (lane >= (uint32_t)off) x += y;
+    }
+    uint32_t b = bp + x - nb;
+    bp += (uint32_t)__shfl((int)x, 63, 64);
+    uint32_t t = tb;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t lb = (uint32_t)(LB >> (4 * j)) & 15u, mb = (uint32_t)(MB >> (4 * j)) & 15u;
+      for (uint32_t k = 0; k < 4; k++) {
+        if (lb >> k & 1u) {
+          const uint32_t lc = S.lcode[(rw[j] >> (8u * k)) & 0xFFu];
+          put_bits(S.outw, b, lc & 0xFFFFu, lc >> 16);
+          b += lc >> 16;
+        } else if (mb >> k & 1u) {
+          const uint32_t tk = tok[t++];
+          uint32_t le, lv, de, dv;
+          const uint32_t ls = len_sym(((tk >> 3) & 0xFFu) + 3u, le, lv);
+          const uint32_t ds = dist_sym(J.cand[tk & 7u], de, dv);
+          const uint32_t lc = S.lcode[ls], dc = S.dcode[ds];
+          const uint32_t l1 = lc >> 16, d1 = dc >> 16;
+          put_bits(S.outw, b, (lc & 0xFFFFu) | (lv << l1), l1 + le);
+          put_bits(S.outw, b + l1 + le, (dc & 0xFFFFu) | (dv << d1), d1 + de);
+          b += l1 + le + d1 + de;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const uint32_t ec = S.lcode[256];
+    if (lane == 0) put_bits(S.outw, bp, ec & 0xFFFFu, ec >> 16);
+    bp += ec >> 16;
+  }
+  if (!final_seg) {
+    bp += 3;  // empty stored block: BFINAL=0, BTYPE=00, then LEN=0, NLEN=0xFFFF on a byte
+    bp = (bp + 7u) & ~7u;
+    if (lane == 0) put_bits(S.outw, bp, 0xFFFF0000u, 32);
+    bp += 32;
+  } else {
+    bp = (bp + 7u) & ~7u;
+  }
+  __syncthreads();
+  const uint32_t nbytes = bp / 8u;
+  uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
+  for (uint32_t w = lane; w < (nbytes + 3u) / 4u; w += 64) d32[w] = S.outw[w];
+  if (lane == 0) J.seg_len[s] = nbytes;
 }
 
 // ---------------------------------------------------------------------------
