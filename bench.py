@@ -264,8 +264,14 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     """Open-loop Poisson arrivals at each QPS level, cycling through the
     request pool (and one saturation run with twice the pool queued at
     t=0).  Latency = completion - scheduled
-    arrival; throughput = pixel-draws counted on the device / wall time."""
+    arrival; throughput = pixel-draws counted on the device / wall time.
+    The submitting thread sleeps between arrivals and the interpreter's
+    switch interval is 0.1 ms, so the completion thread takes its time stamp
+    as a request completes rather than at the next 5 ms GIL hand-over."""
+    import sys
     import threading
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
     srv = gpu.Server(ctx, budget, 64)
     rng = np.random.default_rng(seed)
     out = {}
@@ -290,8 +296,11 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
         t0 = time.perf_counter()
         for i in range(n):
             rq = reqs[i % len(reqs)]
-            while time.perf_counter() - t0 < sched[i]:
-                pass
+            while True:  # wait for the arrival without holding the GIL
+                dt = sched[i] - (time.perf_counter() - t0)
+                if dt <= 0:
+                    break
+                time.sleep(dt - 3e-4 if dt > 5e-4 else 0)
             rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
             if rc:
                 raise RuntimeError(f"server submit rc={rc}")
@@ -314,6 +323,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     out["hbm_budget_GiB"] = budget / 2**30
     out["rejected"] = int(st["rejected"])
     srv.close()
+    sys.setswitchinterval(switch)
     return out
 
 

@@ -62,15 +62,23 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
 
+// Device and pinned host buffers that only grow.  A regrowth frees the old
+// allocation (hipFree waits for the device), so past the first size they
+// grow by half again at least: a workload whose sizes creep up (a server's
+// batches) reallocates a logarithmic number of times, a fixed one once.
+inline size_t grown_size(size_t n, size_t cap) {
+  return std::max(std::max(n, (size_t)1 << 20), cap ? cap + cap / 2 : (size_t)0);
+}
+
 struct DevBuf {
   uint8_t *p = nullptr;
   size_t cap = 0;
   hipError_t ensure(size_t n) {
     if (n <= cap) return hipSuccess;
+    const size_t want = grown_size(n, cap);
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = std::max(n, (size_t)1 << 20);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) return e;
     cap = want;
@@ -88,10 +96,10 @@ struct HostBuf {
   size_t cap = 0;
   hipError_t ensure(size_t n) {
     if (n <= cap) return hipSuccess;
+    const size_t want = grown_size(n, cap);
     if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = std::max(n, (size_t)1 << 20);
     hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), want, hipHostMallocDefault);
     if (e != hipSuccess) return e;
     cap = want;
