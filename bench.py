@@ -273,6 +273,17 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     switch = sys.getswitchinterval()
     sys.setswitchinterval(1e-4)
     srv = gpu.Server(ctx, budget, 64)
+    # untimed warm-up: one burst of the whole pool, so the per-batch buffers
+    # reach their steady-state sizes before any level is measured
+    warm = []
+    for rq in reqs:
+        rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
+        if rc:
+            raise RuntimeError(f"server submit rc={rc}")
+        warm.append(rid)
+    for rid in warm:
+        if srv.wait(rid):
+            raise RuntimeError("server warm-up request failed")
     rng = np.random.default_rng(seed)
     out = {}
     levels = list(qps_levels) + ["saturation"]
