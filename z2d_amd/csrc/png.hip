@@ -153,9 +153,11 @@ struct DeflateShared {
     };
     struct {  // code construction, after it
       uint32_t key[512];  // sort keys: count << 9 | symbol
-      uint16_t hw[576];   // two-queue weights (a segment's counts sum to <= PNG_SEG + 1)
+      union {
+        uint16_t hw[576];  // two-queue weights (a segment's counts sum to <= PNG_SEG + 1)
+        uint8_t hdep[576];  // node depths, once the merge is done with the weights
+      };
       uint16_t hpar[576];
-      uint8_t hdep[576];
     };
     uint8_t cl[320];           // literal/length then distance code lengths, for the run coding
     uint32_t outw[kOutWords];  // the coded block, last
