@@ -14,7 +14,7 @@
 //                  from per-segment sums, zlib and file lengths
 //   k_png_copy     segments into the file, framed as IDAT chunks of
 //                  PNG_IDAT zlib bytes, plus the zlib header and trailer
-//   k_png_crc      one wave per IDAT chunk: CRC-32 by pieces combined in
+//   k_png_crc      one block per IDAT chunk: CRC-32 by pieces combined in
 //                  GF(2); one more wave writes signature, IHDR, gAMA, IEND
 #pragma once
 

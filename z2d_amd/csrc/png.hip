@@ -882,9 +882,17 @@ __host__ __device__ constexpr uint32_t multmodp(uint32_t a, uint32_t b) {
   return p;
 }
 struct CrcTables {
-  uint32_t tab[256];  // byte-at-a-time table
-  uint32_t x2n[32];   // x^(2^k) mod P
-  constexpr CrcTables() : tab(), x2n() {
+  uint32_t tab[256];   // byte-at-a-time table
+  uint32_t x2n[32];    // x^(2^k) mod P
+  uint32_t piece[256];  // x^(8n) mod P for the bytes after thread i's piece of a full chunk
+  uint32_t full;       // x^(8n) mod P, n = PNG_IDAT
+  constexpr uint32_t pow8n(uint64_t n) const {  // x^(8n) mod P
+    uint32_t p = 1u << 31;                      // x^0
+    for (uint32_t k = 3; n; n >>= 1, k++)
+      if (n & 1u) p = multmodp(x2n[k & 31u], p);
+    return p;
+  }
+  constexpr CrcTables() : tab(), x2n(), piece(), full(0) {
     for (uint32_t i = 0; i < 256; i++) {
       uint32_t c = i;
       for (int k = 0; k < 8; k++) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
@@ -896,8 +904,11 @@ struct CrcTables {
       p = multmodp(p, p);
       x2n[k] = p;
     }
+    for (uint32_t i = 0; i < 256; i++) piece[i] = pow8n((uint64_t)(PNG_IDAT / 256) * (255 - i));
+    full = pow8n(PNG_IDAT);
   }
 };
+static_assert(PNG_IDAT % 256 == 0, "a full IDAT chunk splits into 256 equal pieces");
 __constant__ CrcTables kCrc = CrcTables();
 
 // x^(8n) mod P: appending n zero bytes to a CRC register multiplies it by this.
@@ -943,22 +954,24 @@ __device__ uint32_t put_chunk(const uint32_t *tab, uint8_t *o, const char *type,
   return 12 + n;
 }
 
-// k_png_crc: block b < IDAT count frames chunk b (one wave: each lane's
+// k_png_crc: block b < IDAT count frames chunk b (256 threads: each thread's
 // piece CRC register, started at zero, is shifted to the chunk's end and the
-// 64 terms XOR together, the type bytes' register shifted likewise); the last
+// 256 terms XOR together, the type bytes' register shifted likewise; the
+// shifts of a full chunk come from a compile-time table); the last
 // block writes the signature, IHDR, gAMA and IEND (src/export_png.zig:86-145,
 // :422-446).
-__global__ void __launch_bounds__(64) k_png_crc(PngJob J) {
+__global__ void __launch_bounds__(256) k_png_crc(PngJob J) {
   __shared__ uint32_t tab[256];
+  __shared__ uint32_t part[4];
   if (J.meta[3]) return;
-  const uint32_t lane = threadIdx.x;
-  for (uint32_t i = lane; i < 256; i += 64) tab[i] = kCrc.tab[i];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  tab[tid] = kCrc.tab[tid];
   __syncthreads();
   const uint64_t file = J.meta[0], zlib = J.meta[1];
   const uint64_t nidat = (zlib + PNG_IDAT - 1) / PNG_IDAT;
   const uint32_t b = blockIdx.x;
   if (b == gridDim.x - 1) {
-    if (lane != 0) return;
+    if (tid != 0) return;
     uint8_t *o = J.out;
     const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
     for (int i = 0; i < 8; i++) o[i] = sig[i];
@@ -985,15 +998,21 @@ __global__ void __launch_bounds__(64) k_png_crc(PngJob J) {
   const uint64_t co = J.head_len + (uint64_t)b * (PNG_IDAT + 12);
   const uint32_t len = (uint32_t)min((uint64_t)PNG_IDAT, zlib - (uint64_t)b * PNG_IDAT);
   uint8_t *o = J.out + co;
-  const uint32_t P = (len + 63u) / 64u;
-  const uint32_t lo = min(lane * P, len), hi = min(lo + P, len);
+  const uint32_t P = (len + 255u) / 256u;
+  const uint32_t lo = min(tid * P, len), hi = min(lo + P, len);
   uint32_t term = 0;
-  if (hi > lo) term = multmodp(x8nmodp(len - hi), crc_update(tab, 0u, o + 8 + lo, hi - lo));
+  if (hi > lo) {
+    const uint32_t after = len == PNG_IDAT ? kCrc.piece[tid] : x8nmodp(len - hi);
+    term = multmodp(after, crc_update(tab, 0u, o + 8 + lo, hi - lo));
+  }
   for (int off = 32; off > 0; off >>= 1) term ^= __shfl_xor(term, off, 64);
-  if (lane == 0) {
+  if (lane == 0) part[tid >> 6] = term;
+  __syncthreads();
+  if (tid == 0) {
+    term = part[0] ^ part[1] ^ part[2] ^ part[3];
     const uint8_t type[4] = {'I', 'D', 'A', 'T'};
     const uint32_t head = crc_update(tab, 0xFFFFFFFFu, type, 4);
-    const uint32_t crc = ~(multmodp(x8nmodp(len), head) ^ term);
+    const uint32_t crc = ~(multmodp(len == PNG_IDAT ? kCrc.full : x8nmodp(len), head) ^ term);
     put_be32(o, len);
     for (int i = 0; i < 4; i++) o[4 + i] = type[i];
     put_be32(o + 8 + len, crc);
@@ -1013,6 +1032,6 @@ hipError_t z2d_launch_png(const PngJob &J, hipStream_t st) {
   hipLaunchKernelGGL(k_png_deflate, dim3(J.n_seg), dim3(64), 0, st, J);
   hipLaunchKernelGGL(k_png_layout, dim3(1), dim3(1024), 0, st, J);
   hipLaunchKernelGGL(k_png_copy, dim3(J.n_seg), dim3(256), 0, st, J);
-  hipLaunchKernelGGL(k_png_crc, dim3(J.max_idat + 1), dim3(64), 0, st, J);
+  hipLaunchKernelGGL(k_png_crc, dim3(J.max_idat + 1), dim3(256), 0, st, J);
   return hipGetLastError();
 }
