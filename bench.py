@@ -267,7 +267,10 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     arrival; throughput = pixel-draws counted on the device / wall time.
     The submitting thread sleeps between arrivals and the interpreter's
     switch interval is 0.1 ms, so the completion thread takes its time stamp
-    as a request completes rather than at the next 5 ms GIL hand-over."""
+    as a request completes rather than at the next 5 ms GIL hand-over; the
+    Python garbage collector runs between levels, not during one (a full
+    collection holds the GIL long enough to stall both threads)."""
+    import gc
     import sys
     import threading
     switch = sys.getswitchinterval()
@@ -291,6 +294,8 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
         n = 2 * len(reqs) if lvl == "saturation" else int(lvl * seconds)
         gaps = (np.zeros(n) if lvl == "saturation" else rng.exponential(1.0 / lvl, n))
         sched = np.cumsum(gaps)
+        gc.collect()
+        gc.disable()
         st0 = srv.stats()
         done_t = {}
 
@@ -318,6 +323,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
             ids.append(rid)
         th.join()
         t_end = time.perf_counter()
+        gc.enable()
         if len(done_t) < n or any(v[1] for v in done_t.values()):
             raise RuntimeError(f"serving level {lvl}: {len(done_t)}/{n} completed")
         lat = np.array([(done_t[rid][0] - t0 - sched[i]) * 1e3 for i, rid in enumerate(ids)])
