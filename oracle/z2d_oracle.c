@@ -1137,6 +1137,20 @@ static void stops_search(const zstop *st, int n, float offset, zcolor *c0, zcolo
 }
 
 #define ZO_MAX_STOPS 256
+
+/* Stop.List.search over a gradient's stops as added (src/gradient.zig:797-905):
+ * the two colors (after Color.init) and the offset between them. */
+void zo_stop_search(const z2d_gradient_desc *g, float offset, z2d_color *c0, z2d_color *c1,
+                    float *t) {
+  zstop st[ZO_MAX_STOPS];
+  int n = g->n_stops > ZO_MAX_STOPS ? 0 : build_stops(g, st);
+  zcolor a, b;
+  stops_search(st, n, offset, &a, &b, t);
+  c0->space = a.space;
+  c1->space = b.space;
+  memcpy(c0->v, a.v, sizeof(a.v));
+  memcpy(c1->v, b.v, sizeof(b.v));
+}
 static void gradient_sample(const z2d_gradient_desc *g, int32_t x, int32_t y, int encode,
                             rgba16 *oi, float of[4]) {
   zstop st[ZO_MAX_STOPS];
@@ -1411,6 +1425,69 @@ z2d_pixel zo_run_pixel(uint32_t precision, z2d_pixel dst, z2d_pixel src, uint32_
   }
   if (dst.format == Z2D_FORMAT_RGB || dst.format == Z2D_FORMAT_XRGB) out.a = 0;
   return out;
+}
+
+/* Alpha.scaleIntoIntAlpha / shlr (src/pixel.zig:582-630): a `from`-bit
+ * alpha value in a `to`-bit field.  A 1-bit source is multiplied to the
+ * target's maximum; scaling down shifts right; scaling up repeats the bits
+ * ((v << d) + v for a difference of 2 or 4, four copies for 6). */
+static uint32_t shlr_bits(uint32_t v, int from, int to) {
+  const uint32_t max_to = (1u << to) - 1u;
+  if (from == 1) return v * max_to;
+  if (v == 0 || from == to) return v;
+  if (from > to) return v >> (from - to);
+  const int d = to - from;
+  if (d == 6) return (v << 6) | (v << 4) | (v << 2) | v;
+  return (v << d) + v;
+}
+
+static int pixel_alpha_bits(uint32_t f) {
+  return (f == Z2D_FORMAT_ALPHA4) ? 4 : (f == Z2D_FORMAT_ALPHA2) ? 2 : (f == Z2D_FORMAT_ALPHA1) ? 1 : 8;
+}
+
+z2d_pixel zo_pixel_convert(uint32_t to, z2d_pixel p) {
+  z2d_pixel o = {to, 0, 0, 0, 0};
+  if (to == p.format) return p;
+  if (fmt_has_color(to)) {  /* RGB-family fromPixel (src/pixel.zig:398-432) */
+    const int is_rgba = (to == Z2D_FORMAT_ARGB || to == Z2D_FORMAT_RGBA);
+    if (fmt_has_color(p.format)) {
+      o.r = p.r;
+      o.g = p.g;
+      o.b = p.b;
+      if (is_rgba)
+        o.a = (p.format == Z2D_FORMAT_ARGB || p.format == Z2D_FORMAT_RGBA) ? p.a : 255;
+    } else if (is_rgba) {
+      o.a = (uint8_t)shlr_bits(p.a, pixel_alpha_bits(p.format), 8);
+    }
+    return o;
+  }
+  /* Alpha-family fromPixel (src/pixel.zig:569-577): RGB is opaque. */
+  const int tb = pixel_alpha_bits(to);
+  if (p.format == Z2D_FORMAT_RGB || p.format == Z2D_FORMAT_XRGB)
+    o.a = (uint8_t)((1u << tb) - 1u);
+  else
+    o.a = (uint8_t)shlr_bits(p.a, pixel_alpha_bits(p.format), tb);
+  return o;
+}
+
+/* RGBA.multiply / ARGB.multiply (src/pixel.zig:483-491). */
+z2d_pixel zo_pixel_multiply(z2d_pixel p) {
+  z2d_pixel o = p;
+  o.r = (uint8_t)((uint32_t)p.r * p.a / 255u);
+  o.g = (uint8_t)((uint32_t)p.g * p.a / 255u);
+  o.b = (uint8_t)((uint32_t)p.b * p.a / 255u);
+  return o;
+}
+
+/* Pixel.fromColor (src/pixel.zig:119-121):
+ * LinearRGB.fromColor(Color.init(c)).encodeRGBA() (src/color.zig:227-249,
+ * :272-289): linear values, round(255 x), then RGBA.multiply. */
+z2d_pixel zo_pixel_from_color(const z2d_color *c) {
+  float lin[4];
+  color_to_rgb(color_init(*c), 0, lin);
+  rgba16 v = encode_premul(lin);
+  z2d_pixel o = {Z2D_FORMAT_RGBA, (uint8_t)v.r, (uint8_t)v.g, (uint8_t)v.b, (uint8_t)v.a};
+  return o;
 }
 
 void zo_sample_pattern_int(const z2d_pattern_desc *p, int32_t x, int32_t y,
@@ -1720,6 +1797,10 @@ static int is_closed_node_set(const z2d_path_node *nodes, uint64_t n) {
     }
   }
   return closed;
+}
+
+int zo_is_closed_node_set(const z2d_path_node *nodes, uint64_t n) {
+  return is_closed_node_set(nodes, n);
 }
 
 /* ------------------------------------------------------------------------

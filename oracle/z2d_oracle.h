@@ -48,8 +48,27 @@ void zo_composite(z2d_surface_desc *dst, int32_t dst_x, int32_t dst_y,
 z2d_pixel zo_run_pixel(uint32_t precision, z2d_pixel dst, z2d_pixel src,
                        uint32_t op);
 
+/* T.fromPixel (src/pixel.zig:398-432 for the 32-bit formats, :569-630 for
+ * the alpha formats): `px` converted to format `to`. */
+z2d_pixel zo_pixel_convert(uint32_t to, z2d_pixel px);
+
+/* RGBA.multiply (src/pixel.zig:483-491). */
+z2d_pixel zo_pixel_multiply(z2d_pixel px);
+
+/* Pixel.fromColor (src/pixel.zig:119-121): an RGBA pixel. */
+z2d_pixel zo_pixel_from_color(const z2d_color *c);
+
+/* PathNode.isClosedNodeSet (src/internal/path_nodes.zig:23-36). */
+int zo_is_closed_node_set(const z2d_path_node *nodes, uint64_t n);
+
 /* Gradient.getOffset (src/gradient.zig:349-375, :605-652, :731-742). */
 float zo_gradient_offset(const z2d_gradient_desc *g, int32_t x, int32_t y);
+
+/* Stop.List.search (src/gradient.zig:832-905) over the gradient's stops as
+ * added (addAssumeCapacity: clamped, stable by offset): the colors on both
+ * sides of `offset` and the relative offset between them. */
+void zo_stop_search(const z2d_gradient_desc *g, float offset, z2d_color *c0, z2d_color *c1,
+                    float *t);
 
 /* Gradient.getPixel (src/gradient.zig:333-341): pre-multiplied RGBA. */
 z2d_pixel zo_gradient_pixel(const z2d_gradient_desc *g, int32_t x, int32_t y);
