@@ -192,16 +192,11 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     is built.  One persistent launch vs one launch per surface over the same
     prepared batch; the persistent launch also as a replayed hipGraph of the
     whole pipeline.  The band height is picked by the autotuner first."""
-    rng = scenes.SplitMix64(seed)
     b = gpu.Batch(ctx)
     surfs = []
-    for _ in range(n_surfaces):
-        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
-        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
+    for fmt, side, jobs, keep in scenes.uneven_surfaces(n_surfaces, seed):
         s = gpu.DeviceSurface(ctx, fmt, side, side)
         surfs.append(s)
-        jobs, keep = scenes.random_jobs(rng, rng.randint(1, 50), side, side,
-                                        max_side=min(512.0, side))
         surfs.append(keep)
         for kind, pat, nodes, opts in jobs:
             rc = (b.stroke if kind == "stroke" else b.fill)(s, pat, nodes, opts)
@@ -241,17 +236,8 @@ def serving_requests(gpu, n, seed=0xC5):
     form's distribution each (side log-uniform 16-1024 px, 80% RGBA8 / 20%
     alpha8 cleared to 0, 1-50 paths of the 4K scene's distribution, 30% of
     them stroked), half with an alpha8 clip mask."""
-    rng = scenes.SplitMix64(seed)
     reqs = []
-    for _ in range(n):
-        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
-        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
-        jobs, keep = scenes.random_jobs(rng, rng.randint(1, 50), side, side,
-                                        max_side=min(512.0, side))
-        clip = None
-        if rng.uniform() < 0.5:
-            clip = scenes.splitmix_stream(rng.next(), (side * side + 7) // 8).view(np.uint8)
-            clip = clip[: side * side].copy()
+    for fmt, side, jobs, keep, clip in scenes.served_requests(n, seed):
         prep = gpu.Server.prepare(fmt, side, side, [], clear=abi.pixel(fmt), clip=clip,
                                   draws=jobs)
         reqs.append(dict(prep=prep, keep=keep,

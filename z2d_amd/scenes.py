@@ -218,6 +218,41 @@ def random_jobs(rng, n_paths, width, height, max_side=512.0, stroke_share=0.3):
     return jobs, keep
 
 
+def uneven_surfaces(n_surfaces=4096, seed=0xC4):
+    """The batched form's surfaces, in generation order: side log-uniform in
+    16-1024 px, 80% RGBA8 / 20% alpha8 (cleared to 0), 1-50 paths each of the
+    4K scene's distribution with boxes up to min(512, side), 30% stroked.
+    Returns [(fmt, side, jobs, keep)]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n_surfaces):
+        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
+        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
+        jobs, keep = random_jobs(rng, rng.randint(1, 50), side, side,
+                                 max_side=min(512.0, side))
+        out.append((fmt, side, jobs, keep))
+    return out
+
+
+def served_requests(n, seed=0xC5):
+    """The served form's requests: one surface job of the batched form's
+    distribution each, half of them with a uniform alpha8 clip mask.
+    Returns [(fmt, side, jobs, keep, clip or None)]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        side = int(round(np.exp(rng.uniform(np.log(16), np.log(1024)))))
+        fmt = abi.RGBA if rng.uniform() < 0.8 else abi.ALPHA8
+        jobs, keep = random_jobs(rng, rng.randint(1, 50), side, side,
+                                 max_side=min(512.0, side))
+        clip = None
+        if rng.uniform() < 0.5:
+            clip = splitmix_stream(rng.next(), (side * side + 7) // 8).view(np.uint8)
+            clip = clip[: side * side].copy()
+        out.append((fmt, side, jobs, keep, clip))
+    return out
+
+
 def scene_4k(n_paths=10000, width=3840, height=2160, seed=0xC3):
     rng = SplitMix64(seed)
     buf = np.full(width * height * 4, 255, dtype=np.uint8)
