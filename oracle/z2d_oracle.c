@@ -982,6 +982,15 @@ static void gradient_interp(uint32_t method, uint32_t polar, zcolor c0, zcolor c
   }
 }
 
+z2d_pixel zo_interpolate_encode(uint32_t method, uint32_t polar, const z2d_color *a,
+                                const z2d_color *b, float t) {
+  rgba16 v = {0, 0, 0, 0};
+  float f[4];
+  gradient_interp(method, polar, color_init(*a), color_init(*b), t, 1, &v, f);
+  z2d_pixel o = {Z2D_FORMAT_RGBA, (uint8_t)v.r, (uint8_t)v.g, (uint8_t)v.b, (uint8_t)v.a};
+  return o;
+}
+
 /* ------------------------------------------------------------------------
  * gradients (src/gradient.zig)
  * ---------------------------------------------------------------------- */
@@ -2012,6 +2021,8 @@ static void msaa_add_span(uint8_t *cov, uint32_t x, uint32_t len) {
     if (endc > 0) cov[sx + 1 + mid] += (uint8_t)endc;
   }
 }
+
+void zo_msaa_add_span(uint8_t *cov, uint32_t x, uint32_t len) { msaa_add_span(cov, x, len); }
 
 /* multisample.run (src/internal/raster/multisample.zig:22-229). */
 static void raster_msaa(z2d_surface_desc *s, const z2d_pattern_desc *p, zpoly *poly,

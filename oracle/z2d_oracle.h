@@ -70,6 +70,18 @@ float zo_gradient_offset(const z2d_gradient_desc *g, int32_t x, int32_t y);
 void zo_stop_search(const z2d_gradient_desc *g, float offset, z2d_color *c0, z2d_color *c1,
                     float *t);
 
+/* InterpolationMethod.interpolateEncodeVec for one lane
+ * (src/internal/color_vector.zig:59-83, :191-459): the two colors (after
+ * Color.init) interpolated at t in the method's space, encoded to a
+ * pre-multiplied RGBA pixel. */
+z2d_pixel zo_interpolate_encode(uint32_t method, uint32_t polar, const z2d_color *a,
+                                const z2d_color *b, float t);
+
+/* multisample.addSpan (src/internal/raster/multisample.zig:239-279) into a
+ * dense per-pixel coverage array (`cov` indexed by pixel, values 0..16):
+ * x and len in sample columns. */
+void zo_msaa_add_span(uint8_t *cov, uint32_t x, uint32_t len);
+
 /* Gradient.getPixel (src/gradient.zig:333-341): pre-multiplied RGBA. */
 z2d_pixel zo_gradient_pixel(const z2d_gradient_desc *g, int32_t x, int32_t y);
 

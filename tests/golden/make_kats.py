@@ -79,18 +79,121 @@ def color_args(node):
     return [space, vals]
 
 
-def color_init_call(node):
+def component(node):
+    """A color component: a number, or ["powf", x, y] for `math.pow(f32, x,
+    y)` (evaluated by the checker with the oracle's std.math.pow)."""
+    if node[0] == "call" and node[1] == ("id", "math.pow"):
+        assert node[2][0] == ("id", "f32"), node
+        return ["powf", float(num(node[2][1])), float(num(node[2][2]))]
+    return float(num(node))
+
+
+def color_init_call(node, env=None):
     """`color.LinearRGB.init(r, g, b, a).asColor()` / `colorpkg.HSL.init(..)`
-    / `HSL.init(..)` -> [space, [v0..v3]]."""
+    / `HSL.init(..)` / a named constant of those -> [space, [v0..v3]]."""
+    if node[0] == "id" and env and node[1] in env:
+        return env[node[1]]
     if node[0] == "call" and node[1][0] == "field" and node[1][2] == "asColor":
         node = node[1][1]
+    if node[0] == "id" and node[1].endswith(".asColor") and env:
+        return env[node[1][: -len(".asColor")]]
     assert node[0] == "call", node
     callee = node[1][1]
     space = ("hsl" if callee.endswith("HSL.init") else
              "srgb" if callee.endswith("SRGB.init") else
              "linear_rgb" if callee.endswith("LinearRGB.init") else None)
     assert space, callee
-    return [space, [float(num(a)) for a in node[2]]]
+    return [space, [component(a) for a in node[2]]]
+
+
+def interp_table(text, start, method=None):
+    """Color interpolation tables (color.zig): a, b, t, the method (fixed
+    for the per-space tables, from the case for InterpolationMethod) and the
+    expected color; the checked output is interpolateEncode == expected
+    encoded (LinearRGB.fromColor(expected).encodeRGBA())."""
+    body = Z.test_block(text, start)
+    env = {}
+    for m in re.finditer(r"const (\w+) = ((?:color\.)?(?:LinearRGB|SRGB|HSL)\.init\([^;]*\));",
+                         body):
+        env[m.group(1)] = color_init_call(Z.parse(m.group(2)))
+    _, items = Z.cases_table(body)
+    out = []
+    for it in items:
+        f = Z.fields(it)
+        if method is None:
+            (meth, pv), = Z.fields(f["method"]).items()
+            polar = enum(pv) if pv[0] == "enum" else "shorter"
+        elif method == "hsl":
+            meth, polar = "hsl", enum(f["method"])
+        else:
+            meth, polar = method, "shorter"
+        out.append({"name": f["name"][1], "method": meth, "polar": polar,
+                    "a": color_init_call(f["a"], env), "b": color_init_call(f["b"], env),
+                    "t": float(num(f["t"])), "expected": color_init_call(f["expected"], env)})
+    return out
+
+
+def plotter_cases(text, start):
+    """fill_plotter.plot on node lists: edges and their count."""
+    body = Z.test_block(text, start)
+    out, nodes = [], []
+    cur = None
+    for st in flatten_blocks(Z.statements(body)):
+        if re.match(r"^(const alloc|var nodes|defer )", st):
+            if st.startswith("var nodes"):
+                nodes = []
+            continue
+        m = re.match(r"^try nodes\.append\(alloc, (.*)\)$", st, re.S)
+        if m:
+            nodes.append(path_node(Z.parse(m.group(1))))
+            continue
+        m = re.match(r"^var result = try plot\(alloc, nodes\.items, (.*)\)$", st, re.S)
+        if m:
+            sc, tol = Z.parse("f(" + m.group(1) + ")")[2]
+            cur = {"nodes": list(nodes), "scale": float(num(sc)), "tolerance": float(num(tol))}
+            out.append(cur)
+            continue
+        m = re.match(r"^try testing\.expectEqual\((\d+), result\.edges\.items\.len\)$", st)
+        if m:
+            cur["n_edges"] = int(m.group(1))
+            continue
+        m = re.match(r"^try testing\.expectEqualSlices\(Polygon\.Edge, (.*), result\.edges\.items\)$",
+                     st, re.S)
+        if m:
+            lst = Z.parse(m.group(1))
+            assert lst[0] == "addr"
+            cur["edges"] = [[float(num(Z.fields(e)[k])) for k in ("y0", "y1", "x_start", "x_inc")]
+                            for _, e in lst[1][2]]
+            continue
+        raise ValueError(f"statement not understood: {st[:80]}")
+    return out
+
+
+def path_node(node):
+    """`.{ .move_to = .{ .point = .{ .x = 5, .y = 0 } } }` -> [tag, x, y]."""
+    (tag, inner), = Z.fields(node).items()
+    if tag == "close_path":
+        return ["close_path"]
+    pt = Z.fields(Z.fields(inner)["point"])
+    return [tag, float(num(pt["x"])), float(num(pt["y"]))]
+
+
+def closed_sets(text, start):
+    body = Z.test_block(text, start).replace("[_]PathNode{", ".{")
+    out, nodes = [], None
+    for st in flatten_blocks(Z.statements(body)):
+        m = re.match(r"^const nodes = (.*)$", st, re.S)
+        if m:
+            nodes = [path_node(v) for _, v in Z.parse(m.group(1))[2]]
+            continue
+        m = re.match(r"^try testing\.expectEqual\((true|false), PathNode\.isClosedNodeSet\((.*)\)\)$",
+                     st, re.S)
+        if m:
+            arg = m.group(2).strip()
+            out.append({"nodes": [] if arg == "&.{}" else nodes, "expected": m.group(1) == "true"})
+            continue
+        raise ValueError(f"statement not understood: {st[:80]}")
+    return out
 
 
 def matrix_given(node):
@@ -403,6 +506,17 @@ def main():
                            "cases": stop_search(gr, 963) + stop_search(gr, 1098)}
     kats["in_box"] = {"source": "src/internal/tess/Polygon.zig:452-784",
                       "cases": in_box(src("src/internal/tess/Polygon.zig"), 452)}
+    col = src("src/color.zig")
+    kats["interpolate"] = {
+        "source": "src/color.zig:788-886 (linear), :1056-1153 (sRGB), :1334-1511 (HSL), "
+                  ":1513-1779 (InterpolationMethod)",
+        "cases": interp_table(col, 788, "linear_rgb") + interp_table(col, 1056, "srgb") +
+        interp_table(col, 1334, "hsl") + interp_table(col, 1513)}
+    fp = src("src/internal/tess/fill_plotter.zig")
+    kats["fill_plotter"] = {"source": "src/internal/tess/fill_plotter.zig:121-209",
+                            "cases": plotter_cases(fp, 121) + plotter_cases(fp, 153)}
+    kats["closed_node_set"] = {"source": "src/internal/path_nodes.zig:63-137",
+                               "cases": closed_sets(src("src/internal/path_nodes.zig"), 63)}
     with open(OUT, "w") as f:
         json.dump(kats, f, indent=1)
         f.write("\n")

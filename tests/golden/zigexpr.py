@@ -113,7 +113,7 @@ class Parser:
         if kind == "id":
             self.take()
             return ("id", val)
-        if val == ".{":
+        if val == ".{" or val == "{":
             return self.init_list(None)
         if val == "." and self.peek(1)[0] == "id":
             self.take(".")

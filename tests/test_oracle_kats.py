@@ -10,6 +10,17 @@ Zig unit tests into tests/golden/kats.json (by tests/golden/make_kats.py):
   Linear/Radial/Conic.getPixel (incl. HSL)  src/gradient.zig:1252-1779
   Stop.List.search (incl. hard stops)       src/gradient.zig:963-1146
   Polygon.inBox                             src/internal/tess/Polygon.zig:452-784
+  color interpolation, every space and the
+    four HSL polar modes                    src/color.zig:788-1779
+  fill_plotter.plot, degenerate nodes       src/internal/tess/fill_plotter.zig:121-209
+  isClosedNodeSet (oracle and the product's
+    host check through the C ABI)           src/internal/path_nodes.zig:63-137
+  multisample.addSpan                       src/internal/raster/multisample.zig:281-428
+
+The SparseCoverageBuffer tests (src/internal/raster/sparse_coverage.zig:
+214-361) check the run-length layout of the reference's coverage buffer
+with all-zero values; the oracle keeps coverage as a dense per-pixel array,
+so what they pin here is the addSpan result per pixel, checked below.
 
 Every vector is exact (the reference uses expectEqual / expectEqualDeep),
 except Stop.List.search offsets, compared within floatEps(f32) as there."""
@@ -50,6 +61,13 @@ def _setup():
     L.zo_stop_search.argtypes = [C.POINTER(abi.GradientDesc), C.c_float, C.POINTER(abi.Color),
                                  C.POINTER(abi.Color), C.POINTER(C.c_float)]
     L.zo_stop_search.restype = None
+    L.zo_interpolate_encode.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(abi.Color),
+                                        C.POINTER(abi.Color), C.c_float]
+    L.zo_interpolate_encode.restype = abi.Pixel
+    L.zo_msaa_add_span.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+    L.zo_msaa_add_span.restype = None
+    L.zo_is_closed_node_set.argtypes = [C.c_void_p, C.c_uint64]
+    L.zo_is_closed_node_set.restype = C.c_int
     return L
 
 
@@ -86,8 +104,12 @@ def chain(c):
 
 
 def color(spec):
+    """[space, [v0..v3]]; a component ["powf", x, y] is std.math.pow(f32, x, y)
+    (the oracle's restatement of it)."""
     space, v = spec
-    return abi.color(SPACE[space], *v)
+    vals = [float(L.zo_powf(np.float32(c[1]), np.float32(c[2]))) if isinstance(c, list) else c
+            for c in v]
+    return abi.color(SPACE[space], *vals)
 
 
 def _ids(cases, key="name"):
@@ -204,10 +226,104 @@ def test_kat_in_box(case):
     assert bool(got) == case["expected"]
 
 
+@pytest.mark.parametrize("case", KATS["interpolate"]["cases"],
+                         ids=_ids(KATS["interpolate"]["cases"]))
+def test_kat_interpolate_encode(case):
+    """interpolateEncode(a, b, t) == LinearRGB.fromColor(expected).encodeRGBA()."""
+    a, b, e = color(case["a"]), color(case["b"]), color(case["expected"])
+    got = L.zo_interpolate_encode(METHOD[case["method"]], POLAR[case["polar"]], C.byref(a),
+                                  C.byref(b), case["t"])
+    assert px_tuple(got) == px_tuple(L.zo_pixel_from_color(C.byref(e)))
+
+
+def _nodes(spec):
+    tags = {"move_to": abi.MOVE_TO, "line_to": abi.LINE_TO, "close_path": abi.CLOSE_PATH}
+    return abi.nodes_array([(tags[n[0]], *n[1:]) for n in spec])
+
+
+@pytest.mark.parametrize("case", KATS["fill_plotter"]["cases"],
+                         ids=_ids(KATS["fill_plotter"]["cases"], "n_edges"))
+def test_kat_fill_plotter(case):
+    from oracle_backend import plot_fill
+    edges, _ = plot_fill(_nodes(case["nodes"]), case["scale"], case["tolerance"])
+    assert len(edges) == case["n_edges"]
+    got = [[float(e[k]) for k in ("y0", "y1", "x_start", "x_inc")] for e in edges]
+    assert got == case["edges"]
+
+
+@pytest.mark.parametrize("case", KATS["closed_node_set"]["cases"],
+                         ids=_ids(KATS["closed_node_set"]["cases"], "expected"))
+def test_kat_closed_node_set(case):
+    nodes = _nodes(case["nodes"]) if case["nodes"] else np.zeros(0, abi.NODE_DTYPE)
+    assert bool(L.zo_is_closed_node_set(nodes.ctypes.data, len(nodes))) == case["expected"]
+
+
+@pytest.mark.parametrize("case", KATS["closed_node_set"]["cases"],
+                         ids=_ids(KATS["closed_node_set"]["cases"], "expected"))
+def test_kat_closed_node_set_product(case):
+    """The library's host-side painter.fill checks (no device needed): an
+    unclosed set is PathNotClosed before the context is looked at; a closed
+    one passes that check and stops at the missing context; no nodes is the
+    reference's silent no-op (src/painter.zig:73-82)."""
+    from z2d_amd import gpu
+    nodes = _nodes(case["nodes"]) if case["nodes"] else np.zeros(1, abi.NODE_DTYPE)
+    buf = np.zeros(16 * 16 * 4, np.uint8)
+    d = abi.SurfaceDesc(abi.RGBA, 16, 16, 0, buf.ctypes.data, buf.nbytes)
+    pat = abi.solid_pattern(abi.rgba(1, 2, 3, 255))
+    opts = abi.fill_opts()
+    rc = gpu.lib().z2d_gpu_fill(None, C.byref(d), C.byref(pat), nodes.ctypes.data,
+                                len(case["nodes"]), C.byref(opts))
+    if not case["nodes"]:
+        assert rc == abi.OK
+    else:
+        assert rc == (abi.ERR_NO_CONTEXT if case["expected"] else abi.ERR_PATH_NOT_CLOSED)
+
+
+def _cov_runs(cov, n):
+    """Maximal runs of equal coverage over [0, n): [(start, value, length)]."""
+    runs, i = [], 0
+    while i < n:
+        j = i
+        while j < n and cov[j] == cov[i]:
+            j += 1
+        runs.append((i, int(cov[i]), j - i))
+        i = j
+    return runs
+
+
+def test_kat_msaa_add_span():
+    """multisample.zig:281-314: four nested spans of a 100-px cross-section."""
+    cov = np.zeros(1024 + 1, np.uint8)
+    L.zo_msaa_add_span(cov.ctypes.data, 200, 400)
+    assert (cov[50], cov[149]) == (4, 4) and cov[50:150].tolist() == [4] * 100
+    L.zo_msaa_add_span(cov.ctypes.data, 201, 398)
+    assert (cov[50], cov[51:149].tolist(), cov[149]) == (7, [8] * 98, 7)
+    L.zo_msaa_add_span(cov.ctypes.data, 202, 396)
+    assert (cov[50], cov[51:149].tolist(), cov[149]) == (9, [12] * 98, 9)
+    L.zo_msaa_add_span(cov.ctypes.data, 203, 394)
+    assert (cov[50], cov[51:149].tolist(), cov[149]) == (10, [16] * 98, 10)
+    # four runs over the touched range: zeros, 10, 16 x 98, 10
+    assert len(_cov_runs(cov, 150)) == 4
+
+
+@pytest.mark.parametrize("cap", [255, 256, 65535, 65536, 131072])
+def test_kat_msaa_add_span_capacity(cap):
+    """multisample.zig:316-428: spans over the whole capacity, each later one
+    one sample further in at both ends."""
+    cov = np.zeros(cap + 1, np.uint8)
+    full = cap * 4
+    L.zo_msaa_add_span(cov.ctypes.data, 0, full)
+    assert _cov_runs(cov, cap) == [(0, 4, cap)]
+    for k, (edge, mid) in enumerate(((7, 8), (9, 12), (10, 16)), start=1):
+        L.zo_msaa_add_span(cov.ctypes.data, k, full - 2 * k)
+        assert _cov_runs(cov, cap) == [(0, edge, 1), (1, mid, cap - 2), (cap - 1, edge, 1)]
+
+
 def test_kat_counts():
     """Every table the generator reads is present and non-trivial."""
     want = {"compositor_pairs": 76, "compositor_ops": 102, "pixel_from_pixel": 64,
             "pixel_from_color": 6, "dither": 6, "gradient_offset": 12, "gradient_pixel": 17,
-            "stop_search": 18, "in_box": 24}
+            "stop_search": 18, "in_box": 24, "interpolate": 69, "fill_plotter": 3,
+            "closed_node_set": 6}
     for k, n in want.items():
         assert len(KATS[k]["cases"]) == n, k
