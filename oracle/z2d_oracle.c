@@ -1,3 +1,9 @@
+/* SPDX-License-Identifier: MPL-2.0
+ * Restates code of z2d (Copyright (c) 2024-2026 Chris Marchesi, MPL-2.0),
+ * parts of which are derived from the Cairo project (Copyright (c) 2002
+ * University of Southern California; cairo-spline.c).  The Mozilla Public
+ * License v2.0 applies to this file: https://mozilla.org/MPL/2.0/.
+ */
 /*
  * z2d_oracle.c - ORACLE (test infrastructure only).
  *

@@ -1,3 +1,10 @@
+// SPDX-License-Identifier: MPL-2.0
+// Restates code of z2d (Copyright (c) 2024-2026 Chris Marchesi, MPL-2.0),
+// parts of which are derived from the Cairo project (Copyright (c) 2002
+// University of Southern California; cairo-path-stroke-polygon.c,
+// cairo-pen.c, cairo-slope.c, cairo-spline.c, cairo-arc.c).  The Mozilla
+// Public License v2.0 applies to this file: https://mozilla.org/MPL/2.0/.
+//
 // Stroke tessellation on the host, restating the reference's stroker step
 // for step (the polygon is then filled non-zero by the device rasterizer):
 //   stroke_plotter.zig  (plot, join, plotSingle, plotOpenJoined,

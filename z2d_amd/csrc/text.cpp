@@ -1,3 +1,8 @@
+// SPDX-License-Identifier: MPL-2.0
+// Restates code of z2d (Copyright (c) 2024-2026 Chris Marchesi, MPL-2.0).
+// The Mozilla Public License v2.0 applies to this file:
+// https://mozilla.org/MPL/2.0/.
+//
 // text.cpp - text for the device fill path: TrueType outlines (src/Font.zig,
 // src/internal/Glyph.zig) laid out by text.show (src/text.zig:73-196) into
 // one path, whose nodes go to the fill entry points (one fill per call, as
