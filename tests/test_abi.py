@@ -67,3 +67,25 @@ def test_div255_identity():
     x = np.arange(1 << 16, dtype=np.uint64)
     assert np.array_equal((x * 0x8081) >> 23, x // 255)
     assert (66299 * 0x8081) >> 23 != 66299 // 255
+
+
+def test_node_walk_errors_before_the_device():
+    """The node walk's own errors come from the host before any device work:
+    a line before the first move is InvalidState (fill_plotter.zig:52-71);
+    a non-finite coordinate on a node the walk uses (where the reference
+    panics in Polygon.inBox, src/internal/tess/Polygon.zig:146-178, or
+    recurses without end in Spline.decompose) is an invalid argument; the
+    coordinates of a final move, which the walk ignores, are not checked."""
+    red = abi.solid_pattern(abi.rgba(1, 1, 1, 255))
+    inf, nan = float("inf"), float("nan")
+    tri = [(abi.LINE_TO, 1, 1), (abi.LINE_TO, 0, 1), (abi.CLOSE_PATH,)]
+    assert _fill(red, [(abi.MOVE_TO, 0, 0)] + tri) == abi.ERR_NO_CONTEXT
+    assert _fill(red, [(abi.LINE_TO, 0, 0)] + tri) == abi.ERR_INVALID_STATE
+    assert _fill(red, [(abi.MOVE_TO, nan, 0)] + tri) == abi.ERR_INVALID_ARGUMENT
+    assert _fill(red, [(abi.MOVE_TO, 0, 0), (abi.LINE_TO, 1, inf)] + tri[1:]) == \
+        abi.ERR_INVALID_ARGUMENT
+    assert _fill(red, [(abi.MOVE_TO, 0, 0), (abi.CURVE_TO, 1, 1, -inf, 2, 3, 3)] + tri) == \
+        abi.ERR_INVALID_ARGUMENT
+    # a final move is ignored by the walk, whatever its coordinates
+    assert _fill(red, [(abi.MOVE_TO, 0, 0)] + tri + [(abi.MOVE_TO, nan, nan)]) == \
+        abi.ERR_NO_CONTEXT

@@ -160,6 +160,42 @@ def test_server_error_status():
         srv.close()
 
 
+def test_server_bad_curves_fail_their_request_only():
+    """A curve with an infinite control point fails its request at the
+    host's node walk (INVALID_ARGUMENT); a finite curve whose subdivision
+    exceeds the device flattener's visit limit (an arch 1e12 px across at
+    tolerance 0.001: the leftmost branch of Spline.decompose,
+    src/internal/tess/Spline.zig, is 25 halvings deep, so each of the
+    flattener's 16 pieces needs more than 2^20 visits) fails its request
+    with UNSUPPORTED.  The other requests of the same batch render exactly
+    as the oracle does."""
+    rng = scenes.SplitMix64(0xBADC)
+    goods = [make_request(rng, fmt=abi.RGBA, with_clip=False) for _ in range(3)]
+    red = abi.solid_pattern(abi.rgba(200, 0, 0))
+
+    def curve_req(s):
+        rq = make_request(rng, fmt=abi.RGBA, with_clip=False)
+        nodes = abi.nodes_array([(abi.MOVE_TO, 0.0, 0.0),
+                                 (abi.CURVE_TO, 0.0, s, s, s, s, 0.0),
+                                 (abi.LINE_TO, 4.0, 25.0), (abi.CLOSE_PATH,)])
+        rq["draws"] = rq["draws"] + [(red, nodes, abi.fill_opts(tolerance=0.001))]
+        return rq
+
+    inf_req, huge_req = curve_req(float("inf")), curve_req(1e12)
+    srv = _server(64 << 20)
+    try:
+        order = [goods[0], inf_req, goods[1], huge_req, goods[2]]
+        subs = [submit(srv, rq) for rq in order]
+        st = [srv.wait(rid, timeout_ms=120000.0) for rid, _ in subs]
+        assert st[1] == abi.ERR_INVALID_ARGUMENT
+        assert st[3] == abi.ERR_UNSUPPORTED
+        for k in (0, 2, 4):
+            assert st[k] == 0
+            assert not diff_report(subs[k][1], oracle_render(order[k]))
+    finally:
+        srv.close()
+
+
 def test_server_fills_and_strokes():
     """Requests of the batched form's distribution (30% of the paths
     stroked, dashes included) against the CPU path: the host tessellator's

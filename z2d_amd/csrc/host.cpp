@@ -70,47 +70,73 @@ inline size_t grown_size(size_t n, size_t cap) {
   return std::max(std::max(n, (size_t)1 << 20), cap ? cap + cap / 2 : (size_t)0);
 }
 
-struct DevBuf {
+// The buffer owns its allocation: freed by release() or the destructor, so
+// a holder that is destroyed (a batch, a server, a per-call batch on the
+// stack) never leaks it.  Moves transfer ownership; copies are not allowed.
+// A growth that fails at the grown size is retried at the exact size (the
+// old allocation is already gone, so near the memory limit the exact size
+// may still fit).
+template <class Alloc>
+struct OwnedBuf {
   uint8_t *p = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t n) {
+  OwnedBuf() = default;
+  OwnedBuf(const OwnedBuf &) = delete;
+  OwnedBuf &operator=(const OwnedBuf &) = delete;
+  OwnedBuf(OwnedBuf &&o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  OwnedBuf &operator=(OwnedBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
+  ~OwnedBuf() { release(); }
+  // exact: allocate n itself (small fixed-size buffers), not a grown size
+  hipError_t ensure(size_t n, bool exact = false) {
     if (n <= cap) return hipSuccess;
-    const size_t want = grown_size(n, cap);
-    if (p) (void)hipFree(p);
+    const size_t want = exact ? n : grown_size(n, cap);
+    release();
+    hipError_t e = Alloc::alloc(&p, want);
+    if (e == hipSuccess) {
+      cap = want;
+      return hipSuccess;
+    }
     p = nullptr;
-    cap = 0;
-    hipError_t e = hipMalloc(&p, want);
-    if (e != hipSuccess) return e;
-    cap = want;
+    if (want == n) return e;
+    (void)hipGetLastError();  // the failed attempt's error is superseded
+    e = Alloc::alloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    cap = n;
     return hipSuccess;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) Alloc::free(p);
     p = nullptr;
     cap = 0;
   }
 };
-
-struct HostBuf {
-  uint8_t *p = nullptr;
-  size_t cap = 0;
-  hipError_t ensure(size_t n) {
-    if (n <= cap) return hipSuccess;
-    const size_t want = grown_size(n, cap);
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), want, hipHostMallocDefault);
-    if (e != hipSuccess) return e;
-    cap = want;
-    return hipSuccess;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-  }
+struct DeviceAlloc {
+  static hipError_t alloc(uint8_t **p, size_t n) { return hipMalloc(p, n); }
+  static void free(uint8_t *p) { (void)hipFree(p); }
 };
+struct PinnedAlloc {
+  static hipError_t alloc(uint8_t **p, size_t n) {
+    return hipHostMalloc(reinterpret_cast<void **>(p), n, hipHostMallocDefault);
+  }
+  static void free(uint8_t *p) { (void)hipHostFree(p); }
+};
+using DevBuf = OwnedBuf<DeviceAlloc>;
+using HostBuf = OwnedBuf<PinnedAlloc>;
 
 // Sub-allocation plan inside one arena.
 struct Layout {
@@ -582,6 +608,15 @@ struct z2d_gpu_batch {
   uint32_t longest_first = 0;  // strip queue order of the single-launch render
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool render_timed = false;  // the last render recorded ev[1] before the tile render
+  // The counters of the last render, copied to pinned memory at its end
+  // (read after the stream completes: no extra synchronizing copy).
+  HostBuf ctr_host;
+  // isolate_failures: a draw whose curve exceeds the subdivision limit is
+  // dropped (renders nothing) and reported in `failed` as (insertion index,
+  // status) instead of failing the whole batch (the server's requests fail
+  // one by one).
+  bool isolate_failures = false;
+  std::vector<std::pair<uint32_t, int32_t>> failed;
   // hipGraph replay of the render pipeline (z2d_gpu_batch_use_graph)
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -640,16 +675,30 @@ static int32_t fill_checks(const z2d_path_node *nodes, uint64_t n, const z2d_fil
   if (!is_closed_node_set(nodes, n)) return Z2D_ERR_PATH_NOT_CLOSED;
   if (o->op >= Z2D_OP_COUNT || o->fill_rule > 1 || o->anti_aliasing_mode > 3)
     return Z2D_ERR_INVALID_ARGUMENT;
+  // A non-finite coordinate on a node the walk uses makes the reference
+  // panic (Polygon.inBox, src/internal/tess/Polygon.zig:146-178) or recurse
+  // without end (Spline.decompose); here it is an invalid argument of this
+  // call alone.
+  auto finite = [](const z2d_path_node &nd, int k) {
+    for (int i = 0; i < k; i++)
+      if (!std::isfinite(nd.v[i])) return false;
+    return true;
+  };
   bool moved = false;
   for (uint64_t i = 0; i < n; i++) {
     switch (nodes[i].tag) {
       case Z2D_NODE_MOVE_TO:
         if (i == n - 1) return Z2D_OK;  // a final move ends the walk
+        if (!finite(nodes[i], 2)) return Z2D_ERR_INVALID_ARGUMENT;
         moved = true;
         break;
       case Z2D_NODE_LINE_TO:
+        if (!moved) return Z2D_ERR_INVALID_STATE;
+        if (!finite(nodes[i], 2)) return Z2D_ERR_INVALID_ARGUMENT;
+        break;
       case Z2D_NODE_CURVE_TO:
         if (!moved) return Z2D_ERR_INVALID_STATE;
+        if (!finite(nodes[i], 6)) return Z2D_ERR_INVALID_ARGUMENT;
         break;
       case Z2D_NODE_CLOSE_PATH: break;
       default: return Z2D_ERR_INVALID_ARGUMENT;
@@ -862,10 +911,27 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
                                hipMemcpyDeviceToHost, st));
     HIP_OK(ctx, hipStreamSynchronize(st));
     const FlatResult *res = reinterpret_cast<const FlatResult *>(FH + fo_res);
+    b->failed.clear();
     for (size_t k = 0; k < jobs.size(); k++) {
       HostDraw &h = b->draws[job_draw[k]];
       const FlatResult &r = res[k];
-      if (r.flags & FLAT_RUNAWAY) return Z2D_ERR_INVALID_ARGUMENT;  // non-finite curve
+      if (r.flags & FLAT_RUNAWAY) {
+        // A curve needed more than the subdivision visit limit: its control
+        // points lie ~1e10 px apart (the reference would allocate millions
+        // of edges) or are infinite (the reference recurses without end).
+        bool fin = true;
+        for (uint32_t q = 0; q < h.raw_cnt && fin; q++)
+          for (int c = 0; c < 6; c++) fin &= std::isfinite(b->raw_nodes[h.raw_off + q].v[c]) ||
+                                                b->raw_nodes[h.raw_off + q].tag != Z2D_NODE_CURVE_TO;
+        const int32_t code = fin ? Z2D_ERR_UNSUPPORTED : Z2D_ERR_INVALID_ARGUMENT;
+        if (!b->isolate_failures) return code;
+        b->failed.emplace_back(h.seq, code);
+        h.d.node_cnt = 0;  // the draw renders nothing: its emit job is an
+        h.edge_pairs = 0;  // empty walk of the per-draw flattener
+        jobs[k].raw_cnt = 0;
+        jobs[k].serial = 1;
+        continue;
+      }
       h.d.node_cnt = r.n_nodes;
       h.edge_pairs = r.edge_pairs;
       if (r.flags & FLAT_UNBALANCED) h.d.flags |= DRAW_UNBALANCED | DRAW_ROWPROG;
@@ -1003,6 +1069,11 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   if (e != hipSuccess) return fail(ctx, e, "batch arena");
   e = b->stage.ensure(input_bytes);
   if (e != hipSuccess) return fail(ctx, e, "batch staging");
+  if (!b->ctr_host.p) {
+    e = b->ctr_host.ensure(16 * sizeof(uint32_t), true);
+    if (e != hipSuccess) return fail(ctx, e, "batch counters");
+    std::memset(b->ctr_host.p, 0, 16 * sizeof(uint32_t));
+  }
   uint8_t *H = b->stage.p;
   uint8_t *D = b->arena.p;
 
@@ -1067,6 +1138,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
     for (size_t k = 0; k < jobs.size(); k++) {
       hj[k].node_off = b->draws[job_draw[k]].d.node_off;
       hj[k].serial = jobs[k].serial;
+      hj[k].raw_cnt = jobs[k].raw_cnt;
     }
     HIP_OK(ctx, hipMemcpyAsync(b->flat_buf.p + fo_jobs, hj, jobs.size() * sizeof(FlatJob),
                                hipMemcpyHostToDevice, st));
@@ -1164,6 +1236,11 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
       e = z2d_launch_render(J, t0, tn, J.cursors + s, std::min(tn, max_blocks), st);
     }
   }
+  // the counters (work counts, bin-overflow flag) to pinned memory, read
+  // once the stream has completed
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->ctr_host.p, J.counters, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       st);
   return e;
 }
 
@@ -1216,6 +1293,13 @@ static int32_t batch_render(z2d_gpu_batch *b, uint32_t mode) {
 static int32_t batch_wait(z2d_gpu_batch *b) {
   z2d_gpu_ctx *ctx = b->ctx;
   HIP_OK(ctx, hipStreamSynchronize(bstream(b)));
+  // A bin outside the host's capacity bounds drops work: the pixels would be
+  // wrong, so the render is an error (never expected: the bounds are upper
+  // bounds of what the kernels write).
+  if (b->ctr_host.p && reinterpret_cast<const uint32_t *>(b->ctr_host.p)[2]) {
+    ctx->err = "render: a bin exceeded its capacity bound";
+    return Z2D_ERR_DEVICE;
+  }
   if (b->ev[0] && b->ev[2]) {
     float a = 0.0f, r = 0.0f;
     if (hipEventElapsedTime(&a, b->ev[0], b->ev[2]) == hipSuccess) b->last_pipeline_ms = a;
@@ -1826,11 +1910,9 @@ int32_t z2d_gpu_batch_counters(z2d_gpu_batch *b, uint32_t enable, uint64_t out[8
   b->view.count_stats = b->count_stats;
   if (out) {
     for (int i = 0; i < 8; i++) out[i] = 0;
-    if (b->prepared && b->view.counters) {
-      uint32_t c[16];
-      HIP_OK(b->ctx, hipMemcpyAsync(c, b->view.counters, sizeof(c), hipMemcpyDeviceToHost,
-                                    bstream(b)));
+    if (b->prepared && b->ctr_host.p) {
       HIP_OK(b->ctx, hipStreamSynchronize(bstream(b)));
+      const uint32_t *c = reinterpret_cast<const uint32_t *>(b->ctr_host.p);
       out[0] = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
       out[1] = c[6];
       out[2] = c[7];
@@ -1905,8 +1987,8 @@ int32_t z2d_gpu_fill(z2d_gpu_ctx *ctx, z2d_surface_desc *surface, const z2d_patt
   int32_t rc = check_pattern(pattern);
   if (rc) return rc;
   if (n_nodes == 0) return Z2D_OK;
-  if (!nodes) return Z2D_ERR_INVALID_ARGUMENT;
-  if (!is_closed_node_set(nodes, n_nodes)) return Z2D_ERR_PATH_NOT_CLOSED;
+  rc = fill_checks(nodes, n_nodes, opts);  // closed set, then the node walk's errors
+  if (rc) return rc;
   if (!ctx) return Z2D_ERR_NO_CONTEXT;
   FillArgs a{pattern, nodes, n_nodes, opts};
   return single_draw(
@@ -2351,11 +2433,17 @@ struct z2d_gpu_server {
   }
 
   void finalize(SrvSlot &sl) {
-    uint64_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    (void)z2d_gpu_batch_counters(&sl.batch, 1, cnt);
+    // the batch's counters reached pinned memory before its done event
+    uint64_t pix = 0;
+    bool overflow = false;
+    if (sl.batch.prepared && sl.batch.ctr_host.p) {
+      const uint32_t *c = reinterpret_cast<const uint32_t *>(sl.batch.ctr_host.p);
+      pix = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
+      overflow = c[2] != 0;
+    }
     for (size_t i = 0; i < sl.reqs.size(); i++) {
       SrvRequest *r = sl.reqs[i];
-      if (cnt[3] && r->status == Z2D_OK) r->status = Z2D_ERR_DEVICE;  // bin overflow
+      if (overflow && r->status == Z2D_OK) r->status = Z2D_ERR_DEVICE;  // bin overflow
       const uint8_t *res = sl.result_stage.p + sl.stage_off[i];
       if (r->encode && r->status == Z2D_OK) {
         const uint64_t *meta = reinterpret_cast<const uint64_t *>(res);
@@ -2383,7 +2471,7 @@ struct z2d_gpu_server {
       completed.push_back(r->id);
       n_completed++;
     }
-    pixel_draws += cnt[0];
+    pixel_draws += pix;
     sl.reqs.clear();
     sl.busy = false;
     cv_done.notify_all();
@@ -2398,6 +2486,8 @@ struct z2d_gpu_server {
     sl.batch.ctx = ctx;
     sl.batch.stream = st;
     sl.batch.count_stats = 1;
+    sl.batch.isolate_failures = true;
+    std::vector<uint32_t> seq0(take.size() + 1, 0);  // each request's draws: [seq0[i], seq0[i+1])
     size_t stage = 0, files = 0;
     sl.stage_off.clear();
     sl.png_off.clear();
@@ -2410,8 +2500,10 @@ struct z2d_gpu_server {
     }
     hipError_t e = sl.result_stage.ensure(stage);
     if (e == hipSuccess && files) e = sl.png_files.ensure(files);
-    for (SrvRequest *r : take) {
+    for (size_t ri = 0; ri < take.size(); ri++) {
+      SrvRequest *r = take[ri];
       if (e != hipSuccess) break;
+      seq0[ri] = (uint32_t)sl.batch.draws.size();
       // Surface.initPixel: every pixel = T.fromPixel(clear)
       I4 c = pixel_rgba(r->clear);
       uint32_t raw = 0;
@@ -2445,8 +2537,15 @@ struct z2d_gpu_server {
                                     f->nodes.size(), &f->opts);
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
       }
+      seq0[ri + 1] = (uint32_t)sl.batch.draws.size();
     }
     int32_t rc = e == hipSuccess ? batch_prepare(&sl.batch) : Z2D_ERR_DEVICE;
+    // draws the flattener refused fail their own request only
+    for (const auto &fd : sl.batch.failed) {
+      const size_t ri =
+          (size_t)(std::upper_bound(seq0.begin(), seq0.end(), fd.first) - seq0.begin()) - 1;
+      if (ri < take.size() && take[ri]->status == Z2D_OK) take[ri]->status = fd.second;
+    }
     if (rc == Z2D_OK) rc = batch_render(&sl.batch, 0);
     for (size_t i = 0; i < take.size() && rc == Z2D_OK; i++) {
       SrvRequest *r = take[i];
