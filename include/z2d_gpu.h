@@ -88,6 +88,24 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
                           uint32_t precision, z2d_surface_desc *out);
 
+/* z2d_gpu_composite with no allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as z2d_gpu_composite, but every
+ * surface (dst, out, surface parameters) must be a device descriptor
+ * (INVALID_ARGUMENT otherwise), and the operation list with its paint,
+ * gradient and stop tables is staged in a fixed device scratch of
+ * z2d_gpu_composite_noalloc_max_bytes() bytes (1 MiB) that each device
+ * holds for the life of the process; a job that needs more returns
+ * BUDGET_EXCEEDED.  Calls on one device share that scratch: calls that
+ * overlap on one device (from several threads or contexts) race and must
+ * be serialized by the caller.  Synchronous, like z2d_gpu_composite; it
+ * records no device timing (z2d_gpu_composite_timing is not updated). */
+int32_t z2d_gpu_composite_noalloc(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x,
+                                  int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
+                                  uint32_t precision, z2d_surface_desc *out);
+uint64_t z2d_gpu_composite_noalloc_max_bytes(void);
+
 /* ---- text ---------------------------------------------------------------------
  * Font.loadBuffer (src/Font.zig:28-45): validates the font (magic, table
  * checksums, required tables, cmap subtable, loca format) and keeps a copy of

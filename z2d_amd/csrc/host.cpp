@@ -55,6 +55,35 @@ static const uint16_t kBlueNoise[4096] = {
 };
 
 // ---------------------------------------------------------------------------
+// Fixed device scratch of z2d_gpu_composite_noalloc: the operation list
+// (CompJob), its paint / gradient / stop tables and the blue-noise matrix of
+// one call.  Each device holds its own copy for the life of the process; the
+// host reaches it only through the address hipGetSymbolAddress returns on
+// that device (the array's name in host code is a stand-in, not a device
+// address).
+// ---------------------------------------------------------------------------
+constexpr size_t kCompScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) uint8_t z2d_comp_scratch[kCompScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_comp_scratch[kMaxDevices];  // per device, resolved once
+
+// The scratch's device address on `device` (current), resolved at the first
+// context created there.  Null when the lookup failed.
+static uint8_t *comp_scratch_for(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_comp_scratch[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(z2d_comp_scratch)) == hipSuccess)
+      g_comp_scratch[device] = static_cast<uint8_t *>(ptr);
+    else
+      (void)hipGetLastError();
+  }
+  return g_comp_scratch[device];
+}
+
+// ---------------------------------------------------------------------------
 // device memory helpers
 // ---------------------------------------------------------------------------
 namespace {
@@ -168,6 +197,7 @@ struct z2d_gpu_ctx {
   HostBuf png_meta;
   hipEvent_t png_ev[2] = {nullptr, nullptr};
   float last_png_ms = 0.0f;   // device time of the last PNG encode
+  uint8_t *comp_scratch = nullptr;  // this device's fixed composite scratch
 };
 
 struct z2d_gpu_surface {
@@ -1473,12 +1503,34 @@ static bool quad_job(const CompJob &job, QuadJob &q) {
   return false;
 }
 
+// noalloc: the variant that allocates no device or pinned memory: every
+// surface must be a device descriptor, the job lives in the device's fixed
+// scratch (a job larger than it is refused) and is staged from pageable
+// host memory, and no timing events are recorded.
 static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t dst_x,
                               int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
-                              uint32_t precision, z2d_surface_desc *out_d) {
+                              uint32_t precision, z2d_surface_desc *out_d, bool noalloc = false) {
   if (!ctx) return Z2D_ERR_NO_CONTEXT;
   if (!dst_d || (n_ops && !ops) || n_ops > 16) return Z2D_ERR_INVALID_ARGUMENT;
   if (n_ops == 0) return Z2D_OK;
+  if (noalloc) {
+    auto on_device = [](const z2d_surface_desc *d) {
+      return d == nullptr || d->reserved == Z2D_SURFACE_DEVICE;
+    };
+    bool ok = on_device(dst_d) && on_device(out_d);
+    for (uint64_t i = 0; i < n_ops && ok; i++) {
+      if (ops[i].dst.kind == Z2D_PARAM_SURFACE) ok = on_device(ops[i].dst.surface);
+      if (ok && ops[i].src.kind == Z2D_PARAM_SURFACE) ok = on_device(ops[i].src.surface);
+    }
+    if (!ok) {
+      ctx->err = "composite_noalloc: every surface must be a device descriptor";
+      return Z2D_ERR_INVALID_ARGUMENT;
+    }
+    if (!ctx->comp_scratch) {
+      ctx->err = "composite_noalloc: the device scratch address is unavailable";
+      return Z2D_ERR_DEVICE;
+    }
+  }
   if (dst_x >= dst_d->width || dst_y >= dst_d->height) return Z2D_OK;
   for (uint64_t i = 0; i < n_ops; i++)
     if (ops[i].op >= Z2D_OP_COUNT) return Z2D_ERR_INVALID_ARGUMENT;
@@ -1599,12 +1651,28 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
     const size_t o_job = L.take(sizeof(CompJob));
     const PaintTable::Offsets o_pt = pt.take(L);
     const size_t o_bn = L.take(sizeof(kBlueNoise));
-    hipError_t e = ctx->comp_arena.ensure(L.off);
-    if (e == hipSuccess) e = ctx->comp_stage.ensure(L.off);
-    if (e != hipSuccess) {
-      rc = fail(ctx, e, "composite arena");
+    std::vector<uint8_t> host_stage;  // noalloc: pageable staging
+    hipError_t e = hipSuccess;
+    uint8_t *H = nullptr, *D = nullptr;
+    if (noalloc) {
+      if (L.off > kCompScratchBytes) {
+        ctx->err = "composite_noalloc: the job needs " + std::to_string(L.off) +
+                   " bytes of device scratch, more than its " +
+                   std::to_string(kCompScratchBytes);
+        rc = Z2D_ERR_BUDGET_EXCEEDED;
+      } else {
+        host_stage.assign(L.off, 0);
+        H = host_stage.data();
+        D = ctx->comp_scratch;
+      }
     } else {
-      uint8_t *H = ctx->comp_stage.p, *D = ctx->comp_arena.p;
+      e = ctx->comp_arena.ensure(L.off);
+      if (e == hipSuccess) e = ctx->comp_stage.ensure(L.off);
+      if (e != hipSuccess) rc = fail(ctx, e, "composite arena");
+      H = ctx->comp_stage.p;
+      D = ctx->comp_arena.p;
+    }
+    if (rc == Z2D_OK) {
       pt.write(H, D, o_pt);
       const DevPaint *dp = reinterpret_cast<const DevPaint *>(D + o_pt.paints);
       for (uint64_t k = 0; k < n_ops; k++) {
@@ -1630,9 +1698,11 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
             if (q.kind == Z2D_PARAM_SURFACE && (q.base != 0 || q.surf.w != job.dst.w))
               stream = false;
           }
-        if (!ctx->ev[0]) e = hipEventCreate(&ctx->ev[0]);
-        if (e == hipSuccess && !ctx->ev[1]) e = hipEventCreate(&ctx->ev[1]);
-        if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
+        if (!noalloc) {
+          if (!ctx->ev[0]) e = hipEventCreate(&ctx->ev[0]);
+          if (e == hipSuccess && !ctx->ev[1]) e = hipEventCreate(&ctx->ev[1]);
+          if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], ctx->stream);
+        }
         QuadJob qj;
         if (e != hipSuccess) {
         } else if (stream && quad_job(job, qj))
@@ -1642,10 +1712,11 @@ static int32_t composite_impl(z2d_gpu_ctx *ctx, z2d_surface_desc *dst_d, int32_t
           e = z2d_launch_composite(reinterpret_cast<const CompJob *>(D + o_job), job, stream,
                                    ctx->stream);
       }
-      if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+      if (e == hipSuccess && !noalloc) e = hipEventRecord(ctx->ev[1], ctx->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
       float ms = 0.0f;
-      if (e == hipSuccess && hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
+      if (e == hipSuccess && !noalloc &&
+          hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
         ctx->last_comp_ms = ms;
       if (e != hipSuccess) rc = fail(ctx, e, "composite launch");
     }
@@ -1682,6 +1753,7 @@ int32_t z2d_gpu_create(int32_t device, z2d_gpu_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->n_cus = prop.multiProcessorCount;
+  ctx->comp_scratch = comp_scratch_for(device);  // device is current
   *out = ctx;
   return Z2D_OK;
 }
@@ -2024,6 +2096,14 @@ int32_t z2d_gpu_composite(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x
                           z2d_surface_desc *out) {
   return composite_impl(ctx, dst, dst_x, dst_y, ops, n_ops, precision, out);
 }
+
+int32_t z2d_gpu_composite_noalloc(z2d_gpu_ctx *ctx, z2d_surface_desc *dst, int32_t dst_x,
+                                  int32_t dst_y, const z2d_comp_op *ops, uint64_t n_ops,
+                                  uint32_t precision, z2d_surface_desc *out) {
+  return composite_impl(ctx, dst, dst_x, dst_y, ops, n_ops, precision, out, true);
+}
+
+uint64_t z2d_gpu_composite_noalloc_max_bytes(void) { return kCompScratchBytes; }
 
 // ---------------------------------------------------------------------------
 // PNG export (src/export_png.zig:56-360)

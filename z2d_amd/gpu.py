@@ -85,6 +85,9 @@ _SIGS = {
                                 C.c_int32),
     "z2d_gpu_server_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_server_destroy": ([C.c_void_p], None),
+    "z2d_gpu_composite_noalloc": ([C.c_void_p, P(abi.SurfaceDesc), C.c_int32, C.c_int32,
+                                   C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p], C.c_int32),
+    "z2d_gpu_composite_noalloc_max_bytes": ([], C.c_uint64),
 }
 
 
