@@ -37,7 +37,8 @@ hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n
                               hipStream_t st);
 hipError_t z2d_launch_binning(const JobView &J, hipStream_t st);
 hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
-                             uint32_t *cursor, uint32_t n_blocks, hipStream_t st);
+                             uint32_t *cursor, uint32_t *cursor_fast, uint32_t n_blocks,
+                             uint32_t n_fast_blocks, hipStream_t st);
 hipError_t z2d_launch_composite_quad(const z2d::QuadJob &job, uint64_t n_quads, hipStream_t st);
 hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs, uint32_t n_jobs,
                               z2d::FNode *out, z2d::FlatResult *res, uint32_t *rel,
@@ -1084,13 +1085,17 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_blist = L.take(max_entries * sizeof(uint32_t));
   const size_t o_ecnt = L.take((2 * max_entries + 1) * sizeof(uint32_t));  // + cursors
   const size_t o_exr = L.take(max_entries * 2 * sizeof(int32_t));
-  const size_t o_eidx = L.take(max_entry_edges * sizeof(z2d_edge));
+  const size_t o_eidx = L.take(max_entry_edges * sizeof(REdge));
   const size_t o_stripcnt = L.take(((size_t)n_tiles * spt + 1) * sizeof(uint32_t));
   const size_t o_order = L.take(b->longest_first ? (size_t)n_tiles * spt * sizeof(uint32_t) : 0);
   const size_t o_ohist = L.take(b->longest_first ? (ORDER_BUCKETS + 1) * sizeof(uint32_t) : 0);
   const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
+  const size_t o_sslow = L.take((size_t)n_tiles * spt * sizeof(uint32_t));
+  const size_t o_fast = L.take((size_t)nd * sizeof(FastDraw));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
-  const size_t o_cursors = L.take(((size_t)ns + 1) * sizeof(uint32_t));
+  // render cursors: the general renderer's, then the fast one's (one per
+  // launch: the single launch uses slot ns, per-surface launches slot s)
+  const size_t o_cursors = L.take(2 * ((size_t)ns + 1) * sizeof(uint32_t));
   const size_t o_rows = L.take(max_rows * sizeof(RowRec));
   const size_t o_sedges = L.take(scratch * sizeof(z2d_edge));
   const size_t o_sx = L.take(scratch * sizeof(int32_t));
@@ -1208,7 +1213,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.band_list = reinterpret_cast<uint32_t *>(D + o_blist);
   J.egroups = reinterpret_cast<const uint2 *>(D + o_egroups);
   J.n_egroups = (uint32_t)egroups.size();
-  J.entry_edges = reinterpret_cast<z2d_edge *>(D + o_eidx);
+  J.entry_edges = reinterpret_cast<REdge *>(D + o_eidx);
   J.max_entry_edges = (uint32_t)max_entry_edges;
   J.band_h = band_h;
   J.strips_per_tile = spt;
@@ -1217,6 +1222,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.strip_cnt = reinterpret_cast<uint32_t *>(D + o_stripcnt);
   J.strip_entries = reinterpret_cast<uint32_t *>(D + o_sentries);
   J.max_strip_entries = (uint32_t)max_strip_entries;
+  J.strip_slow = reinterpret_cast<uint32_t *>(D + o_sslow);
+  J.fast = reinterpret_cast<FastDraw *>(D + o_fast);
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
   J.count_stats = b->count_stats;
   J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);
@@ -1249,21 +1256,29 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
                            b->d_flat_rel, b->d_flat_curves, b->n_flat_curves, b->d_flat_tasks,
                            b->d_flat_toff, nullptr, true, st);
   if (e == hipSuccess) e = z2d_launch_fill_u32(J.counters, 16, 0u, st);
-  if (e == hipSuccess) e = z2d_launch_fill_u32(J.cursors, ns + 1, 0u, st);
+  if (e == hipSuccess) e = z2d_launch_fill_u32(J.cursors, 2 * (ns + 1), 0u, st);
   if (e == hipSuccess) e = z2d_launch_edge_build(J, st);
   if (e == hipSuccess)
     e = z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,
                            b->d_scratch_x, b->d_rowprog_soff, st);
   if (e == hipSuccess) e = z2d_launch_binning(J, st);
   if (e == hipSuccess && mark_render) e = hipEventRecord(b->ev[1], st);
+  // persistent workers: up to 8 four-wave workgroups of the general renderer
+  // and 16 one-wave workgroups of the fast one per CU (a launch with more
+  // strips than workers drains a shared queue)
   const uint32_t max_blocks = (uint32_t)ctx->n_cus * 8;
+  const uint32_t max_fast = (uint32_t)ctx->n_cus * 16;
+  uint32_t *cur_fast = J.cursors + (ns + 1);
   if (mode == 0) {
-    const uint32_t nb = std::min(J.n_tiles, max_blocks);
-    if (e == hipSuccess) e = z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, nb, st);
+    const uint32_t items = J.n_tiles * J.strips_per_tile;
+    if (e == hipSuccess)
+      e = z2d_launch_render(J, 0, J.n_tiles, J.cursors + ns, cur_fast + ns,
+                            std::min(J.n_tiles, max_blocks), std::min(items, max_fast), st);
   } else {
     for (uint32_t s = 0; s < ns && e == hipSuccess; s++) {
       const uint32_t t0 = b->surf_tile_off[s], tn = b->surf_tile_off[s + 1] - t0;
-      e = z2d_launch_render(J, t0, tn, J.cursors + s, std::min(tn, max_blocks), st);
+      e = z2d_launch_render(J, t0, tn, J.cursors + s, cur_fast + s, std::min(tn, max_blocks),
+                            std::min(tn * J.strips_per_tile, max_fast), st);
     }
   }
   // the counters (work counts, bin-overflow flag) to pinned memory, read

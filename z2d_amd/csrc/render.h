@@ -100,6 +100,7 @@ enum DrawFlags : uint32_t {
   DRAW_ROWPROG = 1u << 4,     // needs the serial row emulation
   DRAW_HAS_EDGES = 1u << 5,
   DRAW_SERIAL_PLOT = 1u << 6,  // a line follows a close: plot node by node
+  DRAW_FAST = 1u << 7,         // rendered by k_render_fast (see FastDraw)
 };
 
 enum DrawMode : uint32_t { MODE_DIRECT = 0, MODE_MSAA = 1, MODE_SSAA = 2 };
@@ -137,6 +138,17 @@ struct RowRec {
   int32_t s, e;    // surviving span [s, e)
 };
 
+// An edge as the tile renderer reads it from a band entry's list: the
+// reference's intercept terms (top, x_start, x_inc, Polygon.Edge,
+// src/internal/tess/Polygon.zig:18-47) and the sample rows y it is active on,
+// top < y + 0.5 <= bottom (the active-edge test, Polygon.zig:284-285), as
+// [ylo, yhi] computed once by the binning pass.  yhi2 = yhi * 2 + (y0 < y1):
+// the low bit is the edge's orientation (its winding direction).
+struct REdge {
+  double top, x_start, x_inc;
+  int32_t ylo, yhi2;
+};
+
 struct BandEntry {
   uint32_t draw;
   uint32_t edge_off;  // into the entry-edge array
@@ -147,6 +159,36 @@ struct BandEntry {
   // pixels)
   int32_t px_lo, px_hi;
 };
+
+// Per-draw constants of k_render_fast, the renderer of the common draw
+// (multisample, src_over in integer precision, a solid paint or a linear /
+// radial / conic gradient interpolated in linear RGB with at most FAST_STOPS
+// stops): one fixed-size block per draw, written by k_edge_build, fetched
+// by the strip worker one entry ahead of its use.
+constexpr uint32_t FAST_STOPS = 8;
+enum : uint32_t {
+  FAST_PAINT_NONE = 0,
+  FAST_PAINT_SOLID = 1,
+  FAST_PAINT_LINEAR = 2,  // + gradient type
+  FAST_PAINT_RADIAL = 3,
+  FAST_PAINT_CONIC = 4,
+};
+struct alignas(16) FastDraw {
+  int32_t rx0, rx1, ry0, ry1;  // composite region
+  uint32_t flags, rule, paint, px;  // DRAW_* flags, fill rule, FAST_PAINT_*, solid RGBA8
+  uint32_t n_stops, identity, pad0, pad1;
+  uint32_t pad2[4];
+  // gradient: geom[0..1], the inverse matrix, the type's constants
+  // (linear: dx, dy, 1/|d|^2, degenerate; radial: r1, r2, cdx, cdy, dr,
+  // min_dr, a, 1/a; conic: the start angle mod 2 pi)
+  double gx, gy;
+  double m[6];
+  double p[8];
+  float offs[FAST_STOPS];   // stop offsets
+  float4 pre[FAST_STOPS];   // stop colors in the interpolation space
+};
+constexpr uint32_t FAST_DRAW_CHUNKS = sizeof(FastDraw) / 16;  // 16-byte lanes
+static_assert(sizeof(FastDraw) % 16 == 0 && FAST_DRAW_CHUNKS <= 64, "FastDraw layout");
 
 struct TileRef {
   uint32_t surface;
@@ -185,7 +227,7 @@ struct JobView {
   int32_t *entry_xhi;
   const uint2 *egroups;      // edge-binning work: draw, first edge slot
   uint32_t n_egroups;
-  z2d_edge *entry_edges;     // the edges of each entry's list, capacity max_entry_edges
+  REdge *entry_edges;        // the edges of each entry's list, capacity max_entry_edges
   uint32_t max_entry_edges;
   uint32_t band_h;           // rows per band
   uint32_t strips_per_tile;  // band_h / ROWS_PER_THREAD
@@ -199,8 +241,14 @@ struct JobView {
   uint32_t *strip_cnt;       // n_tiles * strips_per_tile + 1 (scanned in place)
   uint32_t *strip_entries;   // band entry indices, capacity max_strip_entries
   uint32_t max_strip_entries;
+  // per strip: nonzero when an entry of its list is not a DRAW_FAST draw
+  // with a packed edge list (the general k_render takes it, k_render_fast
+  // the others)
+  uint32_t *strip_slow;
+  FastDraw *fast;            // per draw: k_render_fast's constants
   // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;
-  // [7] tile-draw entries rasterized; [8] entries on the unpacked path
+  // [7] tile-draw entries rasterized; [8] entries on the unpacked path;
+  // [9] strips for the general renderer; [10] strips for the fast one
   uint32_t *counters;
   uint32_t count_stats;  // nonzero: accumulate pixel-draw / edge counts
   uint32_t *cursors;     // tile work cursors, one per render launch

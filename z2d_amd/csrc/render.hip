@@ -254,6 +254,61 @@ __device__ void plot_parallel(const FNode *nodes, uint32_t n, double scale, z2d_
   }
 }
 
+// The fast renderer's paint class of a paint (FAST_PAINT_NONE: the general
+// renderer's).
+__device__ __forceinline__ uint32_t fast_paint(const DevPaint &P) {
+  if (P.kind == Z2D_PATTERN_OPAQUE) return FAST_PAINT_SOLID;
+  if (P.kind == Z2D_PATTERN_GRADIENT && P.gradient) {
+    const DevGradient &G = *P.gradient;
+    if (G.method == Z2D_INTERP_LINEAR_RGB && G.n_stops <= FAST_STOPS && G.type <= 2u)
+      return FAST_PAINT_LINEAR + G.type;
+  }
+  return FAST_PAINT_NONE;
+}
+
+// A fast draw's block: its region and flags, and the paint's constants.
+__device__ void write_fast_draw(FastDraw &F, const DevDraw &D, const DevPaint &P, uint32_t pf) {
+  F.rx0 = D.rx0;
+  F.rx1 = D.rx1;
+  F.ry0 = D.ry0;
+  F.ry1 = D.ry1;
+  F.flags = D.flags;
+  F.rule = D.rule;
+  F.paint = pf;
+  F.px = P.px.r | (P.px.g << 8) | (P.px.b << 16) | (P.px.a << 24);
+  F.n_stops = 0;
+  F.identity = 1;
+  if (pf < FAST_PAINT_LINEAR) return;
+  const DevGradient &G = *P.gradient;
+  F.n_stops = G.n_stops;
+  F.identity = G.identity;
+  F.gx = G.geom[0];
+  F.gy = G.geom[1];
+  for (int i = 0; i < 6; i++) F.m[i] = G.m[i];
+  for (int i = 0; i < 8; i++) F.p[i] = 0.0;
+  if (pf == FAST_PAINT_LINEAR) {
+    F.p[0] = G.lin_dx;
+    F.p[1] = G.lin_dy;
+    F.p[2] = G.lin_inv;
+    F.p[3] = G.lin_degenerate ? 1.0 : 0.0;
+  } else if (pf == FAST_PAINT_RADIAL) {
+    F.p[0] = G.r1;
+    F.p[1] = G.r2;
+    F.p[2] = G.cdx;
+    F.p[3] = G.cdy;
+    F.p[4] = G.dr;
+    F.p[5] = G.min_dr;
+    F.p[6] = G.a;
+    F.p[7] = G.inv_a;
+  } else {
+    F.p[0] = G.conic_angle;
+  }
+  for (uint32_t i = 0; i < G.n_stops; i++) {
+    F.offs[i] = G.offs[i];
+    F.pre[i] = G.pre[i];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_edge_build(JobView J) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t d = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -381,6 +436,15 @@ __global__ void __launch_bounds__(256) k_edge_build(JobView J) {
   }
   D.flags = flags | DRAW_VISIBLE;
   J.draw_rows[d] = make_int2(D.ry0, D.ry1);
+  if (J.fast) {
+    const uint32_t pf = fast_paint(P);
+    if (pf != FAST_PAINT_NONE && D.mode == MODE_MSAA && D.op == Z2D_OP_SRC_OVER &&
+        D.precision == Z2D_PRECISION_INTEGER &&
+        !(D.flags & (DRAW_ROWPROG | DRAW_UNBALANCED | DRAW_UNBOUNDED))) {
+      D.flags |= DRAW_FAST;
+      write_fast_draw(J.fast[d], D, P, pf);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1241,6 +1305,7 @@ __global__ void k_strip_pass(JobView J) {
   const uint32_t spt = J.strips_per_tile;
   const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * (int32_t)J.band_h;
   uint32_t cnt[MAX_STRIPS_PER_TILE], base[MAX_STRIPS_PER_TILE];
+  uint32_t slow_strips = 0;  // strips whose list holds a general-renderer entry
 #pragma unroll
   for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
     cnt[s] = 0;
@@ -1249,10 +1314,12 @@ __global__ void k_strip_pass(JobView J) {
   const uint64_t below = (1ull << lane) - 1ull;
   for (uint32_t e = e0; e < e1; e += 64) {
     uint32_t mask = 0, idx = 0;
+    bool slow = false;  // the entry needs the general renderer
     if (e + lane < e1) {
       idx = J.band_list[e + lane];
       const BandEntry &E = J.entries[idx];
       const DevDraw &D = J.draws[E.draw];
+      slow = !(D.flags & DRAW_FAST) || E.edge_cnt > PACKED_EDGE_LIMIT;
       const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
       const bool beside = (int64_t)E.px_lo >= (int64_t)tx0 + TILE_W + 1 ||
                           (int64_t)E.px_hi <= (int64_t)tx0 - 2;  // see k_band_pass
@@ -1269,6 +1336,7 @@ __global__ void k_strip_pass(JobView J) {
       if ((uint32_t)s >= spt) break;  // uniform
       const bool hit = (mask >> s) & 1u;
       const uint64_t m = __ballot(hit);
+      if (FILL && __ballot(hit && slow)) slow_strips |= 1u << s;
       if (FILL && hit) {
         const uint32_t pos = base[s] + cnt[s] + (uint32_t)__popcll(m & below);
         if (pos < J.max_strip_entries)
@@ -1283,6 +1351,18 @@ __global__ void k_strip_pass(JobView J) {
 #pragma unroll
     for (int s = 0; s < MAX_STRIPS_PER_TILE; s++)
       if ((uint32_t)s < spt) J.strip_cnt[t * spt + s] = cnt[s];
+  }
+  if (FILL && lane == 0) {
+    uint32_t n_slow = 0;
+#pragma unroll
+    for (int s = 0; s < MAX_STRIPS_PER_TILE; s++)
+      if ((uint32_t)s < spt) {
+        const bool sl = (slow_strips >> s) & 1u;
+        J.strip_slow[t * spt + s] = sl ? 1u : 0u;
+        n_slow += sl ? 1u : 0u;
+      }
+    if (n_slow) atomicAdd(&J.counters[9], n_slow);
+    if (spt - n_slow) atomicAdd(&J.counters[10], spt - n_slow);
   }
 }
 
@@ -1305,6 +1385,54 @@ __global__ void k_order_scatter(JobView J, uint32_t n_strips) {
   if (i >= n_strips) return;
   const uint32_t pos = atomicAdd(&J.order_hist[order_bucket(J, i)], 1u);
   if (pos < n_strips) J.strip_order[pos] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Sample rows of an edge: the smallest y with top < y + 0.5 and the largest
+// with y + 0.5 <= bottom (the active-edge test of Polygon.zig:284-285).  The
+// floor estimates are corrected with those exact comparisons; values are
+// clamped to [-2, 2^29] (rows outside every surface behave alike there).
+// ---------------------------------------------------------------------------
+constexpr double ROW_CLAMP = 536870912.0;  // 2^29
+__device__ __forceinline__ int32_t row_lo(double top) {
+  const double f = floor(top - 0.5) + 1.0;
+  if (!(f > -2.0)) return -2;
+  if (f > ROW_CLAMP) return (int32_t)ROW_CLAMP;
+  int32_t y = (int32_t)f;
+  if ((double)(y - 1) + 0.5 > top)
+    y--;
+  else if (!((double)y + 0.5 > top))
+    y++;
+  return y;
+}
+__device__ __forceinline__ int32_t row_hi(double bot) {
+  const double f = floor(bot - 0.5);
+  if (!(f > -2.0)) return -2;
+  if (f > ROW_CLAMP) return (int32_t)ROW_CLAMP;
+  int32_t y = (int32_t)f;
+  if ((double)(y + 1) + 0.5 <= bot)
+    y++;
+  else if (!((double)y + 0.5 <= bot))
+    y--;
+  return y;
+}
+__device__ __forceinline__ REdge make_redge(const z2d_edge &e) {
+  const bool down = e.y0 < e.y1;
+  const double top = down ? e.y0 : e.y1, bot = down ? e.y1 : e.y0;
+  REdge r;
+  r.top = top;
+  r.x_start = e.x_start;
+  r.x_inc = e.x_inc;
+  r.ylo = row_lo(top);
+  r.yhi2 = row_hi(bot) * 2 + (down ? 1 : 0);
+  return r;
+}
+// The crossing of an edge with sample row y: x_start + x_inc * ((y + 0.5) -
+// top), rounded to the nearest integer, ties away from zero (the
+// reference's intercept, src/internal/tess/Polygon.zig:302-307; no fused
+// multiply-add).
+__device__ __forceinline__ double redge_x(double top, double x_start, double x_inc, int32_t y) {
+  return x_start + (x_inc * (((double)y + 0.5) - top));
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,7 +1545,7 @@ __global__ void k_edge_bins(JobView J) {
         if (hit) {
           const uint32_t pos = base + (uint32_t)__popcll(m & below);
           if (pos < J.max_entry_edges)
-            J.entry_edges[pos] = ed;
+            J.entry_edges[pos] = make_redge(ed);
           else
             atomicOr(&J.counters[2], 1u);
         }
@@ -1445,7 +1573,7 @@ struct StripShared {
   float4 spre[STAGED_STOPS];
   union {
     struct {                               // coverage (phase A)
-      z2d_edge cedge[64];                  // staged band edges
+      REdge cedge[64];                     // staged band edges
       uint8_t own[64 * STRIP_SUBROWS];     // owning edge of each work item
     } a;
     struct {                               // composite
@@ -1608,28 +1736,6 @@ __device__ __forceinline__ bool msaa_preclear(const DevDraw &D, int W, int x, in
   return false;
 }
 
-// Active sample rows of an edge inside the tile: the rows sr in [0, SR) with
-// top < ys_base + sr + 0.5 <= bottom (the reference's active-edge test).  The
-// floor estimate is corrected with that exact comparison.
-__device__ __forceinline__ int32_t active_rows(double top, double bot, int32_t ys_base, int SR,
-                                               int32_t &lo_out) {
-  const double flo = floor(top - 0.5 - (double)ys_base) + 1.0;
-  const double fhi = floor(bot - 0.5 - (double)ys_base);
-  if (!(flo <= fhi + 1.0 && flo <= (double)SR && fhi >= -1.0)) return 0;  // NaN too
-  int32_t lo = (int32_t)fmin(fmax(flo, 0.0), (double)SR);
-  int32_t hi = (int32_t)fmin(fmax(fhi, -1.0), (double)(SR - 1));
-  if (lo > 0 && top < (double)(ys_base + lo - 1) + 0.5)
-    lo--;
-  else if (lo < SR && !(top < (double)(ys_base + lo) + 0.5))
-    lo++;
-  if (hi < SR - 1 && (double)(ys_base + hi + 1) + 0.5 <= bot)
-    hi++;
-  else if (hi >= 0 && !((double)(ys_base + hi) + 0.5 <= bot))
-    hi--;
-  lo_out = lo;
-  return hi >= lo ? hi - lo + 1 : 0;
-}
-
 __device__ __forceinline__ bool wind_in(int32_t w, bool eo) { return eo ? (w & 1) != 0 : w != 0; }
 
 // Covered samples (of 4 x 4) per pixel of the lane's column for the strip's
@@ -1638,8 +1744,9 @@ __device__ __forceinline__ bool wind_in(int32_t w, bool eo) { return eo ? (w & 1
 // lanes, two sample rows packed into one scan).  Reads and clears the deltas.
 // CUT: the row records' cutoff columns apply (unbalanced contours).
 template <bool EO, bool CUT>
-__device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, int32_t xs_base,
-                                              int32_t ys_base, const RowRec *R, const DevDraw &D,
+__device__ __forceinline__ void coverage_msaa(uint32_t (&acc)[STRIP_SUBROWS][ACC_WORDS],
+                                              uint32_t lane, int32_t xs_base, int32_t ys_base,
+                                              const RowRec *R, int32_t row_y0, int32_t row_n,
                                               int *kk) {
   const int c_lo = (int)lane * 4;
 #pragma unroll
@@ -1650,7 +1757,7 @@ __device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, in
       int32_t d[2][4], tot[2];
 #pragma unroll
       for (int q = 0; q < 2; q++) {
-        uint2 *slot = reinterpret_cast<uint2 *>(&sh.acc[i * 4 + pr * 2 + q][lane * 2]);
+        uint2 *slot = reinterpret_cast<uint2 *>(&acc[i * 4 + pr * 2 + q][lane * 2]);
         const uint2 v = *slot;
         *slot = make_uint2(0u, 0u);
         unpack_pair(v.x, d[q][0], d[q][1]);
@@ -1665,8 +1772,8 @@ __device__ __forceinline__ void coverage_msaa(StripShared &sh, uint32_t lane, in
       for (int q = 0; q < 2; q++) {
         int32_t cutoff = INT32_MAX;
         if (CUT) {
-          const int32_t rr = ys_base + i * 4 + pr * 2 + q - D.row_y0;
-          if (rr >= 0 && rr < D.row_n) cutoff = load_uniform(&R[rr]).cutoff;
+          const int32_t rr = ys_base + i * 4 + pr * 2 + q - row_y0;
+          if (rr >= 0 && rr < row_n) cutoff = load_uniform(&R[rr]).cutoff;
         }
         int32_t w = base[q];
 #pragma unroll
@@ -1696,6 +1803,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
   StripShared &sh = shw[threadIdx.x >> 6];
   const uint32_t spt = J.strips_per_tile;
   const uint32_t n_items = tile_count * spt;
+  if (load_uniform(&J.counters[9]) == 0) return;  // every strip is k_render_fast's
 
   // The accumulator starts at zero and every entry leaves it at zero.
   for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
@@ -1723,6 +1831,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
       break;
     }
     const uint32_t sidx = ordered ? load_uniform(&J.strip_order[item]) : tile_base * spt + item;
+    if (!load_uniform(&J.strip_slow[sidx])) continue;  // k_render_fast's strip
     const TileRef T = load_uniform(&J.tiles[sidx / spt]);
     const uint32_t strip = sidx % spt;
     const DevSurface S = load_uniform(&J.surfaces[T.surface]);
@@ -1830,7 +1939,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           kk[i] = 1;
         }
       } else {
-        const z2d_edge *edges = J.entry_edges + E.edge_off;
+        const REdge *edges = J.entry_edges + E.edge_off;
         const uint32_t ne = E.edge_off >= J.max_entry_edges
                                 ? 0u
                                 : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
@@ -1846,16 +1955,19 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
           for (uint32_t c0 = 0; c0 < ne; c0 += 64) {
             const uint32_t k = c0 + lane;
             int32_t n_sr = 0, lo = 0;
-            z2d_edge ed{0.0, 0.0, 0.0, 0.0};
+            REdge ed{0.0, 0.0, 0.0, 0, 0};
             if (k < ne) {
               ed = edges[k];
-              const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-              const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-              n_sr = active_rows(top, bot, ys_base, SR, lo);
-              if (n_sr) {
-                // every crossing right of the strip: no effect here
-                const double xb = ed.x_start + ed.x_inc * (bot - top);
-                if (fmin(ed.x_start, xb) > (double)(xs_base + cols) + 2.0) n_sr = 0;
+              const int32_t a = max(ed.ylo, ys_base), b = min(ed.yhi2 >> 1, ys_base + SR - 1);
+              if (a <= b) {
+                n_sr = b - a + 1;
+                lo = a - ys_base;
+                // every crossing right of the strip: no effect here (the
+                // crossings are monotone in the row, so the extremes are at
+                // the first and last active rows)
+                const double xa = redge_x(ed.top, ed.x_start, ed.x_inc, a);
+                const double xb = redge_x(ed.top, ed.x_start, ed.x_inc, b);
+                if (fmin(xa, xb) > (double)(xs_base + cols) + 1.0) n_sr = 0;
               }
             }
             const int32_t incl = wave_incl_scan(n_sr);
@@ -1870,16 +1982,14 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
             wave_lds_sync();
             for (uint32_t it = lane; it < total; it += 64) {
               const uint32_t lj = sh.u.a.own[it];
-              const z2d_edge ce = sh.u.a.cedge[lj];
+              const REdge ce = sh.u.a.cedge[lj];
               const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
-              const double top = ce.y0 < ce.y1 ? ce.y0 : ce.y1;
-              const double mid = (double)(ys_base + sr) + 0.5;
-              const int32_t x = (int32_t)round(ce.x_start + (ce.x_inc * (mid - top)));
+              const int32_t x = (int32_t)round(redge_x(ce.top, ce.x_start, ce.x_inc, ys_base + sr));
               const int32_t xr = x - xs_base;
               if (xr >= cols) continue;
               const int32_t col = xr < 0 ? 0 : xr;
               inner |= (uint32_t)col;
-              const int32_t dv = eo ? 1 : (ce.y0 < ce.y1 ? -1 : 1);
+              const int32_t dv = eo ? 1 : ((ce.yhi2 & 1) ? -1 : 1);
               atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? ((uint32_t)dv << 16) : (uint32_t)dv);
             }
             wave_lds_sync();
@@ -1908,11 +2018,11 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
             kk[i] = 4 * __popcll((in_rows >> (4 * i)) & 0xFull);
         } else if (packed && SC == 4) {
           if (eo) {
-            if (R) coverage_msaa<true, true>(sh, lane, xs_base, ys_base, R, D, kk);
-            else coverage_msaa<true, false>(sh, lane, xs_base, ys_base, R, D, kk);
+            if (R) coverage_msaa<true, true>(sh.acc, lane, xs_base, ys_base, R, D.row_y0, D.row_n, kk);
+            else coverage_msaa<true, false>(sh.acc, lane, xs_base, ys_base, R, 0, 0, kk);
           } else {
-            if (R) coverage_msaa<false, true>(sh, lane, xs_base, ys_base, R, D, kk);
-            else coverage_msaa<false, false>(sh, lane, xs_base, ys_base, R, D, kk);
+            if (R) coverage_msaa<false, true>(sh.acc, lane, xs_base, ys_base, R, D.row_y0, D.row_n, kk);
+            else coverage_msaa<false, false>(sh.acc, lane, xs_base, ys_base, R, 0, 0, kk);
           }
         } else if (packed) {
           // one sample per pixel: rows paired into one scan
@@ -1954,12 +2064,10 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               const double mid = (double)ys + 0.5;
               int32_t w[4] = {0, 0, 0, 0};
               for (uint32_t q = 0; q < ne; q++) {
-                const z2d_edge ed = edges[q];
-                const double top = ed.y0 < ed.y1 ? ed.y0 : ed.y1;
-                const double bot = ed.y0 < ed.y1 ? ed.y1 : ed.y0;
-                if (!(top < mid && bot >= mid)) continue;
-                const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - top)));
-                const int32_t dv = eo ? 1 : (ed.y0 < ed.y1 ? -1 : 1);
+                const REdge ed = edges[q];
+                if (ys < ed.ylo || ys > (ed.yhi2 >> 1)) continue;
+                const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - ed.top)));
+                const int32_t dv = eo ? 1 : ((ed.yhi2 & 1) ? -1 : 1);
                 const int32_t xr = x - xs_base;
   #pragma unroll
                 for (int j = 0; j < 4; j++)
@@ -2058,6 +2166,354 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
         for (int i = 0; i < ROWS_PER_THREAD; i++) pix[i] = tp[i];
       }
       if (J.count_stats) stat_pix += n_cov;  // per lane
+    }
+
+    if (S.clip) {
+      // Clip stage: Surface.composite(dst, mask, .dst_in) after the draws.
+#pragma unroll
+      for (int i = 0; i < ROWS_PER_THREAD; i++) {
+        const int32_t py = ty0 + i;
+        if (!col_ok || py >= H) continue;
+        const I4 m{0, 0, 0, S.clip[(int64_t)py * W + px]};
+        pix[i] = pack(op_int(Z2D_OP_DST_IN, unpack(pix[i], S.fmt), m), S.fmt);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ROWS_PER_THREAD; i++) {
+      const int32_t py = ty0 + i;
+      if (col_ok && py < H && pix[i] != orig[i]) store_raw(S, (int64_t)py * W + px, pix[i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_render_fast: the strip worker for the common draw (DRAW_FAST: multisample,
+// src_over in integer precision, a solid paint or a linear-RGB gradient of
+// at most FAST_STOPS stops, an edge list the packed deltas hold), over the
+// strips whose every entry is such a draw (k_render skips them).  The
+// coverage and composite arithmetic is k_render's; the difference is how the
+// work reaches the wave: one wave per workgroup, the entries of a strip
+// gathered 64 at a time into registers, and each draw's constants (its
+// FastDraw block, with the gradient's stops inline) loaded one entry ahead
+// of their use, so the dependent chain strip list -> entry -> draw -> paint
+// -> gradient no longer stalls every entry.
+// ---------------------------------------------------------------------------
+struct FastShared {
+  // winding deltas, as in StripShared (zero between entries)
+  uint32_t acc[STRIP_SUBROWS][ACC_WORDS];
+  union {
+    struct {                             // coverage (phase A)
+      double top[64], xs[64], xi[64];    // staged edges' intercept terms
+      uint8_t own[64 * STRIP_SUBROWS];   // owning lane of each work item, 0x80: delta -1
+      uint8_t lo[64];                    // first active sample row in the strip
+      uint16_t off[64];                  // exclusive offsets of active-row counts
+    } a;
+    struct {                             // composite
+      uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples (RGBA8)
+      uint8_t plist[ROWS_PER_THREAD * 64];   // covered (row, lane) pairs
+      uint32_t srctab[17];                   // solid paint masked by each coverage count
+    } c;
+  } u;
+  FastDraw blk;  // the current entry's draw constants
+};
+
+// Gradient.getOffset (src/gradient.zig:349-375, :605-652, :731-742) on a
+// fast draw's block: the arithmetic of linear_offset / radial_offset /
+// conic_offset, term for term.
+__device__ __forceinline__ float fast_linear_offset(const FastDraw &g, int x, int y) {
+  double px = (double)x + 0.5, py = (double)y + 0.5;
+  if (!g.identity) user_to_device(g.m, px, py);
+  if (g.p[3] != 0.0) return -1.0f;
+  double spx = px - g.gx, spy = py - g.gy;
+  double dt = 0.0;
+  dt += g.p[0] * spx;
+  dt += g.p[1] * spy;
+  return (float)dclamp01(dt * g.p[2]);
+}
+__device__ __forceinline__ float fast_radial_offset(const FastDraw &g, int x, int y) {
+  double px = (double)x + 0.5, py = (double)y + 0.5;
+  const double r1 = g.p[0], r2 = g.p[1], cdx = g.p[2], cdy = g.p[3], dr = g.p[4],
+               min_dr = g.p[5], a = g.p[6], inv_a = g.p[7];
+  if (r1 == 0.0 && r2 == 0.0) return -1.0f;
+  if (!g.identity) user_to_device(g.m, px, py);
+  double pdx = px - g.gx, pdy = py - g.gy;
+  double b = 0.0;
+  b += pdx * cdx;
+  b += pdy * cdy;
+  b += r1 * dr;
+  double c = 0.0;
+  c += pdx * pdx;
+  c += pdy * pdy;
+  c += (-r1) * r1;
+  if (a == 0.0) {
+    if (b == 0.0) return -1.0f;
+    double t = 0.5 * c / b;
+    if (t * dr >= min_dr) return (float)dclamp01(t);
+    return -1.0f;
+  }
+  double discr = 0.0;
+  discr += b * b;
+  discr += a * (-c);
+  if (discr >= 0.0) {
+    double sq = sqrt(discr);
+    double t0 = (b + sq) * inv_a;
+    double t1 = (b - sq) * inv_a;
+    if (t0 * dr >= min_dr) return (float)dclamp01(t0);
+    if (t1 * dr >= min_dr) return (float)dclamp01(t1);
+  }
+  return -1.0f;
+}
+__device__ __forceinline__ float fast_conic_offset(const FastDraw &g, int x, int y) {
+  double px = (double)x + 0.5, py = (double)y + 0.5;
+  const double two_pi = 3.14159265358979323846 * 2.0;
+  if (!g.identity) user_to_device(g.m, px, py);
+  double ang = zmodd(atan2(py - g.gy, px - g.gx) - g.p[0], two_pi);
+  return (float)(ang / two_pi);
+}
+
+// Paint samples of the covered pixels listed in plist (as dense_gradient).
+template <uint32_t PF>
+__device__ __forceinline__ void fast_gradient(const FastDraw &B, FastShared &sh, uint32_t tot_p,
+                                              uint32_t lane, int32_t tx0, int32_t ty0) {
+#pragma unroll 1
+  for (uint32_t it = lane; it < tot_p; it += 64) {
+    const uint32_t v = sh.u.c.plist[it];
+    const uint32_t pl = v & 63u, pi = v >> 6;
+    const int32_t x = tx0 + (int32_t)pl, y = ty0 + (int32_t)pi;
+    float off;
+    if constexpr (PF == FAST_PAINT_LINEAR)
+      off = fast_linear_offset(B, x, y);
+    else if constexpr (PF == FAST_PAINT_RADIAL)
+      off = fast_radial_offset(B, x, y);
+    else
+      off = fast_conic_offset(B, x, y);
+    int i0, i1;
+    float t;
+    bool transparent;
+    stops_search(B.offs, B.n_stops, off, i0, i1, t, transparent);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 A = transparent ? z : B.pre[i0];
+    const float4 C2 = transparent ? z : B.pre[i1];
+    const uint32_t r = f2u8(255.0f * lerpf_(A.x, C2.x, t));
+    const uint32_t g = f2u8(255.0f * lerpf_(A.y, C2.y, t));
+    const uint32_t b = f2u8(255.0f * lerpf_(A.z, C2.z, t));
+    const uint32_t a = f2u8(255.0f * lerpf_(A.w, C2.w, t));
+    sh.u.c.ppaint[pi][pl] = r | (g << 8) | (b << 16) | (a << 24);
+  }
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t rfl(int32_t v) {
+  return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__global__ void __launch_bounds__(64) k_render_fast(JobView J, uint32_t tile_base,
+                                                     uint32_t tile_count, uint32_t *cursor,
+                                                     uint32_t ordered) {
+  __shared__ FastShared sh;
+  const uint32_t lane = threadIdx.x & 63;
+  if (load_uniform(&J.counters[10]) == 0) return;  // every strip is k_render's
+  const uint32_t spt = J.strips_per_tile;
+  const uint32_t n_items = tile_count * spt;
+  for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
+    (&sh.acc[0][0])[i] = 0u;
+  wave_lds_sync();
+
+  uint32_t stat_pix = 0, stat_entries = 0;  // work counters (count_stats)
+  for (;;) {
+    uint32_t item = 0;
+    if (lane == 0) item = atomicAdd(cursor, 1u);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= n_items) {
+      if (J.count_stats) {
+        unsigned long long wpix = stat_pix;
+        for (int off = 32; off > 0; off >>= 1) wpix += __shfl_xor(wpix, off, 64);
+        if (lane == 0) {
+          if (wpix) atomicAdd(reinterpret_cast<unsigned long long *>(&J.counters[4]), wpix);
+          if (stat_entries) atomicAdd(&J.counters[7], stat_entries);
+        }
+      }
+      break;
+    }
+    const uint32_t sidx = ordered ? load_uniform(&J.strip_order[item]) : tile_base * spt + item;
+    if (load_uniform(&J.strip_slow[sidx])) continue;  // k_render's strip
+    const TileRef T = load_uniform(&J.tiles[sidx / spt]);
+    const uint32_t strip = sidx % spt;
+    const DevSurface S = load_uniform(&J.surfaces[T.surface]);
+    const int32_t W = S.w, H = S.h;
+    const int32_t tx0 = (int32_t)T.tx * TILE_W;
+    const int32_t ty0 = (int32_t)T.ty * (int32_t)J.band_h + (int32_t)strip * ROWS_PER_THREAD;
+    if (ty0 >= H) continue;
+    const uint32_t e0 = min(load_uniform(&J.strip_cnt[sidx]), J.max_strip_entries);
+    const uint32_t e1 = min(load_uniform(&J.strip_cnt[sidx + 1]), J.max_strip_entries);
+    if (e0 == e1 && !S.clip) continue;
+    const int32_t px = tx0 + (int32_t)lane;
+    const bool col_ok = px < W;
+    uint32_t pix[ROWS_PER_THREAD], orig[ROWS_PER_THREAD];
+#pragma unroll
+    for (int i = 0; i < ROWS_PER_THREAD; i++) {
+      const int32_t py = ty0 + i;
+      pix[i] = (col_ok && py < H) ? load_raw(S, (int64_t)py * W + px) : 0u;
+      orig[i] = pix[i];
+    }
+    const int32_t ys_base = ty0 * 4, xs_base = tx0 * 4;
+    constexpr int32_t COLS = TILE_W * 4, SR = ROWS_PER_THREAD * 4;
+
+    for (uint32_t c = e0; c < e1; c += 64) {
+      // this chunk's entries, one per lane
+      const uint32_t nc = min(64u, e1 - c);
+      uint32_t g_draw = 0, g_off = 0, g_cnt = 0;
+      if (lane < nc) {
+        const BandEntry E = J.entries[J.strip_entries[c + lane]];
+        g_draw = E.draw;
+        g_off = E.edge_off;
+        g_cnt = E.edge_off >= J.max_entry_edges ? 0u
+                                                : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
+      }
+      uint4 nb = make_uint4(0u, 0u, 0u, 0u);  // the next entry's block, 16 B per lane
+      {
+        const uint32_t d0 = __builtin_amdgcn_readlane(g_draw, 0);
+        if (lane < FAST_DRAW_CHUNKS) nb = reinterpret_cast<const uint4 *>(J.fast + d0)[lane];
+      }
+      for (uint32_t j = 0; j < nc; j++) {
+        // entry j's block to LDS, entry j+1's in flight
+        wave_lds_sync();
+        if (lane < FAST_DRAW_CHUNKS) reinterpret_cast<uint4 *>(&sh.blk)[lane] = nb;
+        wave_lds_sync();
+        if (j + 1 < nc) {
+          const uint32_t dn = __builtin_amdgcn_readlane(g_draw, j + 1);
+          if (lane < FAST_DRAW_CHUNKS) nb = reinterpret_cast<const uint4 *>(J.fast + dn)[lane];
+        }
+        const FastDraw &B = sh.blk;
+        const int32_t rx0 = rfl(B.rx0), rx1 = rfl(B.rx1), ry0 = rfl(B.ry0), ry1 = rfl(B.ry1);
+        // uniform: the draw's composite region misses this strip
+        if (rx1 <= tx0 || rx0 >= tx0 + TILE_W) continue;
+        if (ry1 <= ty0 || ry0 >= ty0 + ROWS_PER_THREAD) continue;
+        const uint32_t pf = rfl(B.paint), bflags = rfl(B.flags);
+        const bool eo = rfl(B.rule) == Z2D_FILL_EVEN_ODD;
+        const uint32_t ne = __builtin_amdgcn_readlane(g_cnt, j);
+        const REdge *edges = J.entry_edges + __builtin_amdgcn_readlane(g_off, j);
+        if (J.count_stats) stat_entries++;
+
+        // ---- phase A: winding deltas of the band edges (as k_render)
+        uint32_t inner = 0;
+        for (uint32_t c0 = 0; c0 < ne; c0 += 64) {
+          const uint32_t k = c0 + lane;
+          int32_t n_sr = 0, lo = 0;
+          REdge ed{0.0, 0.0, 0.0, 0, 0};
+          if (k < ne) {
+            ed = edges[k];
+            const int32_t a = max(ed.ylo, ys_base), b = min(ed.yhi2 >> 1, ys_base + SR - 1);
+            if (a <= b) {
+              n_sr = b - a + 1;
+              lo = a - ys_base;
+              const double xa = redge_x(ed.top, ed.x_start, ed.x_inc, a);
+              const double xb = redge_x(ed.top, ed.x_start, ed.x_inc, b);
+              if (fmin(xa, xb) > (double)(xs_base + COLS) + 1.0) n_sr = 0;
+            }
+          }
+          const int32_t incl = wave_incl_scan(n_sr);
+          const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+          if (total == 0) continue;  // uniform
+          const uint32_t my_off = (uint32_t)(incl - n_sr);
+          sh.u.a.top[lane] = ed.top;
+          sh.u.a.xs[lane] = ed.x_start;
+          sh.u.a.xi[lane] = ed.x_inc;
+          sh.u.a.lo[lane] = (uint8_t)lo;
+          sh.u.a.off[lane] = (uint16_t)my_off;
+          const uint8_t tag = (uint8_t)(lane | ((!eo && (ed.yhi2 & 1)) ? 0x80u : 0u));
+          for (int t = 0; t < n_sr; t++) sh.u.a.own[my_off + (uint32_t)t] = tag;
+          wave_lds_sync();
+          for (uint32_t it = lane; it < total; it += 64) {
+            const uint32_t v = sh.u.a.own[it];
+            const uint32_t lj = v & 63u;
+            const int sr = (int)sh.u.a.lo[lj] + (int)(it - sh.u.a.off[lj]);
+            const int32_t x = (int32_t)round(
+                redge_x(sh.u.a.top[lj], sh.u.a.xs[lj], sh.u.a.xi[lj], ys_base + sr));
+            const int32_t xr = x - xs_base;
+            if (xr >= COLS) continue;
+            const int32_t col = xr < 0 ? 0 : xr;
+            inner |= (uint32_t)col;
+            const uint32_t dv = (v & 0x80u) ? 0xFFFFFFFFu : 1u;
+            atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? (dv << 16) : dv);
+          }
+          wave_lds_sync();
+        }
+
+        // ---- phase B: covered samples per pixel
+        int kk[ROWS_PER_THREAD];
+        if (__ballot(inner != 0) == 0) {
+          // every delta at column 0: uniform winding across each sample row
+          uint32_t w0 = 0;
+          if (lane < (uint32_t)SR) {
+            w0 = sh.acc[lane][0];
+            sh.acc[lane][0] = 0u;
+          }
+          int32_t wl, wh;
+          unpack_pair(w0, wl, wh);
+          const uint64_t in_rows = __ballot(lane < (uint32_t)SR && wind_in(wl, eo));
+#pragma unroll
+          for (int i = 0; i < ROWS_PER_THREAD; i++)
+            kk[i] = 4 * __popcll((in_rows >> (4 * i)) & 0xFull);
+        } else if (eo) {
+          coverage_msaa<true, false>(sh.acc, lane, xs_base, ys_base, nullptr, 0, 0, kk);
+        } else {
+          coverage_msaa<false, false>(sh.acc, lane, xs_base, ys_base, nullptr, 0, 0, kk);
+        }
+        wave_lds_sync();
+
+        // ---- composite: src_over of the masked paint
+        uint32_t need = 0;  // covered pixels inside the composite region
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          const int32_t py = ty0 + i;
+          if (col_ok && py < H && px >= rx0 && px < rx1 && py >= ry0 && py < ry1 && kk[i] > 0)
+            need |= 1u << i;
+        }
+        if (__ballot(need != 0) == 0) continue;  // nothing covered in this strip (uniform)
+        const uint32_t praw = rfl(B.px);
+        const I4 p_px{praw & 0xFFu, (praw >> 8) & 0xFFu, (praw >> 16) & 0xFFu, praw >> 24};
+        const bool solid = pf == FAST_PAINT_SOLID;
+        if (solid) {
+          if (lane <= 16) {
+            const I4 m = lane >= 16 ? p_px : mask_coverage(p_px, (int)lane);
+            sh.u.c.srctab[lane] = m.r | (m.g << 8) | (m.b << 16) | (m.a << 24);
+          }
+          wave_lds_sync();
+        } else {
+          const int32_t np = __popc(need);
+          const int32_t incl_p = wave_incl_scan(np);
+          const uint32_t tot_p = (uint32_t)__builtin_amdgcn_readlane(incl_p, 63);
+          uint32_t o = (uint32_t)(incl_p - np);
+#pragma unroll
+          for (int i = 0; i < ROWS_PER_THREAD; i++)
+            if ((need >> i) & 1u) sh.u.c.plist[o++] = (uint8_t)((i << 6) | lane);
+          wave_lds_sync();
+          if (pf == FAST_PAINT_LINEAR)
+            fast_gradient<FAST_PAINT_LINEAR>(B, sh, tot_p, lane, tx0, ty0);
+          else if (pf == FAST_PAINT_RADIAL)
+            fast_gradient<FAST_PAINT_RADIAL>(B, sh, tot_p, lane, tx0, ty0);
+          else
+            fast_gradient<FAST_PAINT_CONIC>(B, sh, tot_p, lane, tx0, ty0);
+          wave_lds_sync();
+        }
+        const bool fast_full = (bflags & DRAW_FAST_SRC) != 0;
+        const uint32_t full_raw = fast_full ? pack(p_px, S.fmt) : 0u;
+#pragma unroll
+        for (int i = 0; i < ROWS_PER_THREAD; i++) {
+          if (!((need >> i) & 1u)) continue;
+          stat_pix++;
+          const int k = kk[i];
+          if (fast_full && k >= 16) {
+            pix[i] = full_raw;
+            continue;
+          }
+          const uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
+          I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+          if (!solid && k < 16) src = mask_coverage(src, k);
+          pix[i] = composite_src_over(pix[i], S.fmt, src);
+        }
+      }
     }
 
     if (S.clip) {
@@ -2419,10 +2875,17 @@ hipError_t z2d_launch_flatten(const z2d_path_node *raw, const FlatJob *jobs, uin
   return hipGetLastError();
 }
 
+// The tile render of tiles [tile_base, tile_base + tile_count): k_render_fast
+// (one-wave workgroups) over the strips of fast draws, then k_render
+// (four-wave workgroups) over the others; each takes strips from its own
+// cursor and leaves the other's.
 hipError_t z2d_launch_render(const JobView &J, uint32_t tile_base, uint32_t tile_count,
-                             uint32_t *cursor, uint32_t n_blocks, hipStream_t st) {
+                             uint32_t *cursor, uint32_t *cursor_fast, uint32_t n_blocks,
+                             uint32_t n_fast_blocks, hipStream_t st) {
   if (tile_count == 0) return hipSuccess;
   const uint32_t ordered = (J.strip_order && tile_base == 0 && tile_count == J.n_tiles) ? 1u : 0u;
+  hipLaunchKernelGGL(k_render_fast, dim3(n_fast_blocks), dim3(64), 0, st, J, tile_base,
+                     tile_count, cursor_fast, ordered);
   hipLaunchKernelGGL(k_render, dim3(n_blocks), dim3(RENDER_THREADS), 0, st, J, tile_base,
                      tile_count, cursor, ordered);
   return hipGetLastError();
