@@ -56,6 +56,114 @@ static inline float zmod_f(float x, float y) {
   if (x < 0.0f) return fmodf(a + y, y);
   return a;
 }
+/* std.math.atan / std.math.atan2 for f64 as Zig 0.16's standard library
+ * computes them (lib/std/math/atan.zig, atan2.zig; a third-party dependency
+ * of the reference, not under /root/reference).  Both restate the published
+ * fdlibm algorithm (Sun Microsystems, e_atan2.c / s_atan.c, as carried by
+ * musl): atan reduces |x| to one of five ranges around 0, 0.5, 1, 1.5 and
+ * infinity, evaluates an odd polynomial split into two Horner chains, and
+ * adds atan of the range's centre in two parts; atan2 handles signs, zeros
+ * and infinities and takes atan(|y/x|).  Used by Conic.getOffset
+ * (src/gradient.zig:731-742). */
+static uint32_t hi_word(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return (uint32_t)(u >> 32);
+}
+static uint32_t lo_word(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return (uint32_t)u;
+}
+static double zo_atan(double x) {
+  static const double atanhi[4] = {4.63647609000806093515e-01, 7.85398163397448278999e-01,
+                                   9.82793723247329054082e-01, 1.57079632679489655800e+00};
+  static const double atanlo[4] = {2.26987774529616870924e-17, 3.06161699786838301793e-17,
+                                   1.39033110312309984516e-17, 6.12323399573676603587e-17};
+  static const double aT[11] = {3.33333333333329318027e-01,  -1.99999999998764832476e-01,
+                                1.42857142725034663711e-01,  -1.11111104054623557880e-01,
+                                9.09088713343650656196e-02,  -7.69187620504482999495e-02,
+                                6.66107313738753120669e-02,  -5.83357013379057348645e-02,
+                                4.97687799461593236017e-02,  -3.65315727442169155270e-02,
+                                1.62858201153657823623e-02};
+  uint32_t ix = hi_word(x);
+  const uint32_t sign = ix >> 31;
+  ix &= 0x7FFFFFFFu;
+  if (ix >= 0x44100000u) { /* |x| >= 2^66 */
+    if (isnan(x)) return x;
+    const double z = atanhi[3] + 0x1.0p-120;
+    return sign ? -z : z;
+  }
+  int id;
+  if (ix < 0x3FDC0000u) { /* |x| < 0.4375 */
+    if (ix < 0x3E400000u) return x; /* |x| < 2^-27 */
+    id = -1;
+  } else {
+    x = fabs(x);
+    if (ix < 0x3FF30000u) { /* |x| < 1.1875 */
+      if (ix < 0x3FE60000u) { /* 7/16 <= |x| < 11/16 */
+        id = 0;
+        x = (2.0 * x - 1.0) / (2.0 + x);
+      } else { /* 11/16 <= |x| < 19/16 */
+        id = 1;
+        x = (x - 1.0) / (x + 1.0);
+      }
+    } else if (ix < 0x40038000u) { /* |x| < 2.4375 */
+      id = 2;
+      x = (x - 1.5) / (1.0 + 1.5 * x);
+    } else { /* 2.4375 <= |x| < 2^66 */
+      id = 3;
+      x = -1.0 / x;
+    }
+  }
+  const double z = x * x;
+  const double w = z * z;
+  const double s1 =
+      z * (aT[0] + w * (aT[2] + w * (aT[4] + w * (aT[6] + w * (aT[8] + w * aT[10])))));
+  const double s2 = w * (aT[1] + w * (aT[3] + w * (aT[5] + w * (aT[7] + w * aT[9]))));
+  if (id < 0) return x - x * (s1 + s2);
+  const double zz = atanhi[id] - ((x * (s1 + s2) - atanlo[id]) - x);
+  return sign ? -zz : zz;
+}
+double zo_atan2(double y, double x) {
+  const double pi = 3.1415926535897931160E+00, pi_lo = 1.2246467991473531772E-16;
+  if (isnan(x) || isnan(y)) return x + y;
+  uint32_t ix = hi_word(x), iy = hi_word(y);
+  const uint32_t lx = lo_word(x), ly = lo_word(y);
+  if (((ix - 0x3FF00000u) | lx) == 0) return zo_atan(y); /* x = 1.0 */
+  const uint32_t m = ((iy >> 31) & 1u) | ((ix >> 30) & 2u); /* 2 sign(x) + sign(y) */
+  ix &= 0x7FFFFFFFu;
+  iy &= 0x7FFFFFFFu;
+  if ((iy | ly) == 0) { /* y = 0 */
+    if (m < 2) return y;
+    return m == 2 ? pi : -pi;
+  }
+  if ((ix | lx) == 0) return (m & 1u) ? -pi / 2 : pi / 2; /* x = 0 */
+  if (ix == 0x7FF00000u) { /* x infinite */
+    if (iy == 0x7FF00000u) {
+      static const double q[4] = {3.1415926535897931160E+00 / 4, -3.1415926535897931160E+00 / 4,
+                                  3 * 3.1415926535897931160E+00 / 4,
+                                  -3 * 3.1415926535897931160E+00 / 4};
+      return q[m];
+    }
+    static const double r[4] = {0.0, -0.0, 3.1415926535897931160E+00, -3.1415926535897931160E+00};
+    return r[m];
+  }
+  if (ix + (64u << 20) < iy || iy == 0x7FF00000u) /* |y/x| > 2^64 */
+    return (m & 1u) ? -pi / 2 : pi / 2;
+  double z;
+  if ((m & 2u) && iy + (64u << 20) < ix) /* |y/x| < 2^-64, x < 0 */
+    z = 0.0;
+  else
+    z = zo_atan(fabs(y / x));
+  switch (m) {
+    case 0: return z;
+    case 1: return -z;
+    case 2: return pi - (z - pi_lo);
+    default: return (z - pi_lo) - pi;
+  }
+}
+
 static inline double zmod_d(double x, double y) {
   double a = fmod(x, y);
   if (x < 0.0) return fmod(a + y, y);
@@ -1083,7 +1191,7 @@ float zo_gradient_offset(const z2d_gradient_desc *g, int32_t x, int32_t y) {
       double angle0 = zmod_d(g->geom[2], two_pi);
       if (!is_identity(g->matrix)) user_to_device(g->matrix, &px, &py);
       double dx = px - g->geom[0], dy = py - g->geom[1];
-      double ang = zmod_d(atan2(dy, dx) - angle0, two_pi);
+      double ang = zmod_d(zo_atan2(dy, dx) - angle0, two_pi);
       return (float)(ang / two_pi);
     }
   }

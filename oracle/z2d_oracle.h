@@ -104,6 +104,10 @@ float zo_powf(float x, float y);
  * its zlib compressor.  Returns the stream length; `out` may be NULL. */
 uint64_t zo_png_scanlines(const z2d_surface_desc *s, uint32_t profile, uint8_t *out);
 
+/* std.math.atan2 (f64) of Zig's standard library: the fdlibm algorithm
+ * (Conic.getOffset, src/gradient.zig:731-742). */
+double zo_atan2(double y, double x);
+
 /* Polygon.inBox (src/internal/tess/Polygon.zig:142-201). */
 int zo_in_box(const double extents[4], double scale, int32_t w, int32_t h);
 

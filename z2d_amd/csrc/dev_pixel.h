@@ -536,6 +536,100 @@ __device__ __forceinline__ void user_to_device(const double *m, double &x, doubl
   x = ox + m[4];
   y = oy + m[5];
 }
+// std.math.atan / std.math.atan2 for f64 as Zig's standard library computes
+// them (lib/std/math/atan.zig, atan2.zig, the reference's third-party
+// dependency): the published fdlibm algorithm (Sun Microsystems
+// s_atan.c / e_atan2.c).  atan reduces |x| to one of five ranges around 0,
+// 0.5, 1, 1.5 and infinity, evaluates an odd polynomial as two Horner
+// chains and adds atan of the range's centre in two parts; atan2 handles
+// signs, zeros and infinities and takes atan(|y/x|).  A few live doubles,
+// so a conic gradient's sampler stays small.
+__device__ __forceinline__ double zatan(double x) {
+  const double atanhi0 = 4.63647609000806093515e-01, atanhi1 = 7.85398163397448278999e-01,
+               atanhi2 = 9.82793723247329054082e-01, atanhi3 = 1.57079632679489655800e+00;
+  const double atanlo0 = 2.26987774529616870924e-17, atanlo1 = 3.06161699786838301793e-17,
+               atanlo2 = 1.39033110312309984516e-17, atanlo3 = 6.12323399573676603587e-17;
+  uint32_t ix = (uint32_t)__double2hiint(x);
+  const uint32_t sign = ix >> 31;
+  ix &= 0x7FFFFFFFu;
+  if (ix >= 0x44100000u) {  // |x| >= 2^66
+    if (isnan(x)) return x;
+    const double z = atanhi3 + 0x1.0p-120;
+    return sign ? -z : z;
+  }
+  int id;
+  if (ix < 0x3FDC0000u) {  // |x| < 0.4375
+    if (ix < 0x3E400000u) return x;  // |x| < 2^-27
+    id = -1;
+  } else {
+    x = fabs(x);
+    if (ix < 0x3FF30000u) {    // |x| < 1.1875
+      if (ix < 0x3FE60000u) {  // 7/16 <= |x| < 11/16
+        id = 0;
+        x = (2.0 * x - 1.0) / (2.0 + x);
+      } else {  // 11/16 <= |x| < 19/16
+        id = 1;
+        x = (x - 1.0) / (x + 1.0);
+      }
+    } else if (ix < 0x40038000u) {  // |x| < 2.4375
+      id = 2;
+      x = (x - 1.5) / (1.0 + 1.5 * x);
+    } else {  // 2.4375 <= |x| < 2^66
+      id = 3;
+      x = -1.0 / x;
+    }
+  }
+  const double z = x * x;
+  const double w = z * z;
+  const double s1 =
+      z * (3.33333333333329318027e-01 +
+           w * (1.42857142725034663711e-01 +
+                w * (9.09088713343650656196e-02 +
+                     w * (6.66107313738753120669e-02 +
+                          w * (4.97687799461593236017e-02 + w * 1.62858201153657823623e-02)))));
+  const double s2 =
+      w * (-1.99999999998764832476e-01 +
+           w * (-1.11111104054623557880e-01 +
+                w * (-7.69187620504482999495e-02 +
+                     w * (-5.83357013379057348645e-02 + w * -3.65315727442169155270e-02))));
+  if (id < 0) return x - x * (s1 + s2);
+  const double hi = id == 0 ? atanhi0 : id == 1 ? atanhi1 : id == 2 ? atanhi2 : atanhi3;
+  const double lo = id == 0 ? atanlo0 : id == 1 ? atanlo1 : id == 2 ? atanlo2 : atanlo3;
+  const double zz = hi - ((x * (s1 + s2) - lo) - x);
+  return sign ? -zz : zz;
+}
+__device__ __forceinline__ double zatan2(double y, double x) {
+  const double pi = 3.1415926535897931160E+00, pi_lo = 1.2246467991473531772E-16;
+  if (isnan(x) || isnan(y)) return x + y;
+  uint32_t ix = (uint32_t)__double2hiint(x), iy = (uint32_t)__double2hiint(y);
+  const uint32_t lx = (uint32_t)__double2loint(x), ly = (uint32_t)__double2loint(y);
+  if (((ix - 0x3FF00000u) | lx) == 0) return zatan(y);  // x = 1.0
+  const uint32_t m = ((iy >> 31) & 1u) | ((ix >> 30) & 2u);  // 2 sign(x) + sign(y)
+  ix &= 0x7FFFFFFFu;
+  iy &= 0x7FFFFFFFu;
+  if ((iy | ly) == 0) {  // y = 0
+    if (m < 2) return y;
+    return m == 2 ? pi : -pi;
+  }
+  if ((ix | lx) == 0) return (m & 1u) ? -pi / 2 : pi / 2;  // x = 0
+  if (ix == 0x7FF00000u) {  // x infinite
+    if (iy == 0x7FF00000u)
+      return m == 0 ? pi / 4 : m == 1 ? -pi / 4 : m == 2 ? 3 * pi / 4 : -3 * pi / 4;
+    return m == 0 ? 0.0 : m == 1 ? -0.0 : m == 2 ? pi : -pi;
+  }
+  if (ix + (64u << 20) < iy || iy == 0x7FF00000u)  // |y/x| > 2^64
+    return (m & 1u) ? -pi / 2 : pi / 2;
+  double z;
+  if ((m & 2u) && iy + (64u << 20) < ix)  // |y/x| < 2^-64, x < 0
+    z = 0.0;
+  else
+    z = zatan(fabs(y / x));
+  if (m == 0) return z;
+  if (m == 1) return -z;
+  if (m == 2) return pi - (z - pi_lo);
+  return (z - pi_lo) - pi;
+}
+
 __device__ __forceinline__ double dclamp01(double v) { return fmax(0.0, fmin(v, 1.0)); }
 
 // Gradient.getOffset at pixel (x, y), one function per gradient type.
@@ -586,7 +680,7 @@ __device__ __forceinline__ float conic_offset(const DevGradient &g, int x, int y
   double px = (double)x + 0.5, py = (double)y + 0.5;
   const double two_pi = 3.14159265358979323846 * 2.0;
   if (!g.identity) user_to_device(g.m, px, py);
-  double ang = zmodd(atan2(py - g.geom[1], px - g.geom[0]) - g.conic_angle, two_pi);
+  double ang = zmodd(zatan2(py - g.geom[1], px - g.geom[0]) - g.conic_angle, two_pi);
   return (float)(ang / two_pi);
 }
 __device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
@@ -634,6 +728,51 @@ __device__ inline void stops_search(const float *st, uint32_t n_stops, float off
   i1 = mid + 1;
   float rl = st[mid + 1] - st[mid];
   t = rl != 0.0f ? (o - st[mid]) / rl : 0.0f;
+}
+
+// Stop.List.search for a list of at most FAST_STOPS stops whose offsets every
+// lane reads at the same addresses: when the offset equals no stop offset,
+// the reference's binary search can only end on the one interval
+// st[m] < o < st[m + 1] that holds it (or on an end stop), so m follows from
+// the number of stops below the offset, with no data-dependent loop; an
+// offset equal to some stop offset (where the search path picks among equal
+// stops) goes through stops_search itself.
+__device__ __forceinline__ void stops_find(const float *st, uint32_t n_stops, float offset,
+                                           int &i0, int &i1, float &t, bool &transparent) {
+  const int n = (int)n_stops;
+  if (offset < 0.0f || n == 0) {
+    transparent = true;
+    i0 = i1 = 0;
+    t = 0.0f;
+    return;
+  }
+  transparent = false;
+  const float o = fminz(offset, 1.0f);
+  int below = 0;
+  bool tie = false;
+  for (int i = 0; i < n; i++) {
+    const float s = st[i];
+    below += s < o ? 1 : 0;
+    tie |= s == o;
+  }
+  if (tie) {
+    stops_search(st, n_stops, offset, i0, i1, t, transparent);
+    return;
+  }
+  if (below == n || (n == 1 && below == 0)) {  // the search ends on the last stop
+    i0 = i1 = n - 1;
+    t = o - st[n - 1];
+    return;
+  }
+  if (below == 0) {  // before the first stop
+    i0 = i1 = 0;
+    t = o / st[0];
+    return;
+  }
+  i0 = below - 1;
+  i1 = below;
+  const float rl = st[i1] - st[i0];
+  t = rl != 0.0f ? (o - st[i0]) / rl : 0.0f;
 }
 
 // HSL.toRGB (src/color.zig:470-488).

@@ -2267,7 +2267,7 @@ __device__ __forceinline__ float fast_conic_offset(const FastDraw &g, int x, int
   double px = (double)x + 0.5, py = (double)y + 0.5;
   const double two_pi = 3.14159265358979323846 * 2.0;
   if (!g.identity) user_to_device(g.m, px, py);
-  double ang = zmodd(atan2(py - g.gy, px - g.gx) - g.p[0], two_pi);
+  double ang = zmodd(zatan2(py - g.gy, px - g.gx) - g.p[0], two_pi);
   return (float)(ang / two_pi);
 }
 
@@ -2275,6 +2275,7 @@ __device__ __forceinline__ float fast_conic_offset(const FastDraw &g, int x, int
 template <uint32_t PF>
 __device__ __forceinline__ void fast_gradient(const FastDraw &B, FastShared &sh, uint32_t tot_p,
                                               uint32_t lane, int32_t tx0, int32_t ty0) {
+  const uint32_t n_stops = min(__builtin_amdgcn_readfirstlane(B.n_stops), FAST_STOPS);
 #pragma unroll 1
   for (uint32_t it = lane; it < tot_p; it += 64) {
     const uint32_t v = sh.u.c.plist[it];
@@ -2290,7 +2291,7 @@ __device__ __forceinline__ void fast_gradient(const FastDraw &B, FastShared &sh,
     int i0, i1;
     float t;
     bool transparent;
-    stops_search(B.offs, B.n_stops, off, i0, i1, t, transparent);
+    stops_find(B.offs, n_stops, off, i0, i1, t, transparent);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 A = transparent ? z : B.pre[i0];
     const float4 C2 = transparent ? z : B.pre[i1];
@@ -2307,7 +2308,9 @@ __device__ __forceinline__ int32_t rfl(int32_t v) {
   return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
 }
 
-__global__ void __launch_bounds__(64) k_render_fast(JobView J, uint32_t tile_base,
+// At most 128 VGPRs (four waves per SIMD; LDS allows fourteen per CU).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_fast(JobView J, uint32_t tile_base,
                                                      uint32_t tile_count, uint32_t *cursor,
                                                      uint32_t ordered) {
   __shared__ FastShared sh;
@@ -2349,12 +2352,12 @@ __global__ void __launch_bounds__(64) k_render_fast(JobView J, uint32_t tile_bas
     if (e0 == e1 && !S.clip) continue;
     const int32_t px = tx0 + (int32_t)lane;
     const bool col_ok = px < W;
-    uint32_t pix[ROWS_PER_THREAD], orig[ROWS_PER_THREAD];
+    uint32_t pix[ROWS_PER_THREAD];
+    uint32_t dirty = 0;  // rows whose pixel a draw or the clip changed
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
       const int32_t py = ty0 + i;
       pix[i] = (col_ok && py < H) ? load_raw(S, (int64_t)py * W + px) : 0u;
-      orig[i] = pix[i];
     }
     const int32_t ys_base = ty0 * 4, xs_base = tx0 * 4;
     constexpr int32_t COLS = TILE_W * 4, SR = ROWS_PER_THREAD * 4;
@@ -2504,14 +2507,16 @@ __global__ void __launch_bounds__(64) k_render_fast(JobView J, uint32_t tile_bas
           if (!((need >> i) & 1u)) continue;
           stat_pix++;
           const int k = kk[i];
+          const uint32_t before = pix[i];
           if (fast_full && k >= 16) {
             pix[i] = full_raw;
-            continue;
+          } else {
+            const uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
+            I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+            if (!solid && k < 16) src = mask_coverage(src, k);
+            pix[i] = composite_src_over(pix[i], S.fmt, src);
           }
-          const uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
-          I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
-          if (!solid && k < 16) src = mask_coverage(src, k);
-          pix[i] = composite_src_over(pix[i], S.fmt, src);
+          dirty |= (pix[i] != before ? 1u : 0u) << i;
         }
       }
     }
@@ -2523,13 +2528,15 @@ __global__ void __launch_bounds__(64) k_render_fast(JobView J, uint32_t tile_bas
         const int32_t py = ty0 + i;
         if (!col_ok || py >= H) continue;
         const I4 m{0, 0, 0, S.clip[(int64_t)py * W + px]};
+        const uint32_t before = pix[i];
         pix[i] = pack(op_int(Z2D_OP_DST_IN, unpack(pix[i], S.fmt), m), S.fmt);
+        dirty |= (pix[i] != before ? 1u : 0u) << i;
       }
     }
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
       const int32_t py = ty0 + i;
-      if (col_ok && py < H && pix[i] != orig[i]) store_raw(S, (int64_t)py * W + px, pix[i]);
+      if (col_ok && py < H && ((dirty >> i) & 1u)) store_raw(S, (int64_t)py * W + px, pix[i]);
     }
   }
 }
