@@ -592,7 +592,6 @@ struct HostDraw {
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
   uint32_t tcol_lo, tcol_hi;  // estimated tile-column range (inclusive)
   uint64_t edge_pairs;        // upper bound of (edge, band) pairs
-  bool maybe_fast;            // may take the fast renderer (backdrop blocks reserved)
 };
 
 // Bands [floor(lo / band_h), floor(hi / band_h)] that an edge between pixel
@@ -682,16 +681,6 @@ static int32_t batch_add_common(z2d_gpu_batch *b, z2d_gpu_surface *s,
   d.op = op;
   d.precision = op_requires_float(op) ? Z2D_PRECISION_FLOAT : precision;
   d.paint = b->paints.add(pattern);
-  // the fast renderer's draws (k_edge_build decides; a superset here)
-  {
-    bool paint_ok = pattern && pattern->kind == Z2D_PATTERN_OPAQUE;
-    if (pattern && pattern->kind == Z2D_PATTERN_GRADIENT && pattern->gradient) {
-      const z2d_gradient_desc *g = pattern->gradient;
-      paint_ok = g->method == Z2D_INTERP_LINEAR_RGB && g->n_stops <= FAST_STOPS && g->type <= 2u;
-    }
-    hd.maybe_fast = paint_ok && d.mode == MODE_MSAA && op == Z2D_OP_SRC_OVER &&
-                    d.precision == Z2D_PRECISION_INTEGER;
-  }
   const bool bounded = op_is_bounded(op);
   if (d.mode == MODE_DIRECT && !bounded) d.flags |= DRAW_ROWPROG;
   if (d.flags & DRAW_UNBALANCED) d.flags |= DRAW_ROWPROG;
@@ -1011,7 +1000,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   }
   if (n_slots > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
   uint64_t max_entries = 0, max_rows = 0, scratch = 0, max_entry_edges = 0;
-  uint64_t max_strip_entries = 0, bd_blocks = 0;
+  uint64_t max_strip_entries = 0;
   std::vector<uint32_t> rowprog_list, rowprog_soff;
   std::vector<uint2> egroups;
   for (uint32_t i = 0; i < nd; i++) {
@@ -1045,13 +1034,6 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
       }
     }
     h.d.band_lo = h.band_lo;
-    // backdrop blocks of a possible fast draw: one per (band, tile column)
-    // of its estimate; none otherwise (the device then keeps it off the
-    // fast renderer)
-    h.d.tcol_lo = h.tcol_lo;
-    h.d.n_tcol = h.maybe_fast ? h.tcol_hi - h.tcol_lo + 1 : 0u;
-    h.d.bd_block = (uint32_t)std::min<uint64_t>(bd_blocks, 0xFFFFFFFFull);
-    bd_blocks += (uint64_t)(h.band_hi - h.band_lo + 1) * h.d.n_tcol;
     h.d.band_hi = h.band_hi;
     h.d.entry_base = (uint32_t)std::min<uint64_t>(max_entries, 0xFFFFFFFFull);
     for (uint32_t k = 0; k < h.d.node_cnt; k += EDGE_GROUP) egroups.push_back(make_uint2(i, k));
@@ -1071,7 +1053,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   if (max_entries > 0xFFFFFFF0ull || total_nodes > 0xFFFFFFF0ull ||
       max_entry_edges > 0xFFFFFFF0ull || max_strip_entries > 0xFFFFFFF0ull ||
       (uint64_t)n_tiles * spt > 0xFFFFFFF0ull || egroups.size() > 0xFFFFFFF0ull ||
-      2 * max_entries + 1 > 0xFFFFFFF0ull || bd_blocks > 0xFFFFFFF0ull)
+      2 * max_entries + 1 > 0xFFFFFFF0ull)
     return Z2D_ERR_OUT_OF_MEMORY;
   b->n_rowprog = (uint32_t)rowprog_list.size();
 
@@ -1110,7 +1092,6 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_sentries = L.take(max_strip_entries * sizeof(uint32_t));
   const size_t o_sslow = L.take((size_t)n_tiles * spt * sizeof(uint32_t));
   const size_t o_fast = L.take((size_t)nd * sizeof(FastDraw));
-  const size_t o_bd = L.take((size_t)bd_blocks * band_h * 4 * sizeof(int16_t));
   const size_t o_counters = L.take(16 * sizeof(uint32_t));
   // render cursors: the general renderer's, then the fast one's (one per
   // launch: the single launch uses slot ns, per-surface launches slot s)
@@ -1243,8 +1224,6 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.max_strip_entries = (uint32_t)max_strip_entries;
   J.strip_slow = reinterpret_cast<uint32_t *>(D + o_sslow);
   J.fast = reinterpret_cast<FastDraw *>(D + o_fast);
-  J.backdrop = reinterpret_cast<int16_t *>(D + o_bd);
-  J.n_bd_blocks = (uint32_t)bd_blocks;
   J.counters = reinterpret_cast<uint32_t *>(D + o_counters);
   J.count_stats = b->count_stats;
   J.cursors = reinterpret_cast<uint32_t *>(D + o_cursors);

@@ -128,10 +128,6 @@ struct DevDraw {
   int32_t mask_bits;           // SSAA mask depth
   int32_t row_y0, row_n;       // sub-rows covered by the row records
   uint32_t row_off;            // offset into the row-record array
-  // backdrops (k_backdrop): the host's tile-column estimate [tcol_lo,
-  // tcol_lo + n_tcol) and the draw's first block; entry (band b, tile
-  // column c) owns block bd_block + (b - band_lo) * n_tcol + (c - tcol_lo)
-  uint32_t tcol_lo, n_tcol, bd_block;
 };
 
 // Per sub-row record of the serial emulation (rowprog): unbalanced cutoff
@@ -162,7 +158,6 @@ struct BandEntry {
   // coverage ([INT32_MIN, INT32_MAX] for draws where that may still change
   // pixels)
   int32_t px_lo, px_hi;
-  uint32_t bd_blk0;  // backdrop block of the draw's first tile column in this band
 };
 
 // Per-draw constants of k_render_fast, the renderer of the common draw
@@ -182,7 +177,7 @@ struct alignas(16) FastDraw {
   int32_t rx0, rx1, ry0, ry1;  // composite region
   uint32_t flags, rule, paint, px;  // DRAW_* flags, fill rule, FAST_PAINT_*, solid RGBA8
   uint32_t n_stops, identity, pad0, pad1;
-  uint32_t band_lo, tcol_lo, n_tcol, bd_block;  // backdrop blocks (as DevDraw)
+  uint32_t pad2[4];
   // gradient: geom[0..1], the inverse matrix, the type's constants
   // (linear: dx, dy, 1/|d|^2, degenerate; radial: r1, r2, cdx, cdy, dr,
   // min_dr, a, 1/a; conic: the start angle mod 2 pi)
@@ -251,14 +246,9 @@ struct JobView {
   // the others)
   uint32_t *strip_slow;
   FastDraw *fast;            // per draw: k_render_fast's constants
-  // Per (fast entry, tile column) block of band_h * 4 int16: for each sample
-  // row of the band, the winding of the entry's crossings left of the
-  // tile's first sample column (k_backdrop); null: no backdrops
-  int16_t *backdrop;
-  uint32_t n_bd_blocks;
   // [2] binning overflow flag; [4..5] pixel-draws (u64); [6] edges;
   // [7] tile-draw entries rasterized; [8] entries on the unpacked path;
-  // [9] strips for the general renderer; [10] strips for the fast one
+  // [9] strips for the general renderer
   uint32_t *counters;
   uint32_t count_stats;  // nonzero: accumulate pixel-draw / edge counts
   uint32_t *cursors;     // tile work cursors, one per render launch

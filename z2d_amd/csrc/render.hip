@@ -278,10 +278,6 @@ __device__ void write_fast_draw(FastDraw &F, const DevDraw &D, const DevPaint &P
   F.px = P.px.r | (P.px.g << 8) | (P.px.b << 16) | (P.px.a << 24);
   F.n_stops = 0;
   F.identity = 1;
-  F.band_lo = D.band_lo;
-  F.tcol_lo = D.tcol_lo;
-  F.n_tcol = D.n_tcol;
-  F.bd_block = D.bd_block;
   if (pf < FAST_PAINT_LINEAR) return;
   const DevGradient &G = *P.gradient;
   F.n_stops = G.n_stops;
@@ -443,7 +439,7 @@ __global__ void __launch_bounds__(256) k_edge_build(JobView J) {
   if (J.fast) {
     const uint32_t pf = fast_paint(P);
     if (pf != FAST_PAINT_NONE && D.mode == MODE_MSAA && D.op == Z2D_OP_SRC_OVER &&
-        D.precision == Z2D_PRECISION_INTEGER && D.n_tcol > 0 &&
+        D.precision == Z2D_PRECISION_INTEGER &&
         !(D.flags & (DRAW_ROWPROG | DRAW_UNBALANCED | DRAW_UNBOUNDED))) {
       D.flags |= DRAW_FAST;
       write_fast_draw(J.fast[d], D, P, pf);
@@ -1167,7 +1163,6 @@ __global__ void k_band_pass(JobView J) {
           e.px_hi = (int32_t)floor((double)xh / sc);
         }
       }
-      e.bd_blk0 = Dd.bd_block + (lb - Dd.band_lo) * Dd.n_tcol;
       J.entries[idx] = e;
     } else {
       atomicOr(&J.counters[2], 1u);
@@ -1367,7 +1362,6 @@ __global__ void k_strip_pass(JobView J) {
         n_slow += sl ? 1u : 0u;
       }
     if (n_slow) atomicAdd(&J.counters[9], n_slow);
-    if (spt - n_slow) atomicAdd(&J.counters[10], spt - n_slow);
   }
 }
 
@@ -2192,111 +2186,6 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
 }
 
 // ---------------------------------------------------------------------------
-// k_backdrop: for each listed entry of a fast draw, the winding each tile
-// column of the draw starts with on each sample row of the band: the sum of
-// the directions (1 under even-odd) of the entry's crossings strictly left
-// of the tile's first sample column, crossings computed as the renderer
-// computes them (redge_x, rounded).  The fast renderer seeds a strip's
-// column-0 winding with it and adds only the crossings that fall inside the
-// strip.  One wave per entry: the entry's (edge, sample row) pairs are
-// compacted by a wave scan, each adds its direction to a per-row difference
-// array over the tile columns of a window (BD_WIN columns) in LDS, a per-row
-// prefix turns that into the blocks (int16: an entry on the fast path has at
-// most PACKED_EDGE_LIMIT edges).
-// ---------------------------------------------------------------------------
-constexpr uint32_t BD_WIN = 32;
-constexpr uint32_t BD_ROWS = MAX_BAND_H * 4;
-
-__global__ void __launch_bounds__(64) k_backdrop(JobView J) {
-  __shared__ int32_t diff[BD_ROWS][BD_WIN + 1];
-  __shared__ double c_top[64], c_xs[64], c_xi[64];
-  __shared__ uint8_t c_lo[64];
-  __shared__ uint16_t c_off[64];
-  __shared__ uint8_t own[64 * BD_ROWS];  // owning lane | 0x80: direction -1
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t pos = blockIdx.x;
-  if (pos >= min(load_uniform(&J.band_cnt[J.n_bands]), J.max_entries)) return;
-  const uint32_t idx = load_uniform(&J.band_list[pos]);
-  const BandEntry E = load_uniform(&J.entries[idx]);
-  const DevDraw &D = J.draws[E.draw];
-  const uint32_t dflags = load_uniform(&D.flags);
-  if (!(dflags & DRAW_FAST) || E.edge_cnt > PACKED_EDGE_LIMIT) return;
-  const uint32_t s = load_uniform(&D.surface);
-  const uint32_t lb = E.band - load_uniform(&J.surf_band_off[s]);
-  const bool eo = load_uniform(&D.rule) == Z2D_FILL_EVEN_ODD;
-  const uint32_t n_tcol = load_uniform(&D.n_tcol), tcol_lo = load_uniform(&D.tcol_lo);
-  const int32_t rows = (int32_t)J.band_h * 4;  // sample rows of a band
-  const int32_t y0 = (int32_t)lb * rows;
-  const REdge *edges = J.entry_edges + E.edge_off;
-  const uint32_t ne =
-      E.edge_off >= J.max_entry_edges ? 0u : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
-  if (E.bd_blk0 >= J.n_bd_blocks || n_tcol > J.n_bd_blocks - E.bd_blk0) return;
-  int16_t *out = J.backdrop + (uint64_t)E.bd_blk0 * (uint32_t)rows;
-  constexpr int32_t TCOLS = TILE_W * 4;  // sample columns of a tile
-  for (uint32_t w0 = 0; w0 < n_tcol; w0 += BD_WIN) {
-    const uint32_t nw = min(BD_WIN, n_tcol - w0);
-    const int32_t c0 = (int32_t)(tcol_lo + w0);  // the window's first tile column
-    for (uint32_t i = lane; i < BD_ROWS * (BD_WIN + 1); i += 64) (&diff[0][0])[i] = 0;
-    wave_lds_sync();
-    for (uint32_t k0 = 0; k0 < ne; k0 += 64) {
-      const uint32_t k = k0 + lane;
-      int32_t n_r = 0, lo = 0;
-      REdge ed{0.0, 0.0, 0.0, 0, 0};
-      if (k < ne) {
-        ed = edges[k];
-        const int32_t a = max(ed.ylo, y0), b = min(ed.yhi2 >> 1, y0 + rows - 1);
-        if (a <= b) {
-          n_r = b - a + 1;
-          lo = a - y0;
-          // crossings at or right of the window's last tile start reach no
-          // tile of it
-          const double xa = redge_x(ed.top, ed.x_start, ed.x_inc, a);
-          const double xb = redge_x(ed.top, ed.x_start, ed.x_inc, b);
-          if (fmin(xa, xb) > (double)((int64_t)(c0 + (int32_t)nw) * TCOLS) + 1.0) n_r = 0;
-        }
-      }
-      const int32_t incl = wave_incl_scan(n_r);
-      const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
-      if (total == 0) continue;
-      const uint32_t my_off = (uint32_t)(incl - n_r);
-      c_top[lane] = ed.top;
-      c_xs[lane] = ed.x_start;
-      c_xi[lane] = ed.x_inc;
-      c_lo[lane] = (uint8_t)lo;
-      c_off[lane] = (uint16_t)my_off;
-      const uint8_t tag = (uint8_t)(lane | ((!eo && (ed.yhi2 & 1)) ? 0x80u : 0u));
-      for (int t = 0; t < n_r; t++) own[my_off + (uint32_t)t] = tag;
-      wave_lds_sync();
-      for (uint32_t it = lane; it < total; it += 64) {
-        const uint32_t v = own[it];
-        const uint32_t lj = v & 63u;
-        const int r = (int)c_lo[lj] + (int)(it - c_off[lj]);
-        const int32_t x = (int32_t)round(redge_x(c_top[lj], c_xs[lj], c_xi[lj], y0 + r));
-        // tiles c with c * TCOLS > x: c >= floor(x / TCOLS) + 1
-        const int32_t kstart = max((x >> 8) + 1 - c0, 0);
-        static_assert(TCOLS == 256, "tile columns are 256 sample columns");
-        if (kstart >= (int32_t)nw) continue;
-        atomicAdd(&diff[r][kstart], (v & 0x80u) ? -1 : 1);
-      }
-      wave_lds_sync();
-    }
-    // per-row prefix over the window's tile columns, then the blocks
-    for (int32_t r = (int32_t)lane; r < rows; r += 64) {
-      int32_t acc = 0;
-      for (uint32_t c = 0; c < nw; c++) {
-        acc += diff[r][c];
-        diff[r][c] = acc;
-      }
-    }
-    wave_lds_sync();
-    for (uint32_t c = 0; c < nw; c++)
-      for (int32_t r = (int32_t)lane; r < rows; r += 64)
-        out[(uint64_t)(w0 + c) * (uint32_t)rows + (uint32_t)r] = (int16_t)diff[r][c];
-    wave_lds_sync();
-  }
-}
-
-// ---------------------------------------------------------------------------
 // k_render_fast: the strip worker for the common draw (DRAW_FAST: multisample,
 // src_over in integer precision, a solid paint or a linear-RGB gradient of
 // at most FAST_STOPS stops, an edge list the packed deltas hold), over the
@@ -2325,6 +2214,9 @@ struct FastShared {
     } c;
   } u;
   FastDraw blk;  // the current entry's draw constants
+  // winding of the edges wholly left of the strip, as a difference array
+  // over its sample rows (zero between entries)
+  int32_t ldiff[STRIP_SUBROWS + 1];
 };
 
 // Gradient.getOffset (src/gradient.zig:349-375, :605-652, :731-742) on a
@@ -2428,11 +2320,11 @@ k_render_fast(JobView J, uint32_t tile_base,
                                                      uint32_t ordered) {
   __shared__ FastShared sh;
   const uint32_t lane = threadIdx.x & 63;
-  if (load_uniform(&J.counters[10]) == 0) return;  // every strip is k_render's
   const uint32_t spt = J.strips_per_tile;
   const uint32_t n_items = tile_count * spt;
   for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
     (&sh.acc[0][0])[i] = 0u;
+  if (lane <= (uint32_t)STRIP_SUBROWS) sh.ldiff[lane] = 0;
   wave_lds_sync();
 
   uint32_t stat_pix = 0, stat_entries = 0;  // work counters (count_stats)
@@ -2478,12 +2370,11 @@ k_render_fast(JobView J, uint32_t tile_base,
     for (uint32_t c = e0; c < e1; c += 64) {
       // this chunk's entries, one per lane
       const uint32_t nc = min(64u, e1 - c);
-      uint32_t g_draw = 0, g_off = 0, g_cnt = 0, g_bd = 0;
+      uint32_t g_draw = 0, g_off = 0, g_cnt = 0;
       if (lane < nc) {
         const BandEntry E = J.entries[J.strip_entries[c + lane]];
         g_draw = E.draw;
         g_off = E.edge_off;
-        g_bd = E.bd_blk0;
         g_cnt = E.edge_off >= J.max_entry_edges ? 0u
                                                 : min(E.edge_cnt, J.max_entry_edges - E.edge_off);
       }
@@ -2511,19 +2402,12 @@ k_render_fast(JobView J, uint32_t tile_base,
         const uint32_t ne = __builtin_amdgcn_readlane(g_cnt, j);
         const REdge *edges = J.entry_edges + __builtin_amdgcn_readlane(g_off, j);
         if (J.count_stats) stat_entries++;
-        // the strip's backdrop: each sample row's winding from the entry's
-        // crossings left of the strip (k_backdrop), added after phase A
-        int32_t backdrop = 0;
-        {
-          const uint32_t blk = __builtin_amdgcn_readlane(g_bd, j) +
-                               ((uint32_t)T.tx - (uint32_t)rfl(B.tcol_lo));
-          if (lane < (uint32_t)SR && blk < J.n_bd_blocks)
-            backdrop = J.backdrop[(uint64_t)blk * (J.band_h * 4) + strip * SR + lane];
-        }
 
-        // ---- phase A: winding deltas of the crossings inside the strip
-        // (those left of it are in the backdrop)
+        // ---- phase A: winding deltas of the band edges (as k_render; an
+        // edge wholly left of the strip adds its direction at column 0 of
+        // each of its rows through two entries of a row difference array)
         uint32_t inner = 0;
+        bool any_left = false;
         for (uint32_t c0 = 0; c0 < ne; c0 += 64) {
           const uint32_t k = c0 + lane;
           int32_t n_sr = 0, lo = 0;
@@ -2536,10 +2420,17 @@ k_render_fast(JobView J, uint32_t tile_base,
               lo = a - ys_base;
               const double xa = redge_x(ed.top, ed.x_start, ed.x_inc, a);
               const double xb = redge_x(ed.top, ed.x_start, ed.x_inc, b);
-              // every crossing right of the strip, or left of it: none here
-              if (fmin(xa, xb) > (double)(xs_base + COLS) + 1.0 ||
-                  fmax(xa, xb) < (double)xs_base - 1.0)
+              if (fmin(xa, xb) > (double)(xs_base + COLS) + 1.0) {
+                n_sr = 0;  // every crossing right of the strip: none here
+              } else if (fmax(xa, xb) < (double)xs_base - 1.0) {
+                // every crossing left of the strip (rounded below its first
+                // sample column): column 0 of rows [lo, lo + n_sr)
+                const int32_t dv = (!eo && (ed.yhi2 & 1)) ? -1 : 1;
+                atomicAdd(&sh.ldiff[lo], dv);
+                atomicAdd(&sh.ldiff[lo + n_sr], -dv);
+                any_left = true;
                 n_sr = 0;
+              }
             }
           }
           const int32_t incl = wave_incl_scan(n_sr);
@@ -2560,17 +2451,28 @@ k_render_fast(JobView J, uint32_t tile_base,
             const int sr = (int)sh.u.a.lo[lj] + (int)(it - sh.u.a.off[lj]);
             const int32_t x = (int32_t)round(
                 redge_x(sh.u.a.top[lj], sh.u.a.xs[lj], sh.u.a.xi[lj], ys_base + sr));
-            const int32_t col = x - xs_base;
-            if (col < 0 || col >= COLS) continue;  // left: in the backdrop
+            const int32_t xr = x - xs_base;
+            if (xr >= COLS) continue;
+            const int32_t col = xr < 0 ? 0 : xr;
             inner |= (uint32_t)col;
             const uint32_t dv = (v & 0x80u) ? 0xFFFFFFFFu : 1u;
             atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? (dv << 16) : dv);
           }
           wave_lds_sync();
         }
-        if (lane < (uint32_t)SR && backdrop != 0)
-          atomicAdd(&sh.acc[lane][0], (uint32_t)backdrop);
-        wave_lds_sync();
+
+        if (__ballot(any_left)) {
+          // the left edges' winding per row: prefix of the difference array
+          wave_lds_sync();
+          int32_t dl = 0;
+          if (lane <= (uint32_t)SR) {
+            dl = lane < (uint32_t)SR ? sh.ldiff[lane] : 0;
+            sh.ldiff[lane] = 0;
+          }
+          const int32_t wl = wave_incl_scan(dl);
+          if (lane < (uint32_t)SR && wl != 0) atomicAdd(&sh.acc[lane][0], (uint32_t)wl);
+          wave_lds_sync();
+        }
 
         // ---- phase B: covered samples per pixel
         int kk[ROWS_PER_THREAD];
@@ -2959,8 +2861,6 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   hipLaunchKernelGGL(k_band_pass<false>, grid, dim3(256), 0, st, J);
   scan_u32(J.slot_cnt, J.n_slots, J.scan_tmp, st);
   hipLaunchKernelGGL(k_band_pass<true>, grid, dim3(256), 0, st, J);
-  if (J.backdrop && J.max_entries)
-    hipLaunchKernelGGL(k_backdrop, dim3(J.max_entries), dim3(64), 0, st, J);
   if (J.n_tiles) {
     dim3 tgrid((J.n_tiles + 3) / 4);
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
