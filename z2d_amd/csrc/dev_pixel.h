@@ -730,47 +730,38 @@ __device__ inline void stops_search(const float *st, uint32_t n_stops, float off
   t = rl != 0.0f ? (o - st[mid]) / rl : 0.0f;
 }
 
-// Stop.List.search for a list of at most FAST_STOPS stops whose offsets every
-// lane reads at the same addresses: when the offset equals no stop offset,
-// the reference's binary search can only end on the one interval
-// st[m] < o < st[m + 1] that holds it (or on an end stop), so m follows from
-// the number of stops below the offset, with no data-dependent loop; an
+// Stop.List.search for a list of at most 8 stops whose offsets every lane
+// reads at the same addresses, padded with +infinity up to 8: when the
+// offset equals no stop offset, the reference's binary search can only end
+// on the one interval st[m] < o < st[m + 1] that holds it, or on an end stop
+// (where both colors are that stop's and t does not matter), so m follows
+// from the number of stops below the offset with no data-dependent loop; an
 // offset equal to some stop offset (where the search path picks among equal
 // stops) goes through stops_search itself.
-__device__ __forceinline__ void stops_find(const float *st, uint32_t n_stops, float offset,
-                                           int &i0, int &i1, float &t, bool &transparent) {
+__device__ __forceinline__ void stops_find8(const float *st, uint32_t n_stops, float offset,
+                                            int &i0, int &i1, float &t, bool &transparent) {
   const int n = (int)n_stops;
-  if (offset < 0.0f || n == 0) {
-    transparent = true;
+  transparent = offset < 0.0f || n == 0;
+  const float o = fminz(offset, 1.0f);
+  const float4 a = *reinterpret_cast<const float4 *>(st);
+  const float4 b = *reinterpret_cast<const float4 *>(st + 4);
+  const int below = (a.x < o) + (a.y < o) + (a.z < o) + (a.w < o) + (b.x < o) + (b.y < o) +
+                    (b.z < o) + (b.w < o);
+  const bool tie = (a.x == o) | (a.y == o) | (a.z == o) | (a.w == o) | (b.x == o) |
+                   (b.y == o) | (b.z == o) | (b.w == o);
+  if (transparent) {
     i0 = i1 = 0;
     t = 0.0f;
     return;
-  }
-  transparent = false;
-  const float o = fminz(offset, 1.0f);
-  int below = 0;
-  bool tie = false;
-  for (int i = 0; i < n; i++) {
-    const float s = st[i];
-    below += s < o ? 1 : 0;
-    tie |= s == o;
   }
   if (tie) {
     stops_search(st, n_stops, offset, i0, i1, t, transparent);
     return;
   }
-  if (below == n || (n == 1 && below == 0)) {  // the search ends on the last stop
-    i0 = i1 = n - 1;
-    t = o - st[n - 1];
-    return;
-  }
-  if (below == 0) {  // before the first stop
-    i0 = i1 = 0;
-    t = o / st[0];
-    return;
-  }
-  i0 = below - 1;
-  i1 = below;
+  // below == 0: before the first stop; below == n: after the last; either
+  // way one stop's color, which t leaves unchanged
+  i0 = below > 0 ? below - 1 : 0;
+  i1 = below < n ? below : n - 1;
   const float rl = st[i1] - st[i0];
   t = rl != 0.0f ? (o - st[i0]) / rl : 0.0f;
 }

@@ -303,9 +303,9 @@ __device__ void write_fast_draw(FastDraw &F, const DevDraw &D, const DevPaint &P
   } else {
     F.p[0] = G.conic_angle;
   }
-  for (uint32_t i = 0; i < G.n_stops; i++) {
-    F.offs[i] = G.offs[i];
-    F.pre[i] = G.pre[i];
+  for (uint32_t i = 0; i < FAST_STOPS; i++) {
+    F.offs[i] = i < G.n_stops ? G.offs[i] : __builtin_inff();
+    F.pre[i] = i < G.n_stops ? G.pre[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
 
@@ -2296,7 +2296,7 @@ __device__ __forceinline__ void fast_gradient(const FastDraw &B, FastShared &sh,
     int i0, i1;
     float t;
     bool transparent;
-    stops_find(B.offs, n_stops, off, i0, i1, t, transparent);
+    stops_find8(B.offs, n_stops, off, i0, i1, t, transparent);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 A = transparent ? z : B.pre[i0];
     const float4 C2 = transparent ? z : B.pre[i1];
