@@ -17,6 +17,7 @@
 #include <thread>
 #include <unordered_map>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -47,6 +48,7 @@ hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t z2d_launch_multi_copy(const z2d::CopyCmd *cmds, uint32_t n, hipStream_t st);
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st);
 hipError_t z2d_launch_png(const z2d::PngJob &J, hipStream_t st);  // png.hip
@@ -647,6 +649,11 @@ struct z2d_gpu_batch {
   // status) instead of failing the whole batch (the server's requests fail
   // one by one).
   bool isolate_failures = false;
+  double t_count_wait = 0;  // seconds the last prepare waited for its count pass
+  // async_prepare: prepare returns with its uploads still in flight (the
+  // caller keeps the batch's staging buffers untouched until the stream
+  // passes them: a server slot until its done event)
+  bool async_prepare = false;
   std::vector<std::pair<uint32_t, int32_t>> failed;
   // hipGraph replay of the render pipeline (z2d_gpu_batch_use_graph)
   bool use_graph = false;
@@ -872,6 +879,7 @@ static void batch_drop_graph(z2d_gpu_batch *b) {
 static int32_t batch_prepare(z2d_gpu_batch *b) {
   z2d_gpu_ctx *ctx = b->ctx;
   if (b->prepared) return Z2D_OK;
+  b->t_count_wait = 0;
   HIP_OK(ctx, hipSetDevice(ctx->device));
   batch_drop_graph(b);  // its kernel arguments point into the arena
   const uint32_t ns = (uint32_t)b->surfaces.size();
@@ -940,7 +948,10 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
                                    reinterpret_cast<FlatCurve *>(FD + fo_cv), false, st));
     HIP_OK(ctx, hipMemcpyAsync(FH + fo_res, FD + fo_res, jobs.size() * sizeof(FlatResult),
                                hipMemcpyDeviceToHost, st));
+    const auto cw0 = std::chrono::steady_clock::now();
     HIP_OK(ctx, hipStreamSynchronize(st));
+    b->t_count_wait =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - cw0).count();
     const FlatResult *res = reinterpret_cast<const FlatResult *>(FH + fo_res);
     b->failed.clear();
     for (size_t k = 0; k < jobs.size(); k++) {
@@ -1235,7 +1246,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   b->d_rowprog_soff = reinterpret_cast<uint32_t *>(D + o_rps);
   b->d_scratch_edges = reinterpret_cast<z2d_edge *>(D + o_sedges);
   b->d_scratch_x = reinterpret_cast<int32_t *>(D + o_sx);
-  HIP_OK(ctx, hipStreamSynchronize(bstream(b)));
+  if (!b->async_prepare) HIP_OK(ctx, hipStreamSynchronize(bstream(b)));
   b->prepared = true;
   return Z2D_OK;
 }
@@ -2466,6 +2477,7 @@ struct SrvRequest {
   uint8_t *mask_dev = nullptr;
   int32_t status = Z2D_OK;
   bool done = false;
+  double t_queued = 0;  // when it became ready for admission (steady clock, s)
 };
 
 struct SrvSlot {
@@ -2473,11 +2485,30 @@ struct SrvSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   std::vector<SrvRequest *> reqs;
+  // inputs: the batch's copy commands (surface clears and clip-mask scatters
+  // before the render, result gathers after it) and its clip masks, one
+  // upload; results: gathered on the device, one read-back
+  HostBuf in_stage;
+  DevBuf in_dev;
+  DevBuf result_dev;
   HostBuf result_stage;
   std::vector<size_t> stage_off;
+  std::vector<CopyCmd> pre_cmds, post_cmds;
   DevBuf png_work, png_files;  // PNG encodes of the batch: work, then meta + file
   std::vector<size_t> png_off;
-  bool busy = false;
+  // free -> running (launched) -> copying (results on their way to the
+  // callers) -> free; changed under the server's mutex
+  enum State { FREE, RUNNING, COPYING } state = FREE;
+  uint32_t pending = 0;  // results still being copied to callers (under the server's mutex)
+};
+
+// A finished request's result on its way from the pinned stage to the
+// caller's buffer (done by the server's copy threads).
+struct SrvCopy {
+  SrvRequest *r;
+  SrvSlot *slot;
+  const uint8_t *src;
+  uint64_t n;
 };
 
 }  // namespace
@@ -2495,9 +2526,21 @@ struct z2d_gpu_server {
   uint64_t next_id = 1;
   uint64_t used = 0, peak = 0;
   uint64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_batches = 0, pixel_draws = 0;
+  uint32_t n_running = 0;  // batches launched and not yet finalized (under the mutex)
+  // Continuous batching: a worker with a free slot launches at once while
+  // fewer than kBusyBatches batches are on the device (a light load is
+  // answered immediately); beyond that the device is busy anyway, so it
+  // waits for a full batch, for the device to drain below that count, or
+  // until the oldest queued request has waited kMaxHoldS: a saturated
+  // server runs few large batches.
+  static constexpr uint32_t kBusyBatches = 2;
+  static constexpr double kMaxHoldS = 4e-3;
   bool stop = false;
-  SrvSlot slots[2];
-  std::thread worker;
+  // worker threads, each building and launching batches on its own two
+  // slots (own streams), so batch preparation runs in parallel
+  static constexpr unsigned kWorkers = 4, kSlotsPerWorker = 2;
+  std::vector<std::unique_ptr<SrvSlot>> slots;
+  std::vector<std::thread> workers;
   std::string err;
   // preparation threads: the host geometry of submitted requests (flattened
   // fills, stroke polygons) is built in parallel before admission
@@ -2505,6 +2548,58 @@ struct z2d_gpu_server {
   std::deque<SrvRequest *> prep_q;
   std::condition_variable cv_prep;
   bool stop_prep = false;
+  // copy threads: results from the pinned stage into callers' buffers, in
+  // parallel, off the worker thread
+  std::vector<std::thread> copiers;
+  std::deque<SrvCopy> copy_q;
+  std::condition_variable cv_copy;
+  bool stop_copy = false;
+
+  // Marks a request complete (mutex held).
+  void complete_locked(SrvRequest *r) {
+    r->done = true;
+    completed.push_back(r->id);
+    n_completed++;
+  }
+  // A slot whose results have all reached their callers is free (mutex held).
+  void slot_copied_locked(SrvSlot &sl) {
+    if (--sl.pending == 0) {
+      sl.reqs.clear();
+      sl.state = SrvSlot::FREE;
+      cv_work.notify_all();
+    }
+  }
+
+  void copy_loop() {
+    for (;;) {
+      SrvCopy c;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_copy.wait(lk, [&] { return stop_copy || !copy_q.empty(); });
+        if (copy_q.empty()) return;  // stopping and drained
+        c = copy_q.front();
+        copy_q.pop_front();
+      }
+      const double c0 = now_s();
+      if (c.n) std::memcpy(c.r->out, c.src, c.n);
+      const double c1 = now_s();
+      std::lock_guard<std::mutex> lk(mu);
+      t_copythreads += c1 - c0;
+      complete_locked(c.r);
+      slot_copied_locked(*c.slot);
+      cv_done.notify_all();
+    }
+  }
+  // worker-thread phase times in seconds, summed over the workers (under the
+  // mutex; reported at destroy when the environment sets Z2D_SERVER_TRACE):
+  // batch build, the count pass wait, the rest of prepare, render +
+  // result-copy enqueue, finalize, idle
+  double t_build = 0, t_count = 0, t_prep = 0, t_launch = 0, t_fin = 0, t_idle = 0;
+  double t_prepthreads = 0, t_copythreads = 0;  // busy time of the prep and copy threads
+  static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+  }
 
   void prep_loop() {
     for (;;) {
@@ -2516,18 +2611,28 @@ struct z2d_gpu_server {
         r = prep_q.front();
         prep_q.pop_front();
       }
+      const double p0 = now_s();
       for (auto &f : r->fills)
         f->geo_rc = f->kind == Z2D_DRAW_STROKE
                         ? stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
                                       f->stroke)
                         : fill_checks(f->nodes.data(), f->nodes.size(), &f->opts);
       std::lock_guard<std::mutex> lk(mu);
+      r->t_queued = now_s();
+      t_prepthreads += r->t_queued - p0;
       queue.push_back(r);
       cv_work.notify_all();
     }
   }
 
   void finalize(SrvSlot &sl) {
+    const double f0 = now_s();
+    finalize_impl(sl);
+    const double f1 = now_s();
+    std::lock_guard<std::mutex> lk(mu);
+    t_fin += f1 - f0;
+  }
+  void finalize_impl(SrvSlot &sl) {
     // the batch's counters reached pinned memory before its done event
     uint64_t pix = 0;
     bool overflow = false;
@@ -2536,10 +2641,13 @@ struct z2d_gpu_server {
       pix = (uint64_t)c[4] | ((uint64_t)c[5] << 32);
       overflow = c[2] != 0;
     }
+    std::vector<SrvCopy> copies;
+    copies.reserve(sl.reqs.size());
     for (size_t i = 0; i < sl.reqs.size(); i++) {
       SrvRequest *r = sl.reqs[i];
       if (overflow && r->status == Z2D_OK) r->status = Z2D_ERR_DEVICE;  // bin overflow
       const uint8_t *res = sl.result_stage.p + sl.stage_off[i];
+      SrvCopy c{r, &sl, nullptr, 0};
       if (r->encode && r->status == Z2D_OK) {
         const uint64_t *meta = reinterpret_cast<const uint64_t *>(res);
         const uint64_t flen = meta[0];
@@ -2548,52 +2656,86 @@ struct z2d_gpu_server {
         } else {
           if (r->out_written) *r->out_written = flen;
           if (r->out) {
-            if (flen <= r->out_len) std::memcpy(r->out, res + 256, flen);
-            else r->status = Z2D_ERR_INVALID_ARGUMENT;
+            if (flen <= r->out_len) {
+              c.src = res + 256;
+              c.n = flen;
+            } else {
+              r->status = Z2D_ERR_INVALID_ARGUMENT;
+            }
           }
         }
       } else if (r->status == Z2D_OK) {
-        if (r->out) std::memcpy(r->out, res, std::min<uint64_t>(r->out_len, r->surf.nbytes));
-        if (r->out_written) *r->out_written = r->out ? std::min<uint64_t>(r->out_len, r->surf.nbytes) : 0;
+        const uint64_t n = r->out ? std::min<uint64_t>(r->out_len, r->surf.nbytes) : 0;
+        if (r->out_written) *r->out_written = n;
+        c.src = res;
+        c.n = n;
       }
+      copies.push_back(c);
     }
     std::lock_guard<std::mutex> lk(mu);
-    for (SrvRequest *r : sl.reqs) {
+    for (SrvRequest *r : sl.reqs) {  // the device is done with their memory
       pool.release(r->surf.data, r->surf_bytes);
       if (r->mask_dev) pool.release(r->mask_dev, r->mask_bytes);
       used -= r->surf_bytes + r->mask_bytes;
-      r->done = true;
-      completed.push_back(r->id);
-      n_completed++;
     }
     pixel_draws += pix;
-    sl.reqs.clear();
-    sl.busy = false;
+    n_running--;
+    sl.state = SrvSlot::COPYING;
+    sl.pending = 1;  // released below, after the copies are queued
+    for (const SrvCopy &c : copies) {
+      if (c.n) {
+        sl.pending++;
+        copy_q.push_back(c);
+      } else {
+        complete_locked(c.r);
+      }
+    }
+    slot_copied_locked(sl);
+    cv_copy.notify_all();
     cv_done.notify_all();
     cv_work.notify_all();
   }
 
   // Builds and launches one batch on a free slot (requests already admitted
-  // and their device memory reserved).
+  // and their device memory reserved): one upload of copy commands and clip
+  // masks, one launch for every surface clear and mask scatter, the
+  // batch's pipeline, PNG encodes, one launch gathering every result and one
+  // read-back into the pinned stage.
   void launch(SrvSlot &sl, std::vector<SrvRequest *> &take) {
+    const double l0 = now_s();
     hipStream_t st = sl.stream;
     batch_clear(&sl.batch);
     sl.batch.ctx = ctx;
     sl.batch.stream = st;
     sl.batch.count_stats = 1;
     sl.batch.isolate_failures = true;
+    sl.batch.async_prepare = true;  // this slot's buffers wait for its done event
     std::vector<uint32_t> seq0(take.size() + 1, 0);  // each request's draws: [seq0[i], seq0[i+1])
-    size_t stage = 0, files = 0;
+    size_t stage = 0, files = 0, masks = 0;
     sl.stage_off.clear();
     sl.png_off.clear();
-    for (SrvRequest *r : take) {
+    sl.pre_cmds.clear();
+    sl.post_cmds.clear();
+    std::vector<size_t> mask_off(take.size(), 0);
+    for (size_t ri = 0; ri < take.size(); ri++) {
+      SrvRequest *r = take[ri];
       sl.stage_off.push_back(stage);
       sl.png_off.push_back(files);
       const size_t res = r->encode ? 256 + r->png.bound : r->surf.nbytes;
       stage += (res + 255) & ~(size_t)255;
       if (r->encode) files += (256 + r->png.bound + 255) & ~(size_t)255;
+      if (r->mask_dev) {
+        mask_off[ri] = masks;
+        masks += (r->mask.size() + 255) & ~(size_t)255;
+      }
     }
+    const size_t n_cmd_max = 3 * take.size();
+    const size_t o_masks = (n_cmd_max * sizeof(CopyCmd) + 255) & ~(size_t)255;
+    const size_t in_bytes = o_masks + masks;
     hipError_t e = sl.result_stage.ensure(stage);
+    if (e == hipSuccess) e = sl.result_dev.ensure(stage);
+    if (e == hipSuccess) e = sl.in_stage.ensure(in_bytes);
+    if (e == hipSuccess) e = sl.in_dev.ensure(in_bytes);
     if (e == hipSuccess && files) e = sl.png_files.ensure(files);
     for (size_t ri = 0; ri < take.size(); ri++) {
       SrvRequest *r = take[ri];
@@ -2616,10 +2758,12 @@ struct z2d_gpu_server {
           raw = byte | (byte << 8) | (byte << 16) | (byte << 24);
         } break;
       }
-      e = z2d_launch_fill_u32(reinterpret_cast<uint32_t *>(r->surf.data),
-                              padded_bytes(r->surf.nbytes) / 4, raw, st);
-      if (e == hipSuccess && r->mask_dev)
-        e = hipMemcpyAsync(r->mask_dev, r->mask.data(), r->mask.size(), hipMemcpyHostToDevice, st);
+      sl.pre_cmds.push_back(CopyCmd{nullptr, r->surf.data, padded_bytes(r->surf.nbytes), raw, 0});
+      if (r->mask_dev) {
+        std::memcpy(sl.in_stage.p + o_masks + mask_off[ri], r->mask.data(), r->mask.size());
+        sl.pre_cmds.push_back(CopyCmd{sl.in_dev.p + o_masks + mask_off[ri], r->mask_dev,
+                                      r->mask.size(), 0, 0});
+      }
       r->surf.clip = r->mask_dev;
       // a clipped surface takes part in the render even with no fills
       if (r->mask_dev) sl.batch.slot_of(&r->surf);
@@ -2633,8 +2777,24 @@ struct z2d_gpu_server {
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
       }
       seq0[ri + 1] = (uint32_t)sl.batch.draws.size();
+      // the result: the surface, or the PNG region (meta, then the file at +256)
+      const uint8_t *src = r->encode ? sl.png_files.p + sl.png_off[ri] : r->surf.data;
+      const uint64_t n = r->encode ? 256 + r->png.bound : r->surf.nbytes;
+      sl.post_cmds.push_back(CopyCmd{src, sl.result_dev.p + sl.stage_off[ri], n, 0, 0});
     }
+    const size_t n_pre = sl.pre_cmds.size(), n_post = sl.post_cmds.size();
+    if (e == hipSuccess) {
+      std::memcpy(sl.in_stage.p, sl.pre_cmds.data(), n_pre * sizeof(CopyCmd));
+      std::memcpy(sl.in_stage.p + n_pre * sizeof(CopyCmd), sl.post_cmds.data(),
+                  n_post * sizeof(CopyCmd));
+      e = hipMemcpyAsync(sl.in_dev.p, sl.in_stage.p, in_bytes, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess)
+      e = z2d_launch_multi_copy(reinterpret_cast<const CopyCmd *>(sl.in_dev.p), (uint32_t)n_pre,
+                                st);
+    const double l1 = now_s();
     int32_t rc = e == hipSuccess ? batch_prepare(&sl.batch) : Z2D_ERR_DEVICE;
+    const double l2 = now_s();
     // draws the flattener refused fail their own request only
     for (const auto &fd : sl.batch.failed) {
       const size_t ri =
@@ -2644,51 +2804,72 @@ struct z2d_gpu_server {
     if (rc == Z2D_OK) rc = batch_render(&sl.batch, 0);
     for (size_t i = 0; i < take.size() && rc == Z2D_OK; i++) {
       SrvRequest *r = take[i];
-      hipError_t ce;
-      if (r->encode) {  // the PNG file (meta, then the file at +256) instead of the surface
-        uint8_t *region = sl.png_files.p + sl.png_off[i];
-        ce = png_enqueue(r->surf, r->encode - 1, r->png, sl.png_work, region + 256, r->png.bound,
-                         reinterpret_cast<uint64_t *>(region), st);
-        if (ce == hipSuccess)
-          ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], region, 256 + r->png.bound,
-                              hipMemcpyDeviceToHost, st);
-      } else {
-        ce = hipMemcpyAsync(sl.result_stage.p + sl.stage_off[i], r->surf.data, r->surf.nbytes,
-                            hipMemcpyDeviceToHost, st);
-      }
-      if (ce != hipSuccess) rc = Z2D_ERR_DEVICE;
+      if (!r->encode) continue;
+      uint8_t *region = sl.png_files.p + sl.png_off[i];
+      if (png_enqueue(r->surf, r->encode - 1, r->png, sl.png_work, region + 256, r->png.bound,
+                      reinterpret_cast<uint64_t *>(region), st) != hipSuccess)
+        rc = Z2D_ERR_DEVICE;
     }
+    if (rc == Z2D_OK &&
+        z2d_launch_multi_copy(reinterpret_cast<const CopyCmd *>(sl.in_dev.p) + n_pre,
+                              (uint32_t)n_post, st) != hipSuccess)
+      rc = Z2D_ERR_DEVICE;
+    if (rc == Z2D_OK && stage &&
+        hipMemcpyAsync(sl.result_stage.p, sl.result_dev.p, stage, hipMemcpyDeviceToHost, st) !=
+            hipSuccess)
+      rc = Z2D_ERR_DEVICE;
     if (rc != Z2D_OK)
       for (SrvRequest *r : take)
         if (r->status == Z2D_OK) r->status = rc;
     (void)hipEventRecord(sl.done, st);
     sl.reqs = take;
-    sl.busy = true;
+    const double l3 = now_s();
     std::lock_guard<std::mutex> lk(mu);
+    sl.state = SrvSlot::RUNNING;
+    n_running++;
     n_batches++;
+    t_build += l1 - l0;
+    t_count += sl.batch.t_count_wait;
+    t_prep += (l2 - l1) - sl.batch.t_count_wait;
+    t_launch += l3 - l2;
   }
 
-  void run() {
+  void run(unsigned w) {
     (void)hipSetDevice(ctx->device);
+    SrvSlot *mine[kSlotsPerWorker];
+    for (unsigned k = 0; k < kSlotsPerWorker; k++) mine[k] = slots[w * kSlotsPerWorker + k].get();
     for (;;) {
       bool progressed = false;
-      for (SrvSlot &sl : slots) {
-        if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
+      for (SrvSlot *slp : mine) {
+        SrvSlot &sl = *slp;
+        bool running;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          running = sl.state == SrvSlot::RUNNING;
+        }
+        if (running && hipEventQuery(sl.done) == hipSuccess) {
           finalize(sl);
           progressed = true;
         }
       }
       SrvSlot *free_slot = nullptr;
-      for (SrvSlot &sl : slots)
-        if (!sl.busy) {
-          free_slot = &sl;
-          break;
-        }
       std::vector<SrvRequest *> take;
       {
         std::unique_lock<std::mutex> lk(mu);
-        if (stop && queue.empty() && !slots[0].busy && !slots[1].busy) return;
-        if (free_slot) {
+        bool all_free = true;
+        for (SrvSlot *slp : mine) {
+          if (slp->state == SrvSlot::FREE) {
+            if (!free_slot) free_slot = slp;
+          } else {
+            all_free = false;
+          }
+        }
+        if (stop && queue.empty() && all_free) return;
+        bool hold = false;  // wait for the batch to fill
+        if (free_slot && !queue.empty() && !stop && n_running >= kBusyBatches &&
+            queue.size() < max_batch && now_s() - queue.front()->t_queued < kMaxHoldS)
+          hold = true;
+        if (free_slot && !hold) {
           while (!queue.empty() && take.size() < max_batch) {
             SrvRequest *r = queue.front();
             size_t need = r->surf_bytes + r->mask_bytes;
@@ -2712,12 +2893,13 @@ struct z2d_gpu_server {
           }
         }
         if (take.empty() && !progressed) {
-          bool any_busy = slots[0].busy || slots[1].busy;
-          if (any_busy)
+          const double w0 = now_s();
+          if (!all_free || hold)
             cv_work.wait_for(lk, std::chrono::microseconds(20));
           else
             cv_work.wait_for(lk, std::chrono::milliseconds(5),
                              [&] { return stop || !queue.empty(); });
+          t_idle += now_s() - w0;
         }
       }
       if (!take.empty()) launch(*free_slot, take);
@@ -2755,20 +2937,29 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
   s->max_batch = max_batch ? max_batch : 64;
   HIP_OK(ctx, hipSetDevice(ctx->device));
   hipError_t e = s->pool.init(hbm_budget + 256 * 1024);
-  for (int i = 0; i < 2 && e == hipSuccess; i++) {
-    e = hipStreamCreate(&s->slots[i].stream);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->slots[i].done, hipEventDisableTiming);
-    s->slots[i].batch.ctx = ctx;
+  for (unsigned i = 0; i < z2d_gpu_server::kWorkers * z2d_gpu_server::kSlotsPerWorker; i++)
+    s->slots.push_back(std::make_unique<SrvSlot>());
+  for (auto &sl : s->slots) {
+    if (e != hipSuccess) break;
+    e = hipStreamCreate(&sl->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming);
+    sl->batch.ctx = ctx;
   }
   if (e != hipSuccess) {
+    for (auto &sl : s->slots) {
+      if (sl->done) (void)hipEventDestroy(sl->done);
+      if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    }
     s->pool.destroy();
     delete s;
     return fail(ctx, e, "server create");
   }
-  s->worker = std::thread([s] { s->run(); });
+  for (unsigned w = 0; w < z2d_gpu_server::kWorkers; w++)
+    s->workers.emplace_back([s, w] { s->run(w); });
   const unsigned hc = std::thread::hardware_concurrency();
-  const unsigned np = std::max(2u, std::min(8u, hc ? hc / 2 : 2u));
+  const unsigned np = std::max(2u, std::min(12u, hc ? hc / 2 : 2u));
   for (unsigned i = 0; i < np; i++) s->preppers.emplace_back([s] { s->prep_loop(); });
+  for (unsigned i = 0; i < 4; i++) s->copiers.emplace_back([s] { s->copy_loop(); });
   *out = s;
   return Z2D_OK;
 }
@@ -2928,14 +3119,34 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
     s->stop = true;
     s->cv_work.notify_all();
   }
-  if (s->worker.joinable()) s->worker.join();
-  for (SrvSlot &sl : s->slots) {
+  for (std::thread &t : s->workers)
+    if (t.joinable()) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);  // the worker left every slot free: no copies remain
+    s->stop_copy = true;
+    s->cv_copy.notify_all();
+  }
+  for (std::thread &t : s->copiers)
+    if (t.joinable()) t.join();
+  if (std::getenv("Z2D_SERVER_TRACE"))
+    std::fprintf(stderr,
+                 "[z2d server] batches %llu: build %.3f s, count-pass wait %.3f s, prepare %.3f "
+                 "s, launch %.3f s, finalize %.3f s, idle %.3f s; prep threads %.3f s (%zu), "
+                 "copy threads %.3f s (%zu)\n",
+                 (unsigned long long)s->n_batches, s->t_build, s->t_count, s->t_prep,
+                 s->t_launch, s->t_fin, s->t_idle, s->t_prepthreads, s->preppers.size(),
+                 s->t_copythreads, s->copiers.size());
+  for (auto &slp : s->slots) {
+    SrvSlot &sl = *slp;
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     sl.batch.arena.release();
     sl.batch.stage.release();
     for (int i = 0; i < 3; i++)
       if (sl.batch.ev[i]) (void)hipEventDestroy(sl.batch.ev[i]);
     sl.result_stage.release();
+    sl.result_dev.release();
+    sl.in_stage.release();
+    sl.in_dev.release();
     sl.png_work.release();
     sl.png_files.release();
     if (sl.done) (void)hipEventDestroy(sl.done);

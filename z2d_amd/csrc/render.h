@@ -306,4 +306,15 @@ struct QuadJob {
   QuadSrc src, mask;
 };
 
+// One command of k_multi_copy: copy `bytes` from src to dst, or (src null)
+// fill bytes / 4 words at dst with `value`.  Many such commands go in one
+// launch (a server batch's surface clears, clip-mask scatters and result
+// gathers).
+struct CopyCmd {
+  const uint8_t *src;
+  uint8_t *dst;
+  uint64_t bytes;
+  uint32_t value, pad;
+};
+
 }  // namespace z2d

@@ -2810,6 +2810,38 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Command y of the list, strided over the x blocks: 16-byte moves where the
+// addresses and the size allow, bytes otherwise; fills write whole words.
+__global__ void __launch_bounds__(256) k_multi_copy(const CopyCmd *__restrict__ cmds) {
+  const CopyCmd c = cmds[blockIdx.y];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!c.src) {
+    uint32_t *d = reinterpret_cast<uint32_t *>(c.dst);
+    const uint64_t n = c.bytes / 4;
+    for (uint64_t i = t0; i < n; i += stride) d[i] = c.value;
+    return;
+  }
+  const bool v16 = ((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst)) &
+                    15u) == 0;
+  uint64_t done = 0;
+  if (v16) {
+    const uint64_t n = c.bytes / 16;
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(c.src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(c.dst);
+    for (uint64_t i = t0; i < n; i += stride) d4[i] = s4[i];
+    done = n * 16;
+  }
+  for (uint64_t i = done + t0; i < c.bytes; i += stride) c.dst[i] = c.src[i];
+}
+
+hipError_t z2d_launch_multi_copy(const CopyCmd *cmds, uint32_t n, hipStream_t st) {
+  for (uint32_t c0 = 0; c0 < n; c0 += 65535u) {  // grid y is at most 65535
+    hipLaunchKernelGGL(k_multi_copy, dim3(64, min(65535u, n - c0)), dim3(256), 0, st, cmds + c0);
+  }
+  return hipGetLastError();
+}
+
 hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st) {
   if (n_grads == 0) return hipSuccess;
   hipLaunchKernelGGL(k_grad_setup, dim3((n_grads + 63) / 64), dim3(64), 0, st, grads, n_grads);

@@ -19,7 +19,8 @@
 
 #include <cfloat>
 #include <cmath>
-#include <list>
+#include <cstring>
+#include <vector>
 #include <memory>
 
 namespace z2d {
@@ -29,6 +30,8 @@ struct P {
   double x, y;
 };
 inline bool peq(P a, P b) { return a.x == b.x && a.y == b.y; }
+// the same bits (signed zeros kept apart)
+inline bool pbits(P a, P b) { return std::memcmp(&a, &b, sizeof(P)) == 0; }
 
 // Transformation (src/Transformation.zig): [ax by tx; cx dy ty].
 struct Xf {
@@ -203,7 +206,9 @@ struct Pen {
       int low = 0, high = len;
       i = (low + high) >> 1;
       while (high - low > 1) {
-        if (v[i].slope_cw.compare(from) < 0) low = i; else high = i;
+        const bool lt = v[i].slope_cw.compare(from) < 0;
+        low = lt ? i : low;
+        high = lt ? high : i;
         i = (low + high) >> 1;
       }
       if (v[i].slope_cw.compare(from) < 0) {
@@ -217,7 +222,9 @@ struct Pen {
         i = (low + high) >> 1;
         while (high - low > 1) {
           const int j = i >= len ? i - len : i;
-          if (v[j].slope_cw.compare(to) > 0) high = i; else low = i;
+          const bool gt = v[j].slope_cw.compare(to) > 0;
+          high = gt ? i : high;
+          low = gt ? low : i;
           i = (low + high) >> 1;
         }
         if (i >= len) i -= len;
@@ -227,7 +234,9 @@ struct Pen {
       int low = 0, high = len;
       i = (low + high) >> 1;
       while (high - low > 1) {
-        if (from.compare(v[i].slope_ccw) < 0) low = i; else high = i;
+        const bool lt = from.compare(v[i].slope_ccw) < 0;
+        low = lt ? i : low;
+        high = lt ? high : i;
         i = (low + high) >> 1;
       }
       if (from.compare(v[i].slope_ccw) < 0) {
@@ -241,7 +250,9 @@ struct Pen {
         i = (low + high) >> 1;
         while (high - low > 1) {
           const int j = i >= len ? i - len : i;
-          if (to.compare(v[j].slope_ccw) > 0) high = i; else low = i;
+          const bool gt = to.compare(v[j].slope_ccw) > 0;
+          high = gt ? i : high;
+          low = gt ? low : i;
           i = (low + high) >> 1;
         }
         if (i >= len) i -= len;
@@ -269,17 +280,45 @@ struct Pen {
   }
 };
 
-// Polygon.Contour: a point list plotted at the end, before a given corner,
-// or at the front; points are scaled as they are plotted.
+// Polygon.Contour: a point list plotted at the end or at the front; points
+// are scaled as they are plotted.  The sequence is front (stored reversed)
+// followed by back, so both ends grow in amortized constant time without a
+// node per point.  The reference's insertions before the first corner
+// (a run of points placed, in order, ahead of what the list held) are
+// collected by the caller and prepended at once.
 struct Contour {
-  std::list<P> pts;
+  std::vector<P> front, back;
   double scale = 1.0;
-  using It = std::list<P>::iterator;
-  void plot(P p, It before) { pts.insert(before, P{p.x * scale, p.y * scale}); }
-  void plot(P p) { pts.push_back(P{p.x * scale, p.y * scale}); }
-  void plot_reverse(P p) { pts.push_front(P{p.x * scale, p.y * scale}); }
-  void concat(Contour &o) { pts.splice(pts.end(), o.pts); }
-  void reset() { pts.clear(); }
+  void plot(P p) { back.push_back(P{p.x * scale, p.y * scale}); }
+  void plot_reverse(P p) { front.push_back(P{p.x * scale, p.y * scale}); }
+  // points already scaled, in order, ahead of the current sequence
+  void prepend_scaled(const std::vector<P> &pts) {
+    for (size_t i = pts.size(); i-- > 0;) front.push_back(pts[i]);
+  }
+  P scaled(P p) const { return P{p.x * scale, p.y * scale}; }
+  bool empty() const { return front.empty() && back.empty(); }
+  template <class F>
+  void each(F &&f) const {
+    for (size_t i = front.size(); i-- > 0;) f(front[i]);
+    for (const P &p : back) f(p);
+  }
+  // appends o's sequence and empties o
+  void concat(Contour &o) {
+    back.reserve(back.size() + o.front.size() + o.back.size());
+    for (size_t i = o.front.size(); i-- > 0;) back.push_back(o.front[i]);
+    back.insert(back.end(), o.back.begin(), o.back.end());
+    o.reset();
+  }
+  // takes o's sequence as this one's (this one's points dropped)
+  void take(Contour &o) {
+    front.swap(o.front);
+    back.swap(o.back);
+    o.reset();
+  }
+  void reset() {
+    front.clear();
+    back.clear();
+  }
 };
 
 // Polygon.addEdge / addEdgesFromContour on already scaled points.
@@ -317,14 +356,18 @@ struct Poly {
     out->edges.push_back(e);
   }
   void add_contour(const Contour &c) {
-    if (c.pts.empty()) return;
-    auto it = c.pts.begin();
-    const P first = *it;
-    P last = first;
-    for (++it; it != c.pts.end(); ++it) {
-      add_edge(last, *it);
-      last = *it;
-    }
+    if (c.empty()) return;
+    bool have = false;
+    P first{0, 0}, last{0, 0};
+    c.each([&](const P &p) {
+      if (!have) {
+        first = last = p;
+        have = true;
+      } else {
+        add_edge(last, p);
+        last = p;
+      }
+    });
     add_edge(last, first);
   }
 };
@@ -474,6 +517,16 @@ struct State {
   Poly result{};
   Contour outer, inner;
   Slope current_slope;
+  // the face join built last for its outgoing segment, which is the next
+  // join's incoming one when segments chain (Face.init is a pure function
+  // of its points, the thickness and the transformation)
+  Face last{};
+  bool have_last = false;
+
+  Face face(P a, P b) {
+    if (have_last && pbits(last.p0, a) && pbits(last.p1, b)) return last;
+    return Face::init(a, b, o->thickness, o->ctm);
+  }
 
   void reset_contours() {
     outer.reset();
@@ -482,21 +535,32 @@ struct State {
   }
 };
 
-// stroke_plotter.join.
-void join(State &s, uint32_t mode, P p0, P p1, P p2, Contour::It *before) {
+// stroke_plotter.join.  before: the outer points go, in order, ahead of
+// the outer contour's current points instead of after them.
+void join(State &s, uint32_t mode, P p0, P p1, P p2, bool before) {
   if (peq(p0, p1) || peq(p1, p2)) {
     if (s.cw < 0) s.cw = 0;
     return;
   }
   const Opts &o = *s.o;
-  const Face in = Face::init(p0, p1, o.thickness, o.ctm);
-  const Face out = Face::init(p1, p2, o.thickness, o.ctm);
+  const Face in = s.face(p0, p1);
+  const Face out = s.face(p1, p2);
+  s.last = out;
+  s.have_last = true;
   const bool join_cw = in.dev_slope.compare(out.dev_slope) < 0;
   const bool poly_cw = s.cw >= 0 ? s.cw != 0 : join_cw;
   const bool switched = join_cw != poly_cw;
+  std::vector<P> ahead;  // outer points placed ahead (before == true)
   auto plot_outer = [&](P p) {
-    if (before) s.outer.plot(p, *before); else s.outer.plot(p);
+    if (before) ahead.push_back(s.outer.scaled(p)); else s.outer.plot(p);
   };
+  struct Flush {  // prepends the collected points however join returns
+    Contour &c;
+    std::vector<P> &v;
+    ~Flush() {
+      if (!v.empty()) c.prepend_scaled(v);
+    }
+  } flush{s.outer, ahead};
   auto plot_inner = [&](P p) { s.inner.plot_reverse(p); };
   auto outer_j = [&](P p) { if (switched) plot_inner(p); else plot_outer(p); };
   auto inner_j = [&](P p) { if (switched) plot_outer(p); else plot_inner(p); };
@@ -542,8 +606,10 @@ void plot_open_joined(State &s, P start0, P end0, P start1, P end1) {
   const Face fs = Face::init(start0, end0, o.thickness, o.ctm);
   const Face fe = Face::init(start1, end1, o.thickness, o.ctm);
   const bool clockwise = s.cw >= 0 ? s.cw != 0 : true;
-  const Contour::It before = s.outer.pts.begin();  // the first corner (end if empty)
-  fs.cap_p0([&](P p) { s.outer.plot(p, before); }, o.cap, clockwise, s.pen);
+  // the start cap goes ahead of the first corner, in order
+  std::vector<P> ahead;
+  fs.cap_p0([&](P p) { ahead.push_back(s.outer.scaled(p)); }, o.cap, clockwise, s.pen);
+  s.outer.prepend_scaled(ahead);
   fe.cap_p1([&](P p) { s.outer.plot(p); }, o.cap, clockwise, s.pen);
   s.outer.concat(s.inner);
   s.result.add_contour(s.outer);
@@ -553,10 +619,10 @@ void plot_open_joined(State &s, P start0, P end0, P start1, P end1) {
 void plot_closed_joined(State &s, P initial0, P initial1, P p1, P p2) {
   const uint32_t mode = s.o->join;
   if (!peq(p2, initial0)) {
-    join(s, mode, p1, p2, initial0, nullptr);
-    join(s, mode, p2, initial0, initial1, nullptr);
+    join(s, mode, p1, p2, initial0, false);
+    join(s, mode, p2, initial0, initial1, false);
   } else {
-    join(s, mode, p1, initial0, initial1, nullptr);
+    join(s, mode, p1, initial0, initial1, false);
   }
   s.result.add_contour(s.outer);
   s.result.add_contour(s.inner);
@@ -641,7 +707,7 @@ struct Plotter {
     if (peq(p, s.points.last())) return Z2D_OK;  // degenerate
     s.points.add(p);
     if (s.points.len > 2)
-      join(s, join_mode, s.points.tail(3), s.points.tail(2), s.points.tail(1), nullptr);
+      join(s, join_mode, s.points.tail(3), s.points.tail(2), s.points.tail(1), false);
     return Z2D_OK;
   }
 
@@ -801,7 +867,7 @@ struct DashedPlotter {
       const P dp{first.x + xo, first.y + yo};
       if (!peq(dp, s.points.last())) s.points.add(dp);
       if (dasher.on && s.points.len > 2)
-        join(s, join_mode, s.points.tail(3), s.points.tail(2), s.points.tail(1), nullptr);
+        join(s, join_mode, s.points.tail(3), s.points.tail(2), s.points.tail(1), false);
       if (dasher.step(step)) next_segment(dp);
       step = std::fmin(dasher.remain, remaining);
     }
@@ -844,10 +910,8 @@ struct DashedPlotter {
       init.pen = s.pen;
       init.points = s.points;
       init.cw = s.cw;
-      init.outer.pts.clear();
-      init.inner.pts.clear();
-      init.outer.pts.splice(init.outer.pts.end(), s.outer.pts);
-      init.inner.pts.splice(init.inner.pts.end(), s.inner.pts);
+      init.outer.take(s.outer);
+      init.inner.take(s.inner);
       init.current_slope = s.current_slope;
     } else {
       init_kind = OFF;
@@ -886,15 +950,14 @@ struct DashedPlotter {
     if (init_kind != ON) return Z2D_ERR_INVALID_STATE;
     if (init.points.len < 2) return Z2D_ERR_INVALID_STATE;
     if (s.points.len > 2) {
-      join(s, o.join, s.points.tail(2), init.points.head(0), init.points.head(1), nullptr);
+      join(s, o.join, s.points.tail(2), init.points.head(0), init.points.head(1), false);
       s.outer.concat(init.outer);
       init.inner.concat(s.inner);
-      init.outer.pts.splice(init.outer.pts.end(), s.outer.pts);  // initial outer = combined
+      init.outer.take(s.outer);  // initial outer = combined
       plot_open_joined(init, s.points.head(0), s.points.head(1), init.points.tail(2),
                        init.points.tail(1));
     } else {
-      Contour::It first = init.outer.pts.begin();
-      join(init, o.join, s.points.tail(2), init.points.head(0), init.points.head(1), &first);
+      join(init, o.join, s.points.tail(2), init.points.head(0), init.points.head(1), true);
       plot_open_joined(init, s.points.first(), init.points.first(), init.points.tail(2),
                        init.points.tail(1));
     }
@@ -941,7 +1004,7 @@ struct DashedPlotter {
         case 1: plot_dotted(s.points.first(), s.current_slope); break;
         case 2: plot_single(s, s.points.head(0), s.points.head(1)); break;
         default:
-          join(s, o.join, s.points.tail(2), s.points.head(0), s.points.head(1), nullptr);
+          join(s, o.join, s.points.tail(2), s.points.head(0), s.points.head(1), false);
           s.result.add_contour(s.outer);
           s.result.add_contour(s.inner);
           s.reset_contours();
