@@ -11,12 +11,15 @@ order.
 
   value          pixel-draws per second (Mpix/s): pixels with nonzero
                  coverage summed over all paths, counted on the device
-  roofline       algorithmic bytes of the render kernel (2*bpp per pixel-draw
-                 + 32 B per edge + paint parameters) / its mean duration,
-                 against MI355X HBM3E peak (8 TB/s)
+  roofline       algorithmic bytes of the render stage (k_render_fast +
+                 k_render: 2*bpp per pixel-draw + 32 B per band edge copy +
+                 paint parameters) / its mean duration, against MI355X HBM3E
+                 peak (8 TB/s)
   cpu_baseline   the C oracle (a scalar restatement of the reference
-                 painter) rendering the first 4,000 paths of the same scene
-                 on one host core
+                 painter, not the Zig painter) rendering the first 4,000
+                 paths of the same scene on one host core; all-core and
+                 per-workload legs beside it (star latency, compositor,
+                 batched surfaces)
   extra          512x512 star latency (p50/p99), 4096x4096 compositor
                  passes (GB/s), batched uneven surfaces vs one launch per
                  surface, the request server under open-loop Poisson
@@ -43,6 +46,20 @@ HBM_PEAK_GBS = 8000.0
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 METRIC = "fill+composite Mpix/s (% HBM peak); p50/p99 latency of batched render reqs"
+
+
+KERNEL_SOURCES = ("z2d_amd/csrc/render.hip", "z2d_amd/csrc/render.h", "z2d_amd/csrc/dev_pixel.h")
+
+
+def kernel_sources_hash():
+    """sha256 over the render kernels' sources (the files the render stage
+    is compiled from), to tie a stored profile to the code it measured."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def paint_bytes(pattern):
@@ -120,68 +137,116 @@ def bench_star_latency(gpu, ctx, iters=300):
     return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
-def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3):
+# Bytes per pixel an operator's pass has to move (out-of-place, RGBA8):
+# clear writes out only; src reads src, dst reads dst; every other operator
+# reads both.  The mask variant (dst_in of src by the alpha8 mask, then the
+# operator) adds the mask byte wherever the operator reads src.
+def _op_bytes(name, masked):
+    if name == "clear":
+        return 4
+    if name == "dst":
+        return 8
+    src_side = 4 + (1 if masked else 0)
+    return 4 + src_side + (0 if name == "src" else 4)
+
+
+def _stream_copy(gpu, ctx, nbytes=1 << 30, iters=20, read2=False):
+    """The library's reference streams over 1 GiB buffers (read + write
+    GB/s): a plain copy, or (read2) two reads and one write, the compositor's
+    mix: the achievable-HBM references."""
+    g = C.c_double()
+    f = gpu.lib().z2d_gpu_stream_read2 if read2 else gpu.lib().z2d_gpu_stream_copy
+    ctx.check(f(ctx.h, nbytes, iters, C.byref(g)), "stream")
+    return round(g.value, 1)
+
+
+def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3, sets=3):
     """Compositor sweep at 4096x4096 RGBA8 (seed 0xC2): every operator in
     enum order out-of-place, out = op(src, dst), then the alpha8-mask variant
-    {dst_in dst=src src=mask}, {op}.  GB/s counts 12 B/px (13 with the mask):
-    src and dst read once, out written once."""
+    {dst_in dst=src src=mask}, {op}.  Each operator's bytes are what it has
+    to move (_op_bytes: 4 B/px for clear, 8 for src and dst, 12 otherwise,
+    +1 for the mask where src is read).  The passes rotate over `sets`
+    independent src/dst/out/mask sets (3 x 218 MB), so consecutive passes
+    do not find their inputs in the 256 MB last-level cache; the stream
+    copy rate at the same footprint is reported beside them."""
     src, dst, mask = scenes.compositor_inputs(w, h)
-    s_src = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
-    s_dst = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
-    s_out = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
-    s_mask = gpu.DeviceSurface(ctx, abi.ALPHA8, w, h)
-    s_src.upload(src)
-    s_dst.upload(dst)
-    s_mask.upload(mask)
-    d_src, d_dst, d_out, d_mask = s_src.desc(), s_dst.desc(), s_out.desc(), s_mask.desc()
+    surfs = []
+    for _ in range(sets):
+        s_src = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+        s_dst = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+        s_out = gpu.DeviceSurface(ctx, abi.RGBA, w, h)
+        s_mask = gpu.DeviceSurface(ctx, abi.ALPHA8, w, h)
+        s_src.upload(src)
+        s_dst.upload(dst)
+        s_mask.upload(mask)
+        surfs.append((s_src, s_dst, s_out, s_mask))
+    descs = [tuple(x.desc() for x in st) for st in surfs]
     f = gpu.lib().z2d_gpu_composite
     per_op, per_op_mask, dev_op, dev_mask = {}, {}, {}, {}
     tot_bytes = tot_time = tot_kern = 0.0
     timing = gpu.lib().z2d_gpu_composite_timing
     km = C.c_double(0.0)
+
+    def op_list(name, masked, d_src, d_mask):
+        if masked:
+            m = abi.CompOp()
+            m.op = abi.OP["dst_in"]
+            m.dst.kind = abi.PARAM_SURFACE
+            m.dst.surface = C.pointer(d_src)
+            m.src.kind = abi.PARAM_SURFACE
+            m.src.surface = C.pointer(d_mask)
+            o = abi.CompOp()
+            o.op = abi.OP[name]
+            return (abi.CompOp * 2)(m, o)
+        o = abi.CompOp()
+        o.op = abi.OP[name]
+        o.src.kind = abi.PARAM_SURFACE
+        o.src.surface = C.pointer(d_src)
+        return (abi.CompOp * 1)(o)
+
     for masked in (False, True):
         for name in abi.OPERATORS:
-            if masked:
-                m = abi.CompOp()
-                m.op = abi.OP["dst_in"]
-                m.dst.kind = abi.PARAM_SURFACE
-                m.dst.surface = C.pointer(d_src)
-                m.src.kind = abi.PARAM_SURFACE
-                m.src.surface = C.pointer(d_mask)
-                o = abi.CompOp()
-                o.op = abi.OP[name]
-                arr = (abi.CompOp * 2)(m, o)
-            else:
-                o = abi.CompOp()
-                o.op = abi.OP[name]
-                o.src.kind = abi.PARAM_SURFACE
-                o.src.surface = C.pointer(d_src)
-                arr = (abi.CompOp * 1)(o)
-            n = len(arr)
-            ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
+            lists = [op_list(name, masked, ds[0], ds[3]) for ds in descs]
+            ds0 = descs[-1]
+            arr = lists[-1]
+            ctx.check(f(ctx.h, C.byref(ds0[1]), 0, 0, arr, len(arr), 0, C.byref(ds0[2])),
+                      "composite")
             t = time.perf_counter()
             kern = 0.0
-            for _ in range(reps):
-                ctx.check(f(ctx.h, C.byref(d_dst), 0, 0, arr, n, 0, C.byref(d_out)), "composite")
+            n_pass = reps * sets
+            for i in range(n_pass):
+                ds, arr = descs[i % sets], lists[i % sets]
+                ctx.check(f(ctx.h, C.byref(ds[1]), 0, 0, arr, len(arr), 0, C.byref(ds[2])),
+                          "composite")
                 ctx.check(timing(ctx.h, C.byref(km)), "composite timing")
                 kern += km.value / 1e3
-            dt = (time.perf_counter() - t) / reps
-            kern /= reps
-            nbytes = w * h * (13 if masked else 12)
+            dt = (time.perf_counter() - t) / n_pass
+            kern /= n_pass
+            nbytes = w * h * _op_bytes(name, masked)
             (per_op_mask if masked else per_op)[name] = round(nbytes / dt / 1e9, 1)
             (dev_mask if masked else dev_op)[name] = round(nbytes / kern / 1e9, 1)
             tot_bytes += nbytes
             tot_time += dt
             tot_kern += kern
-    for s in (s_src, s_dst, s_out, s_mask):
-        s.close()
+    for st in surfs:
+        for x in st:
+            x.close()
     return {"GBps_all_passes": round(tot_bytes / tot_time / 1e9, 1), "per_op_GBps": per_op,
             "mask_per_op_GBps": per_op_mask, "passes": 2 * len(abi.OPERATORS),
             "note": "host-timed per call (upload of the op list, launch, sync)",
             "kernel_GBps_all_passes": round(tot_bytes / tot_kern / 1e9, 1),
             "kernel_per_op_GBps": dev_op, "kernel_mask_per_op_GBps": dev_mask,
+            "rotating_sets": sets,
+            "stream_copy_GBps": _stream_copy(gpu, ctx),
+            "stream_read2_GBps": _stream_copy(gpu, ctx, read2=True),
             "kernel_note": "k_composite_quad device time (HIP events around the launch); "
-                           "12 B/px (13 with the mask) over the 8 TB/s HBM peak"}
+                           "bytes each operator moves (clear 4 B/px, src/dst 8, others 12, "
+                           "+1 where the mask is read) over the kernel time, passes "
+                           "rotating over 3 input sets (654 MB) so no pass reads its "
+                           "inputs from the last-level cache; stream_copy_GBps (1 read : "
+                           "1 write) and stream_read2_GBps (2 reads : 1 write, the "
+                           "operators' mix) are the library's reference streams over "
+                           "1 GiB buffers"}
 
 
 def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
@@ -436,23 +501,162 @@ def bench_text(gpu, ctx, lines=100, reps=5):
                                          "scaled to 100"))
 
 
+CPU_LABEL = "C oracle restatement, not the Zig painter"
+
+
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    # the host threads one GPU's job may use (the machine's share per GPU)
+    threads = max(1, min(16, avail))
+    return model, os.cpu_count(), avail, threads
+
+
+def _on_threads(n, fn, items):
+    """fn over items on n threads (the oracle's calls release the GIL);
+    returns the wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(n) as ex:
+        t = time.perf_counter()
+        list(ex.map(fn, items))
+        return time.perf_counter() - t
+
+
 def cpu_baseline(gpu, ctx, sc, n_paths):
+    """The CPU painter on the host cores, on bounded samples of each
+    workload: the C oracle (a scalar restatement of the reference painter;
+    the Zig painter itself cannot be built here), strokes through the host
+    tessellator and the oracle's raster.  Single-core figures, and all-core
+    ones with one job per thread (the reference harness's jobs-over-cores
+    shape).  Pixel-draw counts come from the device's counters over the same
+    sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_backend import OracleBackend  # the checker, timed as the baseline
+    from oracle_backend import OracleBackend, lib as oracle_lib  # the checker, timed as the baseline
+    from stroke_check import StrokeCheckBackend
     from zscene import Surface
+    model, ncpu, avail, threads = _cpu_info()
+    ob = OracleBackend()
+
+    # 4K scene (the headline workload): the first n_paths paths, one core;
+    # all cores: one copy of a shorter prefix per thread
     _, b = build_batch(gpu, ctx, sc, n_paths)
     pix, _ = count_work(b)
     b.close()
-    s = Surface(sc["format"], sc["width"], sc["height"])
-    s.buf[: s.nbytes] = sc["buf"]
-    ob = OracleBackend()
+
+    def scene_job(n):
+        s = Surface(sc["format"], sc["width"], sc["height"])
+        s.buf[: s.nbytes] = sc["buf"]
+        for pat, nodes, opts in sc["draws"][:n]:
+            ob.fill(s, pat, nodes, opts)
+
     t = time.perf_counter()
-    for pat, nodes, opts in sc["draws"][:n_paths]:
-        ob.fill(s, pat, nodes, opts)
+    scene_job(n_paths)
     dt = time.perf_counter() - t
+    n_all = max(1, n_paths // 4)
+    _, b = build_batch(gpu, ctx, sc, n_all)
+    pix_all, _ = count_work(b)
+    b.close()
+    dt_all = _on_threads(threads, lambda i: scene_job(n_all), range(threads))
+
+    # 512x512 star latency: one render per call
+    star = scenes.star_512()
+    lat = []
+    for _ in range(30):
+        s = Surface(star["format"], star["width"], star["height"])
+        s.buf[: s.nbytes] = star["buf"]
+        t0 = time.perf_counter()
+        for pat, nodes, opts in star["draws"]:
+            ob.fill(s, pat, nodes, opts)
+        lat.append((time.perf_counter() - t0) * 1e3)
+
+    # compositor sweep on a 1024x1024 crop of the same distribution: every
+    # operator, out = op(src, dst) with out a copy of dst composited in place
+    cw = ch = 1024
+    src, dst, _mask = scenes.compositor_inputs(cw, ch)
+
+    def comp_sweep(_i=0):
+        so = Surface(abi.RGBA, cw, ch)
+        so.buf[: so.nbytes] = src
+        out = Surface(abi.RGBA, cw, ch)
+        sd = so.desc()
+        times = {}
+        for name in abi.OPERATORS:
+            out.buf[: out.nbytes] = dst
+            o = abi.CompOp()
+            o.op = abi.OP[name]
+            o.src.kind = abi.PARAM_SURFACE
+            o.src.surface = C.pointer(sd)
+            od = out.desc()
+            arr = (abi.CompOp * 1)(o)
+            t0 = time.perf_counter()
+            oracle_lib().zo_composite(C.byref(od), 0, 0, arr, 1, abi.PRECISION_INTEGER)
+            times[name] = time.perf_counter() - t0
+        return times
+
+    ct = comp_sweep()
+    comp_1 = {k: round(cw * ch * _op_bytes(k, False) / v / 1e9, 3) for k, v in ct.items()}
+    tot_b = sum(cw * ch * _op_bytes(k, False) for k in abi.OPERATORS)
+    comp_all = tot_b * threads / _on_threads(threads, comp_sweep, range(threads)) / 1e9
+
+    # batched form: the first 256 of the 4096 uneven surfaces (30% strokes)
+    work = scenes.uneven_surfaces(256, 0xC4)
+    bb = gpu.Batch(ctx)
+    keep = []
+    for fmt, side, jobs, k in work:
+        ds = gpu.DeviceSurface(ctx, fmt, side, side)
+        keep.append(ds)
+        for kind, pat, nodes, opts in jobs:
+            rc = (bb.stroke if kind == "stroke" else bb.fill)(ds, pat, nodes, opts)
+            if rc:
+                raise RuntimeError(f"batch {kind} rc={rc}")
+    bb.prepare()
+    pix_c4, _ = count_work(bb)
+    bb.close()
+    for ds in keep:
+        ds.close()
+    sbe = StrokeCheckBackend()
+
+    def surface_job(item):
+        fmt, side, jobs, _k = item
+        s = Surface(fmt, side, side)
+        for kind, pat, nodes, opts in jobs:
+            (sbe.stroke if kind == "stroke" else sbe.fill)(s, pat, nodes, opts)
+
+    dt_c4 = _on_threads(1, surface_job, work)
+    dt_c4_all = _on_threads(threads, surface_job, work)
+
     return dict(value=round(pix / dt / 1e6, 2), unit="Mpix/s", cores=1, kind="port",
+                label=CPU_LABEL, cpu_model=model, host_cpus=ncpu, cpus_available=avail,
                 sample=f"first {n_paths} of the 10,000 paths of the same 3840x2160 scene, "
-                       f"C oracle (-O2), one core, {dt:.1f} s")
+                       f"one core, {dt:.1f} s",
+                all_cores=dict(value=round(pix_all * threads / dt_all / 1e6, 2),
+                               unit="Mpix/s", cores=threads,
+                               sample=f"one copy of the first {n_all} paths per thread, "
+                                      f"{dt_all:.1f} s wall"),
+                legs={
+                    "star_512_latency_ms": dict(p50=round(float(np.percentile(lat, 50)), 3),
+                                                p99=round(float(np.percentile(lat, 99)), 3),
+                                                cores=1, renders=len(lat)),
+                    "compositor_GBps": dict(per_op=comp_1, cores=1,
+                                            all_ops_all_cores=round(comp_all, 2),
+                                            threads=threads,
+                                            sample="1024x1024 RGBA8 inputs of the 0xC2 "
+                                                   "distribution, bytes as the device "
+                                                   "sweep counts them"),
+                    "uneven_batch_Mpix_s": dict(one_core=round(pix_c4 / dt_c4 / 1e6, 2),
+                                                all_cores=round(pix_c4 / dt_c4_all / 1e6, 2),
+                                                threads=threads, surfaces=len(work),
+                                                sample="first 256 of the 4096 surfaces "
+                                                       "(seed 0xC4), one surface per job"),
+                })
 
 
 def main():
@@ -524,17 +728,25 @@ def main():
                    "strip_draw_pairs_per_step": r["strip_entries"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_render",
+                     "traffic": None,
+                     "kernel": "render stage: k_render_fast + k_render (the two strip "
+                               "workers, back to back on the batch stream; HIP events "
+                               "before the first and after the second)",
                      "kernel_ms": round(r["render_ms"], 4),
                      "pipeline_ms": round(r["pipe_ms"], 4)},
     }
     tj = os.path.join(ROOT, "profiles", "k_render_scene_3840x2160_10k.traffic.json")
     if os.path.exists(tj) and args.paths == 10000:
-        # HBM bytes per k_render launch from rocprofv3 FETCH_SIZE/WRITE_SIZE
-        # runs of this workload, scaled by the stream-copy calibration
+        # HBM bytes per render stage from rocprofv3 FETCH_SIZE/WRITE_SIZE
+        # runs of this workload, scaled by the stream-copy calibration; the
+        # profile records the hash of the kernel sources it was taken with,
+        # so a number older than the kernels shows as stale
         with open(tj) as fh:
-            line["roofline"]["traffic"] = json.load(fh)["traffic_bytes_per_launch"]
+            tr = json.load(fh)
+        line["roofline"]["traffic"] = tr["traffic_bytes_per_launch"]
         line["roofline"]["traffic_source"] = "profiles/k_render_scene_3840x2160_10k.md"
+        line["roofline"]["traffic_kernel_sources"] = tr.get("kernel_sources_sha256")
+        line["roofline"]["traffic_stale"] = tr.get("kernel_sources_sha256") != kernel_sources_hash()
     if not args.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(gpu, ctx, sc, min(args.cpu_paths, args.paths))

@@ -304,6 +304,11 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s);
  * reports read+write GB/s (the achievable-bandwidth reference). */
 int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
+/* Measurement utility (no reference counterpart): the compositor's access
+ * mix as a reference stream, two `bytes` buffers read and one written
+ * (dst = a ^ b), `iters` times; reports read+write GB/s. */
+int32_t z2d_gpu_stream_read2(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
+
 /* Synchronizes the context stream. */
 int32_t z2d_gpu_sync(z2d_gpu_ctx *ctx);
 

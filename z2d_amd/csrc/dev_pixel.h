@@ -203,6 +203,25 @@ __device__ __forceinline__ uint32_t hardlight_ch(int32_t s, int32_t d, int32_t s
   return u16((uint32_t)((s * (255 + da)) / 255 + (d * (255 + sa)) / 255 - (sa * da) / 255 -
                         (2 * s * d) / 255));
 }
+// overlay / hard light on 8-bit lanes (every lane <= 255, the case of
+// unpacked pixels and paints).  Both of the reference's branches are sums
+// of quotients of the same kind:
+//   p:  (2sd)/255 + (s(255-da))/255 + (d(255-sa))/255
+//   !p: (s(255+da))/255 + (d(255+sa))/255 - (2sd)/255 - (da sa)/255
+// so the branch only selects the multipliers: three divisions per channel
+// instead of seven, and (da sa)/255 once per pixel.  With lanes <= 255 every
+// dividend is non-negative and below 2^18, where the unsigned quotient is
+// the signed one; the sum wraps to 16 bits as the signed form's does.
+__device__ __forceinline__ uint32_t blend8_ch(bool p, uint32_t s, uint32_t d, uint32_t sa,
+                                              uint32_t da, uint32_t q_alpha) {
+  const uint32_t t = __umul24(2u * s, d) / 255u;
+  const uint32_t q1 = __umul24(s, p ? 255u - da : 255u + da) / 255u;
+  const uint32_t q2 = __umul24(d, p ? 255u - sa : 255u + sa) / 255u;
+  return u16(p ? t + q1 + q2 : q1 + q2 - t - q_alpha);
+}
+__device__ __forceinline__ bool lanes8(I4 d, I4 s) {
+  return (d.r | d.g | d.b | d.a | s.r | s.g | s.b | s.a) <= 255u;
+}
 
 __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
   I4 o = {0, 0, 0, 0};
@@ -262,8 +281,16 @@ __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
       o.b = u16(s.b + d.b - imul(s.b, d.b)); o.a = ua;
       return o;
     case Z2D_OP_OVERLAY:
-      o.r = overlay_ch(s.r, d.r, sa, da); o.g = overlay_ch(s.g, d.g, sa, da);
-      o.b = overlay_ch(s.b, d.b, sa, da); o.a = ua;
+      if (lanes8(d, s)) {
+        const uint32_t qa = __umul24(da, sa) / 255u;
+        o.r = blend8_ch(2u * d.r <= da, s.r, d.r, sa, da, qa);
+        o.g = blend8_ch(2u * d.g <= da, s.g, d.g, sa, da, qa);
+        o.b = blend8_ch(2u * d.b <= da, s.b, d.b, sa, da, qa);
+      } else {
+        o.r = overlay_ch(s.r, d.r, sa, da); o.g = overlay_ch(s.g, d.g, sa, da);
+        o.b = overlay_ch(s.b, d.b, sa, da);
+      }
+      o.a = ua;
       return o;
     case Z2D_OP_DARKEN:
       o.r = u16(min(imul(s.r, da), imul(d.r, sa)) + iinvmul(s.r, da) + iinvmul(d.r, sa));
@@ -278,8 +305,16 @@ __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
       o.a = ua;
       return o;
     case Z2D_OP_HARD_LIGHT:
-      o.r = hardlight_ch(s.r, d.r, sa, da); o.g = hardlight_ch(s.g, d.g, sa, da);
-      o.b = hardlight_ch(s.b, d.b, sa, da); o.a = ua;
+      if (lanes8(d, s)) {
+        const uint32_t qa = __umul24(da, sa) / 255u;
+        o.r = blend8_ch(2u * s.r <= sa, s.r, d.r, sa, da, qa);
+        o.g = blend8_ch(2u * s.g <= sa, s.g, d.g, sa, da, qa);
+        o.b = blend8_ch(2u * s.b <= sa, s.b, d.b, sa, da, qa);
+      } else {
+        o.r = hardlight_ch(s.r, d.r, sa, da); o.g = hardlight_ch(s.g, d.g, sa, da);
+        o.b = hardlight_ch(s.b, d.b, sa, da);
+      }
+      o.a = ua;
       return o;
     case Z2D_OP_DIFFERENCE:
       o.r = u16(s.r + d.r - 2u * min(imul(s.r, da), imul(d.r, sa)));

@@ -50,6 +50,8 @@ hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob 
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t z2d_launch_multi_copy(const z2d::CopyCmd *cmds, uint32_t n, hipStream_t st);
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+hipError_t z2d_launch_stream_read2(const void *a, const void *b, void *dst, uint64_t bytes,
+                                   hipStream_t st);
 hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream_t st);
 hipError_t z2d_launch_png(const z2d::PngJob &J, hipStream_t st);  // png.hip
 
@@ -2295,37 +2297,49 @@ int32_t z2d_gpu_png_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
   return Z2D_OK;
 }
 
-int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+// The reference stream over `bytes`-sized buffers: a copy, or (read2) two
+// reads and one write; GB/s counts every byte read and written.
+static int32_t stream_rate(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, bool read2,
+                           double *gbps) {
   if (!ctx) return Z2D_ERR_NO_CONTEXT;
   if (bytes < 16 || iters == 0) return Z2D_ERR_INVALID_ARGUMENT;
   bytes &= ~15ull;
   HIP_OK(ctx, hipSetDevice(ctx->device));
-  void *a = nullptr, *b = nullptr;
-  HIP_OK(ctx, hipMalloc(&a, bytes));
-  hipError_t e = hipMalloc(&b, bytes);
-  if (e != hipSuccess) {
-    (void)hipFree(a);
-    return fail(ctx, e, "stream copy alloc");
-  }
+  DevBuf a, b, c;
+  hipError_t e = a.ensure(bytes, true);
+  if (e == hipSuccess) e = b.ensure(bytes, true);
+  if (e == hipSuccess && read2) e = c.ensure(bytes, true);
+  if (e != hipSuccess) return fail(ctx, e, "stream alloc");
+  auto once = [&]() {
+    return read2 ? z2d_launch_stream_read2(a.p, c.p, b.p, bytes, ctx->stream)
+                 : z2d_launch_stream_copy(a.p, b.p, bytes, ctx->stream);
+  };
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  e = hipMemsetAsync(a, 1, bytes, ctx->stream);
+  e = hipMemsetAsync(a.p, 1, bytes, ctx->stream);
+  if (e == hipSuccess && read2) e = hipMemsetAsync(c.p, 2, bytes, ctx->stream);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = z2d_launch_stream_copy(a, b, bytes, ctx->stream);  // warm-up
+  if (e == hipSuccess) e = once();  // warm-up
   if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
-  for (uint32_t i = 0; i < iters && e == hipSuccess; i++)
-    e = z2d_launch_stream_copy(a, b, bytes, ctx->stream);
+  for (uint32_t i = 0; i < iters && e == hipSuccess; i++) e = once();
   if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.0f;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-  if (e == hipSuccess && gbps) *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  if (e == hipSuccess && gbps)
+    *gbps = (read2 ? 3.0 : 2.0) * (double)bytes * iters / (ms * 1e-3) / 1e9;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
-  if (e != hipSuccess) return fail(ctx, e, "stream copy");
+  if (e != hipSuccess) return fail(ctx, e, "stream");
   return Z2D_OK;
+}
+
+int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+  return stream_rate(ctx, bytes, iters, false, gbps);
+}
+
+int32_t z2d_gpu_stream_read2(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+  return stream_rate(ctx, bytes, iters, true, gbps);
 }
 
 struct StrokeArgs {

@@ -1598,6 +1598,16 @@ __device__ __forceinline__ T load_uniform(const T *p) {
 
 // Orders this wave's LDS accesses before and after the call (lanes of one
 // wave exchanging data through LDS).
+// Lane `src`'s value of v (every lane of the wave active).
+__device__ __forceinline__ uint32_t lane_read_u32(uint32_t v, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+__device__ __forceinline__ double lane_read_f64(double v, uint32_t src) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = lane_read_u32((uint32_t)u, src), hi = lane_read_u32((uint32_t)(u >> 32), src);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -2202,10 +2212,10 @@ struct FastShared {
   uint32_t acc[STRIP_SUBROWS][ACC_WORDS];
   union {
     struct {                             // coverage (phase A)
-      double top[64], xs[64], xi[64];    // staged edges' intercept terms
-      uint8_t own[64 * STRIP_SUBROWS];   // owning lane of each work item, 0x80: delta -1
-      uint8_t lo[64];                    // first active sample row in the strip
-      uint16_t off[64];                  // exclusive offsets of active-row counts
+      // owning lane of each work item (0x80: delta -1); the owner's edge
+      // terms are read from its registers (ds_bpermute), not staged here,
+      // which keeps the block within 10 KiB: 16 workgroups per CU
+      uint8_t own[64 * STRIP_SUBROWS];
     } a;
     struct {                             // composite
       uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples (RGBA8)
@@ -2437,20 +2447,23 @@ k_render_fast(JobView J, uint32_t tile_base,
           const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
           if (total == 0) continue;  // uniform
           const uint32_t my_off = (uint32_t)(incl - n_sr);
-          sh.u.a.top[lane] = ed.top;
-          sh.u.a.xs[lane] = ed.x_start;
-          sh.u.a.xi[lane] = ed.x_inc;
-          sh.u.a.lo[lane] = (uint8_t)lo;
-          sh.u.a.off[lane] = (uint16_t)my_off;
+          const uint32_t lo_off = ((uint32_t)lo << 16) | my_off;
           const uint8_t tag = (uint8_t)(lane | ((!eo && (ed.yhi2 & 1)) ? 0x80u : 0u));
           for (int t = 0; t < n_sr; t++) sh.u.a.own[my_off + (uint32_t)t] = tag;
           wave_lds_sync();
-          for (uint32_t it = lane; it < total; it += 64) {
-            const uint32_t v = sh.u.a.own[it];
+          // every lane takes part in each round's register reads (the owner
+          // of an item may be a lane with no item of its own this round)
+          for (uint32_t it0 = 0; it0 < total; it0 += 64) {
+            const uint32_t it = it0 + lane;
+            const bool live = it < total;
+            const uint32_t v = live ? (uint32_t)sh.u.a.own[it] : 0u;
             const uint32_t lj = v & 63u;
-            const int sr = (int)sh.u.a.lo[lj] + (int)(it - sh.u.a.off[lj]);
-            const int32_t x = (int32_t)round(
-                redge_x(sh.u.a.top[lj], sh.u.a.xs[lj], sh.u.a.xi[lj], ys_base + sr));
+            const uint32_t q = lane_read_u32(lo_off, lj);
+            const double etop = lane_read_f64(ed.top, lj), exs = lane_read_f64(ed.x_start, lj),
+                         exi = lane_read_f64(ed.x_inc, lj);
+            if (!live) continue;
+            const int sr = (int)(q >> 16) + (int)(it - (q & 0xFFFFu));
+            const int32_t x = (int32_t)round(redge_x(etop, exs, exi, ys_base + sr));
             const int32_t xr = x - xs_base;
             if (xr >= COLS) continue;
             const int32_t col = xr < 0 ? 0 : xr;
@@ -2789,19 +2802,57 @@ using namespace z2d;
 
 // Plain stream copy (16 B per lane per iteration): the bandwidth reference
 // and the known-size kernel that calibrates HBM byte counters.
-__global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src,
-                                                     uint4 *__restrict__ dst, uint64_t n) {
+// Reference streams for the achievable HBM rate: a copy (1 read : 1 write)
+// and, the compositor's mix, two reads and one write (dst = a ^ b).  Each
+// thread keeps four 16-byte loads in flight before its stores.
+template <bool READ2>
+__global__ void __launch_bounds__(256) k_stream(const uint4 *__restrict__ a,
+                                                const uint4 *__restrict__ b,
+                                                uint4 *__restrict__ dst, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    dst[i] = src[i];
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = a[i + k * stride];
+    if (READ2) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint4 w = b[i + k * stride];
+        v[k].x ^= w.x; v[k].y ^= w.y; v[k].z ^= w.z; v[k].w ^= w.w;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[i + k * stride] = v[k];
+  }
+  for (; i < n; i += stride) {
+    uint4 v = a[i];
+    if (READ2) {
+      const uint4 w = b[i];
+      v.x ^= w.x; v.y ^= w.y; v.z ^= w.z; v.w ^= w.w;
+    }
+    dst[i] = v;
+  }
 }
 
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
   uint64_t n = bytes / 16;
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_stream_copy, dim3((uint32_t)blocks), dim3(256), 0, st,
-                     reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n);
+  uint64_t blocks = (n + 1023) / 1024;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_stream<false>, dim3((uint32_t)blocks), dim3(256), 0, st,
+                     reinterpret_cast<const uint4 *>(src), nullptr, reinterpret_cast<uint4 *>(dst),
+                     n);
+  return hipGetLastError();
+}
+
+hipError_t z2d_launch_stream_read2(const void *a, const void *b, void *dst, uint64_t bytes,
+                                   hipStream_t st) {
+  uint64_t n = bytes / 16;
+  uint64_t blocks = (n + 1023) / 1024;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_stream<true>, dim3((uint32_t)blocks), dim3(256), 0, st,
+                     reinterpret_cast<const uint4 *>(a), reinterpret_cast<const uint4 *>(b),
+                     reinterpret_cast<uint4 *>(dst), n);
   return hipGetLastError();
 }
 

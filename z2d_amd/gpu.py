@@ -74,6 +74,7 @@ _SIGS = {
                            P(C.c_uint64)], C.c_int32),
     "z2d_gpu_png_timing": ([C.c_void_p, P(C.c_double)], C.c_int32),
     "z2d_gpu_stream_copy": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
+    "z2d_gpu_stream_read2": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_double)], C.c_int32),
     "z2d_stroke_polygon": ([C.c_void_p, C.c_uint64, P(abi.StrokeOpts), C.c_uint32, C.c_void_p,
                             C.c_uint64, P(C.c_uint64), P(C.c_double), P(C.c_uint32)], C.c_int32),
     "z2d_gpu_server_create": ([C.c_void_p, C.c_uint64, C.c_uint32, P(C.c_void_p)], C.c_int32),
