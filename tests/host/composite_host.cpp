@@ -1,0 +1,158 @@
+// Host harness for the per-pixel composite of the device renderer
+// (composite_cov in z2d_amd/csrc/composite_px.h), built for the CPU with
+// Z2D_HOST_PIXEL under the address and undefined-behaviour sanitizers and
+// called through a non-inlined wrapper (the "outlined" form).  Every
+// operator, precision and format, coverage counts 0..16 (multisample) and
+// 0..1 (direct), solid paints with and without full alpha, against the
+// oracle: compositeOpaque / compositeOpacity / compositeStride
+// (src/internal/raster/shared.zig:9-117, src/surface.zig:557-581) stated as
+// SurfaceCompositor.run operation lists on a one-pixel surface
+// (zo_composite, src/compositor.zig:302-440).  Test infrastructure only.
+#define Z2D_HOST_PIXEL 1
+#include "../../z2d_amd/csrc/composite_px.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "../../oracle/z2d_oracle.h"
+
+using namespace z2d;
+
+__attribute__((noinline)) uint32_t composite_outlined(uint32_t raw, uint32_t fmt, int k, int kfull,
+                                                      const DevDraw &D, const DevPaint &P,
+                                                      const StopView &sv, const uint16_t *bn,
+                                                      int x, int y) {
+  return composite_cov(raw, fmt, k, kfull, D, P, sv, bn, x, y);
+}
+
+static uint64_t rng_state = 0x5EEDC0DEu;
+static uint64_t next_u64() {  // SplitMix64
+  uint64_t z = (rng_state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static uint32_t rnd(uint32_t n) { return (uint32_t)(next_u64() % n); }
+
+static int fmt_bits_of(uint32_t f) {
+  switch (f) {
+    case Z2D_FORMAT_ALPHA8: return 8;
+    case Z2D_FORMAT_ALPHA4: return 4;
+    case Z2D_FORMAT_ALPHA2: return 2;
+    case Z2D_FORMAT_ALPHA1: return 1;
+    default: return 32;
+  }
+}
+
+// A raw value valid for the format (premultiplied where the format has
+// color and alpha).
+static uint32_t random_raw(uint32_t fmt) {
+  const int bits = fmt_bits_of(fmt);
+  if (bits < 32) return rnd(1u << bits);
+  const uint32_t a = rnd(4) == 0 ? 255u : rnd(256);
+  uint32_t c[3];
+  for (int i = 0; i < 3; i++) c[i] = a ? rnd(a + 1) : 0;
+  switch (fmt) {
+    case Z2D_FORMAT_RGBA: return c[0] | (c[1] << 8) | (c[2] << 16) | (a << 24);
+    case Z2D_FORMAT_ARGB: return c[2] | (c[1] << 8) | (c[0] << 16) | (a << 24);
+    default: return rnd(256) | (rnd(256) << 8) | (rnd(256) << 16);  // RGB, XRGB
+  }
+}
+
+// The oracle's result of `ops` on a one-pixel surface holding raw.
+static uint32_t oracle_run(uint32_t fmt, uint32_t raw, const z2d_comp_op *ops, uint64_t n,
+                           uint32_t precision) {
+  uint8_t buf[8] = {0};
+  const int bits = fmt_bits_of(fmt);
+  if (bits == 32) std::memcpy(buf, &raw, 4); else buf[0] = (uint8_t)raw;
+  z2d_surface_desc s{};
+  s.format = fmt;
+  s.width = 1;
+  s.height = 1;
+  s.data = buf;
+  s.len = bits == 32 ? 4 : 1;
+  zo_composite(&s, 0, 0, ops, n, precision);
+  uint32_t out = 0;
+  if (bits == 32) std::memcpy(&out, buf, 4); else out = buf[0] & ((1u << bits) - 1u);
+  return out;
+}
+
+int main(int argc, char **argv) {
+  const int trials = argc > 1 ? std::atoi(argv[1]) : 24;
+  uint64_t cases = 0, bad = 0;
+  for (uint32_t fmt = 0; fmt < 8; fmt++)
+    for (uint32_t op = 0; op < 28; op++)
+      for (uint32_t prec = 0; prec < 2; prec++)
+        for (int t = 0; t < trials; t++) {
+          const uint32_t raw = random_raw(fmt);
+          z2d_pixel paint{Z2D_FORMAT_RGBA, 0, 0, 0, 0};
+          paint.a = (uint8_t)(rnd(3) == 0 ? 255u : rnd(256));
+          paint.r = (uint8_t)(paint.a ? rnd(paint.a + 1u) : 0u);
+          paint.g = (uint8_t)(paint.a ? rnd(paint.a + 1u) : 0u);
+          paint.b = (uint8_t)(paint.a ? rnd(paint.a + 1u) : 0u);
+          DevPaint P;
+          std::memset(&P, 0, sizeof(P));
+          P.kind = Z2D_PATTERN_OPAQUE;
+          P.px_format = Z2D_FORMAT_RGBA;
+          P.px = I4{paint.r, paint.g, paint.b, paint.a};
+          P.px_opaque = paint.a == 255;
+          // the renderer's flags and precision upgrade (painter.fill's
+          // fillReducesToSource, Operator.requiresFloat)
+          const bool fast = op == Z2D_OP_SRC || (op == Z2D_OP_SRC_OVER && paint.a == 255);
+          const uint32_t precision = op_requires_float(op) ? Z2D_PRECISION_FLOAT : prec;
+          for (int mode = 0; mode < 2; mode++) {
+            DevDraw D;
+            std::memset(&D, 0, sizeof(D));
+            D.mode = mode == 0 ? MODE_MSAA : MODE_DIRECT;
+            D.op = op;
+            D.precision = precision;
+            D.flags = fast ? DRAW_FAST_SRC : 0u;
+            D.mask_bits = 8;
+            const int kfull = mode == 0 ? 16 : 1;
+            const StopView sv{nullptr, nullptr};
+            for (int k = 0; k <= kfull; k++) {
+              const uint32_t got = composite_outlined(raw, fmt, k, kfull, D, P, sv, nullptr, 0, 0);
+              uint32_t want;
+              z2d_comp_op ops[2];
+              std::memset(ops, 0, sizeof(ops));
+              if (k == 0) {
+                want = raw;  // no span covers the pixel
+              } else if (k >= kfull) {
+                // compositeOpaque: paintStride for the fast path, else one
+                // operation with the paint as source
+                ops[0].op = fast ? (uint32_t)Z2D_OP_SRC : op;
+                ops[0].src.kind = Z2D_PARAM_PIXEL;
+                ops[0].src.pixel = paint;
+                want = oracle_run(fmt, raw, ops, 1,
+                                  fast ? (uint32_t)Z2D_PRECISION_INTEGER : precision);
+              } else {
+                // compositeOpacity at the coverage's opacity: the paint
+                // masked by it (dst_in), then the operator; compositeStride
+                // (the fast path) is the same pair in integer precision
+                const uint8_t opacity = (uint8_t)(k * 16 - 1);
+                ops[0].op = Z2D_OP_DST_IN;
+                ops[0].dst.kind = Z2D_PARAM_PIXEL;
+                ops[0].dst.pixel = paint;
+                ops[0].src.kind = Z2D_PARAM_PIXEL;
+                ops[0].src.pixel = z2d_pixel{Z2D_FORMAT_ALPHA8, 0, 0, 0, opacity};
+                ops[1].op = op;
+                want = oracle_run(fmt, raw, ops, 2,
+                                  fast ? (uint32_t)Z2D_PRECISION_INTEGER : precision);
+              }
+              if (op == Z2D_OP_CLEAR && k > 0) want = 0;  // clearStride
+              cases++;
+              if (got != want) {
+                if (bad < 12)
+                  std::printf("mismatch fmt %u op %u prec %u mode %d k %d raw %08x paint "
+                              "%02x%02x%02x%02x: got %08x want %08x\n",
+                              fmt, op, precision, mode, k, raw, paint.r, paint.g, paint.b,
+                              paint.a, got, want);
+                bad++;
+              }
+            }
+          }
+        }
+  std::printf("cases %llu mismatches %llu\n", (unsigned long long)cases, (unsigned long long)bad);
+  return bad != 0;
+}

@@ -17,31 +17,84 @@
 
 #include "../../include/z2d_types.h"
 
+// Z2D_HOST_PIXEL: the pixel functions are compiled for the host as well (a
+// CPU harness runs them there, under the address and undefined-behaviour
+// sanitizers, against the oracle).
+#ifdef Z2D_HOST_PIXEL
+#define Z2D_PX __host__ __device__
+#else
+#define Z2D_PX __device__
+#endif
+
 namespace z2d {
+
+// 24-bit unsigned multiply (low 32 bits of the product of the low 24 bits)
+// and the two 32-bit halves of a double, on either side.
+Z2D_PX __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(a, b);
+#else
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+#endif
+}
+template <typename T>
+Z2D_PX __forceinline__ bool z_isnan(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return isnan(x);
+#else
+  return __builtin_isnan(x);
+#endif
+}
+template <typename T>
+Z2D_PX __forceinline__ bool z_isinf(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return isinf(x);
+#else
+  return __builtin_isinf(x);
+#endif
+}
+Z2D_PX __forceinline__ uint32_t dhi(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint32_t)__double2hiint(x);
+#else
+  uint64_t u;
+  __builtin_memcpy(&u, &x, 8);
+  return (uint32_t)(u >> 32);
+#endif
+}
+Z2D_PX __forceinline__ uint32_t dlo(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint32_t)__double2loint(x);
+#else
+  uint64_t u;
+  __builtin_memcpy(&u, &x, 8);
+  return (uint32_t)u;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // small math helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float fminz(float a, float b) { return fminf(a, b); }
-__device__ __forceinline__ float fmaxz(float a, float b) { return fmaxf(a, b); }
-__device__ __forceinline__ float clampz(float v, float lo, float hi) {
+Z2D_PX __forceinline__ float fminz(float a, float b) { return fminf(a, b); }
+Z2D_PX __forceinline__ float fmaxz(float a, float b) { return fmaxf(a, b); }
+Z2D_PX __forceinline__ float clampz(float v, float lo, float hi) {
   return fmaxz(lo, fminz(v, hi));
 }
 // Float @mod as Zig lowers it: x<0 ? fmod(fmod(x,y)+y, y) : fmod(x,y).
-__device__ __forceinline__ float zmodf(float x, float y) {
+Z2D_PX __forceinline__ float zmodf(float x, float y) {
   float a = fmodf(x, y);
   return x < 0.0f ? fmodf(a + y, y) : a;
 }
 // fmod(x, y) for y > 0: |x| < y gives x and y <= |x| < 2y gives x - y with
 // x's sign, exact by Sterbenz's lemma and equal to fmod's exact remainder;
 // other arguments go to fmod.
-__device__ __forceinline__ double fmod_pos(double x, double y) {
+Z2D_PX __forceinline__ double fmod_pos(double x, double y) {
   const double ax = fabs(x);
   if (ax < y) return x;
   if (ax < 2.0 * y) return x - copysign(y, x);
   return fmod(x, y);
 }
-__device__ __forceinline__ double zmodd(double x, double y) {
+Z2D_PX __forceinline__ double zmodd(double x, double y) {
   if (y > 0.0) {
     const double a = fmod_pos(x, y);
     return x < 0.0 ? fmod_pos(a + y, y) : a;
@@ -51,7 +104,7 @@ __device__ __forceinline__ double zmodd(double x, double y) {
 }
 // round(255*v) to an 8-bit channel; saturates outside the range the
 // reference defines (its @intFromFloat is undefined there).
-__device__ __forceinline__ uint32_t f2u8(float v) {
+Z2D_PX __forceinline__ uint32_t f2u8(float v) {
   float r = roundf(v);
   if (!(r >= 0.0f)) return 0u;
   if (r > 65535.0f) return 65535u;
@@ -60,9 +113,9 @@ __device__ __forceinline__ uint32_t f2u8(float v) {
 
 // std.math.pow(f32) as the reference's stdlib computes it (the Go-derived
 // algorithm: x^frac via exp(frac*log x), x^int by frexp squaring).
-__device__ inline float zpowf(float x, float y) {
+Z2D_PX inline float zpowf(float x, float y) {
   if (y == 0.0f || x == 1.0f) return 1.0f;
-  if (isnan(x) || isnan(y)) return __builtin_nanf("");
+  if (z_isnan(x) || z_isnan(y)) return __builtin_nanf("");
   if (y == 1.0f) return x;
   if (x == 0.0f) {
     float ip;
@@ -70,7 +123,7 @@ __device__ inline float zpowf(float x, float y) {
     if (y < 0.0f) return odd ? copysignf(__builtin_inff(), x) : __builtin_inff();
     return odd ? x : 0.0f;
   }
-  if (isinf(y) || isinf(x)) return powf(x, y);
+  if (z_isinf(y) || z_isinf(x)) return powf(x, y);
   if (y == 0.5f) return sqrtf(x);
   if (y == -0.5f) return 1.0f / sqrtf(x);
   float yi;
@@ -128,17 +181,17 @@ __host__ __device__ __forceinline__ int fmt_bits(uint32_t f) {
 }
 
 // Alpha8.scaleIntoIntAlpha from n bits (bit repetition).
-__device__ __forceinline__ uint32_t alpha_up8(uint32_t v, int bits) {
+Z2D_PX __forceinline__ uint32_t alpha_up8(uint32_t v, int bits) {
   return bits == 1 ? v * 255u : bits == 2 ? v * 85u : bits == 4 ? v * 17u : v;
 }
 // Alpha{4,2,1}.scaleIntoIntAlpha from u8 (shift right).
-__device__ __forceinline__ uint32_t alpha_down(uint32_t a8, int bits) {
+Z2D_PX __forceinline__ uint32_t alpha_down(uint32_t a8, int bits) {
   return (a8 & 0xFFu) >> (8 - bits);
 }
 
 // Raw pixel word (as stored: 4-byte formats as their u32, alpha8 as its byte,
 // packed alpha as its native value) -> RGBA16 lanes.
-__device__ __forceinline__ I4 unpack(uint32_t raw, uint32_t fmt) {
+Z2D_PX __forceinline__ I4 unpack(uint32_t raw, uint32_t fmt) {
   I4 o;
   switch (fmt) {
     case Z2D_FORMAT_RGBA:
@@ -164,7 +217,7 @@ __device__ __forceinline__ I4 unpack(uint32_t raw, uint32_t fmt) {
 }
 // RGBA16 lanes -> raw word for `fmt` (u8 truncation of each lane; rgb/xrgb
 // pad byte zero; packed alpha shifted down).
-__device__ __forceinline__ uint32_t pack(I4 v, uint32_t fmt) {
+Z2D_PX __forceinline__ uint32_t pack(I4 v, uint32_t fmt) {
   uint32_t r = v.r & 0xFF, g = v.g & 0xFF, b = v.b & 0xFF, a = v.a & 0xFF;
   switch (fmt) {
     case Z2D_FORMAT_RGBA: return r | (g << 8) | (b << 16) | (a << 24);
@@ -179,25 +232,25 @@ __device__ __forceinline__ uint32_t pack(I4 v, uint32_t fmt) {
 // ---------------------------------------------------------------------------
 // integer operators (u16 lanes)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t u16(uint32_t x) { return x & 0xFFFFu; }
-__device__ __forceinline__ uint32_t imul(uint32_t a, uint32_t b) { return u16((a * b) / 255u); }
+Z2D_PX __forceinline__ uint32_t u16(uint32_t x) { return x & 0xFFFFu; }
+Z2D_PX __forceinline__ uint32_t imul(uint32_t a, uint32_t b) { return u16((a * b) / 255u); }
 // imul for channels a, b <= 255: (x * 0x8081) >> 23 equals x / 255 for every
 // x < 65536, so two 24-bit multiplies and a shift replace the 32-bit
 // multiply-high division.
-__device__ __forceinline__ uint32_t imul8(uint32_t a, uint32_t b) {
-  return __umul24(__umul24(a, b), 0x8081u) >> 23;
+Z2D_PX __forceinline__ uint32_t imul8(uint32_t a, uint32_t b) {
+  return mul24(mul24(a, b), 0x8081u) >> 23;
 }
-__device__ __forceinline__ uint32_t iinvmul(uint32_t a, uint32_t b) {
+Z2D_PX __forceinline__ uint32_t iinvmul(uint32_t a, uint32_t b) {
   return imul(a, u16(255u - b));
 }
 
-__device__ __forceinline__ uint32_t overlay_ch(int32_t s, int32_t d, int32_t sa, int32_t da) {
+Z2D_PX __forceinline__ uint32_t overlay_ch(int32_t s, int32_t d, int32_t sa, int32_t da) {
   if ((int32_t)u16((uint32_t)d * 2u) <= da)
     return u16((uint32_t)((2 * s * d) / 255 + (s * (255 - da)) / 255 + (d * (255 - sa)) / 255));
   return u16((uint32_t)((s * (255 + da)) / 255 + (d * (255 + sa)) / 255 - (2 * d * s) / 255 -
                         (da * sa) / 255));
 }
-__device__ __forceinline__ uint32_t hardlight_ch(int32_t s, int32_t d, int32_t sa, int32_t da) {
+Z2D_PX __forceinline__ uint32_t hardlight_ch(int32_t s, int32_t d, int32_t sa, int32_t da) {
   if ((int32_t)u16((uint32_t)s * 2u) <= sa)
     return u16((uint32_t)((2 * s * d) / 255 + (s * (255 - da)) / 255 + (d * (255 - sa)) / 255));
   return u16((uint32_t)((s * (255 + da)) / 255 + (d * (255 + sa)) / 255 - (sa * da) / 255 -
@@ -212,18 +265,18 @@ __device__ __forceinline__ uint32_t hardlight_ch(int32_t s, int32_t d, int32_t s
 // instead of seven, and (da sa)/255 once per pixel.  With lanes <= 255 every
 // dividend is non-negative and below 2^18, where the unsigned quotient is
 // the signed one; the sum wraps to 16 bits as the signed form's does.
-__device__ __forceinline__ uint32_t blend8_ch(bool p, uint32_t s, uint32_t d, uint32_t sa,
+Z2D_PX __forceinline__ uint32_t blend8_ch(bool p, uint32_t s, uint32_t d, uint32_t sa,
                                               uint32_t da, uint32_t q_alpha) {
-  const uint32_t t = __umul24(2u * s, d) / 255u;
-  const uint32_t q1 = __umul24(s, p ? 255u - da : 255u + da) / 255u;
-  const uint32_t q2 = __umul24(d, p ? 255u - sa : 255u + sa) / 255u;
+  const uint32_t t = mul24(2u * s, d) / 255u;
+  const uint32_t q1 = mul24(s, p ? 255u - da : 255u + da) / 255u;
+  const uint32_t q2 = mul24(d, p ? 255u - sa : 255u + sa) / 255u;
   return u16(p ? t + q1 + q2 : q1 + q2 - t - q_alpha);
 }
-__device__ __forceinline__ bool lanes8(I4 d, I4 s) {
+Z2D_PX __forceinline__ bool lanes8(I4 d, I4 s) {
   return (d.r | d.g | d.b | d.a | s.r | s.g | s.b | s.a) <= 255u;
 }
 
-__device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
+Z2D_PX inline I4 op_int(uint32_t op, I4 d, I4 s) {
   I4 o = {0, 0, 0, 0};
   const uint32_t sa = s.a, da = d.a;
   const uint32_t ua = u16(sa + da - imul(sa, da));
@@ -282,7 +335,7 @@ __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
       return o;
     case Z2D_OP_OVERLAY:
       if (lanes8(d, s)) {
-        const uint32_t qa = __umul24(da, sa) / 255u;
+        const uint32_t qa = mul24(da, sa) / 255u;
         o.r = blend8_ch(2u * d.r <= da, s.r, d.r, sa, da, qa);
         o.g = blend8_ch(2u * d.g <= da, s.g, d.g, sa, da, qa);
         o.b = blend8_ch(2u * d.b <= da, s.b, d.b, sa, da, qa);
@@ -306,7 +359,7 @@ __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
       return o;
     case Z2D_OP_HARD_LIGHT:
       if (lanes8(d, s)) {
-        const uint32_t qa = __umul24(da, sa) / 255u;
+        const uint32_t qa = mul24(da, sa) / 255u;
         o.r = blend8_ch(2u * s.r <= sa, s.r, d.r, sa, da, qa);
         o.g = blend8_ch(2u * s.g <= sa, s.g, d.g, sa, da, qa);
         o.b = blend8_ch(2u * s.b <= sa, s.b, d.b, sa, da, qa);
@@ -339,20 +392,20 @@ __device__ inline I4 op_int(uint32_t op, I4 d, I4 s) {
 // ---------------------------------------------------------------------------
 // float operators
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float dodge_ch(float s, float d, float sa, float da) {
+Z2D_PX __forceinline__ float dodge_ch(float s, float d, float sa, float da) {
   if (s == sa && d == 0.0f) return s * (1.0f - da);
   if (s == sa) return sa * da + s * (1.0f - da) + d * (1.0f - sa);
   return sa * da * fminz(1.0f, d / da * sa / (sa - s)) + s * (1.0f - da) + d * (1.0f - sa);
 }
 // The reference's p0 receives (sca, dca, sa): its first branch tests
 // sca == 0 && dca == sa.
-__device__ __forceinline__ float burn_ch(float s, float d, float sa, float da) {
+Z2D_PX __forceinline__ float burn_ch(float s, float d, float sa, float da) {
   if (s == 0.0f && d == sa) return sa * da + d * (1.0f - sa);
   if (s == 0.0f) return d * (1.0f - sa);
   return sa * da * (1.0f - fminz(1.0f, (1.0f - d / da) * sa / s)) + s * (1.0f - da) +
          d * (1.0f - sa);
 }
-__device__ __forceinline__ float soft_ch(float s, float d, float sa, float da) {
+Z2D_PX __forceinline__ float soft_ch(float s, float d, float sa, float da) {
   if (da == 0.0f) return s;
   float m = d / da;
   if (2.0f * s <= sa)
@@ -366,12 +419,12 @@ __device__ __forceinline__ float soft_ch(float s, float d, float sa, float da) {
 struct F3 {
   float r, g, b;
 };
-__device__ __forceinline__ float mx3(F3 c) { return fmaxz(fmaxz(c.r, c.g), c.b); }
-__device__ __forceinline__ float mn3(F3 c) { return fminz(fminz(c.r, c.g), c.b); }
-__device__ __forceinline__ float lum3(F3 c) { return c.r * 0.3f + c.g * 0.59f + c.b * 0.11f; }
+Z2D_PX __forceinline__ float mx3(F3 c) { return fmaxz(fmaxz(c.r, c.g), c.b); }
+Z2D_PX __forceinline__ float mn3(F3 c) { return fminz(fminz(c.r, c.g), c.b); }
+Z2D_PX __forceinline__ float lum3(F3 c) { return c.r * 0.3f + c.g * 0.59f + c.b * 0.11f; }
 // clipColor with the reference's vector semantics: the x>a branch wins over
 // the n<0 branch and both read the unclipped channel.
-__device__ __forceinline__ float clip1(float c, float l, float n, float x, float a) {
+Z2D_PX __forceinline__ float clip1(float c, float l, float n, float x, float a) {
   float r = c;
   if (n < 0.0f) {
     float t = l - n;
@@ -383,26 +436,26 @@ __device__ __forceinline__ float clip1(float c, float l, float n, float x, float
   }
   return r;
 }
-__device__ __forceinline__ F3 clip3(F3 c, float a) {
+Z2D_PX __forceinline__ F3 clip3(F3 c, float a) {
   float l = lum3(c), n = mn3(c), x = mx3(c);
   return F3{clip1(c.r, l, n, x, a), clip1(c.g, l, n, x, a), clip1(c.b, l, n, x, a)};
 }
-__device__ __forceinline__ F3 setlum3(F3 c, float l) {
+Z2D_PX __forceinline__ F3 setlum3(F3 c, float l) {
   float d = l - lum3(c);
   return F3{c.r + d, c.g + d, c.b + d};
 }
-__device__ __forceinline__ F3 setsat3(F3 c, float s) {
+Z2D_PX __forceinline__ F3 setsat3(F3 c, float s) {
   float n = mn3(c), x = mx3(c), d = x - n;
   return F3{d == 0.0f ? 0.0f : (c.r - n) * s / d, d == 0.0f ? 0.0f : (c.g - n) * s / d,
             d == 0.0f ? 0.0f : (c.b - n) * s / d};
 }
-__device__ __forceinline__ F4 ns_out(F3 c, F4 d, F4 s) {
+Z2D_PX __forceinline__ F4 ns_out(F3 c, F4 d, F4 s) {
   return F4{s.r * (1.0f - d.a) + d.r * (1.0f - s.a) + c.r,
             s.g * (1.0f - d.a) + d.g * (1.0f - s.a) + c.g,
             s.b * (1.0f - d.a) + d.b * (1.0f - s.a) + c.b, s.a + d.a - s.a * d.a};
 }
 
-__device__ inline F4 op_float(uint32_t op, F4 d, F4 s) {
+Z2D_PX inline F4 op_float(uint32_t op, F4 d, F4 s) {
   const float ua = s.a + d.a - s.a * d.a;
   switch (op) {
     case Z2D_OP_CLEAR: return F4{0, 0, 0, 0};
@@ -512,10 +565,10 @@ __host__ __device__ __forceinline__ bool op_is_bounded(uint32_t op) {
            op == Z2D_OP_DST_ATOP);
 }
 
-__device__ __forceinline__ F4 decode_raw(I4 v) {
+Z2D_PX __forceinline__ F4 decode_raw(I4 v) {
   return F4{(float)v.r / 255.0f, (float)v.g / 255.0f, (float)v.b / 255.0f, (float)v.a / 255.0f};
 }
-__device__ __forceinline__ I4 encode_raw(F4 v) {
+Z2D_PX __forceinline__ I4 encode_raw(F4 v) {
   return I4{f2u8(255.0f * v.r), f2u8(255.0f * v.g), f2u8(255.0f * v.b), f2u8(255.0f * v.a)};
 }
 
@@ -565,7 +618,7 @@ struct DevPaint {
   const DevGradient *gradient;  // gradient pattern or gradient dither source
 };
 
-__device__ __forceinline__ void user_to_device(const double *m, double &x, double &y) {
+Z2D_PX __forceinline__ void user_to_device(const double *m, double &x, double &y) {
   double ox = m[0] * x + m[1] * y;
   double oy = m[2] * x + m[3] * y;
   x = ox + m[4];
@@ -579,16 +632,16 @@ __device__ __forceinline__ void user_to_device(const double *m, double &x, doubl
 // chains and adds atan of the range's centre in two parts; atan2 handles
 // signs, zeros and infinities and takes atan(|y/x|).  A few live doubles,
 // so a conic gradient's sampler stays small.
-__device__ __forceinline__ double zatan(double x) {
+Z2D_PX __forceinline__ double zatan(double x) {
   const double atanhi0 = 4.63647609000806093515e-01, atanhi1 = 7.85398163397448278999e-01,
                atanhi2 = 9.82793723247329054082e-01, atanhi3 = 1.57079632679489655800e+00;
   const double atanlo0 = 2.26987774529616870924e-17, atanlo1 = 3.06161699786838301793e-17,
                atanlo2 = 1.39033110312309984516e-17, atanlo3 = 6.12323399573676603587e-17;
-  uint32_t ix = (uint32_t)__double2hiint(x);
+  uint32_t ix = dhi(x);
   const uint32_t sign = ix >> 31;
   ix &= 0x7FFFFFFFu;
   if (ix >= 0x44100000u) {  // |x| >= 2^66
-    if (isnan(x)) return x;
+    if (z_isnan(x)) return x;
     const double z = atanhi3 + 0x1.0p-120;
     return sign ? -z : z;
   }
@@ -633,11 +686,11 @@ __device__ __forceinline__ double zatan(double x) {
   const double zz = hi - ((x * (s1 + s2) - lo) - x);
   return sign ? -zz : zz;
 }
-__device__ __forceinline__ double zatan2(double y, double x) {
+Z2D_PX __forceinline__ double zatan2(double y, double x) {
   const double pi = 3.1415926535897931160E+00, pi_lo = 1.2246467991473531772E-16;
-  if (isnan(x) || isnan(y)) return x + y;
-  uint32_t ix = (uint32_t)__double2hiint(x), iy = (uint32_t)__double2hiint(y);
-  const uint32_t lx = (uint32_t)__double2loint(x), ly = (uint32_t)__double2loint(y);
+  if (z_isnan(x) || z_isnan(y)) return x + y;
+  uint32_t ix = dhi(x), iy = dhi(y);
+  const uint32_t lx = dlo(x), ly = dlo(y);
   if (((ix - 0x3FF00000u) | lx) == 0) return zatan(y);  // x = 1.0
   const uint32_t m = ((iy >> 31) & 1u) | ((ix >> 30) & 2u);  // 2 sign(x) + sign(y)
   ix &= 0x7FFFFFFFu;
@@ -665,10 +718,10 @@ __device__ __forceinline__ double zatan2(double y, double x) {
   return (z - pi_lo) - pi;
 }
 
-__device__ __forceinline__ double dclamp01(double v) { return fmax(0.0, fmin(v, 1.0)); }
+Z2D_PX __forceinline__ double dclamp01(double v) { return fmax(0.0, fmin(v, 1.0)); }
 
 // Gradient.getOffset at pixel (x, y), one function per gradient type.
-__device__ __forceinline__ float linear_offset(const DevGradient &g, int x, int y) {
+Z2D_PX __forceinline__ float linear_offset(const DevGradient &g, int x, int y) {
   double px = (double)x + 0.5, py = (double)y + 0.5;
   if (!g.identity) user_to_device(g.m, px, py);
   if (g.lin_degenerate) return -1.0f;
@@ -678,7 +731,7 @@ __device__ __forceinline__ float linear_offset(const DevGradient &g, int x, int 
   dt += g.lin_dy * spy;
   return (float)dclamp01(dt * g.lin_inv);
 }
-__device__ __forceinline__ float radial_offset(const DevGradient &g, int x, int y) {
+Z2D_PX __forceinline__ float radial_offset(const DevGradient &g, int x, int y) {
   double px = (double)x + 0.5, py = (double)y + 0.5;
   {
     if (g.r1 == 0.0 && g.r2 == 0.0) return -1.0f;
@@ -711,21 +764,21 @@ __device__ __forceinline__ float radial_offset(const DevGradient &g, int x, int 
     return -1.0f;
   }
 }
-__device__ __forceinline__ float conic_offset(const DevGradient &g, int x, int y) {
+Z2D_PX __forceinline__ float conic_offset(const DevGradient &g, int x, int y) {
   double px = (double)x + 0.5, py = (double)y + 0.5;
   const double two_pi = 3.14159265358979323846 * 2.0;
   if (!g.identity) user_to_device(g.m, px, py);
   double ang = zmodd(zatan2(py - g.geom[1], px - g.geom[0]) - g.conic_angle, two_pi);
   return (float)(ang / two_pi);
 }
-__device__ inline float gradient_offset(const DevGradient &g, int x, int y) {
+Z2D_PX inline float gradient_offset(const DevGradient &g, int x, int y) {
   if (g.type == Z2D_GRADIENT_LINEAR) return linear_offset(g, x, y);
   if (g.type == Z2D_GRADIENT_RADIAL) return radial_offset(g, x, y);
   return conic_offset(g, x, y);
 }
 
 // Stop.List.search.
-__device__ inline void stops_search(const float *st, uint32_t n_stops, float offset, int &i0,
+Z2D_PX inline void stops_search(const float *st, uint32_t n_stops, float offset, int &i0,
                                     int &i1, float &t, bool &transparent) {
   const int n = (int)n_stops;
   transparent = false;
@@ -773,7 +826,7 @@ __device__ inline void stops_search(const float *st, uint32_t n_stops, float off
 // from the number of stops below the offset with no data-dependent loop; an
 // offset equal to some stop offset (where the search path picks among equal
 // stops) goes through stops_search itself.
-__device__ __forceinline__ void stops_find8(const float *st, uint32_t n_stops, float offset,
+Z2D_PX __forceinline__ void stops_find8(const float *st, uint32_t n_stops, float offset,
                                             int &i0, int &i1, float &t, bool &transparent) {
   const int n = (int)n_stops;
   transparent = offset < 0.0f || n == 0;
@@ -802,7 +855,7 @@ __device__ __forceinline__ void stops_find8(const float *st, uint32_t n_stops, f
 }
 
 // HSL.toRGB (src/color.zig:470-488).
-__device__ inline void hsl_to_rgb(const float *h, float *o) {
+Z2D_PX inline void hsl_to_rgb(const float *h, float *o) {
   float hue = fmodf(h[0], 360.0f);
   if (hue < 0.0f) hue += 360.0f;
   const float ns[3] = {0.0f, 8.0f, 4.0f};
@@ -815,7 +868,7 @@ __device__ inline void hsl_to_rgb(const float *h, float *o) {
   o[3] = h[3];
 }
 // HSL.fromRGB (src/color.zig:435-467).
-__device__ inline void hsl_from_rgb(const float *c, float *o) {
+Z2D_PX inline void hsl_from_rgb(const float *c, float *o) {
   float mx = fmaxz(c[0], fmaxz(c[1], c[2])), mn = fminz(c[0], fminz(c[1], c[2]));
   float range = mx - mn, light = (mn + mx) / 2.0f;
   float sat = (light == 0.0f || light == 1.0f) ? 0.0f : (mx - light) / fminz(light, 1.0f - light);
@@ -836,7 +889,7 @@ __device__ inline void hsl_from_rgb(const float *c, float *o) {
   o[0] = hue; o[1] = sat; o[2] = light; o[3] = c[3];
 }
 // RGB(profile).fromColor: srgb=false -> linear, true -> gamma-encoded sRGB.
-__device__ inline void color_to_rgb(const DevStop &s, bool srgb, float *o) {
+Z2D_PX inline void color_to_rgb(const DevStop &s, bool srgb, float *o) {
   if (s.space == Z2D_COLOR_HSL) {
     hsl_to_rgb(s.v, o);
     if (srgb) {
@@ -855,7 +908,7 @@ __device__ inline void color_to_rgb(const DevStop &s, bool srgb, float *o) {
   for (int i = 0; i < 3; i++) o[i] = zpowf(zpowf(s.v[i], gf), igt);
   o[3] = s.v[3];
 }
-__device__ inline void color_to_hsl(const DevStop &s, float *o) {
+Z2D_PX inline void color_to_hsl(const DevStop &s, float *o) {
   if (s.space == Z2D_COLOR_HSL) {
     o[0] = s.v[0]; o[1] = s.v[1]; o[2] = s.v[2]; o[3] = s.v[3];
     return;
@@ -865,9 +918,9 @@ __device__ inline void color_to_hsl(const DevStop &s, float *o) {
   hsl_from_rgb(lin, o);
 }
 
-__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+Z2D_PX __forceinline__ float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
 
-__device__ __forceinline__ I4 encode_premul(const float *v) {
+Z2D_PX __forceinline__ I4 encode_premul(const float *v) {
   I4 o{f2u8(255.0f * v[0]), f2u8(255.0f * v[1]), f2u8(255.0f * v[2]), f2u8(255.0f * v[3])};
   o.r = u16(o.r * o.a / 255u);
   o.g = u16(o.g * o.a / 255u);
@@ -882,7 +935,7 @@ __device__ __forceinline__ I4 encode_premul(const float *v) {
 // HSL with premultiplied saturation and lightness.  Evaluated once per stop
 // (k_grad_setup); a transparent (offset < 0) sample uses all zeros, which is
 // what this gives for the transparent linear stop in every space.
-__device__ inline float4 stop_interp_color(const DevGradient &g, const DevStop &s) {
+Z2D_PX inline float4 stop_interp_color(const DevGradient &g, const DevStop &s) {
   float a[4];
   if (g.method == Z2D_INTERP_LINEAR_RGB || g.method == Z2D_INTERP_SRGB) {
     color_to_rgb(s, g.method == Z2D_INTERP_SRGB, a);
@@ -892,7 +945,7 @@ __device__ inline float4 stop_interp_color(const DevGradient &g, const DevStop &
   return make_float4(a[0], a[1] * a[3], a[2] * a[3], a[3]);
 }
 
-__device__ inline void gradient_sample(const DevGradient &g, const StopView &sv, int x, int y,
+Z2D_PX inline void gradient_sample(const DevGradient &g, const StopView &sv, int x, int y,
                                        bool encode, I4 &oi, F4 &of) {
   int i0, i1;
   float t;
@@ -957,7 +1010,7 @@ __device__ inline void gradient_sample(const DevGradient &g, const StopView &sv,
 }
 
 // Dither threshold matrices (Dither.zig:120-158, color_vector.zig:482-522).
-__device__ __forceinline__ float bayer8(int x, int y) {
+Z2D_PX __forceinline__ float bayer8(int x, int y) {
   int yy = y ^ x;
   uint32_t m = (uint32_t)(((yy & 1) << 5) | ((x & 1) << 4) | ((yy & 2) << 2) | ((x & 2) << 1) |
                           ((yy & 4) >> 1) | ((x & 4) >> 2));
@@ -967,7 +1020,7 @@ __device__ __forceinline__ float bayer8(int x, int y) {
 // fromDitherVec for one lane: linear straight-alpha f32.
 // g: the paint's gradient (p.gradient or a copy of it); read only for a
 // gradient source.
-__device__ inline F4 dither_sample(const DevPaint &p, const DevGradient *g, const StopView &sv,
+Z2D_PX inline F4 dither_sample(const DevPaint &p, const DevGradient *g, const StopView &sv,
                                    const uint16_t *blue_noise, int x, int y) {
   F4 c;
   if (p.dither_source == Z2D_DITHER_SRC_GRADIENT) {
@@ -992,7 +1045,7 @@ __device__ inline F4 dither_sample(const DevPaint &p, const DevGradient *g, cons
 
 // Integer paint sample (premultiplied RGBA, 0-255 per channel); g is the
 // paint's gradient (p.gradient or a copy of it) when it has one.
-__device__ __forceinline__ I4 paint_int_g(const DevPaint &p, const DevGradient *g,
+Z2D_PX __forceinline__ I4 paint_int_g(const DevPaint &p, const DevGradient *g,
                                           const StopView &sv, const uint16_t *bn, int x, int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return p.px;
   if (p.kind == Z2D_PATTERN_GRADIENT) {
@@ -1006,7 +1059,7 @@ __device__ __forceinline__ I4 paint_int_g(const DevPaint &p, const DevGradient *
   return encode_premul(v);
 }
 
-__device__ __noinline__ I4 paint_int(const DevPaint &p, const StopView &sv, const uint16_t *bn, int x,
+Z2D_PX __noinline__ I4 paint_int(const DevPaint &p, const StopView &sv, const uint16_t *bn, int x,
                                int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return p.px;
   if (p.kind == Z2D_PATTERN_GRADIENT) {
@@ -1019,7 +1072,7 @@ __device__ __noinline__ I4 paint_int(const DevPaint &p, const StopView &sv, cons
   float v[4] = {f.r, f.g, f.b, f.a};
   return encode_premul(v);
 }
-__device__ inline F4 paint_float(const DevPaint &p, const StopView &sv, const uint16_t *bn,
+Z2D_PX inline F4 paint_float(const DevPaint &p, const StopView &sv, const uint16_t *bn,
                                  int x, int y) {
   if (p.kind == Z2D_PATTERN_OPAQUE) return decode_raw(p.px);
   if (p.kind == Z2D_PATTERN_GRADIENT) {
@@ -1032,7 +1085,7 @@ __device__ inline F4 paint_float(const DevPaint &p, const StopView &sv, const ui
 }
 
 // The stops of a paint as stored with its gradient.
-__device__ __forceinline__ StopView paint_stops(const DevPaint &p) {
+Z2D_PX __forceinline__ StopView paint_stops(const DevPaint &p) {
   if (!p.gradient) return StopView{nullptr, nullptr};
   return StopView{p.gradient->offs, p.gradient->pre};
 }

@@ -25,6 +25,7 @@
 //   k_composite   SurfaceCompositor.run batches (src/compositor.zig:302-440)
 #include <hip/hip_runtime.h>
 
+#include "composite_px.h"
 #include "render.h"
 
 namespace z2d {
@@ -66,9 +67,6 @@ __device__ __forceinline__ void store_raw(const DevSurface &s, int64_t idx, uint
   }
 }
 
-__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) {
-  return v < lo ? lo : (v > hi ? hi : v);
-}
 
 // Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
 // 16-lane rows, then row broadcasts across rows).
@@ -1621,48 +1619,7 @@ __device__ __forceinline__ void unpack_pair(uint32_t w, int32_t &lo, int32_t &hi
   hi = (int32_t)(w - (uint32_t)lo) >> 16;
 }
 
-__device__ __forceinline__ uint32_t ssaa_mask(int k, int bits) {
-  switch (bits) {
-    case 4: return ((15u * (uint32_t)k) / 16u) * 17u;
-    case 2: return ((3u * (uint32_t)k) / 16u) * 85u;
-    case 1: return ((1u * (uint32_t)k) / 16u) * 255u;
-    default: return (255u * (uint32_t)k) / 16u;
-  }
-}
 
-// One pixel through one draw, given its sample coverage k (of kfull): the
-// paint masked by the coverage (DST_IN), then the draw's operator.  The
-// supersample mode masks every pixel of its box, even at zero coverage; the
-// solid fast path of painter.fill stays in integer precision.
-__device__ __forceinline__ uint32_t composite_cov(uint32_t raw, uint32_t fmt, int k, int kfull,
-                                                  const DevDraw &D, const DevPaint &P,
-                                                  const StopView &sv, const uint16_t *bn, int x,
-                                                  int y) {
-  uint32_t mask_a = 255;
-  bool masked = true;
-  if (D.mode == MODE_SSAA) {
-    mask_a = ssaa_mask(k, D.mask_bits);
-  } else {
-    if (k == 0) return raw;
-    if (D.op == Z2D_OP_CLEAR) return pack(I4{0, 0, 0, 0}, fmt);
-    if (k >= kfull) {
-      if (D.flags & DRAW_FAST_SRC) return pack(P.px, fmt);
-      masked = false;
-    } else {
-      mask_a = (uint32_t)clampi(k * 16 - 1, 0, 255);
-    }
-  }
-  const bool int_path = D.precision == Z2D_PRECISION_INTEGER ||
-                        (D.mode != MODE_SSAA && (D.flags & DRAW_FAST_SRC));
-  if (int_path) {
-    I4 src = P.kind == Z2D_PATTERN_OPAQUE ? P.px : paint_int(P, sv, bn, x, y);
-    if (masked) src = op_int(Z2D_OP_DST_IN, src, I4{0, 0, 0, mask_a});
-    return pack(op_int(D.op, unpack(raw, fmt), src), fmt);
-  }
-  F4 src = P.kind == Z2D_PATTERN_OPAQUE ? decode_raw(P.px) : paint_float(P, sv, bn, x, y);
-  if (masked) src = op_float(Z2D_OP_DST_IN, src, decode_raw(I4{0, 0, 0, mask_a}));
-  return pack(encode_raw(op_float(D.op, decode_raw(unpack(raw, fmt)), src)), fmt);
-}
 
 // composite_cov for the common case, kept inline: src_over in integer
 // precision without a supersample mask, given the paint sample s (the same
