@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "z2d_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(z2d_(?:gpu|stroke|font|text)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(z2d_(?:gpu|stroke|font|text)_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_exports_every_declared_symbol():
