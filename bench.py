@@ -256,17 +256,33 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     uniform, 25% dashed).  Strokes are tessellated on the host when the batch
     is built.  One persistent launch vs one launch per surface over the same
     prepared batch; the persistent launch also as a replayed hipGraph of the
-    whole pipeline.  The band height is picked by the autotuner first."""
-    b = gpu.Batch(ctx)
-    surfs = []
+    whole pipeline.  The band height is picked by the autotuner first.  The
+    host build is timed both as one call per draw and through draw_n
+    (strokes tessellated on host threads)."""
+    surfs, items = [], []
     for fmt, side, jobs, keep in scenes.uneven_surfaces(n_surfaces, seed):
         s = gpu.DeviceSurface(ctx, fmt, side, side)
         surfs.append(s)
         surfs.append(keep)
         for kind, pat, nodes, opts in jobs:
-            rc = (b.stroke if kind == "stroke" else b.fill)(s, pat, nodes, opts)
-            if rc:
-                raise RuntimeError(f"batch {kind} rc={rc}")
+            items.append((s, kind, pat, nodes, opts))
+    # host build of the batch: one call per draw (strokes tessellated on the
+    # caller's thread), then the same draws through draw_n (strokes
+    # tessellated on host threads); the second batch is the one rendered
+    b0 = gpu.Batch(ctx)
+    t = time.perf_counter()
+    for s, kind, pat, nodes, opts in items:
+        rc = (b0.stroke if kind == "stroke" else b0.fill)(s, pat, nodes, opts)
+        if rc:
+            raise RuntimeError(f"batch {kind} rc={rc}")
+    build_calls = time.perf_counter() - t
+    b0.close()
+    b = gpu.Batch(ctx)
+    t = time.perf_counter()
+    st = b.draw_n(items)
+    build_n = time.perf_counter() - t
+    if any(st):
+        raise RuntimeError(f"batch draw_n statuses {set(st)}")
     b.prepare()
     pix, _ = count_work(b)
     res = {}
@@ -289,6 +305,9 @@ def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
     b.use_graph(False)
     res["speedup"] = round(res["batched"] / res["per_surface"], 2)
     res["speedup_graph"] = round(res["batched_graph"] / res["per_surface"], 2)
+    res["host_build_ms"] = {"per_call": round(build_calls * 1e3, 1),
+                            "draw_n": round(build_n * 1e3, 1),
+                            "threads": min(16, os.cpu_count() or 1)}
     res["surfaces"] = n_surfaces
     res["pixel_draws"] = int(pix)
     res["stroke_share"] = 0.3

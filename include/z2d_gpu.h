@@ -262,6 +262,18 @@ typedef struct z2d_draw_item {
   const z2d_stroke_opts *stroke;   /* kind == Z2D_DRAW_STROKE (its dashes are copied) */
 } z2d_draw_item;
 
+/* Batch form of many painter.fill / painter.stroke calls (src/painter.zig:
+ * 66-143, :214-344), item i on surfaces[i], in order: the same draws and the
+ * same per-item status (status[i], optional) as z2d_gpu_batch_fill /
+ * z2d_gpu_batch_stroke called once per item, a failed item left out as a
+ * failed call is.  The strokes' host tessellation runs on `threads` host
+ * threads (0: one per hardware thread, at most 16) before the draws are
+ * appended.  Returns the first item failure, else Z2D_OK.  No reference
+ * counterpart: a caller's loop of painter calls, made parallel. */
+int32_t z2d_gpu_batch_draw_n(z2d_gpu_batch *b, z2d_gpu_surface *const *surfaces,
+                             const z2d_draw_item *items, uint64_t n, uint32_t threads,
+                             int32_t *status);
+
 typedef struct z2d_request_desc {
   uint32_t format;
   int32_t width;

@@ -77,3 +77,51 @@ def test_batch_many_surfaces_bit_exact(mode):
         want = run_draws(OracleBackend(), fmt, w, h, init, draws)
         msg = diff_report(got, want.data())
         assert not msg, msg
+
+
+def test_batch_draw_n_equals_per_item_calls():
+    """z2d_gpu_batch_draw_n (strokes tessellated on host threads) appends the
+    same draws as one fill / stroke call per item: 96 uneven surfaces with
+    30% strokes render to the same bytes, per-item statuses included (a
+    stroke with a singular matrix fails alone, as its single call does)."""
+    from z2d_amd.gpu import Batch, DeviceSurface
+    ctx = gpu_backend().ctx
+    work = scenes.uneven_surfaces(96, 0xC4)
+    bad_opts = None
+    outs = []
+    for use_n in (False, True):
+        b = Batch(ctx)
+        surfs = [DeviceSurface(ctx, fmt, side, side) for fmt, side, _j, _k in work]
+        for s in surfs:
+            s.upload(np.zeros(s.nbytes, np.uint8))
+        jobs = []
+        for s, (fmt, side, js, keep) in zip(surfs, work):
+            for kind, pat, nodes, opts in js:
+                jobs.append((s, kind, pat, nodes, opts))
+        # one failing stroke in the middle: a singular transformation
+        k = next(i for i, j in enumerate(jobs) if j[1] == "stroke" and i > len(jobs) // 2)
+        s, kind, pat, nodes, opts = jobs[k]
+        if bad_opts is None:
+            bad_opts = abi.StrokeOpts.from_buffer_copy(opts)
+            bad_opts.transformation[:] = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+        jobs.insert(k, (s, "stroke", pat, nodes, bad_opts))
+        if use_n:
+            st = b.draw_n(jobs, threads=8)
+        else:
+            st = [(b.stroke if kind == "stroke" else b.fill)(s, pat, nodes, opts)
+                  for s, kind, pat, nodes, opts in jobs]
+        assert st[k] == abi.ERR_INVALID_MATRIX, st[k]
+        assert all(v == 0 for i, v in enumerate(st) if i != k)
+        b.prepare()
+        b.render(0)
+        b.wait()
+        got = []
+        for s in surfs:
+            o = np.empty(s.nbytes, np.uint8)
+            s.download(o)
+            got.append(o)
+            s.close()
+        b.close()
+        outs.append(got)
+    for i, (a, c) in enumerate(zip(*outs)):
+        assert np.array_equal(a, c), f"surface {i}"

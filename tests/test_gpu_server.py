@@ -309,3 +309,47 @@ def test_server_png_responses():
 def gpu_png_bound(rq):
     from z2d_amd import gpu
     return gpu.png_bound(rq["fmt"], rq["w"], rq["h"], rq["prof"])
+
+
+def test_server_resident_gradients_across_requests():
+    """Gradients stay resident in HBM across the server's batches: requests
+    that repeat a gradient (the same descriptor again, and an equal copy of
+    it built separately) render as the oracle does, in batches after the one
+    that first uploaded it and alongside gradients seen once."""
+    rng = scenes.SplitMix64(0x6AD)
+    base = make_request(rng, fmt=abi.RGBA, side_max=200, with_clip=False)
+    stops = [(0.0, abi.color(abi.COLOR_LINEAR_RGB, 0.9, 0.1, 0.2, 1.0)),
+             (0.4, abi.color(abi.COLOR_SRGB, 0.1, 0.8, 0.3, 0.7)),
+             (1.0, abi.color(abi.COLOR_LINEAR_RGB, 0.2, 0.3, 0.9, 0.9))]
+
+    def grad(kind):
+        geom = ([5.0, 7.0, 150.0, 90.0] if kind == abi.GRADIENT_LINEAR
+                else [60.0, 50.0, 4.0, 70.0, 60.0, 80.0])
+        return abi.Gradient(kind, geom, stops, abi.INTERP_LINEAR_RGB, abi.POLAR_SHORTER,
+                            abi.IDENTITY)
+
+    reqs = []
+    for i in range(6):
+        keep = list(base["keep"])
+        draws = []
+        for j, (pat, nodes, opts) in enumerate(base["draws"]):
+            # an equal gradient built again for every request
+            g = grad(abi.GRADIENT_LINEAR if j % 2 == 0 else abi.GRADIENT_RADIAL)
+            keep.append(g)
+            draws.append((g.pattern(), nodes, opts))
+        reqs.append(dict(base, draws=draws, keep=keep))
+    srv = _server(64 << 20)
+    try:
+        for rq in reqs:  # one at a time: each in its own batch
+            rid, out = submit(srv, rq)
+            assert srv.wait(rid) == 0
+            msg = diff_report(out, oracle_render(rq))
+            assert not msg, msg
+        subs = [submit(srv, rq) for rq in reqs + [make_request(rng) for _ in range(8)]]
+        for (rid, out), rq in zip(subs, reqs + [None] * 8):
+            assert srv.wait(rid) == 0
+            if rq is not None:
+                msg = diff_report(out, oracle_render(rq))
+                assert not msg, msg
+    finally:
+        srv.close()

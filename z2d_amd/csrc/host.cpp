@@ -375,6 +375,196 @@ bool can_demultiply(const z2d_pixel &p) {
          (uint32_t)p.b * 255u / p.a <= 255u;
 }
 
+// A gradient's device record (its stop pointers left for the caller) and
+// its stops in Stop.List order, appended to `stops`.
+static void build_gradient(const z2d_gradient_desc *g, DevGradient &d,
+                           std::vector<DevStop> &stops) {
+  std::memset(&d, 0, sizeof(d));
+  d.type = g->type;
+  d.method = g->method;
+  d.polar = g->polar;
+  d.n_stops = g->n_stops;
+  for (int i = 0; i < 6; i++) {
+    d.geom[i] = g->geom[i];
+    d.m[i] = g->matrix[i];
+  }
+  d.identity = (g->matrix[0] == 1.0 && g->matrix[1] == 0.0 && g->matrix[2] == 0.0 &&
+                g->matrix[3] == 1.0 && g->matrix[4] == 0.0 && g->matrix[5] == 0.0);
+  if (g->type == Z2D_GRADIENT_RADIAL) {
+    // Radial.initBuffer (src/gradient.zig:440-470)
+    d.r1 = std::max(0.0, g->geom[2]);
+    d.r2 = std::max(0.0, g->geom[5]);
+    d.cdx = g->geom[3] - g->geom[0];
+    d.cdy = g->geom[4] - g->geom[1];
+    d.dr = d.r2 - d.r1;
+    d.min_dr = -d.r1;
+    double a = 0.0;
+    a += d.cdx * d.cdx;
+    a += d.cdy * d.cdy;
+    a += d.dr * (-d.dr);
+    d.a = a;
+    d.inv_a = a != 0.0 ? 1.0 / a : 0.0;
+  } else if (g->type == Z2D_GRADIENT_LINEAR) {
+    // Linear.getOffset's per-gradient terms (src/gradient.zig)
+    d.lin_dx = g->geom[2] - g->geom[0];
+    d.lin_dy = g->geom[3] - g->geom[1];
+    double dist = 0.0;
+    dist += d.lin_dx * d.lin_dx;
+    dist += d.lin_dy * d.lin_dy;
+    d.lin_degenerate = dist == 0.0;
+    d.lin_inv = dist != 0.0 ? 1.0 / dist : 0.0;
+  } else if (g->type == Z2D_GRADIENT_CONIC) {
+    // Conic.initBuffer: @mod(angle, pi*2)
+    const double two_pi = 3.14159265358979323846 * 2.0;
+    double a = std::fmod(g->geom[2], two_pi);
+    d.conic_angle = g->geom[2] < 0.0 ? std::fmod(a + two_pi, two_pi) : a;
+  }
+  // Stop.List.addAssumeCapacity: clamp, Color.init, stable sort by offset.
+  uint32_t base = (uint32_t)stops.size();
+  for (uint32_t i = 0; i < g->n_stops; i++) {
+    DevStop s = stop_from(g->stops[i]);
+    size_t j = stops.size();
+    stops.push_back(s);
+    while (j > base && stops[j - 1].offset > s.offset) {
+      stops[j] = stops[j - 1];
+      j--;
+    }
+    stops[j] = s;
+  }
+}
+
+// Gradients resident in HBM across batches (a server's requests): each
+// distinct gradient is packed, uploaded and set up (k_grad_setup) once, on
+// the table's own stream, and later draws of an equal gradient (same type,
+// geometry, matrix, method and stops, compared byte for byte) use the
+// resident record.  Entries are never replaced: once the table is full, or
+// for gradients of more than kMaxStops stops, draws use the batch's own
+// tables.  A batch that uses the table makes its stream wait for the
+// table's inserts before its kernels (`ev`, recorded after every insert).
+struct GradCache {
+  static constexpr uint32_t kCap = 4096, kMaxStops = 16;
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  DevBuf dev;     // records, then stops, offsets and per-stop colors
+  HostBuf stage;  // one staging slot per entry (records, stops, offsets)
+  size_t o_stops = 0, o_offs = 0, o_pre = 0, stage_entry = 0;
+  std::unordered_multimap<uint64_t, uint32_t> index;
+  std::vector<std::vector<uint8_t>> keys;
+  uint32_t n = 0;
+  uint64_t hits = 0, misses = 0, full = 0;
+  bool ok = false;
+
+  hipError_t init() {
+    o_stops = kCap * sizeof(DevGradient);
+    o_offs = o_stops + (size_t)kCap * kMaxStops * sizeof(DevStop);
+    o_pre = (o_offs + (size_t)kCap * kMaxStops * sizeof(float) + 255) & ~(size_t)255;
+    const size_t bytes = o_pre + (size_t)kCap * kMaxStops * sizeof(float4);
+    stage_entry = (sizeof(DevGradient) + kMaxStops * (sizeof(DevStop) + sizeof(float)) + 63) &
+                  ~(size_t)63;
+    hipError_t e = dev.ensure(bytes, true);
+    if (e == hipSuccess) e = stage.ensure(stage_entry * kCap, true);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    ok = e == hipSuccess;
+    return e;
+  }
+  void destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+    ev = nullptr;
+    stream = nullptr;
+    dev.release();
+    stage.release();
+    ok = false;
+  }
+  static void key_of(const z2d_gradient_desc *g, std::vector<uint8_t> &k) {
+    k.clear();
+    auto put = [&](const void *p, size_t len) {
+      const uint8_t *b = static_cast<const uint8_t *>(p);
+      k.insert(k.end(), b, b + len);
+    };
+    put(&g->type, sizeof(g->type));
+    put(&g->method, sizeof(g->method));
+    put(&g->polar, sizeof(g->polar));
+    put(&g->n_stops, sizeof(g->n_stops));
+    put(g->geom, sizeof(g->geom));
+    put(g->matrix, sizeof(g->matrix));
+    for (uint32_t i = 0; i < g->n_stops; i++) {
+      put(&g->stops[i].offset, sizeof(float));
+      put(&g->stops[i].color.space, sizeof(uint32_t));
+      put(g->stops[i].color.v, sizeof(g->stops[i].color.v));
+    }
+  }
+  // The resident record of g, inserted if new; null when g does not fit
+  // (the caller packs it into the batch).  `used` is set when a record is
+  // returned.
+  const DevGradient *get(const z2d_gradient_desc *g, bool &used) {
+    if (!ok || g->n_stops > kMaxStops) return nullptr;
+    thread_local std::vector<uint8_t> k;  // the key, built without a new allocation
+    key_of(g, k);
+    uint64_t h = 1469598103934665603ull;
+    for (uint8_t c : k) h = (h ^ c) * 1099511628211ull;
+    std::lock_guard<std::mutex> lk(mu);
+    auto range = index.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it)
+      if (keys[it->second] == k) {
+        hits++;
+        used = true;
+        return reinterpret_cast<const DevGradient *>(dev.p) + it->second;
+      }
+    if (n >= kCap) {
+      full++;
+      return nullptr;
+    }
+    const uint32_t i = n;
+    DevGradient d;
+    std::vector<DevStop> st;
+    build_gradient(g, d, st);
+    DevGradient *dg = reinterpret_cast<DevGradient *>(dev.p) + i;
+    DevStop *ds = reinterpret_cast<DevStop *>(dev.p + o_stops) + (size_t)i * kMaxStops;
+    float *doff = reinterpret_cast<float *>(dev.p + o_offs) + (size_t)i * kMaxStops;
+    float4 *dpre = reinterpret_cast<float4 *>(dev.p + o_pre) + (size_t)i * kMaxStops;
+    d.stops = ds;
+    d.offs = doff;
+    d.pre = dpre;
+    uint8_t *slot = stage.p + (size_t)i * stage_entry;
+    std::memcpy(slot, &d, sizeof(d));
+    DevStop *hs = reinterpret_cast<DevStop *>(slot + sizeof(DevGradient));
+    float *hoff = reinterpret_cast<float *>(hs + kMaxStops);
+    for (size_t j = 0; j < st.size(); j++) {
+      hs[j] = st[j];
+      hoff[j] = st[j].offset;
+    }
+    // the slot is written once and never reused, so the copies may still be
+    // in flight when this returns
+    hipError_t e = hipMemcpyAsync(dg, slot, sizeof(DevGradient), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && !st.empty())
+      e = hipMemcpyAsync(ds, hs, st.size() * sizeof(DevStop), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && !st.empty())
+      e = hipMemcpyAsync(doff, hoff, st.size() * sizeof(float), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = z2d_launch_grad_setup(dg, 1, stream);
+    if (e == hipSuccess) e = hipEventRecord(ev, stream);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      ok = false;  // no further inserts; draws use their batches' tables
+      return nullptr;
+    }
+    n++;
+    misses++;
+    keys.push_back(k);
+    index.emplace(h, i);
+    used = true;
+    return dg;
+  }
+  // The batch stream waits for every insert so far.
+  hipError_t wait_on(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(mu);
+    return hipStreamWaitEvent(st, ev, 0);
+  }
+};
+
 // Accumulates paints, gradients and stops for one upload.
 struct PaintTable {
   std::vector<DevPaint> paints;
@@ -382,63 +572,15 @@ struct PaintTable {
   std::vector<DevStop> stops;
   std::vector<uint32_t> grad_stop_off;  // per gradient
   std::vector<int32_t> paint_grad;      // per paint: gradient index or -1
+  std::vector<const DevGradient *> paint_gdev;  // per paint: a resident gradient or null
+  GradCache *cache = nullptr;           // resident gradients (a server's batches)
+  bool cache_used = false;
 
   uint32_t add_gradient(const z2d_gradient_desc *g) {
     DevGradient d;
-    std::memset(&d, 0, sizeof(d));
-    d.type = g->type;
-    d.method = g->method;
-    d.polar = g->polar;
-    d.n_stops = g->n_stops;
-    for (int i = 0; i < 6; i++) {
-      d.geom[i] = g->geom[i];
-      d.m[i] = g->matrix[i];
-    }
-    d.identity = (g->matrix[0] == 1.0 && g->matrix[1] == 0.0 && g->matrix[2] == 0.0 &&
-                  g->matrix[3] == 1.0 && g->matrix[4] == 0.0 && g->matrix[5] == 0.0);
-    if (g->type == Z2D_GRADIENT_RADIAL) {
-      // Radial.initBuffer (src/gradient.zig:440-470)
-      d.r1 = std::max(0.0, g->geom[2]);
-      d.r2 = std::max(0.0, g->geom[5]);
-      d.cdx = g->geom[3] - g->geom[0];
-      d.cdy = g->geom[4] - g->geom[1];
-      d.dr = d.r2 - d.r1;
-      d.min_dr = -d.r1;
-      double a = 0.0;
-      a += d.cdx * d.cdx;
-      a += d.cdy * d.cdy;
-      a += d.dr * (-d.dr);
-      d.a = a;
-      d.inv_a = a != 0.0 ? 1.0 / a : 0.0;
-    } else if (g->type == Z2D_GRADIENT_LINEAR) {
-      // Linear.getOffset's per-gradient terms (src/gradient.zig)
-      d.lin_dx = g->geom[2] - g->geom[0];
-      d.lin_dy = g->geom[3] - g->geom[1];
-      double dist = 0.0;
-      dist += d.lin_dx * d.lin_dx;
-      dist += d.lin_dy * d.lin_dy;
-      d.lin_degenerate = dist == 0.0;
-      d.lin_inv = dist != 0.0 ? 1.0 / dist : 0.0;
-    } else if (g->type == Z2D_GRADIENT_CONIC) {
-      // Conic.initBuffer: @mod(angle, pi*2)
-      const double two_pi = 3.14159265358979323846 * 2.0;
-      double a = std::fmod(g->geom[2], two_pi);
-      d.conic_angle = g->geom[2] < 0.0 ? std::fmod(a + two_pi, two_pi) : a;
-    }
-    // Stop.List.addAssumeCapacity: clamp, Color.init, stable sort by offset.
-    uint32_t base = (uint32_t)stops.size();
-    for (uint32_t i = 0; i < g->n_stops; i++) {
-      DevStop s = stop_from(g->stops[i]);
-      size_t j = stops.size();
-      stops.push_back(s);
-      while (j > base && stops[j - 1].offset > s.offset) {
-        stops[j] = stops[j - 1];
-        j--;
-      }
-      stops[j] = s;
-    }
+    build_gradient(g, d, stops);
     grads.push_back(d);
-    grad_stop_off.push_back(base);
+    grad_stop_off.push_back((uint32_t)(stops.size() - g->n_stops));
     return (uint32_t)grads.size() - 1;
   }
 
@@ -447,19 +589,22 @@ struct PaintTable {
     std::memset(&d, 0, sizeof(d));
     d.kind = p->kind;
     int32_t gi = -1;
+    const DevGradient *gdev = nullptr;
     if (p->kind == Z2D_PATTERN_OPAQUE) {
       d.px_format = p->pixel.format;
       d.px = pixel_rgba(p->pixel);
       d.px_opaque = pixel_opaque(p->pixel);
     } else if (p->kind == Z2D_PATTERN_GRADIENT) {
-      gi = (int32_t)add_gradient(p->gradient);
+      gdev = cache ? cache->get(p->gradient, cache_used) : nullptr;
+      if (!gdev) gi = (int32_t)add_gradient(p->gradient);
     } else {
       const z2d_dither_desc &dd = p->dither;
       d.dither_type = dd.type;
       d.dither_source = dd.source;
       d.dither_scale = dd.scale;
       if (dd.source == Z2D_DITHER_SRC_GRADIENT) {
-        gi = (int32_t)add_gradient(dd.gradient);
+        gdev = cache ? cache->get(dd.gradient, cache_used) : nullptr;
+        if (!gdev) gi = (int32_t)add_gradient(dd.gradient);
       } else if (dd.source == Z2D_DITHER_SRC_PIXEL) {
         // LinearRGB.decodeRGBA: u16 demultiply, then /255
         I4 v = pixel_rgba(dd.pixel);
@@ -485,6 +630,7 @@ struct PaintTable {
     }
     paints.push_back(d);
     paint_grad.push_back(gi);
+    paint_gdev.push_back(gdev);
     return (uint32_t)paints.size() - 1;
   }
 
@@ -494,6 +640,8 @@ struct PaintTable {
     stops.clear();
     grad_stop_off.clear();
     paint_grad.clear();
+    paint_gdev.clear();
+    cache_used = false;
   }
 
   // Writes the three tables at their device addresses.
@@ -531,7 +679,7 @@ struct PaintTable {
     for (size_t i = 0; i < stops.size(); i++) hoff[i] = stops[i].offset;
     for (size_t i = 0; i < paints.size(); i++) {
       hp[i] = paints[i];
-      hp[i].gradient = paint_grad[i] >= 0 ? dg + paint_grad[i] : nullptr;
+      hp[i].gradient = paint_grad[i] >= 0 ? dg + paint_grad[i] : paint_gdev[i];
     }
   }
 };
@@ -884,6 +1032,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   b->t_count_wait = 0;
   HIP_OK(ctx, hipSetDevice(ctx->device));
   batch_drop_graph(b);  // its kernel arguments point into the arena
+  // resident gradients this batch uses: their uploads and setup first
+  if (b->paints.cache && b->paints.cache_used) HIP_OK(ctx, b->paints.cache->wait_on(bstream(b)));
   const uint32_t ns = (uint32_t)b->surfaces.size();
   // Draws grouped by surface, insertion order kept within a surface.
   std::stable_sort(b->draws.begin(), b->draws.end(), [](const HostDraw &a, const HostDraw &c) {
@@ -2399,6 +2549,61 @@ int32_t z2d_gpu_batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *surface,
   return batch_stroke(b, surface, pattern, nodes, n_nodes, opts);
 }
 
+int32_t z2d_gpu_batch_draw_n(z2d_gpu_batch *b, z2d_gpu_surface *const *surfaces,
+                             const z2d_draw_item *items, uint64_t n, uint32_t threads,
+                             int32_t *status) {
+  if (!b || (n && (!surfaces || !items))) return Z2D_ERR_INVALID_ARGUMENT;
+  // the strokes' checks and polygons, in parallel (pure host work)
+  std::vector<StrokeDraw> sd(n);
+  std::vector<int32_t> src(n, Z2D_OK);
+  std::vector<uint64_t> todo;
+  for (uint64_t i = 0; i < n; i++)
+    if (items[i].kind == Z2D_DRAW_STROKE) todo.push_back(i);
+  auto one = [&](uint64_t i) {
+    const z2d_draw_item &it = items[i];
+    if (!surfaces[i] || !it.stroke) {
+      src[i] = Z2D_ERR_INVALID_ARGUMENT;
+      return;
+    }
+    int32_t rc = check_pattern(it.pattern);
+    if (rc == Z2D_OK) rc = stroke_draw(it.nodes, it.n_nodes, it.stroke, surfaces[i]->format, sd[i]);
+    src[i] = rc;
+  };
+  unsigned nt = threads;
+  if (nt == 0) {
+    const unsigned hc = std::thread::hardware_concurrency();
+    nt = hc ? std::min(hc, 16u) : 4u;
+  }
+  nt = (unsigned)std::min<uint64_t>(nt, todo.size());
+  if (nt <= 1) {
+    for (uint64_t i : todo) one(i);
+  } else {
+    std::atomic<uint64_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++)
+      pool.emplace_back([&] {
+        for (uint64_t k; (k = next.fetch_add(1)) < todo.size();) one(todo[k]);
+      });
+    for (std::thread &t : pool) t.join();
+  }
+  // the draws appended in order
+  int32_t first = Z2D_OK;
+  for (uint64_t i = 0; i < n; i++) {
+    const z2d_draw_item &it = items[i];
+    int32_t rc;
+    if (it.kind == Z2D_DRAW_STROKE)
+      rc = src[i] != Z2D_OK ? src[i]
+                            : batch_add_stroke(b, surfaces[i], it.pattern, sd[i], it.stroke);
+    else if (it.kind == Z2D_DRAW_FILL)
+      rc = batch_fill(b, surfaces[i], it.pattern, it.nodes, it.n_nodes, &it.fill);
+    else
+      rc = Z2D_ERR_INVALID_ARGUMENT;
+    if (status) status[i] = rc;
+    if (rc != Z2D_OK && first == Z2D_OK) first = rc;
+  }
+  return first;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -2530,6 +2735,7 @@ struct SrvCopy {
 struct z2d_gpu_server {
   z2d_gpu_ctx *ctx = nullptr;
   HbmPool pool;
+  GradCache gcache;  // gradients resident across requests
   uint64_t budget = 0;
   uint32_t max_batch = 64;
   std::mutex mu;
@@ -2724,6 +2930,7 @@ struct z2d_gpu_server {
     sl.batch.count_stats = 1;
     sl.batch.isolate_failures = true;
     sl.batch.async_prepare = true;  // this slot's buffers wait for its done event
+    sl.batch.paints.cache = &gcache;
     std::vector<uint32_t> seq0(take.size() + 1, 0);  // each request's draws: [seq0[i], seq0[i+1])
     size_t stage = 0, files = 0, masks = 0;
     sl.stage_off.clear();
@@ -2951,6 +3158,10 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
   s->max_batch = max_batch ? max_batch : 64;
   HIP_OK(ctx, hipSetDevice(ctx->device));
   hipError_t e = s->pool.init(hbm_budget + 256 * 1024);
+  if (e == hipSuccess && s->gcache.init() != hipSuccess) {
+    (void)hipGetLastError();  // the table is optional: batches then pack their own
+    s->gcache.destroy();
+  }
   for (unsigned i = 0; i < z2d_gpu_server::kWorkers * z2d_gpu_server::kSlotsPerWorker; i++)
     s->slots.push_back(std::make_unique<SrvSlot>());
   for (auto &sl : s->slots) {
@@ -2964,6 +3175,7 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
       if (sl->done) (void)hipEventDestroy(sl->done);
       if (sl->stream) (void)hipStreamDestroy(sl->stream);
     }
+    s->gcache.destroy();
     s->pool.destroy();
     delete s;
     return fail(ctx, e, "server create");
@@ -3150,6 +3362,10 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
                  (unsigned long long)s->n_batches, s->t_build, s->t_count, s->t_prep,
                  s->t_launch, s->t_fin, s->t_idle, s->t_prepthreads, s->preppers.size(),
                  s->t_copythreads, s->copiers.size());
+  if (std::getenv("Z2D_SERVER_TRACE"))
+    std::fprintf(stderr, "[z2d server] resident gradients: %u, hits %llu, misses %llu, full %llu\n",
+                 s->gcache.n, (unsigned long long)s->gcache.hits,
+                 (unsigned long long)s->gcache.misses, (unsigned long long)s->gcache.full);
   for (auto &slp : s->slots) {
     SrvSlot &sl = *slp;
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
@@ -3167,6 +3383,7 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   for (auto &kv : s->reqs) delete kv.second;
+  s->gcache.destroy();
   s->pool.destroy();
   delete s;
 }

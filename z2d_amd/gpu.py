@@ -40,6 +40,8 @@ _SIGS = {
                             P(abi.FillOpts)], C.c_int32),
     "z2d_gpu_batch_stroke": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p,
                               C.c_uint64, P(abi.StrokeOpts)], C.c_int32),
+    "z2d_gpu_batch_draw_n": ([C.c_void_p, P(C.c_void_p), P(abi.DrawItem), C.c_uint64, C.c_uint32,
+                              P(C.c_int32)], C.c_int32),
     "z2d_gpu_batch_fill_polygon": ([C.c_void_p, C.c_void_p, P(abi.PatternDesc), C.c_void_p,
                                     C.c_uint64, P(C.c_double), C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_uint32], C.c_int32),
@@ -202,6 +204,29 @@ class Batch:
     def stroke(self, surface, pattern, nodes, opts):
         return lib().z2d_gpu_batch_stroke(self.h, surface.h, C.byref(pattern), nodes.ctypes.data,
                                           len(nodes), C.byref(opts))
+
+    def draw_n(self, jobs, threads=0):
+        """jobs: list of (surface, "fill" | "stroke", pattern, nodes, opts),
+        appended in order as per-item fill / stroke calls would; the
+        strokes' host tessellation runs on `threads` host threads (0: the
+        machine's).  Returns the per-item status list."""
+        n = len(jobs)
+        sfc = (C.c_void_p * max(1, n))()
+        items = (abi.DrawItem * max(1, n))()
+        for i, (surface, kind, pat, nodes, opts) in enumerate(jobs):
+            sfc[i] = surface.h
+            items[i].pattern = C.pointer(pat)
+            items[i].nodes = nodes.ctypes.data
+            items[i].n_nodes = len(nodes)
+            if kind == "stroke":
+                items[i].kind = abi.DRAW_STROKE
+                items[i].stroke = C.pointer(opts)
+            else:
+                items[i].kind = abi.DRAW_FILL
+                items[i].fill = opts
+        st = (C.c_int32 * max(1, n))()
+        lib().z2d_gpu_batch_draw_n(self.h, sfc, items, n, threads, st)
+        return list(st[:n])
 
     def fill_polygon(self, surface, pattern, edges, extents, aa, rule, op, precision):
         ext = (C.c_double * 4)(*extents)
