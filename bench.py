@@ -329,13 +329,14 @@ def serving_requests(gpu, n, seed=0xC5):
     return reqs
 
 
-def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2.0,
-                  seed=0x9015, budget=16 << 30):
+def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 64000),
+                  seconds=2.0, seed=0x9015, budget=16 << 30, n_sub=4):
     """Open-loop Poisson arrivals at each QPS level, cycling through the
-    request pool (and one saturation run with twice the pool queued at
-    t=0).  Latency = completion - scheduled
+    request pool (and one saturation run with eight times the pool queued at
+    t=0), submitted by n_sub client threads (arrival i by thread i mod
+    n_sub).  Latency = completion - scheduled
     arrival; throughput = pixel-draws counted on the device / wall time.
-    The submitting thread sleeps between arrivals and the interpreter's
+    The submitting threads sleep between arrivals and the interpreter's
     switch interval is 0.1 ms, so the completion thread takes its time stamp
     as a request completes rather than at the next 5 ms GIL hand-over; the
     Python garbage collector runs between levels, not during one (a full
@@ -361,7 +362,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
     out = {}
     levels = list(qps_levels) + ["saturation"]
     for lvl in levels:
-        n = 2 * len(reqs) if lvl == "saturation" else int(lvl * seconds)
+        n = 8 * len(reqs) if lvl == "saturation" else int(lvl * seconds)
         gaps = (np.zeros(n) if lvl == "saturation" else rng.exponential(1.0 / lvl, n))
         sched = np.cumsum(gaps)
         gc.collect()
@@ -378,19 +379,33 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000), seconds=2
 
         th = threading.Thread(target=collect)
         th.start()
-        ids = []
+        ids = [0] * n
+        errs = []
         t0 = time.perf_counter()
-        for i in range(n):
-            rq = reqs[i % len(reqs)]
-            while True:  # wait for the arrival without holding the GIL
-                dt = sched[i] - (time.perf_counter() - t0)
-                if dt <= 0:
-                    break
-                time.sleep(dt - 3e-4 if dt > 5e-4 else 0)
-            rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
-            if rc:
-                raise RuntimeError(f"server submit rc={rc}")
-            ids.append(rid)
+
+        def submitter(k):
+            # arrivals k, k + n_sub, ...: the submit call itself runs without
+            # the GIL, so the clients' submissions overlap
+            for i in range(k, n, n_sub):
+                rq = reqs[i % len(reqs)]
+                while True:  # wait for the arrival without holding the GIL
+                    dt = sched[i] - (time.perf_counter() - t0)
+                    if dt <= 0:
+                        break
+                    time.sleep(dt - 3e-4 if dt > 5e-4 else 0)
+                rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
+                if rc:
+                    errs.append(rc)
+                    return
+                ids[i] = rid
+
+        subs = [threading.Thread(target=submitter, args=(k,)) for k in range(n_sub)]
+        for t in subs:
+            t.start()
+        for t in subs:
+            t.join()
+        if errs:
+            raise RuntimeError(f"server submit rc={errs[0]}")
         th.join()
         t_end = time.perf_counter()
         gc.enable()

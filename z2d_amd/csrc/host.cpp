@@ -2697,6 +2697,7 @@ struct SrvRequest {
   int32_t status = Z2D_OK;
   bool done = false;
   double t_queued = 0;  // when it became ready for admission (steady clock, s)
+  double t_submit = 0, t_prep0 = 0, t_taken = 0;  // submitted, prep started, taken into a batch
 };
 
 struct SrvSlot {
@@ -2736,6 +2737,9 @@ struct z2d_gpu_server {
   z2d_gpu_ctx *ctx = nullptr;
   HbmPool pool;
   GradCache gcache;  // gradients resident across requests
+  // batches whose device work has finished, counted by a host function
+  // enqueued behind each batch (wakes the workers instead of polling)
+  uint64_t gpu_completions = 0;
   uint64_t budget = 0;
   uint32_t max_batch = 64;
   std::mutex mu;
@@ -2777,6 +2781,11 @@ struct z2d_gpu_server {
 
   // Marks a request complete (mutex held).
   void complete_locked(SrvRequest *r) {
+    const double t = now_s();
+    q_prep += r->t_prep0 - r->t_submit;
+    q_admit += r->t_taken - r->t_queued;
+    q_batch += t - r->t_taken;
+    q_n++;
     r->done = true;
     completed.push_back(r->id);
     n_completed++;
@@ -2816,6 +2825,10 @@ struct z2d_gpu_server {
   // result-copy enqueue, finalize, idle
   double t_build = 0, t_count = 0, t_prep = 0, t_launch = 0, t_fin = 0, t_idle = 0;
   double t_prepthreads = 0, t_copythreads = 0;  // busy time of the prep and copy threads
+  // summed per request: waiting for a prep thread, waiting for admission
+  // once prepared, and from admission to completion
+  double q_prep = 0, q_admit = 0, q_batch = 0;
+  uint64_t q_n = 0;
   static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
@@ -2832,6 +2845,7 @@ struct z2d_gpu_server {
         prep_q.pop_front();
       }
       const double p0 = now_s();
+      r->t_prep0 = p0;
       for (auto &f : r->fills)
         f->geo_rc = f->kind == Z2D_DRAW_STROKE
                         ? stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
@@ -3043,6 +3057,9 @@ struct z2d_gpu_server {
       for (SrvRequest *r : take)
         if (r->status == Z2D_OK) r->status = rc;
     (void)hipEventRecord(sl.done, st);
+    // a wake-up for the workers once the stream passes this point (they
+    // still check the done event; without the call they poll)
+    (void)hipLaunchHostFunc(st, on_batch_done, this);
     sl.reqs = take;
     const double l3 = now_s();
     std::lock_guard<std::mutex> lk(mu);
@@ -3053,6 +3070,15 @@ struct z2d_gpu_server {
     t_count += sl.batch.t_count_wait;
     t_prep += (l2 - l1) - sl.batch.t_count_wait;
     t_launch += l3 - l2;
+  }
+
+  static void on_batch_done(void *arg) {
+    auto *s = static_cast<z2d_gpu_server *>(arg);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->gpu_completions++;
+    }
+    s->cv_work.notify_all();
   }
 
   void run(unsigned w) {
@@ -3110,14 +3136,19 @@ struct z2d_gpu_server {
             used += need;
             peak = std::max(peak, used);
             queue.pop_front();
+            r->t_taken = now_s();
             take.push_back(r);
           }
         }
         if (take.empty() && !progressed) {
           const double w0 = now_s();
-          if (!all_free || hold)
-            cv_work.wait_for(lk, std::chrono::microseconds(20));
-          else
+          if (!all_free || hold) {
+            const uint64_t seen = gpu_completions;
+            const size_t q0 = queue.size();
+            cv_work.wait_for(lk, std::chrono::microseconds(hold ? 250 : 2000), [&] {
+              return stop || gpu_completions != seen || (free_slot && queue.size() != q0);
+            });
+          } else
             cv_work.wait_for(lk, std::chrono::milliseconds(5),
                              [&] { return stop || !queue.empty(); });
           t_idle += now_s() - w0;
@@ -3263,6 +3294,7 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   r->id = s->next_id++;
   *id = r->id;
   SrvRequest *raw = r.release();
+  raw->t_submit = z2d_gpu_server::now_s();
   s->reqs[raw->id] = raw;
   s->prep_q.push_back(raw);
   s->n_submitted++;
@@ -3362,6 +3394,13 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
                  (unsigned long long)s->n_batches, s->t_build, s->t_count, s->t_prep,
                  s->t_launch, s->t_fin, s->t_idle, s->t_prepthreads, s->preppers.size(),
                  s->t_copythreads, s->copiers.size());
+  if (std::getenv("Z2D_SERVER_TRACE"))
+    std::fprintf(stderr,
+                 "[z2d server] per request (ms): prep queue %.3f, admission queue %.3f, "
+                 "batch %.3f; %.1f requests per batch\n",
+                 s->q_n ? s->q_prep / s->q_n * 1e3 : 0.0, s->q_n ? s->q_admit / s->q_n * 1e3 : 0.0,
+                 s->q_n ? s->q_batch / s->q_n * 1e3 : 0.0,
+                 s->n_batches ? (double)s->q_n / (double)s->n_batches : 0.0);
   if (std::getenv("Z2D_SERVER_TRACE"))
     std::fprintf(stderr, "[z2d server] resident gradients: %u, hits %llu, misses %llu, full %llu\n",
                  s->gcache.n, (unsigned long long)s->gcache.hits,
