@@ -48,7 +48,8 @@ def log(msg):
 METRIC = "fill+composite Mpix/s (% HBM peak); p50/p99 latency of batched render reqs"
 
 
-KERNEL_SOURCES = ("z2d_amd/csrc/render.hip", "z2d_amd/csrc/render.h", "z2d_amd/csrc/dev_pixel.h")
+KERNEL_SOURCES = ("z2d_amd/csrc/render.hip", "z2d_amd/csrc/render.h", "z2d_amd/csrc/dev_pixel.h",
+                  "z2d_amd/csrc/composite_px.h")
 
 
 def kernel_sources_hash():
@@ -151,9 +152,10 @@ def _op_bytes(name, masked):
 
 
 def _stream_copy(gpu, ctx, nbytes=1 << 30, iters=20, read2=False):
-    """The library's reference streams over 1 GiB buffers (read + write
-    GB/s): a plain copy, or (read2) two reads and one write, the compositor's
-    mix: the achievable-HBM references."""
+    """The library's reference streams (read + write GB/s; passes rotating
+    over three buffer sets of nbytes each): a plain copy, or (read2) two
+    reads and one write, the compositor's mix: the achievable-HBM
+    references."""
     g = C.c_double()
     f = gpu.lib().z2d_gpu_stream_read2 if read2 else gpu.lib().z2d_gpu_stream_copy
     ctx.check(f(ctx.h, nbytes, iters, C.byref(g)), "stream")
@@ -237,8 +239,8 @@ def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3, sets=3):
             "kernel_GBps_all_passes": round(tot_bytes / tot_kern / 1e9, 1),
             "kernel_per_op_GBps": dev_op, "kernel_mask_per_op_GBps": dev_mask,
             "rotating_sets": sets,
-            "stream_copy_GBps": _stream_copy(gpu, ctx),
-            "stream_read2_GBps": _stream_copy(gpu, ctx, read2=True),
+            "stream_copy_GBps": _stream_copy(gpu, ctx, nbytes=w * h * 4),
+            "stream_read2_GBps": _stream_copy(gpu, ctx, nbytes=w * h * 4, read2=True),
             "kernel_note": "k_composite_quad device time (HIP events around the launch); "
                            "bytes each operator moves (clear 4 B/px, src/dst 8, others 12, "
                            "+1 where the mask is read) over the kernel time, passes "
@@ -246,7 +248,8 @@ def bench_compositor(gpu, ctx, w=4096, h=4096, reps=3, sets=3):
                            "inputs from the last-level cache; stream_copy_GBps (1 read : "
                            "1 write) and stream_read2_GBps (2 reads : 1 write, the "
                            "operators' mix) are the library's reference streams over "
-                           "1 GiB buffers"}
+                           "buffers of the surfaces' size, rotating over 3 sets the "
+                           "same way"}
 
 
 def bench_uneven_batch(gpu, ctx, n_surfaces=4096, seed=0xC4):
@@ -330,11 +333,11 @@ def serving_requests(gpu, n, seed=0xC5):
 
 
 def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 64000),
-                  seconds=2.0, seed=0x9015, budget=16 << 30, n_sub=4):
+                  seconds=2.0, seed=0x9015, budget=16 << 30, n_sub_max=4):
     """Open-loop Poisson arrivals at each QPS level, cycling through the
     request pool (and one saturation run with eight times the pool queued at
-    t=0), submitted by n_sub client threads (arrival i by thread i mod
-    n_sub).  Latency = completion - scheduled
+    t=0), submitted by client threads (one up to 16k QPS, n_sub_max above;
+    arrival i by thread i mod their number).  Latency = completion - scheduled
     arrival; throughput = pixel-draws counted on the device / wall time.
     The submitting threads sleep between arrivals and the interpreter's
     switch interval is 0.1 ms, so the completion thread takes its time stamp
@@ -381,6 +384,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 640
         th.start()
         ids = [0] * n
         errs = []
+        n_sub = 1 if lvl != "saturation" and lvl <= 16000 else n_sub_max
         t0 = time.perf_counter()
 
         def submitter(k):
@@ -392,7 +396,10 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 640
                     dt = sched[i] - (time.perf_counter() - t0)
                     if dt <= 0:
                         break
-                    time.sleep(dt - 3e-4 if dt > 5e-4 else 0)
+                    if n_sub == 1:
+                        time.sleep(dt - 3e-4 if dt > 5e-4 else 0)
+                    else:  # several clients: sleep rather than spin on the GIL
+                        time.sleep(dt)
                 rc, rid = srv.submit_prepared(rq["prep"], out=rq["out"])
                 if rc:
                     errs.append(rc)

@@ -312,13 +312,16 @@ int32_t z2d_gpu_server_stats(z2d_gpu_server *s, uint64_t out[8]);
 void z2d_gpu_server_destroy(z2d_gpu_server *s);
 
 /* Measurement utility with no reference counterpart: copies `bytes` between
- * two device buffers `iters` times with a plain stream-copy kernel and
- * reports read+write GB/s (the achievable-bandwidth reference). */
+ * two device buffers `iters` times with a plain stream-copy kernel, the
+ * passes rotating over three buffer sets (so no pass reads what the last-
+ * level cache kept), and reports read+write GB/s (the achievable-bandwidth
+ * reference). */
 int32_t z2d_gpu_stream_copy(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
 /* Measurement utility (no reference counterpart): the compositor's access
  * mix as a reference stream, two `bytes` buffers read and one written
- * (dst = a ^ b), `iters` times; reports read+write GB/s. */
+ * (dst = a ^ b), `iters` times over three rotating buffer sets; reports
+ * read+write GB/s. */
 int32_t z2d_gpu_stream_read2(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
 /* Synchronizes the context stream. */

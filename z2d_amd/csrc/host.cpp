@@ -2448,30 +2448,40 @@ int32_t z2d_gpu_png_timing(z2d_gpu_ctx *ctx, double *kernel_ms) {
 }
 
 // The reference stream over `bytes`-sized buffers: a copy, or (read2) two
-// reads and one write; GB/s counts every byte read and written.
+// reads and one write; GB/s counts every byte read and written.  The passes
+// rotate over three independent buffer sets, so a pass does not find its
+// inputs in the last-level cache (as the compositor sweep rotates its
+// inputs).
 static int32_t stream_rate(z2d_gpu_ctx *ctx, uint64_t bytes, uint32_t iters, bool read2,
                            double *gbps) {
   if (!ctx) return Z2D_ERR_NO_CONTEXT;
   if (bytes < 16 || iters == 0) return Z2D_ERR_INVALID_ARGUMENT;
   bytes &= ~15ull;
   HIP_OK(ctx, hipSetDevice(ctx->device));
-  DevBuf a, b, c;
-  hipError_t e = a.ensure(bytes, true);
-  if (e == hipSuccess) e = b.ensure(bytes, true);
-  if (e == hipSuccess && read2) e = c.ensure(bytes, true);
+  constexpr int kSets = 3;
+  DevBuf a[kSets], b[kSets], c[kSets];
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < kSets && e == hipSuccess; k++) {
+    e = a[k].ensure(bytes, true);
+    if (e == hipSuccess) e = b[k].ensure(bytes, true);
+    if (e == hipSuccess && read2) e = c[k].ensure(bytes, true);
+  }
   if (e != hipSuccess) return fail(ctx, e, "stream alloc");
-  auto once = [&]() {
-    return read2 ? z2d_launch_stream_read2(a.p, c.p, b.p, bytes, ctx->stream)
-                 : z2d_launch_stream_copy(a.p, b.p, bytes, ctx->stream);
+  auto once = [&](uint32_t i) {
+    const int k = (int)(i % kSets);
+    return read2 ? z2d_launch_stream_read2(a[k].p, c[k].p, b[k].p, bytes, ctx->stream)
+                 : z2d_launch_stream_copy(a[k].p, b[k].p, bytes, ctx->stream);
   };
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  e = hipMemsetAsync(a.p, 1, bytes, ctx->stream);
-  if (e == hipSuccess && read2) e = hipMemsetAsync(c.p, 2, bytes, ctx->stream);
+  for (int k = 0; k < kSets && e == hipSuccess; k++) {
+    e = hipMemsetAsync(a[k].p, 1, bytes, ctx->stream);
+    if (e == hipSuccess && read2) e = hipMemsetAsync(c[k].p, 2, bytes, ctx->stream);
+  }
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = once();  // warm-up
+  for (uint32_t i = 0; i < kSets && e == hipSuccess; i++) e = once(i);  // warm-up
   if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
-  for (uint32_t i = 0; i < iters && e == hipSuccess; i++) e = once();
+  for (uint32_t i = 0; i < iters && e == hipSuccess; i++) e = once(i);
   if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0.0f;

@@ -19,6 +19,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include <memory>
@@ -356,18 +357,22 @@ struct Poly {
     out->edges.push_back(e);
   }
   void add_contour(const Contour &c) {
-    if (c.empty()) return;
-    bool have = false;
-    P first{0, 0}, last{0, 0};
-    c.each([&](const P &p) {
-      if (!have) {
-        first = last = p;
-        have = true;
-      } else {
-        add_edge(last, p);
-        last = p;
-      }
-    });
+    const size_t nf = c.front.size(), nb = c.back.size();
+    if (nf + nb == 0) return;
+    std::vector<z2d_edge> &E = out->edges;
+    if (E.capacity() < E.size() + nf + nb) E.reserve(std::max(E.size() + nf + nb, 2 * E.capacity()));
+    // the sequence: front (stored reversed), then back
+    const P first = nf ? c.front[nf - 1] : c.back[0];
+    P last = first;
+    for (size_t i = nf; i-- > 0;) {
+      if (i == nf - 1) continue;  // the first point
+      add_edge(last, c.front[i]);
+      last = c.front[i];
+    }
+    for (size_t i = nf ? 0 : 1; i < nb; i++) {
+      add_edge(last, c.back[i]);
+      last = c.back[i];
+    }
     add_edge(last, first);
   }
 };
