@@ -1671,6 +1671,37 @@ __device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fm
   return pack(o, fmt);
 }
 
+// The same two steps on RGBA8 words (red in the low byte), two channels at
+// a time in the 16-bit halves of a word: each product of a channel and an
+// 8-bit factor is below 2^16 (a packed 16-bit multiply), floor(x / 255) is
+// (x + (x >> 8) + 1) >> 8 for every x <= 65025, and no sum carries into the
+// other half, so every channel is the value the per-channel form computes.
+typedef unsigned short z_u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t mul_x2(uint32_t x, uint32_t m) {
+  const z_u16x2 a = __builtin_bit_cast(z_u16x2, x);
+  const z_u16x2 b = {(unsigned short)m, (unsigned short)m};
+  return __builtin_bit_cast(uint32_t, a * b);
+}
+__device__ __forceinline__ uint32_t div255_x2(uint32_t x) {
+  return ((x + ((x >> 8) & 0x00FF00FFu) + 0x00010001u) >> 8) & 0x00FF00FFu;
+}
+// mask_coverage on an RGBA8 word
+__device__ __forceinline__ uint32_t mask_coverage_rgba8(uint32_t s, int k) {
+  const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
+  return div255_x2(mul_x2(s & 0x00FF00FFu, a)) | (div255_x2(mul_x2((s >> 8) & 0x00FF00FFu, a)) << 8);
+}
+// composite_src_over on an RGBA8 surface word d and RGBA8 source s
+__device__ __forceinline__ uint32_t src_over_rgba8(uint32_t d, uint32_t s) {
+  const uint32_t sa = s >> 24, na = 255u - sa;
+  const uint32_t q_rb = div255_x2(mul_x2(d & 0x00FF00FFu, na));
+  const uint32_t q_ga = div255_x2(mul_x2((d >> 8) & 0x00FF00FFu, na));
+  const uint32_t o_rb = ((s & 0x00FF00FFu) + q_rb) & 0x00FF00FFu;
+  const uint32_t o_g = (((s >> 8) & 0xFFu) + (q_ga & 0xFFu)) & 0xFFu;
+  const uint32_t da = d >> 24;
+  const uint32_t o_a = (sa + da - imul8(sa, da)) & 0xFFu;
+  return o_rb | (o_g << 8) | (o_a << 24);
+}
+
 // The rows of the src_over fast composite.  full_raw: the packed source that
 // full coverage stores when the draw takes the solid fast path.
 __device__ __forceinline__ void src_over_rows(uint32_t (&pix)[ROWS_PER_THREAD],
@@ -2503,6 +2534,7 @@ k_render_fast(JobView J, uint32_t tile_base,
         }
         const bool fast_full = (bflags & DRAW_FAST_SRC) != 0;
         const uint32_t full_raw = fast_full ? pack(p_px, S.fmt) : 0u;
+        const bool rgba8 = S.fmt == Z2D_FORMAT_RGBA;  // uniform
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
           if (!((need >> i) & 1u)) continue;
@@ -2512,10 +2544,15 @@ k_render_fast(JobView J, uint32_t tile_base,
           if (fast_full && k >= 16) {
             pix[i] = full_raw;
           } else {
-            const uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
-            I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
-            if (!solid && k < 16) src = mask_coverage(src, k);
-            pix[i] = composite_src_over(pix[i], S.fmt, src);
+            uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
+            if (rgba8) {
+              if (!solid && k < 16) pv = mask_coverage_rgba8(pv, k);
+              pix[i] = src_over_rgba8(pix[i], pv);
+            } else {
+              I4 src{pv & 0xFFu, (pv >> 8) & 0xFFu, (pv >> 16) & 0xFFu, pv >> 24};
+              if (!solid && k < 16) src = mask_coverage(src, k);
+              pix[i] = composite_src_over(pix[i], S.fmt, src);
+            }
           }
           dirty |= (pix[i] != before ? 1u : 0u) << i;
         }
