@@ -104,11 +104,13 @@ Z2D_PX __forceinline__ double zmodd(double x, double y) {
 }
 // round(255*v) to an 8-bit channel; saturates outside the range the
 // reference defines (its @intFromFloat is undefined there).
+// roundf(v) (half away from zero), then 0 below zero or for NaN and 65535
+// above it, without branches: trunc(v + copysign(0.5 - 2^-25, v)) is roundf
+// for every float (checked over all 2^32 of them), and the clamp takes NaN
+// to 0.
 Z2D_PX __forceinline__ uint32_t f2u8(float v) {
-  float r = roundf(v);
-  if (!(r >= 0.0f)) return 0u;
-  if (r > 65535.0f) return 65535u;
-  return (uint32_t)r;
+  const float r = truncf(v + copysignf(0x1.fffffep-2f, v));
+  return (uint32_t)fminf(fmaxf(r, 0.0f), 65535.0f);
 }
 
 // std.math.pow(f32) as the reference's stdlib computes it (the Go-derived

@@ -2295,9 +2295,9 @@ __device__ __forceinline__ void fast_gradient(const FastDraw &B, FastShared &sh,
     float t;
     bool transparent;
     stops_find8(B.offs, n_stops, off, i0, i1, t, transparent);
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float4 A = transparent ? z : B.pre[i0];
-    const float4 C2 = transparent ? z : B.pre[i1];
+    // both stop colors read whole (i0, i1 are valid indices either way)
+    float4 A = B.pre[i0], C2 = B.pre[i1];
+    if (transparent) A = C2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t r = f2u8(255.0f * lerpf_(A.x, C2.x, t));
     const uint32_t g = f2u8(255.0f * lerpf_(A.y, C2.y, t));
     const uint32_t b = f2u8(255.0f * lerpf_(A.z, C2.z, t));
