@@ -1245,7 +1245,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_scantmp = L.take((size_t)(scan_n / 4096 + 2) * sizeof(uint32_t));
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
-  const size_t o_blist = L.take(max_entries * sizeof(uint32_t));
+  const size_t o_blist = L.take(max_entries * sizeof(uint4));
   const size_t o_ecnt = L.take((2 * max_entries + 1) * sizeof(uint32_t));  // + cursors
   const size_t o_exr = L.take(max_entries * 2 * sizeof(int32_t));
   const size_t o_eidx = L.take(max_entry_edges * sizeof(REdge));
@@ -1373,7 +1373,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.entry_cur = J.entry_ecnt + max_entries + 1;
   J.entry_xlo = reinterpret_cast<int32_t *>(D + o_exr);
   J.entry_xhi = J.entry_xlo + max_entries;
-  J.band_list = reinterpret_cast<uint32_t *>(D + o_blist);
+  J.band_list = reinterpret_cast<uint4 *>(D + o_blist);
   J.egroups = reinterpret_cast<const uint2 *>(D + o_egroups);
   J.n_egroups = (uint32_t)egroups.size();
   J.entry_edges = reinterpret_cast<REdge *>(D + o_eidx);

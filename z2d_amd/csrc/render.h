@@ -212,7 +212,11 @@ struct JobView {
   uint32_t n_bands;
   // binning outputs
   uint32_t *band_cnt;    // n_bands + 1: offsets of the band lists in band_list
-  uint32_t *band_list;   // entry indices, capacity max_entries
+  // per band-list position: x = entry index; the tiles of the band whose
+  // column tx0 satisfies y < tx0 < z (int32) take it; w = the strips it
+  // meets (bits 0..7) | needs the general renderer (bit 8).  Capacity
+  // max_entries.
+  uint4 *band_list;
   // band pass slots: a band and 64 consecutive draws of its surface
   const uint2 *slot_info;         // per slot: band, surface
   const uint32_t *band_slot_off;  // n_bands: the band's first slot

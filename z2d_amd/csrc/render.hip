@@ -1133,7 +1133,6 @@ __global__ void k_band_pass(JobView J) {
   if (hit) {
     const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (pos < J.max_entries) {
-      J.band_list[pos] = idx;
       // only indices below the edge capacity were stored
       const uint32_t off = J.entry_ecnt[idx], cnt = J.entry_ecnt[idx + 1] - off;
       BandEntry e;
@@ -1162,6 +1161,22 @@ __global__ void k_band_pass(JobView J) {
         }
       }
       J.entries[idx] = e;
+      // the strip pass's tests for this entry: a tile at column tx0 (>= 0)
+      // takes it when it is not wholly beside the entry's crossings (px_lo
+      // >= tx0 + TILE_W + 1 or px_hi <= tx0 - 2) and meets the composite
+      // region (rx0 < tx0 + TILE_W, tx0 < rx1); a strip when its rows meet
+      // [ry0, ry1)
+      const int64_t lo_x = max(max((int64_t)Dd.rx0 - TILE_W, (int64_t)e.px_lo - TILE_W - 1),
+                               (int64_t)-1);
+      const int64_t hi_x = min(min((int64_t)Dd.rx1, (int64_t)e.px_hi + 2), (int64_t)INT32_MAX);
+      uint32_t smask = 0;
+      for (uint32_t st = 0; st < J.strips_per_tile; st++) {
+        const int32_t sy0 = y0 + (int32_t)st * ROWS_PER_THREAD;
+        if (Dd.ry1 > sy0 && Dd.ry0 < sy0 + ROWS_PER_THREAD) smask |= 1u << st;
+      }
+      const bool slow = !(Dd.flags & DRAW_FAST) || e.edge_cnt > PACKED_EDGE_LIMIT;
+      J.band_list[pos] = make_uint4(idx, (uint32_t)(int32_t)lo_x,
+                                    (uint32_t)(int32_t)max(hi_x, lo_x), smask | (slow ? 256u : 0u));
     } else {
       atomicOr(&J.counters[2], 1u);
     }
@@ -1301,7 +1316,7 @@ __global__ void k_strip_pass(JobView J) {
   const uint32_t e0 = min(J.band_cnt[b], J.max_entries);
   const uint32_t e1 = min(J.band_cnt[b + 1], J.max_entries);
   const uint32_t spt = J.strips_per_tile;
-  const int32_t tx0 = (int32_t)T.tx * TILE_W, ty0 = (int32_t)T.ty * (int32_t)J.band_h;
+  const int32_t tx0 = (int32_t)T.tx * TILE_W;
   uint32_t cnt[MAX_STRIPS_PER_TILE], base[MAX_STRIPS_PER_TILE];
   uint32_t slow_strips = 0;  // strips whose list holds a general-renderer entry
 #pragma unroll
@@ -1314,20 +1329,10 @@ __global__ void k_strip_pass(JobView J) {
     uint32_t mask = 0, idx = 0;
     bool slow = false;  // the entry needs the general renderer
     if (e + lane < e1) {
-      idx = J.band_list[e + lane];
-      const BandEntry &E = J.entries[idx];
-      const DevDraw &D = J.draws[E.draw];
-      slow = !(D.flags & DRAW_FAST) || E.edge_cnt > PACKED_EDGE_LIMIT;
-      const int32_t rx0 = D.rx0, rx1 = D.rx1, ry0 = D.ry0, ry1 = D.ry1;
-      const bool beside = (int64_t)E.px_lo >= (int64_t)tx0 + TILE_W + 1 ||
-                          (int64_t)E.px_hi <= (int64_t)tx0 - 2;  // see k_band_pass
-      if (!beside && rx1 > tx0 && rx0 < tx0 + TILE_W) {
-#pragma unroll
-        for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
-          const int32_t sy0 = ty0 + s * ROWS_PER_THREAD;
-          if ((uint32_t)s < spt && ry1 > sy0 && ry0 < sy0 + ROWS_PER_THREAD) mask |= 1u << s;
-        }
-      }
+      const uint4 it = J.band_list[e + lane];  // see k_band_pass
+      idx = it.x;
+      slow = (it.w & 256u) != 0;
+      if (tx0 > (int32_t)it.y && tx0 < (int32_t)it.z) mask = it.w & 255u;
     }
 #pragma unroll
     for (int s = 0; s < MAX_STRIPS_PER_TILE; s++) {
@@ -1491,6 +1496,7 @@ __global__ void k_edge_bins(JobView J) {
         hi = (int32_t)fhi;
       }
     }
+    const REdge red = make_redge(ed);  // the copy every band of the edge receives
     int32_t ulo = lo, uhi = hi;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1542,7 +1548,7 @@ __global__ void k_edge_bins(JobView J) {
         if (hit) {
           const uint32_t pos = base + (uint32_t)__popcll(m & below);
           if (pos < J.max_entry_edges)
-            J.entry_edges[pos] = make_redge(ed);
+            J.entry_edges[pos] = red;
           else
             atomicOr(&J.counters[2], 1u);
         }
