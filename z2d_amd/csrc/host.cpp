@@ -48,6 +48,7 @@ hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t z2d_launch_fill_set(const FillSet &F, hipStream_t st);
 hipError_t z2d_launch_multi_copy(const z2d::CopyCmd *cmds, uint32_t n, hipStream_t st);
 hipError_t z2d_launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 hipError_t z2d_launch_stream_read2(const void *a, const void *b, void *dst, uint64_t bytes,
@@ -1418,8 +1419,27 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
                            b->d_flat_jobs, b->n_flat_jobs, b->d_flat_out, nullptr,
                            b->d_flat_rel, b->d_flat_curves, b->n_flat_curves, b->d_flat_tasks,
                            b->d_flat_toff, nullptr, true, st);
-  if (e == hipSuccess) e = z2d_launch_fill_u32(J.counters, 16, 0u, st);
-  if (e == hipSuccess) e = z2d_launch_fill_u32(J.cursors, 2 * (ns + 1), 0u, st);
+  if (e == hipSuccess) {
+    // every per-render reset in one launch: counters, render cursors, the
+    // entries' edge counts and cursors, their column ranges (empty until
+    // the edge binning widens them)
+    FillSet F{};
+    auto add = [&](uint32_t *p, uint64_t n, uint32_t v) {
+      if (n == 0) return;
+      F.p[F.count] = p;
+      F.n[F.count] = n;
+      F.v[F.count] = v;
+      F.count++;
+    };
+    add(J.counters, 16, 0u);
+    add(J.cursors, 2 * ((uint64_t)ns + 1), 0u);
+    if (J.n_bands) {
+      add(J.entry_ecnt, 2ull * J.max_entries + 1, 0u);
+      add(reinterpret_cast<uint32_t *>(J.entry_xlo), J.max_entries, 0x7FFFFFFFu);
+      add(reinterpret_cast<uint32_t *>(J.entry_xhi), J.max_entries, 0x80000000u);
+    }
+    e = z2d_launch_fill_set(F, st);
+  }
   if (e == hipSuccess) e = z2d_launch_edge_build(J, st);
   if (e == hipSuccess)
     e = z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,

@@ -310,6 +310,15 @@ struct QuadJob {
   QuadSrc src, mask;
 };
 
+// Up to six word fills in one launch (the zeroing at the start of a render
+// pipeline): fill f writes v[f] to n[f] words at p[f].
+struct FillSet {
+  uint32_t *p[6];
+  uint64_t n[6];
+  uint32_t v[6];
+  uint32_t count;
+};
+
 // One command of k_multi_copy: copy `bytes` from src to dst, or (src null)
 // fill bytes / 4 words at dst with `value`.  Many such commands go in one
 // launch (a server batch's surface clears, clip-mask scatters and result

@@ -2886,6 +2886,25 @@ hipError_t z2d_launch_grad_setup(DevGradient *grads, uint32_t n_grads, hipStream
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) k_fill_set(FillSet F) {
+  const uint32_t f = blockIdx.y;
+  uint32_t *const p = F.p[f];
+  const uint64_t n = F.n[f];
+  const uint32_t v = F.v[f];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t z2d_launch_fill_set(const FillSet &F, hipStream_t st) {
+  if (F.count == 0) return hipSuccess;
+  uint64_t most = 0;
+  for (uint32_t f = 0; f < F.count; f++) most = F.n[f] > most ? F.n[f] : most;
+  const uint64_t blocks = (most + 255) / 256;
+  hipLaunchKernelGGL(k_fill_set, dim3((uint32_t)(blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks)), F.count),
+                     dim3(256), 0, st, F);
+  return hipGetLastError();
+}
+
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
@@ -2910,18 +2929,9 @@ hipError_t z2d_launch_rowprog(const JobView &J, const uint32_t *list, uint32_t n
 
 hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   if (J.n_bands == 0) return hipSuccess;
-  // entry edge lists: counts, offsets, then the copies
-  const uint64_t ne2 = 2ull * J.max_entries + 1;  // entry_ecnt and entry_cur
-  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((ne2 + 255) / 256)), dim3(256), 0, st,
-                     J.entry_ecnt, ne2, 0u);
-  // entry column ranges: empty until k_edge_bins<true> widens them
-  const uint64_t nx = J.max_entries;
-  if (nx) {
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nx + 255) / 256)), dim3(256), 0, st,
-                       reinterpret_cast<uint32_t *>(J.entry_xlo), nx, 0x7FFFFFFFu);
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nx + 255) / 256)), dim3(256), 0, st,
-                       reinterpret_cast<uint32_t *>(J.entry_xhi), nx, 0x80000000u);
-  }
+  // entry edge lists: counts, offsets, then the copies (entry_ecnt,
+  // entry_cur and the entry column ranges were reset by the pipeline's
+  // first fill, z2d_launch_fill_set)
   dim3 ggrid((J.n_egroups + 3) / 4);
   if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<false>, ggrid, dim3(256), 0, st, J);
   scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
