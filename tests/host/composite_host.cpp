@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 
 #include "../../oracle/z2d_oracle.h"
@@ -78,9 +79,83 @@ static uint32_t oracle_run(uint32_t fmt, uint32_t raw, const z2d_comp_op *ops, u
   return out;
 }
 
+// The fast renderer's RGBA8 forms against the per-channel ones they stand
+// for, and the other arithmetic rewrites against the forms they replace.
+static uint64_t check_rewrites(uint64_t &cases) {
+  uint64_t bad = 0;
+  auto unpack8 = [](uint32_t w) { return I4{w & 255u, (w >> 8) & 255u, (w >> 16) & 255u, w >> 24}; };
+  // src_over_rgba8: every (destination channel, source channel, source
+  // alpha), the channel in every byte position, then random words
+  for (uint32_t dc = 0; dc < 256; dc++)
+    for (uint32_t sc = 0; sc < 256; sc++)
+      for (uint32_t sa = 0; sa < 256; sa++) {
+        const uint32_t ds[2] = {dc | (dc << 8) | (dc << 16) | ((255u - dc) << 24),
+                                dc | (dc << 8) | (dc << 16) | (dc << 24)};
+        const uint32_t sw = sc | (sc << 8) | (sc << 16) | (sa << 24);
+        for (uint32_t d : ds) {
+          cases++;
+          if (src_over_rgba8(d, sw) != composite_src_over(d, Z2D_FORMAT_RGBA, unpack8(sw))) bad++;
+        }
+      }
+  for (int i = 0; i < (1 << 22); i++) {
+    const uint64_t r = next_u64();
+    const uint32_t d = (uint32_t)r, sw = (uint32_t)(r >> 32);
+    const int k = (int)(next_u64() % 17u);
+    cases += 2;
+    if (src_over_rgba8(d, sw) != composite_src_over(d, Z2D_FORMAT_RGBA, unpack8(sw))) bad++;
+    const I4 m = mask_coverage(unpack8(sw), k);
+    if (mask_coverage_rgba8(sw, k) != (m.r | (m.g << 8) | (m.b << 16) | (m.a << 24))) bad++;
+  }
+  // f2u8 (truncf of v + copysignf(0.5 - 2^-25, v), clamped) against roundf
+  // with the branches it replaces: every 61st float bit pattern and the
+  // neighbourhoods of the halves 0.5 .. 65535.5
+  auto f2u8_ref = [](float v) -> uint32_t {
+    const float r = std::roundf(v);
+    if (!(r >= 0.0f)) return 0u;
+    if (r > 65535.0f) return 65535u;
+    return (uint32_t)r;
+  };
+  for (uint64_t u = 0; u <= 0xFFFFFFFFull; u += 61) {
+    uint32_t w = (uint32_t)u;
+    float v;
+    std::memcpy(&v, &w, 4);
+    cases++;
+    if (f2u8(v) != f2u8_ref(v)) bad++;
+  }
+  for (uint32_t h = 0; h < 65536; h++) {
+    float v = (float)h + 0.5f;
+    for (int t = 0; t < 3; t++) {
+      const float vs[2] = {v, -v};
+      for (float x : vs) {
+        cases++;
+        if (f2u8(x) != f2u8_ref(x)) bad++;
+      }
+      v = std::nextafterf(v, t == 0 ? 0.0f : 1e9f);
+      if (t == 1) v = std::nextafterf(v, 1e9f);
+    }
+  }
+  // overlay / hard light on 8-bit lanes against the signed per-channel forms
+  for (int i = 0; i < (1 << 24); i++) {
+    const uint64_t r = next_u64();
+    const uint32_t sc = r & 255u, dc = (r >> 8) & 255u, sa = (r >> 16) & 255u, da = (r >> 24) & 255u;
+    const I4 d{dc, dc, dc, da}, sv{sc, sc, sc, sa};
+    cases += 2;
+    if (op_int(Z2D_OP_OVERLAY, d, sv).r != overlay_ch((int32_t)sc, (int32_t)dc, (int32_t)sa, (int32_t)da)) bad++;
+    if (op_int(Z2D_OP_HARD_LIGHT, d, sv).r != hardlight_ch((int32_t)sc, (int32_t)dc, (int32_t)sa, (int32_t)da)) bad++;
+  }
+  return bad;
+}
+
 int main(int argc, char **argv) {
   const int trials = argc > 1 ? std::atoi(argv[1]) : 24;
   uint64_t cases = 0, bad = 0;
+  {
+    uint64_t rc = 0;
+    const uint64_t rb = check_rewrites(rc);
+    std::printf("rewrites: cases %llu mismatches %llu\n", (unsigned long long)rc, (unsigned long long)rb);
+    cases += rc;
+    bad += rb;
+  }
   for (uint32_t fmt = 0; fmt < 8; fmt++)
     for (uint32_t op = 0; op < 28; op++)
       for (uint32_t prec = 0; prec < 2; prec++)

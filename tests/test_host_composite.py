@@ -4,7 +4,13 @@ undefined-behaviour sanitizers and called out of line, against the oracle's
 compositeOpaque / compositeOpacity / compositeStride
 (src/internal/raster/shared.zig:9-117, src/surface.zig:557-581) on a
 one-pixel surface: every operator, both precisions, all eight formats,
-coverage 0..16 (multisample) and 0..1 (direct).  Runs on the CPU."""
+coverage 0..16 (multisample) and 0..1 (direct).  The same harness checks the
+device code's arithmetic rewrites against the forms they replace: src_over
+and the coverage mask on RGBA8 words two channels at a time (every channel
+triple, then random words), the branchless float-to-lane conversion
+(every 61st float and the neighbourhoods of the halves), overlay and hard
+light on 8-bit lanes (random lanes against the signed per-channel forms).
+Runs on the CPU."""
 import os
 import shutil
 import subprocess
@@ -35,5 +41,6 @@ def test_composite_cov_host_sanitized(tmp_path):
                UBSAN_OPTIONS="print_stacktrace=1")
     r = subprocess.run([exe, "48"], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-3000:])
-    assert "mismatches 0" in r.stdout, r.stdout[-3000:]
+    assert "rewrites: cases" in r.stdout and "mismatches 0" in r.stdout, r.stdout[-3000:]
+    assert " mismatches 0\n" in r.stdout.split("rewrites:")[1].splitlines(True)[0], r.stdout
     assert "runtime error" not in r.stderr, r.stderr[-3000:]

@@ -1663,51 +1663,6 @@ __device__ __forceinline__ void dense_gradient(const DevGradient &G, StripShared
   }
 }
 
-// Channels of s (each <= 255) masked by coverage count k.
-__device__ __forceinline__ I4 mask_coverage(I4 s, int k) {
-  const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
-  return I4{imul8(s.r, a), imul8(s.g, a), imul8(s.b, a), imul8(s.a, a)};
-}
-// s: the paint already masked by the pixel's coverage (channels <= 255).
-__device__ __forceinline__ uint32_t composite_src_over(uint32_t raw, uint32_t fmt, I4 s) {
-  const I4 d = unpack(raw, fmt);
-  const uint32_t sa = s.a, na = 255u - sa;
-  const I4 o{u16(s.r + imul8(d.r, na)), u16(s.g + imul8(d.g, na)), u16(s.b + imul8(d.b, na)),
-             u16(sa + d.a - imul8(sa, d.a))};
-  return pack(o, fmt);
-}
-
-// The same two steps on RGBA8 words (red in the low byte), two channels at
-// a time in the 16-bit halves of a word: each product of a channel and an
-// 8-bit factor is below 2^16 (a packed 16-bit multiply), floor(x / 255) is
-// (x + (x >> 8) + 1) >> 8 for every x <= 65025, and no sum carries into the
-// other half, so every channel is the value the per-channel form computes.
-typedef unsigned short z_u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t mul_x2(uint32_t x, uint32_t m) {
-  const z_u16x2 a = __builtin_bit_cast(z_u16x2, x);
-  const z_u16x2 b = {(unsigned short)m, (unsigned short)m};
-  return __builtin_bit_cast(uint32_t, a * b);
-}
-__device__ __forceinline__ uint32_t div255_x2(uint32_t x) {
-  return ((x + ((x >> 8) & 0x00FF00FFu) + 0x00010001u) >> 8) & 0x00FF00FFu;
-}
-// mask_coverage on an RGBA8 word
-__device__ __forceinline__ uint32_t mask_coverage_rgba8(uint32_t s, int k) {
-  const uint32_t a = (uint32_t)clampi(k * 16 - 1, 0, 255);
-  return div255_x2(mul_x2(s & 0x00FF00FFu, a)) | (div255_x2(mul_x2((s >> 8) & 0x00FF00FFu, a)) << 8);
-}
-// composite_src_over on an RGBA8 surface word d and RGBA8 source s
-__device__ __forceinline__ uint32_t src_over_rgba8(uint32_t d, uint32_t s) {
-  const uint32_t sa = s >> 24, na = 255u - sa;
-  const uint32_t q_rb = div255_x2(mul_x2(d & 0x00FF00FFu, na));
-  const uint32_t q_ga = div255_x2(mul_x2((d >> 8) & 0x00FF00FFu, na));
-  const uint32_t o_rb = ((s & 0x00FF00FFu) + q_rb) & 0x00FF00FFu;
-  const uint32_t o_g = (((s >> 8) & 0xFFu) + (q_ga & 0xFFu)) & 0xFFu;
-  const uint32_t da = d >> 24;
-  const uint32_t o_a = (sa + da - imul8(sa, da)) & 0xFFu;
-  return o_rb | (o_g << 8) | (o_a << 24);
-}
-
 // The rows of the src_over fast composite.  full_raw: the packed source that
 // full coverage stores when the draw takes the solid fast path.
 __device__ __forceinline__ void src_over_rows(uint32_t (&pix)[ROWS_PER_THREAD],
