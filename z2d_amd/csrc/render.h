@@ -33,7 +33,7 @@ constexpr uint32_t PACKED_EDGE_LIMIT = 32767;
 // share the first class).
 constexpr uint32_t ORDER_BUCKETS = 1024;
 // Edge slots of one draw per k_edge_bins work item.
-constexpr uint32_t EDGE_GROUP = 512;
+constexpr uint32_t EDGE_GROUP = 64;
 
 // Flattened path node (move_to / line_to / close_path only; curves are
 // flattened on the host by the reference's Spline algorithm first).
