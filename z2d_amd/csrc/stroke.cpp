@@ -325,22 +325,20 @@ struct Contour {
 // Polygon.addEdge / addEdgesFromContour on already scaled points.
 struct Poly {
   StrokePolygon *out;
+  // The two orientations differ only in which end is the lower one, so the
+  // ends are selected and one expression serves both (the same operands as
+  // the reference's two branches); the extents take the same comparisons as
+  // selects rather than branches.
   void add_edge(P a, P b) {
+    const bool down = a.y < b.y;
+    if (!down && !(a.y > b.y)) return;  // horizontal (or NaN)
+    const P lo = down ? a : b, hi = down ? b : a;
     z2d_edge e;
-    if (a.y < b.y) {
-      e.y0 = a.y;
-      e.y1 = b.y;
-      e.x_start = a.x;
-      e.x_inc = (b.x - a.x) / (b.y - a.y);
-    } else if (a.y > b.y) {
-      e.y0 = a.y;
-      e.y1 = b.y;
-      e.x_start = b.x;
-      e.x_inc = (a.x - b.x) / (a.y - b.y);
-    } else {
-      return;
-    }
-    const double top = e.y0 < e.y1 ? e.y0 : e.y1, bottom = e.y0 < e.y1 ? e.y1 : e.y0;
+    e.y0 = a.y;
+    e.y1 = b.y;
+    e.x_start = lo.x;
+    e.x_inc = (hi.x - lo.x) / (hi.y - lo.y);
+    const double top = lo.y, bottom = hi.y;
     const double left = a.x < b.x ? a.x : b.x, right = a.x < b.x ? b.x : a.x;
     double *x = out->ext;
     if (out->edges.empty()) {
@@ -349,10 +347,10 @@ struct Poly {
       x[2] = left;
       x[3] = right;
     } else {
-      if (top < x[0]) x[0] = top;
-      if (bottom > x[1]) x[1] = bottom;
-      if (left < x[2]) x[2] = left;
-      if (right > x[3]) x[3] = right;
+      x[0] = top < x[0] ? top : x[0];
+      x[1] = bottom > x[1] ? bottom : x[1];
+      x[2] = left < x[2] ? left : x[2];
+      x[3] = right > x[3] ? right : x[3];
     }
     out->edges.push_back(e);
   }
@@ -409,10 +407,14 @@ struct Face {
   Slope dev_slope, user_slope;
   double half_width;
   P p0_cw, p0_ccw, p1_cw, p1_ccw;
-  Xf ctm;
+  const Xf *ctm = nullptr;  // the stroke's transformation (outlives every face)
 
   static Face make(P p0, P p1, Slope dev, double thickness, const Xf &ctm) {
     Face f;
+    make_into(f, p0, p1, dev, thickness, ctm);
+    return f;
+  }
+  static void make_into(Face &f, P p0, P p1, Slope dev, double thickness, const Xf &ctm) {
     const double half = thickness / 2;
     double ox, oy;
     Slope user = dev;
@@ -444,13 +446,17 @@ struct Face {
     f.p0_ccw = P{p0.x + ccwx, p0.y + ccwy};
     f.p1_cw = P{p1.x + cwx, p1.y + cwy};
     f.p1_ccw = P{p1.x + ccwx, p1.y + ccwy};
-    f.ctm = ctm;
-    return f;
+    f.ctm = &ctm;
   }
   static Face init(P p0, P p1, double thickness, const Xf &ctm) {
     Slope s = Slope::init(p0, p1);
     s.normalize();
     return make(p0, p1, s, thickness, ctm);
+  }
+  static void init_into(Face &f, P p0, P p1, double thickness, const Xf &ctm) {
+    Slope s = Slope::init(p0, p1);
+    s.normalize();
+    make_into(f, p0, p1, s, thickness, ctm);
   }
   static Face init_single(P pt, Slope dev, double thickness, const Xf &ctm) {
     return make(pt, pt, dev, thickness, ctm);
@@ -475,7 +481,7 @@ struct Face {
   void cap(Sink &&line_to, uint32_t mode, bool clockwise, const Pen *pen) const {
     if (mode == Z2D_CAP_SQUARE) {
       double ox = user_slope.dx * half_width, oy = user_slope.dy * half_width;
-      ctm.u2d_dist(ox, oy);
+      ctm->u2d_dist(ox, oy);
       if (clockwise) {
         line_to(p1_ccw);
         line_to(P{p1_ccw.x + ox, p1_ccw.y + oy});
@@ -504,7 +510,7 @@ struct Face {
   }
   template <typename Sink>
   void cap_p0(Sink &&line_to, uint32_t mode, bool clockwise, const Pen *pen) const {
-    Face::init(p1, p0, width, ctm).cap(line_to, mode, clockwise, pen);
+    Face::init(p1, p0, width, *ctm).cap(line_to, mode, clockwise, pen);
   }
   template <typename Sink>
   void cap_p1(Sink &&line_to, uint32_t mode, bool clockwise, const Pen *pen) const {
@@ -522,16 +528,12 @@ struct State {
   Poly result{};
   Contour outer, inner;
   Slope current_slope;
-  // the face join built last for its outgoing segment, which is the next
-  // join's incoming one when segments chain (Face.init is a pure function
-  // of its points, the thickness and the transformation)
-  Face last{};
-  bool have_last = false;
-
-  Face face(P a, P b) {
-    if (have_last && pbits(last.p0, a) && pbits(last.p1, b)) return last;
-    return Face::init(a, b, o->thickness, o->ctm);
-  }
+  // join's two faces; `last` is the slot holding the face the last join
+  // built for its outgoing segment, which is the next join's incoming one
+  // when segments chain (Face.init is a pure function of its points, the
+  // thickness and the transformation)
+  Face slot[2];
+  int last = -1;
 
   void reset_contours() {
     outer.reset();
@@ -548,10 +550,16 @@ void join(State &s, uint32_t mode, P p0, P p1, P p2, bool before) {
     return;
   }
   const Opts &o = *s.o;
-  const Face in = s.face(p0, p1);
-  const Face out = s.face(p1, p2);
-  s.last = out;
-  s.have_last = true;
+  int in_slot = s.last;
+  if (in_slot < 0 || !pbits(s.slot[in_slot].p0, p0) || !pbits(s.slot[in_slot].p1, p1)) {
+    in_slot = 0;
+    Face::init_into(s.slot[0], p0, p1, o.thickness, o.ctm);
+  }
+  const int out_slot = 1 - in_slot;
+  Face::init_into(s.slot[out_slot], p1, p2, o.thickness, o.ctm);
+  s.last = out_slot;
+  const Face &in = s.slot[in_slot];
+  const Face &out = s.slot[out_slot];
   const bool join_cw = in.dev_slope.compare(out.dev_slope) < 0;
   const bool poly_cw = s.cw >= 0 ? s.cw != 0 : join_cw;
   const bool switched = join_cw != poly_cw;
