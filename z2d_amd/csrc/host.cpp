@@ -745,6 +745,13 @@ struct HostDraw {
   uint32_t band_lo, band_hi;  // estimated band range (inclusive)
   uint32_t tcol_lo, tcol_hi;  // estimated tile-column range (inclusive)
   uint64_t edge_pairs;        // upper bound of (edge, band) pairs
+  // polygon draws whose edges stay with their owner until the batch is
+  // prepared (a server request's stroke polygon): read from there, not
+  // copied into the batch
+  const z2d_edge *ext_edges;
+  // edge_pairs computed by the owner for band height pairs_bh (0: none)
+  uint64_t pairs_hint;
+  uint32_t pairs_bh;
 };
 
 // Bands [floor(lo / band_h), floor(hi / band_h)] that an edge between pixel
@@ -762,6 +769,7 @@ struct z2d_gpu_batch {
   std::vector<z2d_gpu_surface *> surfaces;
   std::vector<z2d_path_node> raw_nodes;  // fill draws' path nodes
   std::vector<z2d_edge> in_edges;
+  uint64_t n_ext_edges = 0;  // edges of the draws that borrow their edges (HostDraw::ext_edges)
   uint64_t n_nodes = 0;  // flattened nodes and edge slots of the prepared batch
   DevBuf flat_buf;       // device flattening inputs and results
   HostBuf flat_stage;
@@ -929,7 +937,8 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
                                   const z2d_pattern_desc *pattern, const z2d_edge *edges,
                                   uint64_t n_edges, const double ext[4], uint32_t aa,
                                   uint32_t rule, uint32_t op, uint32_t precision,
-                                  bool balanced) {
+                                  bool balanced, bool borrow = false, uint64_t pairs_hint = 0,
+                                  uint32_t pairs_bh = 0) {
   if (!b || !s || !ext) return Z2D_ERR_INVALID_ARGUMENT;
   int32_t rc = check_pattern(pattern);
   if (rc) return rc;
@@ -940,8 +949,15 @@ static int32_t batch_fill_polygon(z2d_gpu_batch *b, z2d_gpu_surface *s,
   if (n_edges > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
   hd.d.node_cnt = (uint32_t)n_edges;
   hd.d.input_edges = 1;
-  hd.d.in_edge_off = (uint32_t)b->in_edges.size();
-  b->in_edges.insert(b->in_edges.end(), edges, edges + n_edges);
+  if (borrow) {
+    hd.ext_edges = edges;  // in_edge_off is given at prepare
+    b->n_ext_edges += n_edges;
+    hd.pairs_hint = pairs_hint;
+    hd.pairs_bh = pairs_bh;
+  } else {
+    hd.d.in_edge_off = (uint32_t)b->in_edges.size();
+    b->in_edges.insert(b->in_edges.end(), edges, edges + n_edges);
+  }
   for (int i = 0; i < 4; i++) hd.d.in_ext[i] = ext[i];
   if (!balanced) hd.d.flags |= DRAW_UNBALANCED;
   // extents are in scaled space
@@ -989,6 +1005,10 @@ struct StrokeDraw {
   StrokePolygon poly;
   uint32_t aa = 0;
   bool empty = false;
+  // (edge, band) bound of the polygon for band height pairs_bh (0: not
+  // computed), as batch_prepare would compute it
+  uint64_t pairs = 0;
+  uint32_t pairs_bh = 0;
 };
 
 static int32_t stroke_draw(const z2d_path_node *nodes, uint64_t n, const z2d_stroke_opts *o,
@@ -1003,12 +1023,25 @@ static int32_t stroke_draw(const z2d_path_node *nodes, uint64_t n, const z2d_str
   return stroke_polygon(nodes, n, o, format, d.aa, d.poly);
 }
 
+// borrow: the batch reads d's edges at prepare instead of copying them (d
+// must outlive that).
 static int32_t batch_add_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s,
                                 const z2d_pattern_desc *pattern, const StrokeDraw &d,
-                                const z2d_stroke_opts *o) {
+                                const z2d_stroke_opts *o, bool borrow = false) {
   if (d.empty) return Z2D_OK;
   return batch_fill_polygon(b, s, pattern, d.poly.edges.data(), d.poly.edges.size(),
-                            d.poly.ext, d.aa, Z2D_FILL_NON_ZERO, o->op, o->precision, true);
+                            d.poly.ext, d.aa, Z2D_FILL_NON_ZERO, o->op, o->precision, true,
+                            borrow, d.pairs, d.pairs_bh);
+}
+
+// The (edge, band) bound batch_prepare computes for a polygon draw on a
+// surface of height h with band height bh.
+static uint64_t polygon_pairs(const StrokeDraw &d, uint32_t format, int32_t h, uint32_t bh) {
+  const double sc = (d.aa == Z2D_AA_NONE || format == Z2D_FORMAT_ALPHA1) ? 1.0 : 4.0;
+  const uint32_t nb = (uint32_t)((h + bh - 1) / bh);
+  uint64_t pairs = 0;
+  for (const z2d_edge &e : d.poly.edges) pairs += band_span(e.y0 / sc, e.y1 / sc, nb, bh);
+  return pairs;
 }
 
 static int32_t batch_stroke(z2d_gpu_batch *b, z2d_gpu_surface *s, const z2d_pattern_desc *pattern,
@@ -1137,6 +1170,16 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
                                 std::isfinite(r.box[2]) && std::isfinite(r.box[3]));
     }
   }
+  // borrowed polygon edges follow the copied ones in the input-edge array
+  {
+    uint64_t ext_off = b->in_edges.size();
+    for (HostDraw &h : b->draws)
+      if (h.ext_edges) {
+        h.d.in_edge_off = (uint32_t)std::min<uint64_t>(ext_off, 0xFFFFFFFFull);
+        ext_off += h.d.node_cnt;
+      }
+    if (ext_off > 0xFFFFFFF0ull) return Z2D_ERR_OUT_OF_MEMORY;
+  }
   // node ranges: flattened nodes of fills, one edge slot per edge of polygons
   uint64_t total_nodes = 0;
   for (HostDraw &h : b->draws) {
@@ -1190,9 +1233,11 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
         h.tcol_lo = (uint32_t)clamped(std::floor(h.box[0]) - 2.0, s->w) / TILE_W;
         h.tcol_hi = (uint32_t)clamped(std::ceil(h.box[1]) + 2.0, s->w) / TILE_W;
       }
-      if (h.d.input_edges) {  // (fills: from the flattener's count pass)
+      if (h.d.input_edges && h.pairs_bh == bh) {  // computed by the edges' owner
+        h.edge_pairs = h.pairs_hint;
+      } else if (h.d.input_edges) {  // (fills: from the flattener's count pass)
         h.edge_pairs = 0;
-        const z2d_edge *ed = b->in_edges.data() + h.d.in_edge_off;
+        const z2d_edge *ed = h.ext_edges ? h.ext_edges : b->in_edges.data() + h.d.in_edge_off;
         for (uint32_t k = 0; k < h.d.node_cnt; k++)
           h.edge_pairs += band_span(ed[k].y0 / h.edge_scale, ed[k].y1 / h.edge_scale, nb, bh);
       }
@@ -1223,7 +1268,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
 
   // ---- arena layout
   Layout L;
-  const size_t o_inedges = L.take(b->in_edges.size() * sizeof(z2d_edge));
+  const size_t o_inedges = L.take((b->in_edges.size() + b->n_ext_edges) * sizeof(z2d_edge));
   const size_t o_draws = L.take(nd * sizeof(DevDraw));
   const size_t o_surfs = L.take(ns * sizeof(DevSurface));
   const PaintTable::Offsets o_pt = b->paints.take(L);
@@ -1278,6 +1323,10 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
 
   if (!b->in_edges.empty())
     std::memcpy(H + o_inedges, b->in_edges.data(), b->in_edges.size() * sizeof(z2d_edge));
+  for (const HostDraw &h : b->draws)
+    if (h.ext_edges && h.d.node_cnt)
+      std::memcpy(H + o_inedges + (size_t)h.d.in_edge_off * sizeof(z2d_edge), h.ext_edges,
+                  (size_t)h.d.node_cnt * sizeof(z2d_edge));
   DevDraw *hdraws = reinterpret_cast<DevDraw *>(H + o_draws);
   for (uint32_t i = 0; i < nd; i++) hdraws[i] = b->draws[i].d;
   DevSurface *hs = reinterpret_cast<DevSurface *>(H + o_surfs);
@@ -1544,6 +1593,7 @@ static void batch_clear(z2d_gpu_batch *b) {
   b->raw_nodes.clear();
   b->n_nodes = 0;
   b->in_edges.clear();
+  b->n_ext_edges = 0;
   b->draws.clear();
   b->paints.clear();
   b->prepared = false;
@@ -2876,11 +2926,19 @@ struct z2d_gpu_server {
       }
       const double p0 = now_s();
       r->t_prep0 = p0;
-      for (auto &f : r->fills)
-        f->geo_rc = f->kind == Z2D_DRAW_STROKE
-                        ? stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
-                                      f->stroke)
-                        : fill_checks(f->nodes.data(), f->nodes.size(), &f->opts);
+      for (auto &f : r->fills) {
+        if (f->kind == Z2D_DRAW_STROKE) {
+          f->geo_rc = stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
+                                  f->stroke);
+          // server batches keep the default band height
+          if (f->geo_rc == Z2D_OK && !f->stroke.empty) {
+            f->stroke.pairs = polygon_pairs(f->stroke, r->format, r->surf.h, TILE_H);
+            f->stroke.pairs_bh = TILE_H;
+          }
+        } else {
+          f->geo_rc = fill_checks(f->nodes.data(), f->nodes.size(), &f->opts);
+        }
+      }
       std::lock_guard<std::mutex> lk(mu);
       r->t_queued = now_s();
       t_prepthreads += r->t_queued - p0;
@@ -3036,7 +3094,7 @@ struct z2d_gpu_server {
         int32_t rc = f->geo_rc;
         if (rc == Z2D_OK)
           rc = f->kind == Z2D_DRAW_STROKE
-                   ? batch_add_stroke(&sl.batch, &r->surf, &f->pat, f->stroke, &f->sopts)
+                   ? batch_add_stroke(&sl.batch, &r->surf, &f->pat, f->stroke, &f->sopts, true)
                    : batch_add_fill(&sl.batch, &r->surf, &f->pat, f->nodes.data(),
                                     f->nodes.size(), &f->opts);
         if (rc != Z2D_OK && r->status == Z2D_OK) r->status = rc;
