@@ -300,12 +300,27 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
  * can never fit. */
 int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *req, void *out,
                               uint64_t out_len, uint64_t *id);
+/* Many requests submitted in one call (a front end handing over the
+ * requests that arrived together): request i is reqs[i] with outs[i] and
+ * out_lens[i] (outs and out_lens optional), checked and queued as
+ * z2d_gpu_server_submit would, in order and under one lock.  ids[i]
+ * receives its id (0 if refused) and status[i] (optional) its submit
+ * status.  Returns the first refusal, else Z2D_OK. */
+int32_t z2d_gpu_server_submit_n(z2d_gpu_server *s, const z2d_request_desc *const *reqs,
+                                void *const *outs, const uint64_t *out_lens, uint64_t n,
+                                uint64_t *ids, int32_t *status);
 /* 1 = done (status filled, request released), 0 = pending. */
 int32_t z2d_gpu_server_poll(z2d_gpu_server *s, uint64_t id, int32_t *status);
 int32_t z2d_gpu_server_wait(z2d_gpu_server *s, uint64_t id, int32_t *status, double timeout_ms);
 /* Next completed request in completion order (1), or 0 on timeout. */
 int32_t z2d_gpu_server_wait_any(z2d_gpu_server *s, uint64_t *id, int32_t *status,
                                 double timeout_ms);
+/* Up to `max` completed requests in completion order, waiting up to
+ * timeout_ms for the first: ids, statuses (optional) and completion times
+ * (optional, seconds of CLOCK_MONOTONIC, stamped when the result reached
+ * the caller's buffer).  Returns their count, 0 on timeout. */
+int32_t z2d_gpu_server_wait_some(z2d_gpu_server *s, uint64_t *ids, int32_t *status,
+                                 double *t_done, uint32_t max, double timeout_ms);
 /* submitted, completed, rejected, batches, pixel-draws, bytes in use, peak
  * bytes, queued. */
 int32_t z2d_gpu_server_stats(z2d_gpu_server *s, uint64_t out[8]);

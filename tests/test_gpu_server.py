@@ -112,6 +112,55 @@ def test_server_same_alone_and_in_mix():
         srv.close()
 
 
+def test_server_submit_many_and_wait_some():
+    """z2d_gpu_server_submit_n: requests handed over in one call render as
+    submitted one by one (the oracle's surfaces); a request that can never
+    fit is refused in place (id 0, BUDGET_EXCEEDED) without stopping the
+    others.  z2d_gpu_server_wait_some returns every completion once, in
+    completion order, stamped on CLOCK_MONOTONIC between submit and return."""
+    import time
+    from z2d_amd.gpu import Server
+    rng = scenes.SplitMix64(0x5B7)
+    reqs = [make_request(rng, side_max=200) for _ in range(30)]
+    srv = _server(64 << 20)
+    try:
+        preps, outs = [], []
+        for i, rq in enumerate(reqs):
+            if i == 11:  # larger than the whole budget
+                preps.append(Server.prepare(abi.RGBA, 5000, 5000, []))
+                outs.append(None)
+            preps.append(Server.prepare(rq["fmt"], rq["w"], rq["h"], rq["draws"],
+                                        clear=rq["clear"], clip=rq["clip"]))
+            outs.append(np.zeros(abi.surface_nbytes(rq["fmt"], rq["w"], rq["h"]), np.uint8))
+        t_sub = time.monotonic()
+        rc, ids, st = srv.submit_many(preps, outs)
+        assert rc == abi.ERR_BUDGET_EXCEEDED
+        assert ids[11] == 0 and st[11] == abi.ERR_BUDGET_EXCEEDED
+        live = [i for i in range(len(preps)) if i != 11]
+        assert all(st[i] == 0 and ids[i] != 0 for i in live)
+        c_ids = np.zeros(len(live), np.uint64)
+        c_st = np.zeros(len(live), np.int32)
+        c_t = np.zeros(len(live), np.float64)
+        got = 0
+        while got < len(live):  # several calls of at most 7 completions
+            k = srv.wait_some(c_ids[got:got + 7], c_st[got:got + 7], c_t[got:got + 7], 60000.0)
+            assert k > 0, "server stalled"
+            got += k
+        t_end = time.monotonic()
+        assert sorted(c_ids.tolist()) == sorted(int(ids[i]) for i in live)
+        assert not c_st.any()
+        assert np.all(np.diff(c_t) >= 0) and t_sub <= c_t[0] and c_t[-1] <= t_end
+        for i in live:
+            rq = reqs[i if i < 11 else i - 1]
+            msg = diff_report(outs[i], oracle_render(rq))
+            assert not msg, f"request {i}: {msg}"
+        st = srv.stats()
+        assert st["completed"] == st["submitted"] == len(live)
+        assert st["rejected"] == 1
+    finally:
+        srv.close()
+
+
 def test_server_budget_admission():
     """A 1 MiB budget admits a few requests at a time; the rest wait in the
     queue and still complete.  A request that can never fit is refused."""

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <memory>
 #include <new>
 #include <string>
@@ -2778,6 +2779,7 @@ struct SrvRequest {
   bool done = false;
   double t_queued = 0;  // when it became ready for admission (steady clock, s)
   double t_submit = 0, t_prep0 = 0, t_taken = 0;  // submitted, prep started, taken into a batch
+  double t_done = 0;  // completed
 };
 
 struct SrvSlot {
@@ -2866,6 +2868,7 @@ struct z2d_gpu_server {
     q_admit += r->t_taken - r->t_queued;
     q_batch += t - r->t_taken;
     q_n++;
+    r->t_done = t;
     r->done = true;
     completed.push_back(r->id);
     n_completed++;
@@ -2905,13 +2908,17 @@ struct z2d_gpu_server {
   // result-copy enqueue, finalize, idle
   double t_build = 0, t_count = 0, t_prep = 0, t_launch = 0, t_fin = 0, t_idle = 0;
   double t_prepthreads = 0, t_copythreads = 0;  // busy time of the prep and copy threads
+  double t_strokes = 0;  // of which stroke tessellation
+  uint64_t n_strokes = 0, n_stroke_edges = 0;
   // summed per request: waiting for a prep thread, waiting for admission
   // once prepared, and from admission to completion
   double q_prep = 0, q_admit = 0, q_batch = 0;
   uint64_t q_n = 0;
+  // seconds of CLOCK_MONOTONIC (the clock completion times are reported on)
   static double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
   }
 
   void prep_loop() {
@@ -2926,10 +2933,16 @@ struct z2d_gpu_server {
       }
       const double p0 = now_s();
       r->t_prep0 = p0;
+      double ts = 0;
+      uint64_t ns = 0, ne = 0;
       for (auto &f : r->fills) {
         if (f->kind == Z2D_DRAW_STROKE) {
+          const double s0 = now_s();
           f->geo_rc = stroke_draw(f->nodes.data(), f->nodes.size(), &f->sopts, r->format,
                                   f->stroke);
+          ts += now_s() - s0;
+          ns++;
+          ne += f->stroke.poly.edges.size();
           // server batches keep the default band height
           if (f->geo_rc == Z2D_OK && !f->stroke.empty) {
             f->stroke.pairs = polygon_pairs(f->stroke, r->format, r->surf.h, TILE_H);
@@ -2942,6 +2955,9 @@ struct z2d_gpu_server {
       std::lock_guard<std::mutex> lk(mu);
       r->t_queued = now_s();
       t_prepthreads += r->t_queued - p0;
+      t_strokes += ts;
+      n_strokes += ns;
+      n_stroke_edges += ne;
       queue.push_back(r);
       cv_work.notify_all();
     }
@@ -3301,17 +3317,28 @@ int32_t z2d_gpu_server_create(z2d_gpu_ctx *ctx, uint64_t hbm_budget, uint32_t ma
   }
   for (unsigned w = 0; w < z2d_gpu_server::kWorkers; w++)
     s->workers.emplace_back([s, w] { s->run(w); });
+  // preparation threads: three quarters of the hardware threads less one
+  // (2 to 12; the workers, copy threads and clients take the rest), or
+  // Z2D_SERVER_PREP_THREADS (1 to 64)
   const unsigned hc = std::thread::hardware_concurrency();
-  const unsigned np = std::max(2u, std::min(12u, hc ? hc / 2 : 2u));
+  unsigned np = std::max(2u, std::min(12u, hc >= 4 ? 3 * hc / 4 - 1 : 2u));
+  if (const char *env = std::getenv("Z2D_SERVER_PREP_THREADS")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 1 && v <= 64) np = (unsigned)v;
+  }
   for (unsigned i = 0; i < np; i++) s->preppers.emplace_back([s] { s->prep_loop(); });
   for (unsigned i = 0; i < 4; i++) s->copiers.emplace_back([s] { s->copy_loop(); });
   *out = s;
   return Z2D_OK;
 }
 
-int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, void *out,
-                              uint64_t out_len, uint64_t *id) {
-  if (!s || !rq || !id) return Z2D_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+// A request's copy for the server (z2d_gpu_server_submit's checks), not yet
+// queued.
+static int32_t server_request(z2d_gpu_server *s, const z2d_request_desc *rq, void *out,
+                              uint64_t out_len, std::unique_ptr<SrvRequest> &res) {
+  if (!rq) return Z2D_ERR_INVALID_ARGUMENT;
   if (rq->width < 1) return Z2D_ERR_INVALID_WIDTH;
   if (rq->height < 1) return Z2D_ERR_INVALID_HEIGHT;
   if (rq->format > Z2D_FORMAT_ALPHA1 || (rq->n_fills && !rq->fills) ||
@@ -3374,7 +3401,13 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
     }
     r->fills.push_back(std::move(f));
   }
-  std::lock_guard<std::mutex> lk(s->mu);
+  res = std::move(r);
+  return Z2D_OK;
+}
+
+// Queues a built request for preparation (mutex held).
+static int32_t server_enqueue_locked(z2d_gpu_server *s, std::unique_ptr<SrvRequest> &r,
+                                     double t, uint64_t *id) {
   if (r->surf_bytes + r->mask_bytes > s->budget) {
     s->n_rejected++;
     return Z2D_ERR_BUDGET_EXCEEDED;
@@ -3382,20 +3415,64 @@ int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, voi
   r->id = s->next_id++;
   *id = r->id;
   SrvRequest *raw = r.release();
-  raw->t_submit = z2d_gpu_server::now_s();
+  raw->t_submit = t;
   s->reqs[raw->id] = raw;
   s->prep_q.push_back(raw);
   s->n_submitted++;
-  s->cv_prep.notify_one();
   return Z2D_OK;
 }
 
-static int32_t take_done(z2d_gpu_server *s, uint64_t id, int32_t *status) {
+extern "C" {
+
+int32_t z2d_gpu_server_submit(z2d_gpu_server *s, const z2d_request_desc *rq, void *out,
+                              uint64_t out_len, uint64_t *id) {
+  if (!s || !rq || !id) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<SrvRequest> r;
+  int32_t rc = server_request(s, rq, out, out_len, r);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  rc = server_enqueue_locked(s, r, z2d_gpu_server::now_s(), id);
+  if (rc == Z2D_OK) s->cv_prep.notify_one();
+  return rc;
+}
+
+int32_t z2d_gpu_server_submit_n(z2d_gpu_server *s, const z2d_request_desc *const *reqs,
+                                void *const *outs, const uint64_t *out_lens, uint64_t n,
+                                uint64_t *ids, int32_t *status) {
+  if (!s || (n && (!reqs || !ids))) return Z2D_ERR_INVALID_ARGUMENT;
+  std::vector<std::unique_ptr<SrvRequest>> built(n);
+  std::vector<int32_t> rcs(n, Z2D_OK);
+  for (uint64_t i = 0; i < n; i++)
+    rcs[i] = server_request(s, reqs[i], outs ? outs[i] : nullptr, out_lens ? out_lens[i] : 0,
+                            built[i]);
+  int32_t first = Z2D_OK;
+  uint64_t queued = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double t = z2d_gpu_server::now_s();
+    for (uint64_t i = 0; i < n; i++) {
+      ids[i] = 0;
+      if (rcs[i] == Z2D_OK) rcs[i] = server_enqueue_locked(s, built[i], t, &ids[i]);
+      if (rcs[i] == Z2D_OK) queued++;
+      if (status) status[i] = rcs[i];
+      if (rcs[i] != Z2D_OK && first == Z2D_OK) first = rcs[i];
+    }
+  }
+  if (queued == 1) s->cv_prep.notify_one();
+  else if (queued > 1) s->cv_prep.notify_all();
+  return first;
+}
+
+
+// Takes a completed request out of the server (mutex held); `gone` receives
+// it, so that its memory is released after the mutex is.
+static int32_t take_done(z2d_gpu_server *s, uint64_t id, int32_t *status,
+                         std::unique_ptr<SrvRequest> &gone) {
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return Z2D_ERR_INVALID_ARGUMENT;
   if (!it->second->done) return 0;
   if (status) *status = it->second->status;
-  delete it->second;
+  gone.reset(it->second);
   s->reqs.erase(it);
   auto c = std::find(s->completed.begin(), s->completed.end(), id);
   if (c != s->completed.end()) s->completed.erase(c);
@@ -3404,25 +3481,29 @@ static int32_t take_done(z2d_gpu_server *s, uint64_t id, int32_t *status) {
 
 int32_t z2d_gpu_server_poll(z2d_gpu_server *s, uint64_t id, int32_t *status) {
   if (!s) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<SrvRequest> gone;  // released after the mutex
   std::lock_guard<std::mutex> lk(s->mu);
-  return take_done(s, id, status);
+  return take_done(s, id, status, gone);
 }
 
 int32_t z2d_gpu_server_wait(z2d_gpu_server *s, uint64_t id, int32_t *status, double timeout_ms) {
   if (!s) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<SrvRequest> gone;  // released after the mutex
   std::unique_lock<std::mutex> lk(s->mu);
   auto deadline = std::chrono::steady_clock::now() +
                   std::chrono::microseconds((int64_t)(timeout_ms * 1000.0));
   for (;;) {
-    int32_t rc = take_done(s, id, status);
+    int32_t rc = take_done(s, id, status, gone);
     if (rc != 0) return rc;
-    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) return take_done(s, id, status);
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout)
+      return take_done(s, id, status, gone);
   }
 }
 
 int32_t z2d_gpu_server_wait_any(z2d_gpu_server *s, uint64_t *id, int32_t *status,
                                 double timeout_ms) {
   if (!s || !id) return Z2D_ERR_INVALID_ARGUMENT;
+  std::unique_ptr<SrvRequest> gone;  // released after the mutex
   std::unique_lock<std::mutex> lk(s->mu);
   auto deadline = std::chrono::steady_clock::now() +
                   std::chrono::microseconds((int64_t)(timeout_ms * 1000.0));
@@ -3430,8 +3511,35 @@ int32_t z2d_gpu_server_wait_any(z2d_gpu_server *s, uint64_t *id, int32_t *status
     if (!s->completed.empty()) {
       uint64_t got = s->completed.front();
       *id = got;
-      return take_done(s, got, status);
+      return take_done(s, got, status, gone);
     }
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && s->completed.empty())
+      return 0;
+  }
+}
+
+int32_t z2d_gpu_server_wait_some(z2d_gpu_server *s, uint64_t *ids, int32_t *status,
+                                 double *t_done, uint32_t max, double timeout_ms) {
+  if (!s || !ids || max == 0) return Z2D_ERR_INVALID_ARGUMENT;
+  std::vector<std::unique_ptr<SrvRequest>> gone;  // released after the mutex
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto deadline = std::chrono::steady_clock::now() +
+                  std::chrono::microseconds((int64_t)(timeout_ms * 1000.0));
+  for (;;) {
+    uint32_t k = 0;
+    while (k < max && !s->completed.empty()) {
+      const uint64_t got = s->completed.front();
+      s->completed.pop_front();
+      auto it = s->reqs.find(got);
+      if (it == s->reqs.end()) continue;
+      ids[k] = got;
+      if (status) status[k] = it->second->status;
+      if (t_done) t_done[k] = it->second->t_done;
+      gone.emplace_back(it->second);
+      s->reqs.erase(it);
+      k++;
+    }
+    if (k) return (int32_t)k;
     if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && s->completed.empty())
       return 0;
   }
@@ -3489,6 +3597,13 @@ void z2d_gpu_server_destroy(z2d_gpu_server *s) {
                  s->q_n ? s->q_prep / s->q_n * 1e3 : 0.0, s->q_n ? s->q_admit / s->q_n * 1e3 : 0.0,
                  s->q_n ? s->q_batch / s->q_n * 1e3 : 0.0,
                  s->n_batches ? (double)s->q_n / (double)s->n_batches : 0.0);
+  if (std::getenv("Z2D_SERVER_TRACE"))
+    std::fprintf(stderr,
+                 "[z2d server] strokes %llu: tessellation %.3f s (%.1f us each, %.0f edges "
+                 "each)\n",
+                 (unsigned long long)s->n_strokes, s->t_strokes,
+                 s->n_strokes ? s->t_strokes / (double)s->n_strokes * 1e6 : 0.0,
+                 s->n_strokes ? (double)s->n_stroke_edges / (double)s->n_strokes : 0.0);
   if (std::getenv("Z2D_SERVER_TRACE"))
     std::fprintf(stderr, "[z2d server] resident gradients: %u, hits %llu, misses %llu, full %llu\n",
                  s->gcache.n, (unsigned long long)s->gcache.hits,

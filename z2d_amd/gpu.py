@@ -86,6 +86,10 @@ _SIGS = {
     "z2d_gpu_server_wait": ([C.c_void_p, C.c_uint64, P(C.c_int32), C.c_double], C.c_int32),
     "z2d_gpu_server_wait_any": ([C.c_void_p, P(C.c_uint64), P(C.c_int32), C.c_double],
                                 C.c_int32),
+    "z2d_gpu_server_submit_n": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                 C.c_void_p, C.c_void_p], C.c_int32),
+    "z2d_gpu_server_wait_some": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_double], C.c_int32),
     "z2d_gpu_server_stats": ([C.c_void_p, P(C.c_uint64)], C.c_int32),
     "z2d_gpu_server_destroy": ([C.c_void_p], None),
     "z2d_gpu_composite_noalloc": ([C.c_void_p, P(abi.SurfaceDesc), C.c_int32, C.c_int32,
@@ -476,6 +480,52 @@ class Server:
         if rc == 0:
             self._pending[rid.value] = (out, w)  # keeps the result buffer alive
         return rc, rid.value
+
+    def submit_many(self, preps, outs=None):
+        """Submits prepared requests in one call (z2d_gpu_server_submit_n).
+        Returns (rc, ids, statuses): numpy arrays, id 0 where refused.  Each
+        out (numpy uint8) must stay alive until its request completes."""
+        n = len(preps)
+        rq = np.empty(n, np.uint64)
+        op = np.zeros(n, np.uint64)
+        ol = np.zeros(n, np.uint64)
+        counters = []
+        for i, prep in enumerate(preps):
+            w = C.c_uint64()
+            prep[0].out_written = C.pointer(w)  # copied at submit
+            counters.append(w)
+            rq[i] = C.addressof(prep[0])
+            if outs is not None and outs[i] is not None:
+                op[i] = outs[i].ctypes.data
+                ol[i] = outs[i].nbytes
+        ids = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        rc = self.submit_n_raw(rq.ctypes.data, op.ctypes.data, ol.ctypes.data, n,
+                               ids.ctypes.data, st.ctypes.data)
+        for i in range(n):
+            if ids[i]:
+                self._pending[int(ids[i])] = (outs[i] if outs is not None else None, counters[i])
+        return rc, ids, st
+
+    def submit_n_raw(self, req_ptrs, out_ptrs, out_lens, n, ids, status):
+        """z2d_gpu_server_submit_n on addresses (arrays of descriptor
+        pointers, output pointers and lengths; ids and statuses written).
+        The caller keeps every descriptor and output alive; results of
+        requests submitted this way are not tracked by `written`."""
+        return lib().z2d_gpu_server_submit_n(self.h, req_ptrs, out_ptrs, out_lens, n, ids, status)
+
+    def wait_some(self, ids, status, t_done, timeout_ms=60000.0):
+        """Up to len(ids) completed requests (numpy uint64 / int32 / float64
+        arrays filled in place; t_done in seconds of CLOCK_MONOTONIC, the
+        clock of time.monotonic()).  Returns their count, 0 on timeout."""
+        k = lib().z2d_gpu_server_wait_some(self.h, ids.ctypes.data, status.ctypes.data,
+                                           t_done.ctypes.data, len(ids), timeout_ms)
+        if k < 0:
+            raise GpuError(f"server_wait_some returned {k}")
+        if self._pending:
+            for i in range(k):
+                self._done(int(ids[i]))
+        return k
 
     def _done(self, rid):
         entry = self._pending.pop(rid, None)
