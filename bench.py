@@ -335,7 +335,7 @@ def serving_requests(gpu, n, seed=0xC5):
 def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 64000),
                   seconds=2.0, seed=0x9015, budget=16 << 30, n_sub_max=2):
     """Open-loop Poisson arrivals at each QPS level, cycling through the
-    request pool, and one saturation run with eight times the pool queued at
+    request pool, and one saturation run with 24 times the pool queued at
     t=0.  Client threads (one up to 16k QPS, n_sub_max above; arrival i by
     thread i mod their number) hand each arrival to the server at its
     scheduled time; the arrivals already due when a client wakes go over in
@@ -426,7 +426,7 @@ def bench_serving(gpu, ctx, reqs, qps_levels=(250, 1000, 4000, 16000, 32000, 640
     out = {}
     levels = list(qps_levels) + ["saturation"]
     for lvl in levels:
-        n = 8 * P if lvl == "saturation" else int(lvl * seconds)
+        n = 24 * P if lvl == "saturation" else int(lvl * seconds)
         gaps = (np.zeros(n) if lvl == "saturation" else rng.exponential(1.0 / lvl, n))
         sched = np.cumsum(gaps)
         n_sub = 1 if lvl != "saturation" and lvl <= 16000 else n_sub_max

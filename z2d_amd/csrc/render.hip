@@ -69,14 +69,27 @@ __device__ __forceinline__ void store_raw(const DevSurface &s, int64_t idx, uint
 
 
 // Inclusive prefix sum over the 64 lanes of a wave (DPP row shifts within
-// 16-lane rows, then row broadcasts across rows).
+// 16-lane rows, then row broadcasts across rows), one DPP add per step: a
+// shift reads 0 past the row's start (bound_ctrl), a broadcast leaves the
+// rows outside its row mask as they were.  Written out because the
+// compiler keeps some of the builtin form's steps as a DPP move and an add.
+// Each step reads the previous one's result through DPP, which needs two
+// wait states after the VALU write (s_nop 1); the first covers the
+// caller's write of x.  Every lane of the wave must be active.
 __device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
-  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
-  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
-  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
-  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
-  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
-  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  asm("s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_add_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(x));
   return x;
 }
 
@@ -2318,10 +2331,13 @@ k_render_fast(JobView J, uint32_t tile_base,
     const bool col_ok = px < W;
     uint32_t pix[ROWS_PER_THREAD];
     uint32_t dirty = 0;  // rows whose pixel a draw or the clip changed
+    uint32_t vrows = 0;  // rows whose pixel of this lane's column is on the surface
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
       const int32_t py = ty0 + i;
-      pix[i] = (col_ok && py < H) ? load_raw(S, (int64_t)py * W + px) : 0u;
+      const bool on = col_ok && py < H;
+      pix[i] = on ? load_raw(S, (int64_t)py * W + px) : 0u;
+      vrows |= (on ? 1u : 0u) << i;
     }
     const int32_t ys_base = ty0 * 4, xs_base = tx0 * 4;
     constexpr int32_t COLS = TILE_W * 4, SR = ROWS_PER_THREAD * 4;
@@ -2459,13 +2475,14 @@ k_render_fast(JobView J, uint32_t tile_base,
         wave_lds_sync();
 
         // ---- composite: src_over of the masked paint
-        uint32_t need = 0;  // covered pixels inside the composite region
+        // covered pixels inside the composite region: rows [r_lo, r_hi) of
+        // the strip lie in [ry0, ry1) (uniform; the region meets the strip)
+        const int32_t r_lo = max(ry0, ty0) - ty0, r_hi = min(ry1, ty0 + ROWS_PER_THREAD) - ty0;
+        const uint32_t rmask = (1u << r_hi) - (1u << r_lo);
+        uint32_t kbits = 0;
 #pragma unroll
-        for (int i = 0; i < ROWS_PER_THREAD; i++) {
-          const int32_t py = ty0 + i;
-          if (col_ok && py < H && px >= rx0 && px < rx1 && py >= ry0 && py < ry1 && kk[i] > 0)
-            need |= 1u << i;
-        }
+        for (int i = 0; i < ROWS_PER_THREAD; i++) kbits |= (kk[i] > 0 ? 1u : 0u) << i;
+        const uint32_t need = (px >= rx0 && px < rx1) ? (vrows & rmask & kbits) : 0u;
         if (__ballot(need != 0) == 0) continue;  // nothing covered in this strip (uniform)
         const uint32_t praw = rfl(B.px);
         const I4 p_px{praw & 0xFFu, (praw >> 8) & 0xFFu, (praw >> 16) & 0xFFu, praw >> 24};
