@@ -1762,6 +1762,57 @@ __device__ __forceinline__ void coverage_msaa(uint32_t (&acc)[STRIP_SUBROWS][ACC
   }
 }
 
+// Row-paired winding deltas (the fast strip worker's layout): word [p][c]
+// holds sample column c of sample rows 2p (low half) and 2p + 1 (high half)
+// as lo + hi * 65536 in 32-bit arithmetic (the LDS atomics carry between the
+// halves).  Every winding and partial sum of a draw whose band entry has at
+// most PACKED_EDGE_LIMIT edges lies in [-32767, 32767], so the 32-bit sums
+// are exact, and U = W + 0x8000 for a sum W = lo + hi * 65536 holds hi in its
+// high half (mod 2^16) and lo + 0x8000 in its low half with no borrow:
+// lo is odd iff bit 0 of U is set, hi iff bit 16; U ^ 0x8000 holds lo and hi
+// as 16-bit halves, each zero iff that winding is.
+constexpr int FAST_ACC_ROWS = STRIP_SUBROWS / 2, FAST_ACC_COLS = TILE_W * 4;
+
+// coverage_msaa over the row-paired deltas: per pair of sample rows one
+// 16-byte read of the lane's four columns, one wave scan, and per column an
+// add and a packed test of both rows, counted by popcount.
+// 1 in each 16-bit half of v that is nonzero, 0 in each that is zero (a
+// packed unsigned min with 1; written out because the compiler otherwise
+// expands the packed min into per-half compares).
+__device__ __forceinline__ uint32_t pk_nonzero_u16(uint32_t v) {
+  uint32_t r;
+  asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(v), "s"(0x00010001u));
+  return r;
+}
+
+template <bool EO>
+__device__ __forceinline__ void coverage_msaa_rp(uint32_t (&acc)[FAST_ACC_ROWS][FAST_ACC_COLS],
+                                                 uint32_t lane, int *kk) {
+#pragma unroll
+  for (int i = 0; i < ROWS_PER_THREAD; i++) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int pr = 0; pr < 2; pr++) {
+      uint4 *slot = reinterpret_cast<uint4 *>(&acc[i * 2 + pr][lane * 4]);
+      const uint4 p = *slot;
+      *slot = make_uint4(0u, 0u, 0u, 0u);
+      const uint32_t tot = p.x + p.y + p.z + p.w;
+      const uint32_t exc = (uint32_t)wave_incl_scan((int32_t)tot) - tot;
+      const uint32_t col[4] = {p.x, p.y, p.z, p.w};
+      uint32_t u = exc + 0x8000u;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        u += col[j];
+        if (EO)
+          k += (uint32_t)__popc(u & 0x00010001u);
+        else
+          k += (uint32_t)__popc(pk_nonzero_u16(u ^ 0x8000u));
+      }
+    }
+    kk[i] = (int)k;
+  }
+}
+
 // Persistent strip workers: each wave takes (tile, strip) items from the
 // queue and renders every draw of the tile's band over its ROWS_PER_THREAD x
 // TILE_W pixels, which stay in registers from the first draw to the last.
@@ -2170,8 +2221,8 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
 // -> gradient no longer stalls every entry.
 // ---------------------------------------------------------------------------
 struct FastShared {
-  // winding deltas, as in StripShared (zero between entries)
-  uint32_t acc[STRIP_SUBROWS][ACC_WORDS];
+  // winding deltas, row-paired (coverage_msaa_rp; zero between entries)
+  uint32_t acc[FAST_ACC_ROWS][FAST_ACC_COLS];
   union {
     struct {                             // coverage (phase A)
       // owning lane of each work item (0x80: delta -1); the owner's edge
@@ -2294,7 +2345,7 @@ k_render_fast(JobView J, uint32_t tile_base,
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t spt = J.strips_per_tile;
   const uint32_t n_items = tile_count * spt;
-  for (uint32_t i = lane; i < (uint32_t)(STRIP_SUBROWS * ACC_WORDS); i += 64)
+  for (uint32_t i = lane; i < (uint32_t)(FAST_ACC_ROWS * FAST_ACC_COLS); i += 64)
     (&sh.acc[0][0])[i] = 0u;
   if (lane <= (uint32_t)STRIP_SUBROWS) sh.ldiff[lane] = 0;
   wave_lds_sync();
@@ -2434,7 +2485,7 @@ k_render_fast(JobView J, uint32_t tile_base,
             const int32_t col = xr < 0 ? 0 : xr;
             inner |= (uint32_t)col;
             const uint32_t dv = (v & 0x80u) ? 0xFFFFFFFFu : 1u;
-            atomicAdd(&sh.acc[sr][col >> 1], (col & 1) ? (dv << 16) : dv);
+            atomicAdd(&sh.acc[sr >> 1][col], (sr & 1) ? (dv << 16) : dv);
           }
           wave_lds_sync();
         }
@@ -2448,7 +2499,8 @@ k_render_fast(JobView J, uint32_t tile_base,
             sh.ldiff[lane] = 0;
           }
           const int32_t wl = wave_incl_scan(dl);
-          if (lane < (uint32_t)SR && wl != 0) atomicAdd(&sh.acc[lane][0], (uint32_t)wl);
+          if (lane < (uint32_t)SR && wl != 0)
+            atomicAdd(&sh.acc[lane >> 1][0], (lane & 1) ? ((uint32_t)wl << 16) : (uint32_t)wl);
           wave_lds_sync();
         }
 
@@ -2456,21 +2508,22 @@ k_render_fast(JobView J, uint32_t tile_base,
         int kk[ROWS_PER_THREAD];
         if (__ballot(inner != 0) == 0) {
           // every delta at column 0: uniform winding across each sample row
+          // (lane = sample row; the even lane clears the pair's word)
           uint32_t w0 = 0;
-          if (lane < (uint32_t)SR) {
-            w0 = sh.acc[lane][0];
-            sh.acc[lane][0] = 0u;
-          }
+          if (lane < (uint32_t)SR) w0 = sh.acc[lane >> 1][0];
+          wave_lds_sync();
+          if (lane < (uint32_t)SR && !(lane & 1)) sh.acc[lane >> 1][0] = 0u;
           int32_t wl, wh;
           unpack_pair(w0, wl, wh);
-          const uint64_t in_rows = __ballot(lane < (uint32_t)SR && wind_in(wl, eo));
+          const int32_t w_row = (lane & 1) ? wh : wl;
+          const uint64_t in_rows = __ballot(lane < (uint32_t)SR && wind_in(w_row, eo));
 #pragma unroll
           for (int i = 0; i < ROWS_PER_THREAD; i++)
             kk[i] = 4 * __popcll((in_rows >> (4 * i)) & 0xFull);
         } else if (eo) {
-          coverage_msaa<true, false>(sh.acc, lane, xs_base, ys_base, nullptr, 0, 0, kk);
+          coverage_msaa_rp<true>(sh.acc, lane, kk);
         } else {
-          coverage_msaa<false, false>(sh.acc, lane, xs_base, ys_base, nullptr, 0, 0, kk);
+          coverage_msaa_rp<false>(sh.acc, lane, kk);
         }
         wave_lds_sync();
 
