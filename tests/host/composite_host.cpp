@@ -106,7 +106,7 @@ static uint64_t check_rewrites(uint64_t &cases) {
     const I4 m = mask_coverage(unpack8(sw), k);
     if (mask_coverage_rgba8(sw, k) != (m.r | (m.g << 8) | (m.b << 16) | (m.a << 24))) bad++;
   }
-  // f2u8 (truncf of v + copysignf(0.5 - 2^-25, v), clamped) against roundf
+  // f2u8 (truncf of v + (0.5 - 2^-25), clamped) against roundf
   // with the branches it replaces: every 61st float bit pattern and the
   // neighbourhoods of the halves 0.5 .. 65535.5
   auto f2u8_ref = [](float v) -> uint32_t {

@@ -105,11 +105,12 @@ Z2D_PX __forceinline__ double zmodd(double x, double y) {
 // round(255*v) to an 8-bit channel; saturates outside the range the
 // reference defines (its @intFromFloat is undefined there).
 // roundf(v) (half away from zero), then 0 below zero or for NaN and 65535
-// above it, without branches: trunc(v + copysign(0.5 - 2^-25, v)) is roundf
-// for every float (checked over all 2^32 of them), and the clamp takes NaN
-// to 0.
+// above it, without branches: trunc(v + (0.5 - 2^-25)) is roundf for every
+// v >= 0 (checked over all 2^32 floats with the sign copied onto the
+// addend), and for v < 0 it is at most 0 as roundf(v) is, so the clamp
+// gives 0 either way; the clamp takes NaN to 0.
 Z2D_PX __forceinline__ uint32_t f2u8(float v) {
-  const float r = truncf(v + copysignf(0x1.fffffep-2f, v));
+  const float r = truncf(v + 0x1.fffffep-2f);
   return (uint32_t)fminf(fmaxf(r, 0.0f), 65535.0f);
 }
 

@@ -2381,7 +2381,7 @@ k_render_fast(JobView J, uint32_t tile_base,
     const int32_t px = tx0 + (int32_t)lane;
     const bool col_ok = px < W;
     uint32_t pix[ROWS_PER_THREAD];
-    uint32_t dirty = 0;  // rows whose pixel a draw or the clip changed
+    uint32_t dirty = 0;  // rows whose pixel a draw composited on or the clip changed
     uint32_t vrows = 0;  // rows whose pixel of this lane's column is on the surface
 #pragma unroll
     for (int i = 0; i < ROWS_PER_THREAD; i++) {
@@ -2566,12 +2566,14 @@ k_render_fast(JobView J, uint32_t tile_base,
         const bool fast_full = (bflags & DRAW_FAST_SRC) != 0;
         const uint32_t full_raw = fast_full ? pack(p_px, S.fmt) : 0u;
         const bool rgba8 = S.fmt == Z2D_FORMAT_RGBA;  // uniform
+        // the pixels composited on are stored (a composite that leaves a
+        // pixel as it was stores the same value)
+        stat_pix += (uint32_t)__popc(need);
+        dirty |= need;
 #pragma unroll
         for (int i = 0; i < ROWS_PER_THREAD; i++) {
           if (!((need >> i) & 1u)) continue;
-          stat_pix++;
           const int k = kk[i];
-          const uint32_t before = pix[i];
           if (fast_full && k >= 16) {
             pix[i] = full_raw;
           } else {
@@ -2585,7 +2587,6 @@ k_render_fast(JobView J, uint32_t tile_base,
               pix[i] = composite_src_over(pix[i], S.fmt, src);
             }
           }
-          dirty |= (pix[i] != before ? 1u : 0u) << i;
         }
       }
     }
