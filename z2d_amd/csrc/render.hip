@@ -93,6 +93,44 @@ __device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
   return x;
 }
 
+// Minimum / maximum over the 64 lanes of a wave as a wave-uniform value.
+// The DPP steps have no bound_ctrl, so a lane whose source lies outside its
+// row (or outside the broadcast's row mask) is not written and keeps its
+// value; lane 63 ends with the extreme of the whole wave.  Every lane of
+// the wave must be active.
+__device__ __forceinline__ int32_t wave_min_i32(int32_t x) {
+  asm("s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_min_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(x));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+__device__ __forceinline__ int32_t wave_max_i32(int32_t x) {
+  asm("s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(x));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+
 // ---------------------------------------------------------------------------
 // k_edge_build: one wave per draw.
 // ---------------------------------------------------------------------------
@@ -1510,12 +1548,7 @@ __global__ void k_edge_bins(JobView J) {
       }
     }
     const REdge red = make_redge(ed);  // the copy every band of the edge receives
-    int32_t ulo = lo, uhi = hi;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      ulo = min(ulo, __shfl_xor(ulo, off, 64));
-      uhi = max(uhi, __shfl_xor(uhi, off, 64));
-    }
+    const int32_t ulo = wave_min_i32(lo), uhi = wave_max_i32(hi);
     // bands in windows of 64: lane j holds the count (then the list
     // position) of band w + j, so each window takes one vector atomic
     for (int32_t w = ulo; w <= uhi; w += 64) {
@@ -1545,14 +1578,10 @@ __global__ void k_edge_bins(JobView J) {
         const bool hit = lo <= bnd && bnd <= hi && top < ys_hi && bot >= ys_lo;
         const uint64_t m = __ballot(hit);
         if (m == 0) continue;
-        const uint32_t base = (uint32_t)__shfl((int)at, bnd - w, 64);
+        const uint32_t base = __builtin_amdgcn_readlane(at, bnd - w);
         // the entry's column range, widened by this wave's edges in the band
-        int32_t rlo = hit ? xlo : INT32_MAX, rhi = hit ? xhi : INT32_MIN;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          rlo = min(rlo, __shfl_xor(rlo, off, 64));
-          rhi = max(rhi, __shfl_xor(rhi, off, 64));
-        }
+        const int32_t rlo = wave_min_i32(hit ? xlo : INT32_MAX);
+        const int32_t rhi = wave_max_i32(hit ? xhi : INT32_MIN);
         if (lane == 0) {
           const uint32_t eidx = ebase + ((uint32_t)bnd - blo);
           atomicMin(&J.entry_xlo[eidx], rlo);
