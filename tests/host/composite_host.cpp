@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -142,6 +143,37 @@ static uint64_t check_rewrites(uint64_t &cases) {
     cases += 2;
     if (op_int(Z2D_OP_OVERLAY, d, sv).r != overlay_ch((int32_t)sc, (int32_t)dc, (int32_t)sa, (int32_t)da)) bad++;
     if (op_int(Z2D_OP_HARD_LIGHT, d, sv).r != hardlight_ch((int32_t)sc, (int32_t)dc, (int32_t)sa, (int32_t)da)) bad++;
+  }
+  // stops_find8 against the reference's binary search (stops_search): the
+  // interpolated color (both ends' colors at t) must be bit-identical; stop
+  // lists of 1..8 sorted offsets in [0, 1] drawn from a small set (ties,
+  // repeated stops, 0 and 1), padded with +inf, and offsets at every stop,
+  // between stops, below 0, above 1, and at their float neighbours
+  for (int i = 0; i < (1 << 18); i++) {
+    const uint32_t n = 1u + (uint32_t)(next_u64() % 8u);
+    float st[8], col[8];
+    const float grid[9] = {0.0f, 0.125f, 0.25f, 0.3f, 0.5f, 0.7f, 0.75f, 0.9f, 1.0f};
+    for (uint32_t k = 0; k < 8; k++) {
+      st[k] = k < n ? ((next_u64() & 3u) ? grid[next_u64() % 9u] : (float)(next_u64() % 1000u) / 999.0f)
+                    : __builtin_inff();
+      col[k] = (float)(next_u64() % 100000u) / 99999.0f;
+    }
+    std::sort(st, st + n);
+    float offs[6] = {st[next_u64() % n], (float)(next_u64() % 1000u) / 999.0f, -0.25f, 1.5f, 0.0f, 1.0f};
+    for (float o0 : offs) {
+      const float ov[3] = {o0, std::nextafterf(o0, -1e9f), std::nextafterf(o0, 1e9f)};
+      for (float o : ov) {
+        int a0, a1, b0, b1;
+        float ta, tb;
+        bool xa, xb;
+        stops_find8(st, n, o, a0, a1, ta, xa);
+        stops_search(st, n, o, b0, b1, tb, xb);
+        cases++;
+        const float ca = xa ? 0.0f : col[a0] + (col[a1] - col[a0]) * ta;
+        const float cb = xb ? 0.0f : col[b0] + (col[b1] - col[b0]) * tb;
+        if (xa != xb || __builtin_bit_cast(uint32_t, ca) != __builtin_bit_cast(uint32_t, cb)) bad++;
+      }
+    }
   }
   return bad;
 }

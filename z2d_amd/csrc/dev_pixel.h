@@ -836,10 +836,20 @@ Z2D_PX __forceinline__ void stops_find8(const float *st, uint32_t n_stops, float
   const float o = fminz(offset, 1.0f);
   const float4 a = *reinterpret_cast<const float4 *>(st);
   const float4 b = *reinterpret_cast<const float4 *>(st + 4);
-  const int below = (a.x < o) + (a.y < o) + (a.z < o) + (a.w < o) + (b.x < o) + (b.y < o) +
-                    (b.z < o) + (b.w < o);
-  const bool tie = (a.x == o) | (a.y == o) | (a.z == o) | (a.w == o) | (b.x == o) |
-                   (b.y == o) | (b.z == o) | (b.w == o);
+  // The offset is never NaN (fminf) and the stops are finite or +inf, so
+  // each difference st[k] - o is +0 exactly when st[k] == o (x - x), has its
+  // sign bit set exactly when st[k] < o (a difference of distinct floats is
+  // never +0, and a flushed one keeps its sign), and is +inf for padding:
+  // the stops below the offset are the differences' sign bits, and a tie is
+  // a zero smallest magnitude.
+  const float d[8] = {a.x - o, a.y - o, a.z - o, a.w - o, b.x - o, b.y - o, b.z - o, b.w - o};
+  uint32_t nb = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) nb += __builtin_bit_cast(uint32_t, d[k]) >> 31;
+  const int below = (int)nb;
+  const float dmin = fminf(fminf(fminf(fabsf(d[0]), fabsf(d[1])), fminf(fabsf(d[2]), fabsf(d[3]))),
+                           fminf(fminf(fabsf(d[4]), fabsf(d[5])), fminf(fabsf(d[6]), fabsf(d[7]))));
+  const bool tie = dmin == 0.0f;
   if (transparent) {
     i0 = i1 = 0;
     t = 0.0f;
