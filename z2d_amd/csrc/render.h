@@ -186,6 +186,9 @@ struct alignas(16) FastDraw {
   double p[8];
   float offs[FAST_STOPS];   // stop offsets
   float4 pre[FAST_STOPS];   // stop colors in the interpolation space
+  // solid paint: the RGBA8 source masked by each coverage count 0..16
+  // (mask_coverage; entry 16 is the paint itself)
+  uint32_t srctab[20];
 };
 constexpr uint32_t FAST_DRAW_CHUNKS = sizeof(FastDraw) / 16;  // 16-byte lanes
 static_assert(sizeof(FastDraw) % 16 == 0 && FAST_DRAW_CHUNKS <= 64, "FastDraw layout");

@@ -315,7 +315,17 @@ __device__ __forceinline__ uint32_t fast_paint(const DevPaint &P) {
   return FAST_PAINT_NONE;
 }
 
-// A fast draw's block: its region and flags, and the paint's constants.
+// A solid paint's RGBA8 word (FastDraw::px) masked by coverage count k
+// (mask_coverage; k = 16: the paint itself), FastDraw::srctab[k].
+__device__ __forceinline__ uint32_t fast_srctab_entry(const DevPaint &P, uint32_t k) {
+  const uint32_t praw = P.px.r | (P.px.g << 8) | (P.px.b << 16) | (P.px.a << 24);
+  const I4 p_px{praw & 0xFFu, (praw >> 8) & 0xFFu, (praw >> 16) & 0xFFu, praw >> 24};
+  const I4 m = k >= 16 ? p_px : mask_coverage(p_px, (int)k);
+  return m.r | (m.g << 8) | (m.b << 16) | (m.a << 24);
+}
+
+// A fast draw's block: its region and flags, and the paint's constants
+// (the solid paint's table is written by the draw's lanes 0..16 before).
 __device__ void write_fast_draw(FastDraw &F, const DevDraw &D, const DevPaint &P, uint32_t pf) {
   F.rx0 = D.rx0;
   F.rx1 = D.rx1;
@@ -376,6 +386,10 @@ __global__ void __launch_bounds__(256) k_edge_build(JobView J) {
   } else if (!(flags & DRAW_SERIAL_PLOT)) {
     plot_parallel(J.nodes + D.node_off, D.node_cnt, scale, out, acc);
   }
+  // a solid paint's coverage table, for the fast renderer should the draw
+  // be one (lane k: entry k)
+  if (J.fast && lane <= 16 && J.paints[D.paint].kind == Z2D_PATTERN_OPAQUE)
+    J.fast[d].srctab[lane] = fast_srctab_entry(J.paints[D.paint], lane);
   if (lane != 0) return;
   J.draw_rows[d] = make_int2(0, 0);  // not visible unless set below
   if (!D.input_edges && (flags & DRAW_SERIAL_PLOT)) {
@@ -2262,7 +2276,6 @@ struct FastShared {
     struct {                             // composite
       uint32_t ppaint[ROWS_PER_THREAD][64];  // paint samples (RGBA8)
       uint8_t plist[ROWS_PER_THREAD * 64];   // covered (row, lane) pairs
-      uint32_t srctab[17];                   // solid paint masked by each coverage count
     } c;
   } u;
   FastDraw blk;  // the current entry's draw constants
@@ -2569,13 +2582,7 @@ k_render_fast(JobView J, uint32_t tile_base,
         const uint32_t praw = rfl(B.px);
         const I4 p_px{praw & 0xFFu, (praw >> 8) & 0xFFu, (praw >> 16) & 0xFFu, praw >> 24};
         const bool solid = pf == FAST_PAINT_SOLID;
-        if (solid) {
-          if (lane <= 16) {
-            const I4 m = lane >= 16 ? p_px : mask_coverage(p_px, (int)lane);
-            sh.u.c.srctab[lane] = m.r | (m.g << 8) | (m.b << 16) | (m.a << 24);
-          }
-          wave_lds_sync();
-        } else {
+        if (!solid) {
           const int32_t np = __popc(need);
           const int32_t incl_p = wave_incl_scan(np);
           const uint32_t tot_p = (uint32_t)__builtin_amdgcn_readlane(incl_p, 63);
@@ -2606,7 +2613,7 @@ k_render_fast(JobView J, uint32_t tile_base,
           if (fast_full && k >= 16) {
             pix[i] = full_raw;
           } else {
-            uint32_t pv = solid ? sh.u.c.srctab[k] : sh.u.c.ppaint[i][lane];
+            uint32_t pv = solid ? B.srctab[k] : sh.u.c.ppaint[i][lane];
             if (rgba8) {
               if (!solid && k < 16) pv = mask_coverage_rgba8(pv, k);
               pix[i] = src_over_rgba8(pix[i], pv);
