@@ -184,6 +184,7 @@ struct Pen {
       }
     }
     const bool reflect = ctm.det() < 0;
+    v.reserve((size_t)n);
     for (int i = 0; i < n; i++) {
       double t = 2 * M_PI * (double)i / (double)n;
       if (reflect) t = -t;
@@ -328,20 +329,20 @@ struct Poly {
   // The two orientations differ only in which end is the lower one, so the
   // ends are selected and one expression serves both (the same operands as
   // the reference's two branches); the extents take the same comparisons as
-  // selects rather than branches.
-  void add_edge(P a, P b) {
+  // selects rather than branches.  w: the next edge's slot; x: the extents
+  // (valid once the polygon has an edge).  Returns the slot after the
+  // edge's (w itself for a horizontal edge, which is not added).
+  static z2d_edge *add_edge(z2d_edge *w, double (&x)[4], bool first, P a, P b) {
     const bool down = a.y < b.y;
-    if (!down && !(a.y > b.y)) return;  // horizontal (or NaN)
+    if (!down && !(a.y > b.y)) return w;  // horizontal (or NaN)
     const P lo = down ? a : b, hi = down ? b : a;
-    z2d_edge e;
-    e.y0 = a.y;
-    e.y1 = b.y;
-    e.x_start = lo.x;
-    e.x_inc = (hi.x - lo.x) / (hi.y - lo.y);
+    w->y0 = a.y;
+    w->y1 = b.y;
+    w->x_start = lo.x;
+    w->x_inc = (hi.x - lo.x) / (hi.y - lo.y);
     const double top = lo.y, bottom = hi.y;
     const double left = a.x < b.x ? a.x : b.x, right = a.x < b.x ? b.x : a.x;
-    double *x = out->ext;
-    if (out->edges.empty()) {
+    if (first) {
       x[0] = top;
       x[1] = bottom;
       x[2] = left;
@@ -352,26 +353,33 @@ struct Poly {
       x[2] = left < x[2] ? left : x[2];
       x[3] = right > x[3] ? right : x[3];
     }
-    out->edges.push_back(e);
+    return w + 1;
   }
   void add_contour(const Contour &c) {
     const size_t nf = c.front.size(), nb = c.back.size();
     if (nf + nb == 0) return;
     std::vector<z2d_edge> &E = out->edges;
-    if (E.capacity() < E.size() + nf + nb) E.reserve(std::max(E.size() + nf + nb, 2 * E.capacity()));
+    const size_t n0 = E.size();
+    if (E.capacity() < n0 + nf + nb) E.reserve(std::max(n0 + nf + nb, 2 * E.capacity()));
+    E.resize(n0 + nf + nb);  // at most one edge per point
+    z2d_edge *const w0 = E.data() + n0;
+    z2d_edge *w = w0;
+    double x[4] = {out->ext[0], out->ext[1], out->ext[2], out->ext[3]};
     // the sequence: front (stored reversed), then back
     const P first = nf ? c.front[nf - 1] : c.back[0];
     P last = first;
     for (size_t i = nf; i-- > 0;) {
       if (i == nf - 1) continue;  // the first point
-      add_edge(last, c.front[i]);
+      w = add_edge(w, x, n0 == 0 && w == w0, last, c.front[i]);
       last = c.front[i];
     }
     for (size_t i = nf ? 0 : 1; i < nb; i++) {
-      add_edge(last, c.back[i]);
+      w = add_edge(w, x, n0 == 0 && w == w0, last, c.back[i]);
       last = c.back[i];
     }
-    add_edge(last, first);
+    w = add_edge(w, x, n0 == 0 && w == w0, last, first);
+    E.resize(n0 + (size_t)(w - w0));
+    for (int k = 0; k < 4; k++) out->ext[k] = x[k];
   }
 };
 
@@ -534,6 +542,8 @@ struct State {
   // thickness and the transformation)
   Face slot[2];
   int last = -1;
+  // scratch for the points plotted ahead of a contour (empty between uses)
+  std::vector<P> ahead;
 
   void reset_contours() {
     outer.reset();
@@ -563,7 +573,7 @@ void join(State &s, uint32_t mode, P p0, P p1, P p2, bool before) {
   const bool join_cw = in.dev_slope.compare(out.dev_slope) < 0;
   const bool poly_cw = s.cw >= 0 ? s.cw != 0 : join_cw;
   const bool switched = join_cw != poly_cw;
-  std::vector<P> ahead;  // outer points placed ahead (before == true)
+  std::vector<P> &ahead = s.ahead;  // outer points placed ahead (before == true)
   auto plot_outer = [&](P p) {
     if (before) ahead.push_back(s.outer.scaled(p)); else s.outer.plot(p);
   };
@@ -572,6 +582,7 @@ void join(State &s, uint32_t mode, P p0, P p1, P p2, bool before) {
     std::vector<P> &v;
     ~Flush() {
       if (!v.empty()) c.prepend_scaled(v);
+      v.clear();
     }
   } flush{s.outer, ahead};
   auto plot_inner = [&](P p) { s.inner.plot_reverse(p); };
@@ -620,9 +631,10 @@ void plot_open_joined(State &s, P start0, P end0, P start1, P end1) {
   const Face fe = Face::init(start1, end1, o.thickness, o.ctm);
   const bool clockwise = s.cw >= 0 ? s.cw != 0 : true;
   // the start cap goes ahead of the first corner, in order
-  std::vector<P> ahead;
+  std::vector<P> &ahead = s.ahead;
   fs.cap_p0([&](P p) { ahead.push_back(s.outer.scaled(p)); }, o.cap, clockwise, s.pen);
   s.outer.prepend_scaled(ahead);
+  ahead.clear();
   fe.cap_p1([&](P p) { s.outer.plot(p); }, o.cap, clockwise, s.pen);
   s.outer.concat(s.inner);
   s.result.add_contour(s.outer);
