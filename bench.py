@@ -810,6 +810,17 @@ def main():
         line["roofline"]["traffic_source"] = "profiles/k_render_scene_3840x2160_10k.md"
         line["roofline"]["traffic_kernel_sources"] = tr.get("kernel_sources_sha256")
         line["roofline"]["traffic_stale"] = tr.get("kernel_sources_sha256") != kernel_sources_hash()
+        if "valu_insts_per_launch" in tr:
+            # the stage is bound by VALU issue: the counted VALU instructions
+            # of k_render_fast (SQ_INSTS_VALU, same profile) over this run's
+            # stage time, against one wave64 VALU instruction per SIMD per 4
+            # cycles on 1,024 SIMDs at 2.4 GHz
+            slots = 1024 * 2.4e9 / 4 * r["render_ms"] * 1e-3
+            line["roofline"]["valu_issue"] = {
+                "insts": tr["valu_insts_per_launch"],
+                "frac": round(tr["valu_insts_per_launch"] / slots, 4),
+                "frac_traced": tr.get("valu_issue_frac_traced"),
+                "source": "profiles/k_render_scene_3840x2160_10k.md"}
     if not args.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(gpu, ctx, sc, min(args.cpu_paths, args.paths))
