@@ -744,7 +744,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py renders on one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not run", file=sys.stderr)
+              "runs across several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
     from z2d_amd import gpu
     ctx = gpu.Context()

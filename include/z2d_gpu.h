@@ -6,7 +6,7 @@
  * SurfaceCompositor.run (src/compositor.zig:302-440) and Surface.composite
  * (src/surface.zig:225-241).  A Zig shim keeping those signatures marshals
  * its tagged unions into the POD descriptors of z2d_types.h and calls the
- * functions below; INTEGRATION.md shows that binding.
+ * functions below (z2d_amd/zig/painter_gpu.zig is that binding).
  *
  * All functions return a z2d_status code (0 = ok).  Argument checks run on
  * the host before any device work, in the reference's order.
