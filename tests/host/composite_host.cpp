@@ -135,6 +135,26 @@ static uint64_t check_rewrites(uint64_t &cases) {
       if (t == 1) v = std::nextafterf(v, 1e9f);
     }
   }
+  // round_i32 (truncation of v + copysign(0.5 - 2^-54, v)) against
+  // (int32_t)round(v): integers, halves and quarters at every scale from
+  // 2^-60 to 2^30 with their float neighbours, both signs
+  for (int e = -60; e <= 30; e++)
+    for (int64_t k = 0; k < 1024; k++) {
+      const double base = std::ldexp((double)k, e);
+      for (double h : {base, base + 0.5, base - 0.5, base + 0.25, base + 0.75}) {
+        for (int dir = 0; dir < 2; dir++) {
+          double v = h;
+          for (int st = 0; st < 4; st++) {
+            if (std::fabs(v) < 2.0e9) {
+              cases += 2;
+              if (round_i32(v) != (int32_t)std::round(v)) bad++;
+              if (round_i32(-v) != (int32_t)std::round(-v)) bad++;
+            }
+            v = std::nextafter(v, dir ? 1e300 : -1e300);
+          }
+        }
+      }
+    }
   // overlay / hard light on 8-bit lanes against the signed per-channel forms
   for (int i = 0; i < (1 << 24); i++) {
     const uint64_t r = next_u64();

@@ -114,6 +114,15 @@ Z2D_PX __forceinline__ uint32_t f2u8(float v) {
   return (uint32_t)fminf(fmaxf(r, 0.0f), 65535.0f);
 }
 
+// (int32_t)round(v) (half away from zero) as one add and the truncating
+// conversion: trunc(v + copysign(0.5 - 2^-54, v)) is round(v) for every
+// double (v + 0.5 - 2^-54 reaches the next integer exactly when v's
+// fraction is at least one half); out-of-range values convert as the
+// rounded ones do.
+Z2D_PX __forceinline__ int32_t round_i32(double v) {
+  return (int32_t)(v + copysign(0x1.fffffffffffffp-2, v));
+}
+
 // std.math.pow(f32) as the reference's stdlib computes it (the Go-derived
 // algorithm: x^frac via exp(frac*log x), x^int by frexp squaring).
 Z2D_PX inline float zpowf(float x, float y) {

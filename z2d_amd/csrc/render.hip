@@ -552,7 +552,7 @@ __global__ void k_rowprog(JobView J, const uint32_t *draw_list, uint32_t n_list,
     // inc
     for (uint32_t k = 0; k < to; k++) {
       double top = E[k].y0 < E[k].y1 ? E[k].y0 : E[k].y1;
-      X[k] = (int32_t)round(E[k].x_start + (E[k].x_inc * (mid - top)));
+      X[k] = round_i32(E[k].x_start + (E[k].x_inc * (mid - top)));
     }
     // sort (insertion, stable)
     for (uint32_t a = 1; a < to; a++) {
@@ -2049,7 +2049,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               const uint32_t lj = sh.u.a.own[it];
               const REdge ce = sh.u.a.cedge[lj];
               const int sr = sh.clo[lj] + (int)(it - sh.coff[lj]);
-              const int32_t x = (int32_t)round(redge_x(ce.top, ce.x_start, ce.x_inc, ys_base + sr));
+              const int32_t x = round_i32(redge_x(ce.top, ce.x_start, ce.x_inc, ys_base + sr));
               const int32_t xr = x - xs_base;
               if (xr >= cols) continue;
               const int32_t col = xr < 0 ? 0 : xr;
@@ -2131,7 +2131,7 @@ __global__ void __launch_bounds__(RENDER_THREADS, 3) k_render(JobView J, uint32_
               for (uint32_t q = 0; q < ne; q++) {
                 const REdge ed = edges[q];
                 if (ys < ed.ylo || ys > (ed.yhi2 >> 1)) continue;
-                const int32_t x = (int32_t)round(ed.x_start + (ed.x_inc * (mid - ed.top)));
+                const int32_t x = round_i32(ed.x_start + (ed.x_inc * (mid - ed.top)));
                 const int32_t dv = eo ? 1 : ((ed.yhi2 & 1) ? -1 : 1);
                 const int32_t xr = x - xs_base;
   #pragma unroll
@@ -2521,7 +2521,7 @@ k_render_fast(JobView J, uint32_t tile_base,
                          exi = lane_read_f64(ed.x_inc, lj);
             if (!live) continue;
             const int sr = (int)(q >> 16) + (int)(it - (q & 0xFFFFu));
-            const int32_t x = (int32_t)round(redge_x(etop, exs, exi, ys_base + sr));
+            const int32_t x = round_i32(redge_x(etop, exs, exi, ys_base + sr));
             const int32_t xr = x - xs_base;
             if (xr >= COLS) continue;
             const int32_t col = xr < 0 ? 0 : xr;
