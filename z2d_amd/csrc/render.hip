@@ -183,15 +183,34 @@ __device__ bool in_box(const double *ext, double scale, int32_t w, int32_t h) {
   return true;
 }
 
-// Inclusive max over the lanes of a wave up to this one.
+// Inclusive max over the lanes of a wave up to this one, by the DPP steps
+// of wave_incl_scan with a max (no bound_ctrl: a lane without a source keeps
+// its value).  Every lane of the wave must be active.
 __device__ __forceinline__ int32_t wave_max_scan(int32_t v) {
-  const int32_t lane = (int32_t)(threadIdx.x & 63);
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int32_t t = __shfl_up(v, off, 64);
-    if (lane >= off) v = max(v, t);
-  }
+  asm("s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(v));
   return v;
+}
+// A lane's value read by every lane (src wave-uniform).
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, uint32_t src) {
+  return __builtin_amdgcn_readlane(v, src);
+}
+__device__ __forceinline__ double lane_f64(double v, uint32_t src) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // fill_plotter.plot (src/internal/tess/fill_plotter.zig:21-97) with the
@@ -271,19 +290,19 @@ __device__ void plot_parallel(const FNode *nodes, uint32_t n, double scale, z2d_
     }
     if (ebase == 0 && m != 0) {
       const int fl = __ffsll((unsigned long long)m) - 1;
-      f_top = __shfl(top, fl, 64);
-      f_bot = __shfl(bot, fl, 64);
-      f_left = __shfl(left, fl, 64);
-      f_right = __shfl(right, fl, 64);
+      f_top = lane_f64(top, (uint32_t)fl);
+      f_bot = lane_f64(bot, (uint32_t)fl);
+      f_left = lane_f64(left, (uint32_t)fl);
+      f_right = lane_f64(right, (uint32_t)fl);
     }
     ebase += (uint32_t)__popcll(m);
-    c_px = __shfl(nd.x, 63, 64);
-    c_py = __shfl(nd.y, 63, 64);
-    c_ptag = __shfl(tag, 63, 64);
-    c_fx = __shfl(fx, 63, 64);
-    c_fy = __shfl(fy, 63, 64);
-    c_have = __shfl(have ? 1 : 0, 63, 64) != 0;
-    c_cnt = __shfl(cnt_before + a, 63, 64);
+    c_px = lane_f64(nd.x, 63);
+    c_py = lane_f64(nd.y, 63);
+    c_ptag = lane_u32(tag, 63);
+    c_fx = lane_f64(fx, 63);
+    c_fy = lane_f64(fy, 63);
+    c_have = lane_u32(have ? 1u : 0u, 63) != 0;
+    c_cnt = (int32_t)lane_u32((uint32_t)(cnt_before + a), 63);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -902,11 +921,7 @@ __global__ void __launch_bounds__(256) k_flatten_wave(const z2d_path_node *__res
     // S: running count of lines/curves whose end point differs from the
     // previous end point
     uint32_t S = ((tag == Z2D_NODE_LINE_TO || tag == Z2D_NODE_CURVE_TO) && !fpeq(p, pp)) ? 1u : 0u;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(S, off, 64);
-      if (lane >= (uint32_t)off) S += y;
-    }
-    S += c_S;
+    S = (uint32_t)wave_incl_scan((int32_t)S) + c_S;
     // the contour's move: the last move at or before this node, and the last
     // one before it
     const uint64_t mv_mask = __ballot(tag == Z2D_NODE_MOVE_TO);
@@ -960,21 +975,17 @@ __global__ void __launch_bounds__(256) k_flatten_wave(const z2d_path_node *__res
     }
     c_cur += (uint32_t)__popcll(cm);
     if (!EMIT) {
-      uint32_t x = cnt;
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= (uint32_t)off) x += y;
-      }
+      const uint32_t x = (uint32_t)wave_incl_scan((int32_t)cnt);
       if (act) rl[i] = c_out + x - cnt;
-      c_out += __shfl(x, 63, 64);
+      c_out += lane_u32(x, 63);
     }
-    c_tag = __shfl(tag, 63, 64);
-    c_pt = FPt{__shfl(p.x, 63, 64), __shfl(p.y, 63, 64)};
-    c_S = __shfl(S, 63, 64);
+    c_tag = lane_u32(tag, 63);
+    c_pt = FPt{lane_f64(p.x, 63), lane_f64(p.y, 63)};
+    c_S = lane_u32(S, 63);
     if (mv_mask) {
-      const int last = 63 - __builtin_clzll(mv_mask);
-      c_mv = FPt{__shfl(p.x, last, 64), __shfl(p.y, last, 64)};
-      c_mvS = __shfl(S, last, 64);
+      const uint32_t last = 63u - (uint32_t)__builtin_clzll(mv_mask);
+      c_mv = FPt{lane_f64(p.x, last), lane_f64(p.y, last)};
+      c_mvS = lane_u32(S, last);
       c_mv_ok = true;
     }
   }
@@ -1270,12 +1281,7 @@ __global__ void __launch_bounds__(1024) k_scan(uint32_t *a, uint32_t n) {
       sum += v[k];
     }
     // wave inclusive scan of the lane sums
-    uint32_t x = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(x, off, 64);
-      if (lane >= (uint32_t)off) x += y;
-    }
+    const uint32_t x = (uint32_t)wave_incl_scan((int32_t)sum);
     if (lane == 63) wsum[w] = x;
     __syncthreads();
     if (t < 16) {
@@ -1314,8 +1320,7 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t *a, uint32_t
   uint32_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 16; k++) sum += i0 + k < n ? a[i0 + k] : 0u;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  sum = __builtin_amdgcn_readlane((uint32_t)wave_incl_scan((int32_t)sum), 63);
   if ((t & 63) == 0) ws[t >> 6] = sum;
   __syncthreads();
   if (t == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
@@ -1334,12 +1339,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(uint32_t *a, uint32_t n,
     v[k] = i0 + k < n ? a[i0 + k] : 0u;
     sum += v[k];
   }
-  uint32_t x = sum;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off, 64);
-    if (lane >= (uint32_t)off) x += y;
-  }
+  const uint32_t x = (uint32_t)wave_incl_scan((int32_t)sum);
   if (lane == 63) ws[w] = x;
   __syncthreads();
   uint32_t run = boff[blockIdx.x] + x - sum;
