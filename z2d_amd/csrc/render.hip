@@ -212,6 +212,48 @@ __device__ __forceinline__ double lane_f64(double v, uint32_t src) {
   const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
+// The value of the lane below (DPP wave_shr:1); lane 0, which has none,
+// gets `first`.
+__device__ __forceinline__ uint32_t lane_prev_u32(uint32_t v, uint32_t first) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double lane_prev_f64(double v, double first) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v), f = (uint64_t)__double_as_longlong(first);
+  const uint32_t lo = lane_prev_u32((uint32_t)u, (uint32_t)f);
+  const uint32_t hi = lane_prev_u32((uint32_t)(u >> 32), (uint32_t)(f >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// v from the lane `k` below within the lane's row of 16 (DPP row_shr:k);
+// the first k lanes of each row get `fill`.
+template <int K>
+__device__ __forceinline__ double row_shr_f64(double v, double fill) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v), f = (uint64_t)__double_as_longlong(fill);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)u,
+                                                            0x110 + K, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(
+      (int)(uint32_t)(f >> 32), (int)(uint32_t)(u >> 32), 0x110 + K, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// fmin / fmax of a double over the 64 lanes as a wave-uniform value: a
+// prefix within each row of 16, then the four rows' last lanes.  (fmin and
+// fmax drop NaN operands in any order, so the order of the steps does not
+// change the result.)  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_fmin_f64(double v) {
+  const double inf = __builtin_huge_val();
+  v = fmin(v, row_shr_f64<1>(v, inf));
+  v = fmin(v, row_shr_f64<2>(v, inf));
+  v = fmin(v, row_shr_f64<4>(v, inf));
+  v = fmin(v, row_shr_f64<8>(v, inf));
+  return fmin(fmin(lane_f64(v, 15), lane_f64(v, 31)), fmin(lane_f64(v, 47), lane_f64(v, 63)));
+}
+__device__ __forceinline__ double wave_fmax_f64(double v) {
+  const double ninf = -__builtin_huge_val();
+  v = fmax(v, row_shr_f64<1>(v, ninf));
+  v = fmax(v, row_shr_f64<2>(v, ninf));
+  v = fmax(v, row_shr_f64<4>(v, ninf));
+  v = fmax(v, row_shr_f64<8>(v, ninf));
+  return fmax(fmax(lane_f64(v, 15), lane_f64(v, 31)), fmax(lane_f64(v, 47), lane_f64(v, 63)));
+}
 
 // fill_plotter.plot (src/internal/tess/fill_plotter.zig:21-97) with the
 // PointBuffer(1, 3) of src/internal/tess/point_buffer.zig, one lane per
@@ -241,13 +283,8 @@ __device__ void plot_parallel(const FNode *nodes, uint32_t n, double scale, z2d_
     if (k < n) nd = nodes[k];
     uint32_t tag = nd.tag;
     if (tag == Z2D_NODE_MOVE_TO && k == n - 1) tag = 0xFFFFFFFFu;  // a final move is ignored
-    double px = __shfl_up(nd.x, 1, 64), py = __shfl_up(nd.y, 1, 64);
-    uint32_t ptag = __shfl_up(tag, 1, 64);
-    if (lane == 0) {
-      px = c_px;
-      py = c_py;
-      ptag = c_ptag;
-    }
+    const double px = lane_prev_f64(nd.x, c_px), py = lane_prev_f64(nd.y, c_py);
+    const uint32_t ptag = lane_prev_u32(tag, c_ptag);
     const int32_t head = wave_max_scan(tag == Z2D_NODE_MOVE_TO ? (int32_t)lane : -1);
     const int32_t hsrc = head >= 0 ? head : 0;
     const double hx = __shfl(nd.x, hsrc, 64), hy = __shfl(nd.y, hsrc, 64);
@@ -304,13 +341,10 @@ __device__ void plot_parallel(const FNode *nodes, uint32_t n, double scale, z2d_
     c_have = lane_u32(have ? 1u : 0u, 63) != 0;
     c_cnt = (int32_t)lane_u32((uint32_t)(cnt_before + a), 63);
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    m_top = fmin(m_top, __shfl_xor(m_top, off, 64));
-    m_bot = fmax(m_bot, __shfl_xor(m_bot, off, 64));
-    m_left = fmin(m_left, __shfl_xor(m_left, off, 64));
-    m_right = fmax(m_right, __shfl_xor(m_right, off, 64));
-  }
+  m_top = wave_fmin_f64(m_top);
+  m_bot = wave_fmax_f64(m_bot);
+  m_left = wave_fmin_f64(m_left);
+  m_right = wave_fmax_f64(m_right);
   acc.n = ebase;
   if (ebase) {
     acc.top = f_top != f_top ? f_top : m_top;
@@ -911,12 +945,8 @@ __global__ void __launch_bounds__(256) k_flatten_wave(const z2d_path_node *__res
       else if (tag == Z2D_NODE_CURVE_TO) p = FPt{nd.v[4], nd.v[5]};
     }
     // the predecessor's tag and end point
-    uint32_t ptag = __shfl_up(tag, 1, 64);
-    FPt pp{__shfl_up(p.x, 1, 64), __shfl_up(p.y, 1, 64)};
-    if (lane == 0) {
-      ptag = c_tag;
-      pp = c_pt;
-    }
+    const uint32_t ptag = lane_prev_u32(tag, c_tag);
+    const FPt pp{lane_prev_f64(p.x, c_pt.x), lane_prev_f64(p.y, c_pt.y)};
     const uint32_t ntag = (act && i + 1 < n) ? nodes[i + 1].tag : NONE;
     // S: running count of lines/curves whose end point differs from the
     // previous end point
