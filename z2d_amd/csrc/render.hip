@@ -1058,6 +1058,10 @@ __device__ __forceinline__ void spline_task(FPt a, FPt b, FPt c, FPt d, double t
   uint32_t depth = FLAT_TASK_DEPTH;
   uint64_t path = t;
   uint32_t visits = 0;
+  // the visited piece's parent (set once the walk is below the task's
+  // root): a first half's sibling is the parent's second half, the same
+  // value the recomputation from the root gives
+  FKnots par = k;
   for (;;) {
     bool leaf = knots_err_sq(k) < tol_sq || depth > 64;
     bool runaway = false;
@@ -1075,6 +1079,12 @@ __device__ __forceinline__ void spline_task(FPt a, FPt b, FPt c, FPt d, double t
       if (!fpeq(k.a, a)) pt(k.a);
     }
     if (leaf) {
+      if (depth > FLAT_TASK_DEPTH && !(path & 1u)) {
+        if (runaway) break;
+        path |= 1u;  // the second half of the same parent
+        k = knots_half(par, true);
+        continue;
+      }
       while (depth > FLAT_TASK_DEPTH && (path & 1u)) {
         path >>= 1;
         depth--;
@@ -1082,10 +1092,14 @@ __device__ __forceinline__ void spline_task(FPt a, FPt b, FPt c, FPt d, double t
       if (depth == FLAT_TASK_DEPTH || runaway) break;
       path |= 1u;
       k = root;
-      for (uint32_t i = 0; i < depth; i++) k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
+      for (uint32_t i = 0; i < depth; i++) {
+        if (i + 1 == depth) par = k;
+        k = knots_half(k, (path >> (depth - 1 - i)) & 1u);
+      }
     } else {
       depth++;
       path <<= 1;
+      par = k;
       k = knots_half(k, false);
     }
   }
