@@ -1289,7 +1289,10 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   const size_t o_slotcnt = L.take(((size_t)n_slots + 1) * sizeof(uint32_t));
   const uint64_t scan_n = std::max<uint64_t>(
       std::max<uint64_t>(n_slots, max_entries), (uint64_t)n_tiles * spt);
-  const size_t o_scantmp = L.take((size_t)(scan_n / 4096 + 2) * sizeof(uint32_t));
+  // per scan: a ticket and a status word per workgroup (at most
+  // scan_n / SCAN_BLOCK + 1 workgroups)
+  const uint32_t scan_words = (uint32_t)(scan_n / SCAN_BLOCK + 2);
+  const size_t o_scantmp = L.take((size_t)3 * scan_words * sizeof(uint64_t));
   const size_t o_drows = L.take((size_t)nd * sizeof(int2));
   const size_t o_entries = L.take(max_entries * sizeof(BandEntry));
   const size_t o_blist = L.take(max_entries * sizeof(uint4));
@@ -1417,7 +1420,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   J.band_slot_off = reinterpret_cast<const uint32_t *>(D + o_bso);
   J.n_slots = (uint32_t)n_slots;
   J.slot_cnt = reinterpret_cast<uint32_t *>(D + o_slotcnt);
-  J.scan_tmp = reinterpret_cast<uint32_t *>(D + o_scantmp);
+  J.scan_tmp = reinterpret_cast<uint64_t *>(D + o_scantmp);
+  J.scan_words = scan_words;
   J.entries = reinterpret_cast<BandEntry *>(D + o_entries);
   J.max_entries = (uint32_t)max_entries;
   J.entry_ecnt = reinterpret_cast<uint32_t *>(D + o_ecnt);
@@ -1487,6 +1491,7 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
       add(J.entry_ecnt, 2ull * J.max_entries + 1, 0u);
       add(reinterpret_cast<uint32_t *>(J.entry_xlo), J.max_entries, 0x7FFFFFFFu);
       add(reinterpret_cast<uint32_t *>(J.entry_xhi), J.max_entries, 0x80000000u);
+      add(reinterpret_cast<uint32_t *>(J.scan_tmp), 6ull * J.scan_words, 0u);
     }
     e = z2d_launch_fill_set(F, st);
   }
@@ -1575,7 +1580,7 @@ static int32_t batch_wait(z2d_gpu_batch *b) {
   // wrong, so the render is an error (never expected: the bounds are upper
   // bounds of what the kernels write).
   if (b->ctr_host.p && reinterpret_cast<const uint32_t *>(b->ctr_host.p)[2]) {
-    ctx->err = "render: a bin exceeded its capacity bound";
+    ctx->err = "render: a bin exceeded its capacity bound, or a scan's look-back timed out";
     return Z2D_ERR_DEVICE;
   }
   if (b->ev[0] && b->ev[2]) {

@@ -57,6 +57,9 @@ struct FlatJob {
 // depth belongs to the leftmost piece under it).
 constexpr uint32_t FLAT_TASK_DEPTH = 4;
 constexpr uint32_t FLAT_TASKS = 1u << FLAT_TASK_DEPTH;
+
+// Values per workgroup of the binning's exclusive scans (256 lanes x 16).
+constexpr uint32_t SCAN_BLOCK = 4096;
 struct FlatTask {  // count pass, per curve piece
   uint32_t cnt, flags;
   uint64_t pairs;   // (edge, band) bound between the piece's own points
@@ -225,7 +228,10 @@ struct JobView {
   const uint32_t *band_slot_off;  // n_bands: the band's first slot
   uint32_t n_slots;
   uint32_t *slot_cnt;             // n_slots + 1 (scanned in place)
-  uint32_t *scan_tmp;             // block sums of the scans
+  // the scans' look-back words: three regions of scan_words u64 (a ticket,
+  // then one status word per workgroup), zero at the start of each render
+  uint64_t *scan_tmp;
+  uint32_t scan_words;
   BandEntry *entries;    // per draw and estimated band, capacity max_entries
   uint32_t max_entries;
   uint32_t *entry_ecnt;      // max_entries + 1: per-entry edge counts, scanned in place

@@ -1351,62 +1351,112 @@ __global__ void __launch_bounds__(1024) k_scan(uint32_t *a, uint32_t n) {
   if (t == 0) a[n] = carry;
 }
 
-// Parallel form of k_scan for long arrays: block sums (k_scan_reduce), their
-// scan (k_scan), then each block's values scanned from its offset
-// (k_scan_apply).  SCAN_BLOCK values per block, 16 per lane.
-constexpr uint32_t SCAN_BLOCK = 4096;
+// Exclusive scan of a[0..n) in place with a[n] = total, in one pass
+// (decoupled look-back): SCAN_BLOCK values per workgroup, the workgroups
+// numbered by a ticket taken when they start, so each waits only on
+// workgroups already running.  A workgroup publishes its aggregate, then its
+// inclusive prefix, in its status word (flag << 32 | value); its first wave
+// sums its predecessors' words, 64 at a time, back to the nearest inclusive
+// prefix.  stat[0] is the ticket and stat[1 + b] the status of workgroup b,
+// all zero at launch.  A wait that outlasts SCAN_SPIN_LIMIT polls (never
+// expected) sets bit 1 of *err and goes on, so every workgroup ends.
+constexpr uint64_t SCAN_AGG = 1ull << 32, SCAN_INCL = 2ull << 32;
+constexpr uint32_t SCAN_SPIN_LIMIT = 1u << 20;
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t *a, uint32_t n,
-                                                     uint32_t *bsum) {
+__global__ void __launch_bounds__(256) k_scan_lb(uint32_t *a, uint32_t n, uint64_t *stat,
+                                                 uint32_t *err) {
   __shared__ uint32_t ws[4];
-  const uint32_t t = threadIdx.x;
-  const uint32_t i0 = blockIdx.x * SCAN_BLOCK + t * 16;
-  uint32_t sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 16; k++) sum += i0 + k < n ? a[i0 + k] : 0u;
-  sum = __builtin_amdgcn_readlane((uint32_t)wave_incl_scan((int32_t)sum), 63);
-  if ((t & 63) == 0) ws[t >> 6] = sum;
-  __syncthreads();
-  if (t == 0) bsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
-}
-
-// boff: the scanned block sums; the last lane of the last block writes a[n].
-__global__ void __launch_bounds__(256) k_scan_apply(uint32_t *a, uint32_t n,
-                                                    const uint32_t *boff) {
-  __shared__ uint32_t ws[4];
+  __shared__ uint32_t s_bid, s_off;
   const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const uint32_t i0 = blockIdx.x * SCAN_BLOCK + t * 16;
+  if (t == 0) s_bid = (uint32_t)atomicAdd(reinterpret_cast<unsigned long long *>(stat), 1ull);
+  __syncthreads();
+  const uint32_t bid = s_bid;
+  if (bid >= gridDim.x) return;  // (uniform) the ticket starts at zero
+  const uint32_t i0 = bid * SCAN_BLOCK + t * 16;
+  // a lane's 16 values: four 16-byte loads when all lie below n (a is
+  // 16-byte aligned and i0 a multiple of 16)
+  const bool whole = i0 + 16 <= n;
   uint32_t v[16];
+  if (whole) {
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+      const uint4 u = reinterpret_cast<const uint4 *>(a + i0)[q];
+      v[4 * q] = u.x;
+      v[4 * q + 1] = u.y;
+      v[4 * q + 2] = u.z;
+      v[4 * q + 3] = u.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) v[k] = i0 + k < n ? a[i0 + k] : 0u;
+  }
   uint32_t sum = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < 16; k++) {
-    v[k] = i0 + k < n ? a[i0 + k] : 0u;
-    sum += v[k];
-  }
+  for (uint32_t k = 0; k < 16; k++) sum += v[k];
   const uint32_t x = (uint32_t)wave_incl_scan((int32_t)sum);
   if (lane == 63) ws[w] = x;
   __syncthreads();
-  uint32_t run = boff[blockIdx.x] + x - sum;
+  if (w == 0) {
+    const uint32_t total = ws[0] + ws[1] + ws[2] + ws[3];
+    uint64_t *const st = stat + 1;
+    uint32_t off = 0;
+    if (lane == 0)
+      __hip_atomic_store(&st[bid], (bid == 0 ? SCAN_INCL : SCAN_AGG) | total, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (bid != 0) {
+      uint32_t spins = 0;
+      for (int32_t base = (int32_t)bid - 1;; base -= 64) {
+        const int32_t j = base - (int32_t)lane;  // lane 0 the nearest predecessor
+        uint64_t s = SCAN_INCL;                  // before workgroup 0: an inclusive zero
+        for (;;) {
+          if (j >= 0) s = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (__ballot((s >> 32) == 0) == 0) break;
+          if (++spins > SCAN_SPIN_LIMIT) {
+            if (lane == 0) atomicOr(err, 2u);
+            s = SCAN_INCL | (s & 0xFFFFFFFFull);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t pm = __ballot((s >> 32) == 2);
+        const uint32_t first = pm ? (uint32_t)__builtin_ctzll(pm) : 63u;
+        const uint32_t part = (uint32_t)wave_incl_scan(lane <= first ? (int32_t)(uint32_t)s : 0);
+        off += (uint32_t)__builtin_amdgcn_readlane((int32_t)part, 63);
+        if (pm) break;
+      }
+      if (lane == 0)
+        __hip_atomic_store(&st[bid], SCAN_INCL | (off + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_off = off;
+  }
+  __syncthreads();
+  uint32_t run = s_off + x - sum;
   for (uint32_t j = 0; j < w; j++) run += ws[j];
+  uint32_t o[16];
 #pragma unroll
   for (uint32_t k = 0; k < 16; k++) {
-    if (i0 + k < n) a[i0 + k] = run;
+    o[k] = run;
     run += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && t == 255) a[n] = run;
+  if (whole) {
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++)
+      reinterpret_cast<uint4 *>(a + i0)[q] =
+          make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++)
+      if (i0 + k < n) a[i0 + k] = o[k];
+  }
+  if (bid == gridDim.x - 1 && t == 255) a[n] = run;
 }
 
-// Exclusive scan of a[0..n) in place with a[n] = total; tmp holds
-// n / SCAN_BLOCK + 2 values.
-static void scan_u32(uint32_t *a, uint32_t n, uint32_t *tmp, hipStream_t st) {
-  const uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  if (nb <= 1) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, a, n);
-    return;
-  }
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, st, a, n, tmp);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, tmp, nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, a, n, tmp);
+// Exclusive scan of a[0..n) in place with a[n] = total; stat holds
+// n / SCAN_BLOCK + 2 zeroed words.
+static void scan_u32(uint32_t *a, uint32_t n, uint64_t *stat, uint32_t *err, hipStream_t st) {
+  const uint32_t nb = n == 0 ? 1u : (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  hipLaunchKernelGGL(k_scan_lb, dim3(nb), dim3(256), 0, st, a, n, stat, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -3040,17 +3090,18 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
   // first fill, z2d_launch_fill_set)
   dim3 ggrid((J.n_egroups + 3) / 4);
   if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<false>, ggrid, dim3(256), 0, st, J);
-  scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, st);
+  scan_u32(J.entry_ecnt, J.max_entries, J.scan_tmp, J.counters + 2, st);
   if (J.n_egroups) hipLaunchKernelGGL(k_edge_bins<true>, ggrid, dim3(256), 0, st, J);
   // band lists and entry records
   dim3 grid((J.n_slots + 3) / 4);
   hipLaunchKernelGGL(k_band_pass<false>, grid, dim3(256), 0, st, J);
-  scan_u32(J.slot_cnt, J.n_slots, J.scan_tmp, st);
+  scan_u32(J.slot_cnt, J.n_slots, J.scan_tmp + J.scan_words, J.counters + 2, st);
   hipLaunchKernelGGL(k_band_pass<true>, grid, dim3(256), 0, st, J);
   if (J.n_tiles) {
     dim3 tgrid((J.n_tiles + 3) / 4);
     hipLaunchKernelGGL(k_strip_pass<false>, tgrid, dim3(256), 0, st, J);
-    scan_u32(J.strip_cnt, J.n_tiles * J.strips_per_tile, J.scan_tmp, st);
+    scan_u32(J.strip_cnt, J.n_tiles * J.strips_per_tile, J.scan_tmp + 2 * J.scan_words,
+             J.counters + 2, st);
     hipLaunchKernelGGL(k_strip_pass<true>, tgrid, dim3(256), 0, st, J);
     if (J.strip_order) {
       const uint32_t ns = J.n_tiles * J.strips_per_tile;
