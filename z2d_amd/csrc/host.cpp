@@ -45,7 +45,8 @@ hipError_t z2d_launch_composite_quad(const z2d::QuadJob &job, uint64_t n_quads, 
 hipError_t z2d_launch_flatten(const z2d_path_node *raw, const z2d::FlatJob *jobs, uint32_t n_jobs,
                               z2d::FNode *out, z2d::FlatResult *res, uint32_t *rel,
                               const uint2 *curves, uint32_t n_curves, z2d::FlatTask *tasks,
-                              uint32_t *toff, z2d::FlatCurve *cv, bool emit, hipStream_t st);
+                              uint32_t *toff, z2d::FlatCurve *cv, bool emit, hipStream_t st,
+                              const z2d::FillSet *fill, bool any_serial);
 hipError_t z2d_launch_composite(const z2d::CompJob *dev_job, const z2d::CompJob &host_job,
                                 bool stream, hipStream_t st);
 hipError_t z2d_launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
@@ -775,6 +776,7 @@ struct z2d_gpu_batch {
   DevBuf flat_buf;       // device flattening inputs and results
   HostBuf flat_stage;
   uint32_t n_flat_jobs = 0;  // draws flattened on the device at every render
+  uint32_t n_flat_serial = 0;  // of which by the per-draw walk
   const FlatJob *d_flat_jobs = nullptr;
   FNode *d_flat_out = nullptr;
   uint32_t *d_flat_rel = nullptr;
@@ -1132,7 +1134,8 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
                                    reinterpret_cast<const uint2 *>(FD + fo_curves), n_curves,
                                    reinterpret_cast<FlatTask *>(FD + fo_tasks),
                                    reinterpret_cast<uint32_t *>(FD + fo_toff),
-                                   reinterpret_cast<FlatCurve *>(FD + fo_cv), false, st));
+                                   reinterpret_cast<FlatCurve *>(FD + fo_cv), false, st,
+                                   nullptr, true));
     HIP_OK(ctx, hipMemcpyAsync(FH + fo_res, FD + fo_res, jobs.size() * sizeof(FlatResult),
                                hipMemcpyDeviceToHost, st));
     const auto cw0 = std::chrono::steady_clock::now();
@@ -1383,6 +1386,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
   JobView &J = b->view;
   J.nodes = reinterpret_cast<const FNode *>(D + o_nodes);
   b->n_flat_jobs = 0;
+  b->n_flat_serial = 0;
   if (!jobs.empty()) {
     // the emit pass runs at the start of every render (batch_enqueue): the
     // flattened nodes at their draws' offsets, from the resident raw nodes
@@ -1391,6 +1395,7 @@ static int32_t batch_prepare(z2d_gpu_batch *b) {
       hj[k].node_off = b->draws[job_draw[k]].d.node_off;
       hj[k].serial = jobs[k].serial;
       hj[k].raw_cnt = jobs[k].raw_cnt;
+      b->n_flat_serial += jobs[k].serial ? 1u : 0u;
     }
     HIP_OK(ctx, hipMemcpyAsync(b->flat_buf.p + fo_jobs, hj, jobs.size() * sizeof(FlatJob),
                                hipMemcpyHostToDevice, st));
@@ -1467,34 +1472,34 @@ static hipError_t batch_enqueue(z2d_gpu_batch *b, uint32_t mode, bool mark_rende
   JobView &J = b->view;
   const uint32_t ns = J.n_surfaces;
   const hipStream_t st = bstream(b);
+  // every per-render reset: counters, render cursors, the entries' edge
+  // counts and cursors, their column ranges (empty until the edge binning
+  // widens them) and the scans' look-back words; done by the flattening's
+  // per-node kernel when there is one, else in a launch of their own
+  FillSet F{};
+  auto add = [&](uint32_t *p, uint64_t n, uint32_t v) {
+    if (n == 0) return;
+    F.p[F.count] = p;
+    F.n[F.count] = n;
+    F.v[F.count] = v;
+    F.count++;
+  };
+  add(J.counters, 16, 0u);
+  add(J.cursors, 2 * ((uint64_t)ns + 1), 0u);
+  if (J.n_bands) {
+    add(J.entry_ecnt, 2ull * J.max_entries + 1, 0u);
+    add(reinterpret_cast<uint32_t *>(J.entry_xlo), J.max_entries, 0x7FFFFFFFu);
+    add(reinterpret_cast<uint32_t *>(J.entry_xhi), J.max_entries, 0x80000000u);
+    add(reinterpret_cast<uint32_t *>(J.scan_tmp), 6ull * J.scan_words, 0u);
+  }
   hipError_t e = hipSuccess;
   if (b->n_flat_jobs)
     e = z2d_launch_flatten(reinterpret_cast<const z2d_path_node *>(b->flat_buf.p),
                            b->d_flat_jobs, b->n_flat_jobs, b->d_flat_out, nullptr,
                            b->d_flat_rel, b->d_flat_curves, b->n_flat_curves, b->d_flat_tasks,
-                           b->d_flat_toff, nullptr, true, st);
-  if (e == hipSuccess) {
-    // every per-render reset in one launch: counters, render cursors, the
-    // entries' edge counts and cursors, their column ranges (empty until
-    // the edge binning widens them)
-    FillSet F{};
-    auto add = [&](uint32_t *p, uint64_t n, uint32_t v) {
-      if (n == 0) return;
-      F.p[F.count] = p;
-      F.n[F.count] = n;
-      F.v[F.count] = v;
-      F.count++;
-    };
-    add(J.counters, 16, 0u);
-    add(J.cursors, 2 * ((uint64_t)ns + 1), 0u);
-    if (J.n_bands) {
-      add(J.entry_ecnt, 2ull * J.max_entries + 1, 0u);
-      add(reinterpret_cast<uint32_t *>(J.entry_xlo), J.max_entries, 0x7FFFFFFFu);
-      add(reinterpret_cast<uint32_t *>(J.entry_xhi), J.max_entries, 0x80000000u);
-      add(reinterpret_cast<uint32_t *>(J.scan_tmp), 6ull * J.scan_words, 0u);
-    }
+                           b->d_flat_toff, nullptr, true, st, &F, b->n_flat_serial != 0);
+  else
     e = z2d_launch_fill_set(F, st);
-  }
   if (e == hipSuccess) e = z2d_launch_edge_build(J, st);
   if (e == hipSuccess)
     e = z2d_launch_rowprog(J, b->d_rowprog_list, b->n_rowprog, b->d_scratch_edges,

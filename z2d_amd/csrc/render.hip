@@ -912,8 +912,17 @@ __global__ void __launch_bounds__(256) k_flatten_wave(const z2d_path_node *__res
                                                       const FlatJob *__restrict__ jobs,
                                                       uint32_t n_jobs, FNode *out,
                                                       FlatResult *res, uint32_t *rel,
-                                                      const FlatCurve *__restrict__ cv) {
+                                                      const FlatCurve *__restrict__ cv,
+                                                      FillSet fill) {
   constexpr uint32_t NONE = 0xFFFFFFFFu;
+  if (EMIT) {
+    // the render's per-step resets (none of them is read by the flattening),
+    // here instead of a launch of their own
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t f = 0; f < fill.count; f++)
+      for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fill.n[f]; i += stride)
+        fill.p[f][i] = fill.v[f];
+  }
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= n_jobs) return;
@@ -3119,22 +3128,28 @@ hipError_t z2d_launch_binning(const JobView &J, hipStream_t st) {
 // Count pass (emit = false): the curve pieces, their combination per curve,
 // the per-node pass for every draw, then the per-draw walk for the draws it
 // marked FLAT_NONSIMPLE.  Emit pass: each draw by the form its
-// FlatJob.serial names, curves by pieces.
+// FlatJob.serial names (the per-draw walk launched only when any_serial),
+// curves by pieces, and the fills of *fill (when given) done by the per-node
+// kernel's workgroups before their draws.
 hipError_t z2d_launch_flatten(const z2d_path_node *raw, const FlatJob *jobs, uint32_t n_jobs,
                               FNode *out, FlatResult *res, uint32_t *rel, const uint2 *curves,
                               uint32_t n_curves, FlatTask *tasks, uint32_t *toff, FlatCurve *cv,
-                              bool emit, hipStream_t st) {
+                              bool emit, hipStream_t st, const FillSet *fill,
+                              bool any_serial) {
   if (n_jobs == 0) return hipSuccess;
+  const FillSet no_fill{};
   const dim3 wgrid((n_jobs + 3) / 4), sgrid((n_jobs + 63) / 64);
   const dim3 tgrid((uint32_t)(((uint64_t)n_curves * FLAT_TASKS + 255) / 256)),
       cgrid((n_curves + 255) / 256);
   if (emit) {
     hipLaunchKernelGGL(k_flatten_wave<true>, wgrid, dim3(256), 0, st, raw, jobs, n_jobs, out, res,
-                       rel, cv);
+                       rel, cv, fill ? *fill : no_fill);
     if (n_curves)
       hipLaunchKernelGGL(k_curve_tasks<true>, tgrid, dim3(256), 0, st, raw, jobs, curves, n_curves,
                          tasks, toff, rel, out);
-    hipLaunchKernelGGL(k_flatten_serial<true>, sgrid, dim3(64), 0, st, raw, jobs, n_jobs, out, res);
+    if (any_serial)
+      hipLaunchKernelGGL(k_flatten_serial<true>, sgrid, dim3(64), 0, st, raw, jobs, n_jobs, out,
+                         res);
   } else {
     if (n_curves) {
       hipLaunchKernelGGL(k_curve_tasks<false>, tgrid, dim3(256), 0, st, raw, jobs, curves,
@@ -3143,7 +3158,7 @@ hipError_t z2d_launch_flatten(const z2d_path_node *raw, const FlatJob *jobs, uin
                          tasks, toff, cv);
     }
     hipLaunchKernelGGL(k_flatten_wave<false>, wgrid, dim3(256), 0, st, raw, jobs, n_jobs, out, res,
-                       rel, cv);
+                       rel, cv, no_fill);
     hipLaunchKernelGGL(k_flatten_serial<false>, sgrid, dim3(64), 0, st, raw, jobs, n_jobs, out,
                        res);
   }
